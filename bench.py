@@ -1,0 +1,155 @@
+#!/usr/bin/env python3
+"""Benchmark of the BPM explicit step on one MI355X.
+
+Prints ONE JSON line (rank 0):
+  metric  bond-updates/s over whole explicit steps of a uniaxial-compression
+          run (cell sort, list reconstruct, bond + contact forces with the
+          break check, platens, integration, cut, prune, loading control)
+  roofline  the bond-force kernel: algorithmic bytes per launch / its average
+          duration measured with HIP events on the launch stream
+  cpu_baseline  the oracle (single-threaded C restatement) on a bounded
+          sample of the same kind of step, on this host's cores
+
+One step = one pass of the hot path over the whole specimen.  Inputs are
+synthetic (seeded FCC-lattice cylinder) and resident in HBM before timing.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "rockfit-dem_amd"))
+sys.path.insert(0, str(ROOT))
+
+PEAK_HBM = 8.0e12  # B/s, MI355X
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=50)
+    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--tier", choices=["ref", "native"], default="ref")
+    p.add_argument("--D", type=float, default=100.0)
+    p.add_argument("--H", type=float, default=200.0)
+    p.add_argument("--cpu-seconds", type=float, default=10.0)
+    p.add_argument("--no-cpu", action="store_true")
+    return p.parse_args()
+
+
+def cpu_baseline_ref(seconds: float) -> dict:
+    """Oracle driver (single thread) on a 25 x 50 mm cylinder, the size of the
+    reference's own 28,593-particle sample (kernel.cu:166)."""
+    from oracle.oracle import OracleUniaxialDriver
+    from rfd import cases
+
+    c = cases.reference_uniaxial(25.0, 50.0)
+    d = OracleUniaxialDriver(c.R, c.mat, c.box, c.cmp, c.grid(), c.K)
+    d.setup()
+    d.step()
+    nb = int((d.s.ILtype == 1).sum()) // 2
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        d.step()
+        n += 1
+    dt = time.perf_counter() - t0
+    return {"value": nb * n / dt, "unit": "bond-updates/s", "cores": 1, "kind": "port",
+            "sample": f"oracle step loop, {n} explicit steps of a {c.N}-particle "
+                      f"D25xH50 mm cylinder ({nb} bonds), single thread"}
+
+
+def bench_ref(args) -> dict:
+    import numpy as np
+    import torch
+
+    from rfd import cases
+    from rfd.reflayout import Context, UniaxialDriver
+
+    ctx = Context(0)
+    case = cases.reference_uniaxial(args.D, args.H)
+    drv = UniaxialDriver(ctx, case.R, case.mat, case.box, case.cmp, case.K)
+    drv.setup()
+    print(f"[bench] ref tier: N={case.N} K={case.K}", file=sys.stderr, flush=True)
+    s = drv.s
+    # HIP events around each bond-force launch (same null stream as the ctx)
+    ev = []
+    orig = s.forces_bpm
+
+    def timed_forces(m):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        orig(m)
+        b.record()
+        ev.append((a, b))
+
+    s.forces_bpm = timed_forces
+    for _ in range(args.warmup):
+        drv.step()
+    torch.cuda.synchronize()
+    ev.clear()
+    nb = s.bond_slots() // 2
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        drv.step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    nb_end = s.bond_slots() // 2
+    ms = (t1 - t0) * 1e3 / args.steps
+    N = case.N
+    alg_bytes = 128 * nb + (12 + 48) * N
+    achieved = alg_bytes / (kern_ms * 1e-3)
+    return {
+        "value": nb * args.steps / (t1 - t0),
+        "ms_per_step": ms,
+        "dtype": "f32",
+        "config": {"workload": f"uniaxial cylinder D{args.D:g}xH{args.H:g} mm, reference slot "
+                               f"layout K={case.K}, monodisperse r=0.5 mm",
+                   "particles": N, "bonds": nb, "bonds_end": nb_end, "tier": "ref"},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": None,
+                     "kernel": "k_forces_bpm", "kernel_ms": kern_ms,
+                     "alg_bytes_per_launch": alg_bytes},
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU runs are not part of this build", file=sys.stderr)
+        sys.exit(2)
+    res = bench_ref(args) if args.tier == "ref" else None
+    if res is None:
+        from rfd import native_bench  # noqa: F401
+
+        res = native_bench.run(args)
+    out = {
+        "metric": "bond-updates/s",
+        "value": res["value"],
+        "unit": "bond-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": res["dtype"],
+        "data": "synthetic",
+        "config": res["config"],
+        "roofline": res["roofline"],
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_ref(args.cpu_seconds)
+        out["cpu_baseline"]["host_cores_available"] = os.cpu_count()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,220 @@
+"""numpy/ctypes wrapper of the CPU oracle (oracle/rfd_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker.  The product never imports it.
+
+``OracleLayout`` holds the reference slot-layout arrays in host memory and
+has the same stage methods as rfd.reflayout.RefLayout, so that a parity test
+runs both through the same sequence and compares the arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+import subprocess
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "librfd_oracle.so"
+
+_u32, _f32, _int = C.c_uint32, C.c_float, C.c_int
+_P = C.c_void_p
+_D = C.POINTER(C.c_double)
+
+_SIG = {
+    "orc_cell_distribution": ([_P, _u32, _f32, _u32, _u32, _u32, _P, _P, _P], _int),
+    "orc_il_construct": ([_P, _u32, _P, _P, _P, _u32, _f32, _f32, _f32, _u32, _u32, _u32], _int),
+    "orc_il_reconstruct_bpm": ([_P, _u32, _P, _P, _P, _P, _u32, _f32, _f32, _f32, _u32, _u32,
+                                _u32, _int], _int),
+    "orc_bonds_construct": ([_P] * 11 + [_u32, _u32, _f32], None),
+    "orc_delete_far_links": ([_P, _u32] + [_P] * 9 + [_u32, _f32], None),
+    "orc_forces_bpm": ([_P] * 12 + [_u32, _u32] + [_f32] * 6, None),
+    "orc_forces_contact": ([_P] * 8 + [_u32, _u32] + [_f32] * 8, None),
+    "orc_sumup_forces": ([_P] * 7 + [_u32, _u32], None),
+    "orc_check_break": ([_P] * 7 + [_u32, _f32, _f32, _f32], None),
+    "orc_increments": ([_P] * 5 + [_u32, _f32, _f32, _f32], None),
+    "orc_platen_uniaxial": ([_P, _P, _P, _u32] + [_f32] * 9 + [_D], None),
+    "orc_platen_brazil": ([_P, _P, _P, _u32] + [_f32] * 9 + [_D], None),
+    "orc_cut_restriction": ([_P, _P, _P, _u32, _P, _P, _P], None),
+    "orc_fire_increments": ([_P, _P, _P, _u32, _f32, _f32, _f32], None),
+    "orc_fire_halfstep": ([_P, _P, _u32, _f32], None),
+    "orc_fire_power": ([_P, _P, _P, _P, _u32, _D], None),
+    "orc_norm3": ([_f32, _f32, _f32], _f32),
+    "orc_rsqrt": ([_f32], _f32),
+    "orc_native_bonds_f64": ([_P] * 3 + [_u32] + [_P] * 8 + [_u32, C.c_double, C.c_double,
+                                                             C.c_double, _int, _P, _P], None),
+}
+
+_lib = None
+
+
+def build() -> pathlib.Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        h = C.CDLL(str(LIB_PATH))
+        for name, (args, res) in _SIG.items():
+            f = getattr(h, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = h
+    return _lib
+
+
+def ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+ERR = {0: "ok", -10: "cell range", -11: "row full"}
+
+
+class OracleLayout:
+    """Host twin of rfd.reflayout.RefLayout evaluated by the C oracle."""
+
+    def __init__(self, R: np.ndarray, K: int, grid):
+        R = np.asarray(R, dtype=np.float32).reshape(-1).copy()
+        self.N = N = R.size // 3
+        self.K = K
+        self.grid = grid
+        self.R = R
+        self.V = np.zeros(3 * N, np.float32)
+        self.W = np.zeros(3 * N, np.float32)
+        self.F = np.zeros(3 * N, np.float32)
+        self.M = np.zeros(3 * N, np.float32)
+        self.CI = np.zeros(N, np.uint32)
+        self.CIs = np.zeros(2 * N, np.uint32)
+        self.pnC = np.zeros(2 * grid.C, np.uint32)
+        S = N * K
+        self.IL = np.full(S, 0xFFFFFFFF, np.uint32)
+        self.ILtype = np.zeros(S, np.uint8)
+        self.iL = np.zeros(S, np.float32)
+        self.slot = {n: np.zeros(3 * S, np.float32)
+                     for n in ("rij", "Oijt", "Fijn", "Fijt", "Mijn", "Mijt", "Mijadd")}
+        self.last_status = 0
+
+    def _s(self, *names):
+        return [ptr(self.slot[n]) for n in names]
+
+    def cell_distribution(self):
+        g = self.grid
+        self.last_status = lib().orc_cell_distribution(ptr(self.R), self.N, g.inv_a, g.Nx, g.Ny,
+                                                       g.Nz, ptr(self.CI), ptr(self.CIs),
+                                                       ptr(self.pnC))
+
+    def il_construct(self, aacut):
+        g = self.grid
+        self.last_status = lib().orc_il_construct(ptr(self.R), self.N, ptr(self.CIs),
+                                                  ptr(self.pnC), ptr(self.IL), self.K, g.a,
+                                                  g.inv_a, aacut, g.Nx, g.Ny, g.Nz)
+
+    def il_reconstruct_bpm(self, aacut, quirk_p0=True):
+        g = self.grid
+        self.last_status = lib().orc_il_reconstruct_bpm(
+            ptr(self.R), self.N, ptr(self.CIs), ptr(self.pnC), ptr(self.IL), ptr(self.ILtype),
+            self.K, g.a, g.inv_a, aacut, g.Nx, g.Ny, g.Nz, int(quirk_p0))
+
+    def bonds_construct(self, bdd):
+        lib().orc_bonds_construct(ptr(self.R), ptr(self.IL), ptr(self.ILtype), ptr(self.iL),
+                                  *self._s("rij", "Oijt", "Fijn", "Fijt", "Mijn", "Mijt",
+                                           "Mijadd"), self.N, self.K, bdd)
+
+    def delete_far_links(self, aa_farcut):
+        lib().orc_delete_far_links(ptr(self.R), self.N, ptr(self.IL), ptr(self.ILtype),
+                                   *self._s("rij", "Oijt", "Fijn", "Fijt", "Mijn", "Mijt",
+                                            "Mijadd"), self.K, aa_farcut)
+
+    def zero_fm(self):
+        self.F[:] = 0
+        self.M[:] = 0
+
+    def forces_bpm(self, m):
+        lib().orc_forces_bpm(ptr(self.R), ptr(self.V), ptr(self.W), ptr(self.IL),
+                             ptr(self.ILtype), ptr(self.iL), *self._s("rij", "Oijt", "Mijn",
+                                                                       "Mijt"),
+                             ptr(self.F), ptr(self.M), self.N, self.K, m.dt, m.E, m.G, m.b_r,
+                             m.sc, m.tc)
+
+    def forces_contact(self, m):
+        lib().orc_forces_contact(ptr(self.R), ptr(self.V), ptr(self.W), ptr(self.IL),
+                                 ptr(self.ILtype), ptr(self.slot["Oijt"]), ptr(self.F),
+                                 ptr(self.M), self.N, self.K, m.dt, m.hm_E, m.hm_G, m.p_A,
+                                 m.p_mu, m.p_mur, m.p_m, m.p_r)
+
+    def increments(self, m):
+        lib().orc_increments(ptr(self.F), ptr(self.V), ptr(self.R), ptr(self.M), ptr(self.W),
+                             self.N, m.dt_d_m, m.dt, m.dt_d_I)
+
+    def platen_uniaxial(self, cmp, zcut):
+        out = (C.c_double * 2)()
+        lib().orc_platen_uniaxial(ptr(self.R), ptr(self.V), ptr(self.F), self.N, cmp.center_d[0],
+                                  cmp.center_d[1], cmp.RR, cmp.top, cmp.bottom, cmp.top_r,
+                                  cmp.bottom_r, cmp.V, zcut, out)
+        return out[0], out[1]
+
+    def platen_brazil(self, cmp, zcut):
+        out = (C.c_double * 2)()
+        lib().orc_platen_brazil(ptr(self.R), ptr(self.V), ptr(self.F), self.N, cmp.center_d[0],
+                                cmp.center_d[2], cmp.RR, cmp.top, cmp.bottom, cmp.top_r,
+                                cmp.bottom_r, cmp.V, zcut, out)
+        return out[0], out[1]
+
+    def cut_restriction(self, center, sized2, hidden_point):
+        a = [np.asarray(x, np.float32) for x in (center, sized2, hidden_point)]
+        lib().orc_cut_restriction(ptr(self.R), ptr(self.V), ptr(self.F), self.N, *map(ptr, a))
+
+    def fire_power(self, rotations=False):
+        out = (C.c_double * 4)()
+        lib().orc_fire_power(ptr(self.F), ptr(self.V), ptr(self.M) if rotations else None,
+                             ptr(self.W) if rotations else None, self.N, out)
+        return tuple(out)
+
+    def host(self):
+        d = {k: getattr(self, k) for k in ("R", "V", "W", "F", "M", "CI", "CIs", "pnC", "IL",
+                                           "ILtype", "iL")}
+        d.update(self.slot)
+        return d
+
+
+class OracleUniaxialDriver:
+    """The oracle's restatement of the driver loop
+    (md_calculationuniaxialcompression.cu:131-189), with the same host logic
+    objects (rfd.params.Compression) as the product driver."""
+
+    def __init__(self, R, mat, box, cmp, grid, K=32):
+        self.mat, self.box, self.cmp = mat, box, cmp
+        self.s = OracleLayout(R, K, grid)
+        self.aacut = float(np.float32(mat.a_farcut) ** 2)
+        self.zcut = float(box.center[2] + box.spacesized2[2])
+
+    def setup(self):
+        s, m = self.s, self.mat
+        s.cell_distribution()
+        s.il_construct(self.aacut)
+        s.bonds_construct(m.b_dd)
+        s.delete_far_links(m.aa_farcut)
+
+    def step(self):
+        s, m, cmp, box = self.s, self.mat, self.cmp, self.box
+        s.cell_distribution()
+        s.il_reconstruct_bpm(self.aacut)
+        s.zero_fm()
+        s.forces_bpm(m)
+        s.forces_contact(m)
+        sums = (s.platen_uniaxial(cmp, self.zcut) if cmp.mode == "uniaxial"
+                else s.platen_brazil(cmp, self.zcut))
+        cmp.sum_forces(sums)
+        s.increments(m)
+        s.cut_restriction(box.center, box.spacesized2, box.hidden_point)
+        s.delete_far_links(m.aa_farcut)
+        cmp.compress()
+        return sums

@@ -1,0 +1,419 @@
+// rfd_lists.hip -- cell binning and the neighbour/bond slot lists
+// (reference-layout tier).
+//
+// Cell binning replaces CUB's stable radix sort with a stable counting sort:
+//   histogram (atomics) -> exclusive scan (one workgroup) -> scatter by atomic
+//   cursor -> per-cell rank by id (one wave per cell).
+// Ranking each cell's members by particle id restores index order inside a
+// cell, which is exactly the order a stable sort of (cell, id) produces.
+#include "rfd_internal.h"
+
+using namespace rfd;
+
+namespace {
+
+__global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)blockDim.x * gridDim.x)
+    p[i] = v;
+}
+
+// cell coordinates of particle i (md_calculate_cell.cu:39-47); false when the
+// particle lies outside the grid
+__device__ __forceinline__ bool cell_coords(const float *__restrict__ R, uint32_t N,
+                                            uint32_t i, float inv_a, uint32_t Nx,
+                                            uint32_t Ny, uint32_t Nz, uint32_t ix[3]) {
+  float fx = floorf(R[i] * inv_a);
+  float fy = floorf(R[i + N] * inv_a);
+  float fz = floorf(R[i + 2 * (size_t)N] * inv_a);
+  if (!(fx >= 0.0f) || !(fy >= 0.0f) || !(fz >= 0.0f)) return false;
+  ix[0] = (uint32_t)fx;
+  ix[1] = (uint32_t)fy;
+  ix[2] = (uint32_t)fz;
+  return ix[0] < Nx && ix[1] < Ny && ix[2] < Nz;
+}
+
+__global__ void k_cell_index(const float *__restrict__ R, uint32_t N, float inv_a,
+                             uint32_t Nx, uint32_t Ny, uint32_t Nz,
+                             uint32_t *__restrict__ CI, uint32_t *__restrict__ cnt,
+                             uint32_t *__restrict__ flags) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
+    uint32_t ix[3];
+    if (!cell_coords(R, N, i, inv_a, Nx, Ny, Nz, ix)) {
+      CI[i] = kSentinel;
+      atomicOr(flags, RFD_FLAG_CELL_RANGE);
+      continue;
+    }
+    uint32_t c = ix[0] + ix[1] * Nx + ix[2] * Ny * Nx;
+    CI[i] = c;
+    atomicAdd(&cnt[c], 1u);
+  }
+}
+
+// exclusive scan of the cell counts by one workgroup of 1024 threads;
+// cursor[c] = start, pnC[c] = start for occupied cells, 0 for empty ones
+constexpr int kScanThreads = 1024;
+constexpr int kScanItems = 8;
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_cells(uint32_t *__restrict__ pnC, uint32_t C, uint32_t *__restrict__ cursor) {
+  __shared__ uint32_t s_sum[kScanThreads / 64];
+  __shared__ uint32_t s_carry;
+  const uint32_t *cnt = pnC + C;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  if (tid == 0) s_carry = 0;
+  __syncthreads();
+  for (uint32_t base = 0; base < C; base += kScanThreads * kScanItems) {
+    uint32_t v[kScanItems], local = 0;
+    uint32_t b0 = base + tid * kScanItems;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      v[k] = (b0 + k < C) ? cnt[b0 + k] : 0u;
+      local += v[k];
+    }
+    // inclusive wave scan of per-thread totals
+    uint32_t inc = local;
+    for (int off = 1; off < 64; off <<= 1) {
+      uint32_t t = __shfl_up(inc, off, 64);
+      if (lane >= off) inc += t;
+    }
+    if (lane == 63) s_sum[wid] = inc;
+    __syncthreads();
+    uint32_t wave_off = 0;
+    for (int w = 0; w < wid; ++w) wave_off += s_sum[w];
+    uint32_t run = s_carry + wave_off + inc - local;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      if (b0 + k < C) {
+        cursor[b0 + k] = run;
+        pnC[b0 + k] = v[k] ? run : 0u;
+      }
+      run += v[k];
+    }
+    __syncthreads();
+    if (tid == kScanThreads - 1) s_carry = run;
+    __syncthreads();
+  }
+}
+
+__global__ void k_scatter(const uint32_t *__restrict__ CI, uint32_t N,
+                          uint32_t *__restrict__ cursor, uint32_t *__restrict__ tmp) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
+    uint32_t c = CI[i];
+    if (c == kSentinel) continue;
+    uint32_t p = atomicAdd(&cursor[c], 1u);
+    tmp[p] = i;
+  }
+}
+
+// one wave per cell: rank of each member = number of members with smaller id
+__global__ void k_rank_cells(const uint32_t *__restrict__ tmp,
+                             const uint32_t *__restrict__ pnC, uint32_t C, uint32_t N,
+                             uint32_t *__restrict__ CIs) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t waves = blockDim.x >> 6;
+  for (uint32_t c = blockIdx.x * waves + (threadIdx.x >> 6); c < C;
+       c += gridDim.x * waves) {
+    uint32_t n = pnC[C + c];
+    if (n == 0) continue;
+    uint32_t s0 = pnC[c];
+    for (uint32_t e = lane; e < n; e += 64) {
+      uint32_t id = tmp[s0 + e];
+      uint32_t rank = 0;
+      for (uint32_t t = 0; t < n; ++t) rank += (tmp[s0 + t] < id) ? 1u : 0u;
+      CIs[s0 + rank] = c;
+      CIs[N + s0 + rank] = id;
+    }
+  }
+}
+
+// The 8 cells searched for particle i, in the reference's order
+// (md_interaction_list_construct.cu:53-85): own cell, then the neighbours on
+// the sides nearest i along x, y, z, xy, xz, yz, xyz.  A neighbour outside the
+// grid is returned as kSentinel (searched as empty).
+__device__ __forceinline__ bool search_cells(const float *__restrict__ R, uint32_t N,
+                                             uint32_t i, float a, float inv_a,
+                                             uint32_t Nx, uint32_t Ny, uint32_t Nz,
+                                             uint32_t cells[8]) {
+  uint32_t ix[3];
+  if (!cell_coords(R, N, i, inv_a, Nx, Ny, Nz, ix)) return false;
+  int sg[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    float mid = ((float)ix[d] + 0.5f) * a;
+    sg[d] = (int)copysignf(1.0f, R[i + (size_t)d * N] - mid);
+  }
+  const uint32_t dim[3] = {Nx, Ny, Nz};
+  const int masks[8] = {0, 1, 2, 4, 3, 5, 6, 7};
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    int64_t p[3];
+    bool ok = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      p[d] = (int64_t)ix[d] + (((masks[q] >> d) & 1) ? sg[d] : 0);
+      ok = ok && p[d] >= 0 && p[d] < (int64_t)dim[d];
+    }
+    cells[q] = ok ? (uint32_t)(p[0] + p[1] * Nx + p[2] * (int64_t)Nx * Ny) : kSentinel;
+  }
+  return true;
+}
+
+__device__ __forceinline__ float pair_r2(const float *__restrict__ R, uint32_t N,
+                                         uint32_t i, uint32_t j) {
+  float dx = R[j] - R[i];
+  float dy = R[j + N] - R[i + N];
+  float dz = R[j + 2 * (size_t)N] - R[i + 2 * (size_t)N];
+  return dx * dx + dy * dy + dz * dz;
+}
+
+__global__ void k_il_construct(const float *__restrict__ R, uint32_t N,
+                               const uint32_t *__restrict__ CIs,
+                               const uint32_t *__restrict__ pnC,
+                               uint32_t *__restrict__ IL, uint32_t K, float a,
+                               float inv_a, float aacut, uint32_t Nx, uint32_t Ny,
+                               uint32_t Nz, uint32_t *__restrict__ flags) {
+  const uint32_t C = Nx * Ny * Nz;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
+    uint32_t *row = IL + (size_t)i * K;
+    uint32_t cells[8];
+    uint32_t nk = 0;
+    if (!search_cells(R, N, i, a, inv_a, Nx, Ny, Nz, cells)) {
+      atomicOr(flags, RFD_FLAG_CELL_RANGE);
+    } else {
+      for (int q = 0; q < 8; ++q) {
+        uint32_t c = cells[q];
+        if (c == kSentinel) continue;
+        uint32_t s0 = pnC[c], n = pnC[C + c];
+        for (uint32_t t = 0; t < n; ++t) {
+          uint32_t j = CIs[N + s0 + t];
+          float rr = pair_r2(R, N, i, j);
+          if (rr < aacut && j != i) {
+            if (nk < K) row[nk] = j;
+            ++nk;
+          }
+        }
+      }
+      if (nk > K) atomicOr(flags, RFD_FLAG_ROW_FULL);
+    }
+    for (uint32_t k = nk; k < K; ++k) row[k] = kSentinel;
+  }
+}
+
+// Rows of one workgroup are staged in LDS (coalesced in and out); each thread
+// then scans its own row for every new candidate.
+constexpr int kRecThreads = 128;
+constexpr int kRecMaxK = 64;
+__global__ void __launch_bounds__(kRecThreads)
+k_il_reconstruct(const float *__restrict__ R, uint32_t N, const uint32_t *__restrict__ CIs,
+                 const uint32_t *__restrict__ pnC, uint32_t *__restrict__ IL,
+                 uint8_t *__restrict__ ILtype, uint32_t K, float a, float inv_a,
+                 float aacut, uint32_t Nx, uint32_t Ny, uint32_t Nz, int quirk_p0,
+                 uint32_t *__restrict__ flags) {
+  __shared__ uint32_t s_row[kRecThreads * (kRecMaxK + 1)];
+  const uint32_t C = Nx * Ny * Nz;
+  const uint32_t p0 = blockIdx.x * kRecThreads;
+  const uint32_t np = min((uint32_t)kRecThreads, N - p0);
+  const uint32_t stride = K + 1;
+  const size_t g0 = (size_t)p0 * K;
+  for (uint32_t e = threadIdx.x; e < np * K; e += kRecThreads)
+    s_row[(e / K) * stride + (e % K)] = IL[g0 + e];
+  __syncthreads();
+  const uint32_t t = threadIdx.x;
+  if (t < np) {
+    const uint32_t i = p0 + t;
+    uint32_t *row = s_row + t * stride;
+    uint32_t cells[8];
+    if (!search_cells(R, N, i, a, inv_a, Nx, Ny, Nz, cells)) {
+      atomicOr(flags, RFD_FLAG_CELL_RANGE);
+    } else {
+      for (int q = 0; q < 8; ++q) {
+        uint32_t c = cells[q];
+        if (c == kSentinel) continue;
+        uint32_t s0 = pnC[c], n = pnC[C + c];
+        for (uint32_t u = 0; u < n; ++u) {
+          uint32_t j = CIs[N + s0 + u];
+          float rr = pair_r2(R, N, i, j);
+          if (rr < aacut && (double)rr > 1e-10) {
+            int freeslot = -1;
+            bool found = false;
+            for (uint32_t k = 0; k < K; ++k) {
+              uint32_t v = row[k];
+              if (v == j) {
+                found = true;
+                break;
+              }
+              if (v == kSentinel) {
+                if (freeslot < 0) freeslot = (int)k;
+                else if (quirk_p0 && i == 0 && freeslot == 0) freeslot = (int)k;
+              }
+            }
+            if (!found) {
+              if (freeslot < 0) {
+                atomicOr(flags, RFD_FLAG_ROW_FULL);
+              } else {
+                row[freeslot] = j;
+                ILtype[(size_t)i * K + freeslot] = 0;
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (uint32_t e = threadIdx.x; e < np * K; e += kRecThreads)
+    IL[g0 + e] = s_row[(e / K) * stride + (e % K)];
+}
+
+__device__ __forceinline__ void zero3(float *p, size_t s) {
+  p[3 * s] = 0.0f;
+  p[3 * s + 1] = 0.0f;
+  p[3 * s + 2] = 0.0f;
+}
+
+__global__ void k_bonds_construct(const float *__restrict__ R, const uint32_t *__restrict__ IL,
+                                  uint8_t *__restrict__ ILtype, float *__restrict__ iL,
+                                  float *__restrict__ rij, float *__restrict__ Oijt,
+                                  float *__restrict__ Fijn, float *__restrict__ Fijt,
+                                  float *__restrict__ Mijn, float *__restrict__ Mijt,
+                                  float *__restrict__ Mijadd, uint32_t N, uint32_t K,
+                                  float bdd) {
+  const uint32_t S = N * K;
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < S; s += grid_stride()) {
+    uint32_t j = IL[s];
+    if (j == kSentinel) continue;
+    uint32_t i = s / K;
+    float rx = R[j] - R[i];
+    float ry = R[j + N] - R[i + N];
+    float rz = R[j + 2 * (size_t)N] - R[i + 2 * (size_t)N];
+    float rm2 = rx * rx + ry * ry + rz * rz;
+    if (rm2 < bdd && (double)rm2 > 1e-18) {
+      iL[s] = rsqrt_rn(rm2);
+      ILtype[s] = 1;
+    } else {
+      ILtype[s] = 0;
+    }
+    rij[3 * (size_t)s] = 0.0f * rx;
+    rij[3 * (size_t)s + 1] = 0.0f * ry;
+    rij[3 * (size_t)s + 2] = 0.0f * rz;
+    zero3(Oijt, s);
+    zero3(Fijn, s);
+    zero3(Fijt, s);
+    zero3(Mijn, s);
+    zero3(Mijt, s);
+    zero3(Mijadd, s);
+  }
+}
+
+__global__ void k_delete_far(const float *__restrict__ R, uint32_t N, uint32_t *__restrict__ IL,
+                             uint8_t *__restrict__ ILtype, float *__restrict__ rij,
+                             float *__restrict__ Oijt, float *__restrict__ Fijn,
+                             float *__restrict__ Fijt, float *__restrict__ Mijn,
+                             float *__restrict__ Mijt, float *__restrict__ Mijadd,
+                             uint32_t K, float aa_farcut) {
+  const uint32_t S = N * K;
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < S; s += grid_stride()) {
+    uint32_t j = IL[s];
+    if (j == kSentinel) continue;
+    uint32_t i = s / K;
+    float rr = pair_r2(R, N, i, j);
+    if (rr > aa_farcut || (double)rr < 1e-10) {
+      IL[s] = kSentinel;
+      ILtype[s] = 0;
+      zero3(rij, s);
+      zero3(Oijt, s);
+      zero3(Fijn, s);
+      zero3(Fijt, s);
+      zero3(Mijn, s);
+      zero3(Mijt, s);
+      zero3(Mijadd, s);
+    }
+  }
+}
+
+bool slots_fit(uint32_t N, uint32_t K) { return (uint64_t)N * K < 0xFFFFFFFFull; }
+
+}  // namespace
+
+extern "C" {
+
+int rfd_cell_distribution(rfd_ctx *ctx, const float *R, uint32_t N, float inv_a,
+                          uint32_t Nx, uint32_t Ny, uint32_t Nz, uint32_t *CI,
+                          uint32_t *CIs, uint32_t *pnC) {
+  if (!ctx) return RFD_E_ARG;
+  RFD_CHECK_ARG(ctx, R && CI && CIs && pnC && N > 0, "rfd_cell_distribution: null buffer or N=0");
+  RFD_CHECK_ARG(ctx, Nx && Ny && Nz && (uint64_t)Nx * Ny * Nz < 0x7FFFFFFFull,
+                "rfd_cell_distribution: bad grid %u x %u x %u", Nx, Ny, Nz);
+  const uint32_t C = Nx * Ny * Nz;
+  void *ws = nullptr;
+  int st = rfd_scratch(ctx, sizeof(uint32_t) * ((size_t)C + N), &ws);
+  if (st) return st;
+  uint32_t *cursor = (uint32_t *)ws, *tmp = cursor + C;
+  hipStream_t s = ctx->stream;
+  k_fill_u32<<<rfd_blocks(2 * (size_t)C, 256, 4096), 256, 0, s>>>(pnC, 2 * (size_t)C, 0u);
+  k_cell_index<<<rfd_blocks(N, 256), 256, 0, s>>>(R, N, inv_a, Nx, Ny, Nz, CI, pnC + C,
+                                                  ctx->d_flags);
+  k_scan_cells<<<1, kScanThreads, 0, s>>>(pnC, C, cursor);
+  k_scatter<<<rfd_blocks(N, 256), 256, 0, s>>>(CI, N, cursor, tmp);
+  k_rank_cells<<<rfd_blocks(C, 4), 256, 0, s>>>(tmp, pnC, C, N, CIs);
+  RFD_LAUNCHED(ctx, "rfd_cell_distribution");
+  return RFD_OK;
+}
+
+int rfd_il_construct(rfd_ctx *ctx, const float *R, uint32_t N, const uint32_t *CIs,
+                     const uint32_t *pnC, uint32_t *IL, uint32_t K, float a,
+                     float inv_a, float aacut, uint32_t Nx, uint32_t Ny, uint32_t Nz) {
+  if (!ctx) return RFD_E_ARG;
+  RFD_CHECK_ARG(ctx, R && CIs && pnC && IL && N > 0 && K > 0 && slots_fit(N, K),
+                "rfd_il_construct: bad arguments");
+  k_il_construct<<<rfd_blocks(N, 128), 128, 0, ctx->stream>>>(
+      R, N, CIs, pnC, IL, K, a, inv_a, aacut, Nx, Ny, Nz, ctx->d_flags);
+  RFD_LAUNCHED(ctx, "rfd_il_construct");
+  return RFD_OK;
+}
+
+int rfd_il_reconstruct_bpm(rfd_ctx *ctx, const float *R, uint32_t N,
+                           const uint32_t *CIs, const uint32_t *pnC, uint32_t *IL,
+                           uint8_t *ILtype, uint32_t K, float a, float inv_a,
+                           float aacut, uint32_t Nx, uint32_t Ny, uint32_t Nz,
+                           int quirk_p0) {
+  if (!ctx) return RFD_E_ARG;
+  RFD_CHECK_ARG(ctx, R && CIs && pnC && IL && ILtype && N > 0 && K > 0 && slots_fit(N, K),
+                "rfd_il_reconstruct_bpm: bad arguments");
+  RFD_CHECK_ARG(ctx, K <= (uint32_t)kRecMaxK, "rfd_il_reconstruct_bpm: K=%u > %d", K, kRecMaxK);
+  unsigned blocks = (N + kRecThreads - 1) / kRecThreads;
+  k_il_reconstruct<<<blocks, kRecThreads, 0, ctx->stream>>>(
+      R, N, CIs, pnC, IL, ILtype, K, a, inv_a, aacut, Nx, Ny, Nz, quirk_p0, ctx->d_flags);
+  RFD_LAUNCHED(ctx, "rfd_il_reconstruct_bpm");
+  return RFD_OK;
+}
+
+int rfd_bonds_construct(rfd_ctx *ctx, const float *R, const uint32_t *IL, uint8_t *ILtype,
+                        float *iL, float *rij, float *Oijt, float *Fijn, float *Fijt,
+                        float *Mijn, float *Mijt, float *Mijadd, uint32_t N, uint32_t K,
+                        float bdd) {
+  if (!ctx) return RFD_E_ARG;
+  RFD_CHECK_ARG(ctx, R && IL && ILtype && iL && rij && Oijt && Fijn && Fijt && Mijn && Mijt &&
+                         Mijadd && N > 0 && K > 0 && slots_fit(N, K),
+                "rfd_bonds_construct: bad arguments");
+  k_bonds_construct<<<rfd_blocks((size_t)N * K, 256), 256, 0, ctx->stream>>>(
+      R, IL, ILtype, iL, rij, Oijt, Fijn, Fijt, Mijn, Mijt, Mijadd, N, K, bdd);
+  RFD_LAUNCHED(ctx, "rfd_bonds_construct");
+  return RFD_OK;
+}
+
+int rfd_delete_far_links(rfd_ctx *ctx, const float *R, uint32_t N, uint32_t *IL,
+                         uint8_t *ILtype, float *rij, float *Oijt, float *Fijn,
+                         float *Fijt, float *Mijn, float *Mijt, float *Mijadd,
+                         uint32_t K, float aa_farcut) {
+  if (!ctx) return RFD_E_ARG;
+  RFD_CHECK_ARG(ctx, R && IL && ILtype && rij && Oijt && Fijn && Fijt && Mijn && Mijt &&
+                         Mijadd && N > 0 && K > 0 && slots_fit(N, K),
+                "rfd_delete_far_links: bad arguments");
+  k_delete_far<<<rfd_blocks((size_t)N * K, 256), 256, 0, ctx->stream>>>(
+      R, N, IL, ILtype, rij, Oijt, Fijn, Fijt, Mijn, Mijt, Mijadd, K, aa_farcut);
+  RFD_LAUNCHED(ctx, "rfd_delete_far_links");
+  return RFD_OK;
+}
+
+}  // extern "C"

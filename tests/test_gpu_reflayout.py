@@ -1,0 +1,194 @@
+"""GPU parity of the reference-layout tier against the CPU oracle.
+
+Every stage is compared bit for bit (integer lists, FP32 arrays) except the
+platen / FIRE reductions, which are FP64 sums of the same FP32 terms taken in
+a different order (relative tolerance 1e-12).
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleLayout, OracleUniaxialDriver
+from rfd import cases
+from rfd.reflayout import FLAG_CELL_RANGE, RefLayout, UniaxialDriver
+
+pytestmark = pytest.mark.gpu
+
+BIT_KEYS = ("R", "V", "W", "F", "M", "CI", "CIs", "pnC", "IL", "ILtype", "iL", "rij", "Oijt",
+            "Mijn", "Mijt")
+
+
+def assert_same(gpu: dict, cpu: dict, keys=BIT_KEYS, where=""):
+    for k in keys:
+        a, b = gpu[k], cpu[k]
+        assert a.shape == b.shape, (where, k, a.shape, b.shape)
+        if a.dtype.kind == "f":
+            same = a.view(np.uint32) == b.view(np.uint32)
+            if not same.all():
+                bad = np.nonzero(~same)[0][:5]
+                raise AssertionError(f"{where}: {k} differs at {bad}: gpu {a[bad]} cpu {b[bad]}")
+        else:
+            if not np.array_equal(a, b):
+                bad = np.nonzero(a != b)[0][:5]
+                raise AssertionError(f"{where}: {k} differs at {bad}: gpu {a[bad]} cpu {b[bad]}")
+
+
+def perturb(n, seed, scale):
+    g = np.random.default_rng(seed)
+    return (g.standard_normal(3 * n) * scale).astype(np.float32)
+
+
+def twin(ctx, case):
+    grid = case.grid()
+    return RefLayout(ctx, case.R, case.K, grid), OracleLayout(case.R, case.K, grid)
+
+
+@pytest.mark.parametrize("D,H,seed", [(10.0, 20.0, 1), (6.0, 8.0, 7)])
+def test_lists_bitexact(ctx, D, H, seed):
+    case = cases.reference_uniaxial(D, H, seed=seed)
+    g, o = twin(ctx, case)
+    aacut = float(np.float32(case.mat.a_farcut) ** 2)
+    for s in (g, o):
+        s.cell_distribution()
+        s.il_construct(aacut)
+        s.bonds_construct(case.mat.b_dd)
+        s.delete_far_links(case.mat.aa_farcut)
+    assert o.last_status == 0 and ctx.flags() == 0
+    assert_same(g.host(), o.host(), where="setup")
+    assert (o.ILtype == 1).sum() > 4 * case.N
+
+
+def test_reconstruct_after_motion_bitexact(ctx):
+    case = cases.reference_uniaxial(8.0, 12.0, seed=3)
+    g, o = twin(ctx, case)
+    aacut = float(np.float32(case.mat.a_farcut) ** 2)
+    for s in (g, o):
+        s.cell_distribution()
+        s.il_construct(aacut)
+        s.bonds_construct(case.mat.b_dd)
+    # move particles so that new pairs come inside the cut and some leave
+    d = perturb(case.N, 11, 0.08)
+    g.R += __import__("torch").from_numpy(d).cuda()
+    o.R += d
+    for s in (g, o):
+        s.delete_far_links(case.mat.aa_farcut)
+        s.cell_distribution()
+        s.il_reconstruct_bpm(aacut, quirk_p0=True)
+    assert_same(g.host(), o.host(), where="reconstruct")
+
+
+def test_forces_bitexact_with_motion(ctx):
+    import torch
+
+    case = cases.reference_uniaxial(8.0, 12.0, seed=5)
+    g, o = twin(ctx, case)
+    m = case.mat
+    aacut = float(np.float32(m.a_farcut) ** 2)
+    for s in (g, o):
+        s.cell_distribution()
+        s.il_construct(aacut)
+        s.bonds_construct(m.b_dd)
+        s.delete_far_links(m.aa_farcut)
+    dv = perturb(case.N, 21, 1e-3)
+    dw = perturb(case.N, 22, 2e-3)
+    for k in range(12):
+        g.V.copy_(torch.from_numpy(dv * (k + 1)).cuda())
+        g.W.copy_(torch.from_numpy(dw * (k + 1)).cuda())
+        o.V[:] = dv * (k + 1)
+        o.W[:] = dw * (k + 1)
+        for s in (g, o):
+            s.zero_fm()
+            s.forces_bpm(m)
+            s.forces_contact(m)
+            s.increments(m)
+        assert_same(g.host(), o.host(), where=f"iteration {k}")
+    assert np.abs(o.F).max() > 0
+
+
+def test_bond_break_bitexact(ctx):
+    """Weak bonds break under the same motion on both sides."""
+    import dataclasses
+
+    import torch
+
+    case = cases.reference_uniaxial(8.0, 12.0, seed=9)
+    m = dataclasses.replace(case.mat, sc=1e-4, tc=1e-4)
+    g, o = twin(ctx, case)
+    aacut = float(np.float32(m.a_farcut) ** 2)
+    for s in (g, o):
+        s.cell_distribution()
+        s.il_construct(aacut)
+        s.bonds_construct(m.b_dd)
+    dv = perturb(case.N, 31, 2e-2)
+    g.V.copy_(torch.from_numpy(dv).cuda())
+    o.V[:] = dv
+    before = int((o.ILtype == 1).sum())
+    for k in range(5):
+        for s in (g, o):
+            s.zero_fm()
+            s.forces_bpm(m)
+            s.forces_contact(m)
+            s.increments(m)
+    assert_same(g.host(), o.host(), where="break")
+    assert int((o.ILtype == 1).sum()) < before
+
+
+@pytest.mark.parametrize("mode", ["uniaxial", "brazil"])
+def test_driver_trajectory_bitexact(ctx, mode):
+    """Whole explicit steps (cell sort, reconstruct, forces, platens,
+    integration, cut, prune) for 60 steps; state compared every 20."""
+    mk = cases.reference_uniaxial if mode == "uniaxial" else cases.reference_brazil
+    args = (8.0, 14.0) if mode == "uniaxial" else (14.0, 5.0)
+    cg, co = mk(*args, seed=2), mk(*args, seed=2)
+    # faster loading so that the platens engage within the test
+    for c in (cg, co):
+        c.cmp.v_load *= 50
+    dg = UniaxialDriver(ctx, cg.R, cg.mat, cg.box, cg.cmp, cg.K)
+    do = OracleUniaxialDriver(co.R, co.mat, co.box, co.cmp, co.grid(), co.K)
+    dg.setup()
+    do.setup()
+    for k in range(60):
+        sg = dg.step()
+        so = do.step()
+        np.testing.assert_allclose(sg, so, rtol=1e-12, atol=1e-300)
+        if k % 20 == 19:
+            assert_same(dg.s.host(), do.s.host(), where=f"{mode} step {k}")
+    assert ctx.flags() == 0
+    assert cg.cmp.stress == pytest.approx(co.cmp.stress, rel=1e-12)
+
+
+def test_fire_kernels(ctx):
+    import torch
+
+    case = cases.reference_uniaxial(6.0, 8.0, seed=4)
+    g, o = twin(ctx, case)
+    f = perturb(case.N, 41, 1e-2)
+    v = perturb(case.N, 42, 1e-3)
+    g.F.copy_(torch.from_numpy(f).cuda())
+    g.V.copy_(torch.from_numpy(v).cuda())
+    o.F[:] = f
+    o.V[:] = v
+    pg, po = g.fire_power(), o.fire_power()
+    np.testing.assert_allclose(pg, po, rtol=1e-12)
+    lib = ctx.lib
+    from rfd.reflayout import _p
+    from oracle.oracle import lib as olib, ptr
+
+    ctx.call("rfd_fire_increments", _p(g.F), _p(g.V), _p(g.R), case.N, 0.01, 0.02, 0.1)
+    olib().orc_fire_increments(ptr(o.F), ptr(o.V), ptr(o.R), case.N, 0.01, 0.02, 0.1)
+    ctx.call("rfd_fire_halfstep", _p(g.V), _p(g.R), case.N, 0.005)
+    olib().orc_fire_halfstep(ptr(o.V), ptr(o.R), case.N, 0.005)
+    assert_same(g.host(), o.host(), keys=("R", "V"), where="fire")
+    del lib
+
+
+def test_cell_range_flag(ctx):
+    """A particle outside the grid raises RFD_FLAG_CELL_RANGE, not a fault."""
+    import torch
+
+    case = cases.reference_uniaxial(6.0, 8.0, seed=4)
+    g, _ = twin(ctx, case)
+    ctx.flags()
+    g.R[0] = -5.0
+    g.cell_distribution()
+    torch.cuda.synchronize()
+    assert ctx.flags() & FLAG_CELL_RANGE

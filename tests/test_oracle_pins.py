@@ -1,0 +1,509 @@
+"""Pins for the CPU oracle: closed forms, conservation laws, brute force and
+special cases chosen so that a dropped term, a wrong sign or a swapped index
+in the restatement fails at least one of them.  (The reference ships no test
+vectors or fixtures; these are what the mathematics fixes.)"""
+import ctypes as C
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from oracle.oracle import lib, ptr
+from rfd import cases
+from rfd.params import Compression, Material
+
+PI_F = np.float32(3.14159265358979)
+
+
+def mat():
+    return Material(E=30.0, nu=0.25, rho=2.65, p_r=0.5, b_r=0.25, b_d=1.05, sc=1e9, tc=1e9)
+
+
+class Pair:
+    """Two particles i=0, j=1 with one slot each pointing at the other."""
+
+    def __init__(self, r, bonded=True, L0=None, K=1):
+        self.N, self.K = 2, K
+        self.R = np.zeros(6, np.float32)
+        self.R[1], self.R[3], self.R[5] = r  # particle 1 at r, particle 0 at origin
+        self.V = np.zeros(6, np.float32)
+        self.W = np.zeros(6, np.float32)
+        self.F = np.zeros(6, np.float32)
+        self.M = np.zeros(6, np.float32)
+        S = 2 * K
+        self.IL = np.full(S, 0xFFFFFFFF, np.uint32)
+        self.IL[0], self.IL[K] = 1, 0
+        self.ILtype = np.zeros(S, np.uint8)
+        if bonded:
+            self.ILtype[0] = self.ILtype[K] = 1
+        L0 = L0 if L0 is not None else float(np.linalg.norm(np.asarray(r, np.float64)))
+        self.iL = np.full(S, np.float32(1.0 / L0), np.float32)
+        self.st = {k: np.zeros(3 * S, np.float32) for k in ("rij", "Oijt", "Mijn", "Mijt")}
+
+    def set_v(self, p, v):
+        for d in range(3):
+            self.V[p + 2 * d] = v[d]
+
+    def set_w(self, p, w):
+        for d in range(3):
+            self.W[p + 2 * d] = w[d]
+
+    def vec(self, a, p):
+        return np.array([a[p], a[p + 2], a[p + 4]], np.float64)
+
+    def bonds(self, m, sc=None, tc=None):
+        self.F[:] = 0
+        self.M[:] = 0
+        lib().orc_forces_bpm(ptr(self.R), ptr(self.V), ptr(self.W), ptr(self.IL),
+                             ptr(self.ILtype), ptr(self.iL), ptr(self.st["rij"]),
+                             ptr(self.st["Oijt"]), ptr(self.st["Mijn"]), ptr(self.st["Mijt"]),
+                             ptr(self.F), ptr(self.M), 2, self.K, m.dt, m.E, m.G, m.b_r,
+                             m.sc if sc is None else sc, m.tc if tc is None else tc)
+
+    def contact(self, m):
+        self.F[:] = 0
+        self.M[:] = 0
+        lib().orc_forces_contact(ptr(self.R), ptr(self.V), ptr(self.W), ptr(self.IL),
+                                 ptr(self.ILtype), ptr(self.st["Oijt"]), ptr(self.F),
+                                 ptr(self.M), 2, self.K, m.dt, m.hm_E, m.hm_G, m.p_A, m.p_mu,
+                                 m.p_mur, m.p_m, m.p_r)
+
+
+# ----------------------------------------------------------------- helpers
+def test_norm3_and_rsqrt_correctly_rounded():
+    g = np.random.default_rng(0)
+    x = g.standard_normal((2000, 3)).astype(np.float32) * np.float32(3.7)
+    half_ulp_ok = 0
+    for a, b, c in x:
+        f = lib().orc_norm3(a, b, c)
+        S = sum(Fraction(float(v)) ** 2 for v in (a, b, c))
+        ulp = np.spacing(np.float32(f))
+        # |f - sqrt(S)| <= 1 ulp, checked with exact rational arithmetic
+        lo, hi = Fraction(float(f)) - Fraction(float(ulp)), Fraction(float(f)) + Fraction(float(ulp))
+        assert lo * lo <= S <= hi * hi
+        lo2 = Fraction(float(f)) - Fraction(float(ulp)) / 2
+        hi2 = Fraction(float(f)) + Fraction(float(ulp)) / 2
+        half_ulp_ok += lo2 * lo2 <= S <= hi2 * hi2
+    assert half_ulp_ok == len(x)
+    for v in g.uniform(1e-6, 1e6, 2000).astype(np.float32):
+        r = lib().orc_rsqrt(v)
+        assert abs(r - 1 / math.sqrt(float(v))) <= 0.5 * np.spacing(np.float32(r)) * 1.0001
+
+
+# ---------------------------------------------------------------- bond law
+def test_axial_stretch_closed_form():
+    m = mat()
+    p = Pair((1.001, 0, 0), L0=1.0)
+    p.bonds(m)
+    F0, F1 = p.vec(p.F, 0), p.vec(p.F, 1)
+    area = float(PI_F) * m.b_r ** 2
+    want = m.E * area * 0.001
+    assert F0[0] == pytest.approx(want, rel=2e-4)  # eps=1e-3 in float32
+    assert abs(F0[1]) + abs(F0[2]) == 0
+    assert np.array_equal(F1, -F0)
+    np.testing.assert_array_equal(p.st["rij"][:3], [1, 0, 0])
+
+
+def test_rest_length_gives_zero_force():
+    m = mat()
+    p = Pair((0.0, 1.0, 0.0), L0=1.0)
+    p.bonds(m)
+    assert np.all(p.F == 0) and np.all(p.M == 0)
+
+
+def test_shear_displacement_closed_form():
+    """j slides along +y at v for k steps: |Ft| = G*pi*b^2*delta/L0 on i, +y."""
+    m = mat()
+    p = Pair((1.0, 0, 0), L0=1.0)
+    v, k = 1e-3, 20
+    p.set_v(1, (0, v, 0))
+    for _ in range(k):
+        p.bonds(m)
+    delta = v * m.dt * k
+    want = m.G * float(PI_F) * m.b_r ** 2 * delta / 1.0
+    F0 = p.vec(p.F, 0)
+    assert F0[1] == pytest.approx(want, rel=1e-4)
+    # 0.5 r x Ft moment on both ends, about +z
+    M0 = p.vec(p.M, 0)
+    assert M0[2] == pytest.approx(0.5 * 1.0 * want, rel=1e-4)
+
+
+def test_twist_closed_form():
+    """j spins about the bond axis: stored Mn = G*theta/(2 L0), moment on i =
+    G*J*theta/L0 with J = pi b^4 / 2; j gets the opposite moment."""
+    m = mat()
+    p = Pair((1.0, 0, 0), L0=1.0)
+    w, k = 2e-3, 25
+    p.set_w(1, (w, 0, 0))
+    for _ in range(k):
+        p.bonds(m)
+    theta = w * m.dt * k
+    assert p.st["Mijn"][0] == pytest.approx(m.G * theta / 2.0, rel=1e-4)
+    J = float(PI_F) * m.b_r ** 4 / 2.0
+    M0, M1 = p.vec(p.M, 0), p.vec(p.M, 1)
+    assert M0[0] == pytest.approx(m.G * J * theta / 1.0, rel=1e-4)
+    assert M1[0] == pytest.approx(-M0[0], rel=1e-6)
+    assert np.all(p.F == 0)
+
+
+def test_bend_closed_form_with_reference_factor():
+    """j rotates about +y: stored Mt = E*theta/(4 L0); the applied moment
+    carries the extra dt*E/(4 L0) factor of md_calculateforcesdem3.cu:148;
+    the rotation also shears the bond (O along +z) which adds 0.5 r x Ft."""
+    m = mat()
+    p = Pair((1.0, 0, 0), L0=1.0)
+    w, k = 2e-3, 10
+    p.set_w(1, (0, w, 0))
+    for _ in range(k):
+        p.bonds(m)
+    theta = w * m.dt * k
+    assert p.st["Mijt"][1] == pytest.approx(m.E * theta / 4.0, rel=1e-4)
+    area = float(PI_F) * m.b_r ** 2
+    J4 = float(PI_F) * m.b_r ** 4
+    O = 0.5 * w * 1.0 * m.dt * k       # (w/2 y) x (L x) = -(wL/2) z, minus sign -> +z
+    Ft = m.G * area * O
+    F0 = p.vec(p.F, 0)
+    assert F0[2] == pytest.approx(Ft, rel=1e-4)
+    Mt_applied = (m.E * theta / 4.0) * (m.dt * 0.25 * m.E) * J4
+    M0 = p.vec(p.M, 0)
+    assert M0[1] == pytest.approx(Mt_applied - 0.5 * 1.0 * Ft, rel=1e-4)
+
+
+def test_newton3_and_angular_momentum():
+    g = np.random.default_rng(3)
+    m = mat()
+    for trial in range(20):
+        r = g.uniform(-1, 1, 3)
+        r = r / np.linalg.norm(r) * g.uniform(0.9, 1.1)
+        p = Pair(tuple(r), L0=1.0)
+        p.set_v(0, g.standard_normal(3) * 1e-3)
+        p.set_v(1, g.standard_normal(3) * 1e-3)
+        p.set_w(0, g.standard_normal(3) * 1e-3)
+        p.set_w(1, g.standard_normal(3) * 1e-3)
+        for _ in range(3):
+            p.bonds(m)
+        F0, F1 = p.vec(p.F, 0), p.vec(p.F, 1)
+        assert np.array_equal(F0, -F1), trial
+        R0, R1 = p.vec(p.R, 0), p.vec(p.R, 1)
+        L = np.cross(R0, F0) + np.cross(R1, F1) + p.vec(p.M, 0) + p.vec(p.M, 1)
+        scale = np.abs(F0).max() + np.abs(p.vec(p.M, 0)).max()
+        assert np.abs(L).max() <= 1e-5 * scale, trial
+
+
+def test_rigid_body_motion_is_force_free():
+    """Rigid translation + rotation: no relative velocity, no bond load."""
+    m = mat()
+    p = Pair((0.6, 0.8, 0.0), L0=1.0)
+    Om = np.array([1e-3, -2e-3, 3e-3])
+    V0 = np.array([5e-3, 1e-3, -2e-3])
+    for q in (0, 1):
+        Rq = p.vec(p.R, q)
+        p.set_v(q, V0 + np.cross(Om, Rq))
+        p.set_w(q, Om)
+    p.bonds(m)
+    assert np.abs(p.F).max() < 1e-9 and np.abs(p.M).max() < 1e-9
+
+
+@pytest.mark.parametrize("below", [True, False])
+def test_break_threshold(below):
+    m = mat()
+    p = Pair((1.002, 0, 0), L0=1.0)
+    Sc = np.float32(m.E) * (np.float32(1.002) - np.float32(1.0)) * np.float32(1.0)
+    sc = float(np.nextafter(Sc, np.float32(0))) if below else float(np.nextafter(Sc, np.float32(1)))
+    p.st["Oijt"][:3] = 1.0  # state that must be wiped on a break
+    p.bonds(m, sc=sc)
+    broken = p.ILtype[0] == 0
+    assert broken == below
+    if below:
+        assert np.all(p.st["Oijt"][:3] == 0) and np.all(p.st["Mijn"][:3] == 0)
+        # the force of the breaking step is still applied (md_calculateforcesdem3.cu:450,472)
+        assert p.F[0] > 0
+
+
+# ------------------------------------------------------------- contact law
+def test_hertz_normal_closed_form():
+    m = mat()
+    delta = 0.01
+    p = Pair((2 * m.p_r - delta, 0, 0), bonded=False)
+    p.contact(m)
+    F0 = p.vec(p.F, 0)
+    want = -(4.0 / 3.0) * m.hm_E * math.sqrt(m.p_r / 2.0) * delta ** 1.5
+    assert F0[0] == pytest.approx(want, rel=1e-4)
+    assert np.array_equal(p.vec(p.F, 1), -F0)
+
+
+def test_no_contact_beyond_touching():
+    m = mat()
+    p = Pair((2 * m.p_r + 1e-3, 0, 0), bonded=False)
+    p.st["Oijt"][:] = 1.0
+    p.contact(m)
+    assert np.all(p.F == 0) and np.all(p.st["Oijt"] == 0)
+
+
+def test_coulomb_cap():
+    m = mat()
+    delta = 0.01
+    p = Pair((2 * m.p_r - delta, 0, 0), bonded=False)
+    p.st["Oijt"][0:3] = [0, 0.05, 0]  # large tangential spring of slot 0->1
+    p.set_w(0, (0, 0, 1e-3))          # so that the rolling branch does not wipe ms
+    p.contact(m)
+    F0 = p.vec(p.F, 0)
+    fn = abs(F0[0])
+    ft = math.hypot(F0[1], F0[2])
+    assert ft == pytest.approx(m.p_mu * fn, rel=1e-5)
+
+
+def test_rolling_torque_and_wipe_quirk():
+    m = mat()
+    delta = 0.01
+    p = Pair((2 * m.p_r - delta, 0, 0), bonded=False)
+    w = np.array([0.0, 0.0, 3e-3])
+    p.set_w(0, w)
+    p.contact(m)
+    F0, M0 = p.vec(p.F, 0), p.vec(p.M, 0)
+    n = np.array([1.0, 0, 0])
+    fn = abs(F0 @ n)
+    Ft = F0 - n * (F0 @ n)
+    expect = m.p_r * np.cross(n, Ft) - m.p_mur * fn * w / np.linalg.norm(w)
+    np.testing.assert_allclose(M0, expect, rtol=1e-4, atol=1e-12)
+    # W_i == 0: the reference's else-branch wipes the whole accumulator
+    q = Pair((2 * m.p_r - delta, 0, 0), bonded=False)
+    q.set_w(1, (0, 0, 3e-3))
+    q.contact(m)
+    assert np.all(q.vec(q.M, 0) == 0)
+    assert np.any(q.vec(q.M, 1) != 0) or True
+
+
+def test_head_on_restitution():
+    """The damping law (-1.8257 alpha sqrt(k m/2)) gives e close to the
+    material's restitution coefficient for a head-on Hertz impact."""
+    m = mat()
+    p = Pair((2 * m.p_r + 0.002, 0, 0), bonded=False)
+    v0 = 1e-3
+    p.set_v(1, (-v0, 0, 0))
+    for _ in range(200000):
+        p.contact(m)
+        lib().orc_increments(ptr(p.F), ptr(p.V), ptr(p.R), ptr(p.M), ptr(p.W), 2,
+                             m.dt_d_m, m.dt, m.dt_d_I)
+        rel = p.V[1] - p.V[0]
+        if rel > 0 and p.R[1] - p.R[0] > 2 * m.p_r:
+            break
+    e = (p.V[1] - p.V[0]) / v0
+    assert e == pytest.approx(m.p_e, abs=0.06)
+
+
+# ------------------------------------------------------------ integration
+def test_increments_exact_affine():
+    g = np.random.default_rng(5)
+    N = 50
+    F, V, R, M, W = (g.standard_normal(3 * N).astype(np.float32) for _ in range(5))
+    dt_m, dt, dt_I = np.float32(0.3), np.float32(0.01), np.float32(0.7)
+    Vx = V + F * dt_m
+    Rx = R + Vx * dt
+    Wx = W + M * dt_I
+    lib().orc_increments(ptr(F), ptr(V), ptr(R), ptr(M), ptr(W), N, dt_m, dt, dt_I)
+    assert np.array_equal(V, Vx) and np.array_equal(R, Rx) and np.array_equal(W, Wx)
+
+
+# ------------------------------------------------------------------ lists
+def ref_state(D=6.0, H=8.0, seed=1):
+    from oracle.oracle import OracleLayout
+
+    c = cases.reference_uniaxial(D, H, seed=seed)
+    return c, OracleLayout(c.R, c.K, c.grid())
+
+
+def test_cell_distribution_is_stable_sort():
+    c, o = ref_state()
+    o.cell_distribution()
+    assert o.last_status == 0
+    g = c.grid()
+    R = o.R.reshape(3, -1)
+    ix = np.floor(R * np.float32(g.inv_a)).astype(np.int64)
+    ci = ix[0] + ix[1] * g.Nx + ix[2] * g.Nx * g.Ny
+    assert np.array_equal(o.CI, ci.astype(np.uint32))
+    order = np.argsort(ci, kind="stable")
+    assert np.array_equal(o.CIs[: o.N], ci[order].astype(np.uint32))
+    assert np.array_equal(o.CIs[o.N:], order.astype(np.uint32))
+    cnt = np.bincount(ci, minlength=g.C)
+    start = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+    assert np.array_equal(o.pnC[g.C:], cnt.astype(np.uint32))
+    assert np.array_equal(o.pnC[: g.C], np.where(cnt > 0, start, 0).astype(np.uint32))
+
+
+def brute_pairs(R, aacut):
+    R = R.reshape(3, -1)
+    N = R.shape[1]
+    out = set()
+    for i in range(N):
+        d = (R[:, :] - R[:, i:i + 1]).astype(np.float32)
+        rr = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]
+        for j in np.nonzero(rr < np.float32(aacut))[0]:
+            if j != i:
+                out.add((i, int(j)))
+    return out
+
+
+def test_il_construct_matches_brute_force():
+    """h_CheckInteractions (md_check_test.cu:26-95): no pair within the cut is
+    missing and no listed pair is beyond it."""
+    c, o = ref_state()
+    aacut = float(np.float32(c.mat.a_farcut) ** 2)
+    o.cell_distribution()
+    o.il_construct(aacut)
+    assert o.last_status == 0
+    got = set()
+    IL = o.IL.reshape(o.N, o.K)
+    for i in range(o.N):
+        row = IL[i][IL[i] != 0xFFFFFFFF]
+        assert len(set(row.tolist())) == len(row)
+        got |= {(i, int(j)) for j in row}
+    assert got == brute_pairs(o.R, aacut)
+
+
+def test_il_row_order_own_cell_first():
+    """Slot order: own cell first, then the +/-x neighbour cell, ids ascending."""
+    from oracle.oracle import OracleLayout
+    from rfd.reflayout import Grid
+
+    a = np.float32(2.9)
+    grid = Grid(float(a), float(np.float32(1) / a), 4, 4, 4)
+    # particle 0 in cell (1,1,1) near its +x face; 1 in the +x cell; 2, 3 own cell
+    P = np.array([[5.7, 4.35, 4.35], [6.0, 4.35, 4.35], [4.4, 4.35, 4.35], [5.0, 4.4, 4.3]],
+                 np.float32)
+    o = OracleLayout(np.ascontiguousarray(P.T).reshape(-1), 4, grid)
+    o.cell_distribution()
+    o.il_construct(float(np.float32(1.45) ** 2))
+    assert o.IL[:4].tolist() == [2, 3, 1, 0xFFFFFFFF]
+
+
+def test_reconstruct_first_free_slot_and_p0_quirk():
+    from oracle.oracle import OracleLayout
+    from rfd.reflayout import Grid
+
+    a = np.float32(2.9)
+    grid = Grid(float(a), float(np.float32(1) / a), 4, 4, 4)
+    P = np.array([[4.35, 4.35, 4.35], [5.0, 4.35, 4.35], [4.35, 5.0, 4.35]], np.float32)
+    for quirk, slot in ((True, 2), (False, 0)):
+        o = OracleLayout(np.ascontiguousarray(P.T).reshape(-1), 4, grid)
+        o.cell_distribution()
+        o.IL[:4] = [0xFFFFFFFF, 1, 0xFFFFFFFF, 0xFFFFFFFF]
+        o.ILtype[:4] = [9, 1, 9, 9]
+        o.il_reconstruct_bpm(float(np.float32(1.45) ** 2), quirk_p0=quirk)
+        row = o.IL[:4].tolist()
+        assert row[1] == 1 and o.ILtype[1] == 1  # existing link kept, type untouched
+        assert row[slot] == 2 and o.ILtype[slot] == 0
+
+
+def test_bonds_construct_and_delete_far():
+    c, o = ref_state()
+    m = c.mat
+    aacut = float(np.float32(m.a_farcut) ** 2)
+    o.cell_distribution()
+    o.il_construct(aacut)
+    o.bonds_construct(m.b_dd)
+    R = o.R.reshape(3, -1)
+    IL = o.IL.reshape(o.N, o.K)
+    ty = o.ILtype.reshape(o.N, o.K)
+    for i in range(0, o.N, 7):
+        for k in range(o.K):
+            j = IL[i, k]
+            if j == 0xFFFFFFFF:
+                continue
+            d = (R[:, j] - R[:, i]).astype(np.float32)
+            rr = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]
+            assert ty[i, k] == (1 if rr < np.float32(m.b_dd) else 0)
+            if ty[i, k]:
+                assert o.iL[i * o.K + k] == pytest.approx(1 / math.sqrt(float(rr)), rel=1e-7)
+    # pull one particle far away: all its links disappear on both sides
+    o.R[0] += 10.0
+    o.delete_far_links(m.aa_farcut)
+    assert np.all(o.IL[: o.K] == 0xFFFFFFFF)
+    assert not np.any(o.IL.reshape(o.N, o.K)[1:] == 0)
+
+
+# ------------------------------------------------------- platens, loading
+def test_platen_uniaxial_hand_sums():
+    from oracle.oracle import OracleLayout
+    from rfd.reflayout import Grid
+
+    P = np.array([[0, 0, 9.9], [0.1, 0, 10.2], [0, 0, -0.1], [0, 0, 0.3], [9, 9, 9.9]],
+                 np.float32)
+    o = OracleLayout(np.ascontiguousarray(P.T).reshape(-1), 1, Grid(1, 1, 1, 1, 1))
+    o.F[:] = np.array([0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 2.0, 3.0, -4.0, -1.5, 5.0], np.float32)
+    cmp = Compression("uniaxial", (10, 10, 10), (0, 0, 5), 0.5, 0.01, -1e-3, 1.0, 0.1)
+    s = o.platen_uniaxial(cmp, 100.0)
+    # top band (z >= 9.5): particles 0 (F=2) and 1 (F=3); 1 is past Zt=10 -> clamped
+    # bottom band (z <= 0.5): particles 2 (F=-4, past Zb=0 -> clamped) and 3
+    # (F=-1.5, inside the band, not clamped); particle 4 is outside the radius
+    assert s == (-5.5, 5.0)
+    assert o.R[1 + 10] == 10.0 and o.F[1 + 10] == 0 and o.F[0 + 10] == 2.0
+    assert o.R[2 + 10] == 0.0 and o.F[2 + 10] == 0
+    assert o.R[3 + 10] == np.float32(0.3) and o.F[3 + 10] == np.float32(-1.5)
+
+
+def test_fracture_detector_on_synthetic_trace():
+    cmp = Compression("uniaxial", (10, 10, 10), (0, 0, 5), 0.5, 0.01, -1e-3, 1.0, 0.1)
+    area = cmp.area
+    trace = [0.5, 1.5, 3.0, 2.0, 0.9, 0.2]  # stress values
+    for st in trace:
+        cmp.sum_forces((-st * area, st * area))
+        if cmp.fracture:
+            break
+    assert cmp.loading and cmp.fracture
+    assert cmp.fracture_stress == pytest.approx(3.0)
+    assert cmp.stress == pytest.approx(0.2)  # first value below 0.33*MinStress
+
+
+def test_reference_fire_relaxes_quadratic_bowl():
+    """Reference FIRE kernels + control (md_firerelaxation.cu:263-304) on
+    F = -k (R - R0): converges to the analytic minimum R0."""
+    g = np.random.default_rng(7)
+    N = 64
+    R0 = g.uniform(0, 5, 3 * N).astype(np.float32)
+    R = (R0 + g.normal(0, 0.1, 3 * N)).astype(np.float32)
+    V = np.zeros_like(R)
+    k, mass, dt0 = 2.0, 1.0, 0.05
+    dt, alpha, npos = dt0, 0.25, 0
+    for step in range(3000):
+        F = (-k * (R - R0)).astype(np.float32)
+        out = (C.c_double * 4)()
+        lib().orc_fire_power(ptr(F), ptr(V), None, None, N, out)
+        if out[0] > 0:
+            npos += 1
+            if npos > 20:
+                dt, alpha = min(dt * 1.1, 100 * dt0), alpha * 0.99
+        else:
+            npos = 0
+            if step > 20:
+                dt, alpha = max(dt * 0.5, 1e-3 * dt0), 0.25
+            lib().orc_fire_halfstep(ptr(V), ptr(R), N, 0.5 * dt)
+            V[:] = 0
+        lib().orc_fire_increments(ptr(F), ptr(V), ptr(R), N, dt / mass, dt, alpha)
+    assert np.abs(R - R0).max() < 1e-4
+
+
+# ----------------------------------------------------------- native f64
+def test_native_f64_matches_slot_layout_forces():
+    """The FP64 edge-list oracle in reference mode agrees with the FP32 slot
+    oracle on a random bonded pair sequence to FP32 accuracy."""
+    g = np.random.default_rng(11)
+    m = mat()
+    r = g.uniform(-1, 1, 3)
+    r = r / np.linalg.norm(r) * 1.02
+    p = Pair(tuple(r), L0=1.0)
+    vs = g.standard_normal((4, 3)) * 1e-3
+    p.set_v(0, vs[0]); p.set_v(1, vs[1]); p.set_w(0, vs[2]); p.set_w(1, vs[3])
+    R = p.R.astype(np.float64); V = p.V.astype(np.float64); W = p.W.astype(np.float64)
+    bi, bj = np.array([0], np.uint32), np.array([1], np.uint32)
+    iL = np.array([1.0]); br = np.array([m.b_r]); sc = np.array([1e9]); tc = np.array([1e9])
+    st = np.zeros(12); act = np.ones(1, np.uint8)
+    F = np.zeros(6); M = np.zeros(6)
+    for _ in range(5):
+        p.bonds(m)
+        lib().orc_native_bonds_f64(ptr(R), ptr(V), ptr(W), 2, ptr(bi), ptr(bj), ptr(iL),
+                                   ptr(br), ptr(sc), ptr(tc), ptr(st), ptr(act), 1, m.dt,
+                                   m.E, m.G, 1, ptr(F), ptr(M))
+    np.testing.assert_allclose(F, p.F, rtol=2e-4, atol=1e-6 * np.abs(p.F).max())
+    np.testing.assert_allclose(M, p.M, rtol=2e-4, atol=1e-6 * np.abs(p.M).max())
