@@ -170,6 +170,104 @@ int rfd_fire_halfstep(rfd_ctx *ctx, const float *V, float *R, uint32_t N,
 int rfd_fire_power(rfd_ctx *ctx, const float *F, const float *V,
                    const float *M, const float *W, uint32_t N, double out[4]);
 
+/* =========================== tier 2: native system ======================
+ * Particles: R in FP64 (position accumulation), V, W, F, M in FP32, SoA by
+ * component; per-particle radius (polydisperse).  Bonds: one record per
+ * undirected pair (i<j) in SoA, grouped by owner i (CSR), with per-bond
+ * radius, rest length and strengths; bond state n_prev, O, Mn, Mt (FP32,
+ * seen from the owner) double-buffered so the partner end re-evaluates the
+ * bond from the same old state without a race.  A broken bond record keeps
+ * acting through the contact law (as a reference slot with ILtype 0 does).
+ * Never-bonded neighbours are held in per-particle contact rows rebuilt every
+ * `rebuild_every` steps within a skin.  Platens, integration, the cut box and
+ * the loading control (SumForcesULoading + CompressU/B) run on the device, so
+ * an explicit step needs no host synchronisation. */
+
+typedef struct rfd_system rfd_system;
+
+typedef struct rfd_system_params {
+  double E, nu, rho;     /* elastic modulus, Poisson ratio, density        */
+  double mu, mur, e;     /* sliding friction, rolling friction, restitution */
+  double dt;             /* time step                                       */
+  double farcut;         /* far-link distance as a multiple of r_i + r_j    */
+  double skin;           /* contact-row skin (length)                       */
+  uint32_t rebuild_every;/* contact-row rebuild period in steps             */
+  uint32_t contact_capacity; /* contact slots per particle                  */
+  int refmode;           /* 1: reference bending-moment factor
+                            (md_calculateforcesdem3.cu:148) and rolling-moment
+                            wipe (:330-335); 0: physical law                */
+} rfd_system_params;
+
+/* loading control, device resident; mirrors compression_data
+ * (md_data_types.h:188-294) and the fracture logic of md_results.cu:34-74 */
+typedef struct rfd_loading {
+  int mode;              /* 0 none, 1 uniaxial (z), 2 Brazilian (y)         */
+  int cut;               /* 1: park particles leaving the cut box           */
+  double cx, cy, cz;     /* platen axis centre                              */
+  double RR;             /* squared radius of the platen footprint          */
+  double zcut;           /* particles above zcut are ignored by the platens */
+  double top, bottom, top_r, bottom_r, band;
+  double v, v_load;      /* current and nominal platen speed                */
+  double size, size0;    /* current and initial specimen size on load axis  */
+  double inv_area, min_stress, zero_stress, max_strain;
+  double stress, strain, fracture_stress, fracture_strain;
+  double sum_bottom, sum_top;
+  int loading, fracture;
+  uint32_t steps;        /* explicit steps taken                            */
+  uint32_t hist_len;     /* history records written                         */
+  double box_c[3], box_h[3], hidden[3];  /* cut box and parking point       */
+} rfd_loading;
+
+int rfd_system_create(rfd_ctx *ctx, const rfd_system_params *p, uint32_t N,
+                      rfd_system **out);
+int rfd_system_destroy(rfd_system *sys);
+/* host SoA positions R[3N] (FP64) and radii[N]; V = W = 0 */
+int rfd_system_set_particles(rfd_system *sys, const double *R, const float *radius);
+/* GPU neighbour search: pairs (i<j) whose surface gap |rij|-ri-rj < gap, in
+ * (i, j) order, written to host pairs[2*cap]; *count = number found (which
+ * may exceed cap: then call again with a larger buffer) */
+int rfd_system_find_pairs(rfd_system *sys, double gap, uint32_t *pairs_host,
+                          uint32_t cap, uint32_t *count);
+/* bonds (host arrays, sorted by (bi, bj), bi < bj); rest lengths L0 (NULL:
+ * the current distance, as d_ConstructBoundInteractions sets 1/L0 from the
+ * current positions, md_boundinteractions_construct.cu:65-66); state zero;
+ * all active */
+int rfd_system_set_bonds(rfd_system *sys, uint32_t Nb, const uint32_t *bi,
+                         const uint32_t *bj, const float *br, const float *sc,
+                         const float *tc, const float *L0);
+/* bond state, host arrays: state[12*Nb] = n_prev, O, Mn, Mt per bond (owner
+ * view, bond-major), flag[Nb] = 1 bond, 0 broken (contact law), 2 removed */
+int rfd_system_set_bond_state(rfd_system *sys, const float *state, const uint8_t *flag);
+int rfd_system_get_bond_state(rfd_system *sys, float *state, uint8_t *flag);
+int rfd_system_set_velocities(rfd_system *sys, const float *V, const float *W);
+int rfd_system_set_loading(rfd_system *sys, const rfd_loading *L);
+int rfd_system_get_loading(rfd_system *sys, rfd_loading *L);
+/* host copies; any pointer may be NULL */
+int rfd_system_get_state(rfd_system *sys, double *R, float *V, float *W, float *F,
+                         float *M);
+/* one force evaluation (bonds, broken-bond contacts, contact rows) into F, M;
+ * commits the bond / contact state like a step does */
+int rfd_system_compute_forces(rfd_system *sys);
+/* n explicit steps: [contact-row rebuild every rebuild_every steps], forces,
+ * platen + integration + cut pass, loading control.  Asynchronous. */
+int rfd_system_step(rfd_system *sys, uint32_t nsteps);
+/* stress-strain history (strain, bottom stress, top stress) per step */
+int rfd_system_get_history(rfd_system *sys, double *hist, uint32_t cap, uint32_t *n);
+/* counts of bond records by flag: out[0]=bonds, out[1]=broken, out[2]=removed */
+int rfd_system_bond_counts(rfd_system *sys, uint32_t out[3]);
+/* number of particles and bond records */
+int rfd_system_sizes(rfd_system *sys, uint32_t *N, uint32_t *Nb);
+/* HIP-event timing of the force kernel launched by rfd_system_step /
+ * rfd_system_compute_forces, recorded on the context stream: enable (also
+ * clears), then read the summed kernel time in ms and the launch count */
+int rfd_system_timing(rfd_system *sys, int enable);
+int rfd_system_timing_read(rfd_system *sys, double *kernel_ms, uint32_t *launches);
+/* device pointers of the particle arrays (for callers that keep data in HBM):
+ * P is N records of 4 doubles {x, y, z, radius}; V, W, F, M are N records of
+ * 4 floats {x, y, z, 0} */
+int rfd_system_device_arrays(rfd_system *sys, double **P, float **V, float **W,
+                             float **F, float **M);
+
 #ifdef __cplusplus
 }
 #endif

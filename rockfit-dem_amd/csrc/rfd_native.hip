@@ -1,0 +1,1383 @@
+// rfd_native.hip -- native tier: AoS particle records (FP64 position +
+// radius), undirected edge-list bonds with owner/partner CSR, slot-major
+// contact rows, fused platen + integration pass, device-side loading control.
+//
+// Memory layout (all in HBM, sized for one GPU's share of a specimen):
+//   particle  P4 {x, y, z, r} (double, 32 B), V4 / W4 / F4 / M4 (float4),
+//             inv2 {dt/m, dt/I} (float2)
+//   bond      bij {i, j} (uint2), bp {L0, br, sc, tc} (float4),
+//             state {n_prev, O, Mn, Mt} (3 x float4 = 48 B, double-buffered),
+//             flag (uint8, double-buffered: 1 bond, 0 broken, 2 removed)
+//   owner CSR boff[N+1] over bonds sorted by (i, j); partner CSR ioff[N+1],
+//             iq[] = bond ids grouped by j
+//   contacts  slot-major rows cj[k*N+i], cO[k*N+i] (float4), ccnt[i]
+// A particle record or a bond record is one contiguous segment, so a gather
+// of a neighbour costs one or two cache lines.
+//
+// Force pass: one workgroup per 64 particles.  Its bond ends (owner ends, a
+// contiguous range of bonds, then partner ends) are evaluated one per lane in
+// chunks of 512; each lane writes its end's load to LDS, then each particle's
+// thread adds its ends in a fixed order (owner ends, then partner ends).  The
+// owner lane writes the new bond state; the partner lane evaluates the same
+// bond from the same old state (bit-identical) and writes nothing.  No
+// atomics: the pass is deterministic.
+//
+// Bond law: md_calculateforcesdem3.cu:38-197 with per-bond radius; stretch in
+// FP64 from FP64 positions, the rest in FP32, break criterion (:20-36) in
+// FP64.  A bond that breaks also feels the contact law in the same pass,
+// because the reference runs its contact kernel after the bond kernel on the
+// just-freed slot (md_calculationuniaxialcompression.cu:173-176).  Broken
+// records act through the Hertz-Mindlin law of :199-336 with R* = ri rj /
+// (ri + rj) and m* = mi mj / (mi + mj), which is the reference's law for
+// equal radii.
+#include <vector>
+
+#include "rfd_internal.h"
+
+using namespace rfd;
+
+namespace {
+
+constexpr int kFP = 64;    // particles per force workgroup
+constexpr int kFT = 256;   // threads per force workgroup
+constexpr int kCH = 512;   // bond ends per LDS chunk
+constexpr uint8_t kBond = 1, kBroken = 0, kRemoved = 2;
+constexpr int kHistCap = 1 << 20;
+
+struct alignas(32) P4 {
+  double x, y, z, r;
+};
+struct alignas(16) BS {
+  float4 a, b, c;  // (npx,npy,npz,Ox) (Oy,Oz,Mnx,Mny) (Mnz,Mtx,Mty,Mtz)
+};
+
+struct v3 {
+  float x, y, z;
+};
+__device__ __forceinline__ v3 operator+(v3 a, v3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ v3 operator-(v3 a, v3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ v3 operator*(v3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ v3 cross(v3 a, v3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ double dnorm(v3 a) {
+  double x = a.x, y = a.y, z = a.z;
+  return __builtin_sqrt(x * x + y * y + z * z);
+}
+__device__ __forceinline__ v3 xyz(float4 q) { return {q.x, q.y, q.z}; }
+__device__ __forceinline__ float4 f4(v3 v) { return make_float4(v.x, v.y, v.z, 0.f); }
+__device__ constexpr v3 kZero = {0.f, 0.f, 0.f};
+
+struct Mat {
+  float E, G, Estar, Gstar, pA, mu, mur, rho43pi, dt;
+  int refmode;
+};
+
+// first-order rotation of md_calculateforcesdem3.cu:80-88 built from
+// t = n_prev x n and phi
+struct Rot {
+  float m[9];
+  __device__ __forceinline__ void build(v3 t, v3 ph) {
+    m[0] = 1.0f + t.z * ph.z + t.y * ph.y;
+    m[1] = ph.z - t.z - t.y * ph.x;
+    m[2] = -ph.y - t.z * ph.x + t.y;
+    m[3] = t.z - ph.z - t.x * ph.y;
+    m[4] = t.z * ph.z + 1.0f + t.x * ph.x;
+    m[5] = -t.z * ph.y + ph.x - t.x;
+    m[6] = -t.y - t.x * ph.z + ph.y;
+    m[7] = -t.y * ph.z + t.x - ph.x;
+    m[8] = t.y * ph.y + t.x * ph.x + 1.0f;
+  }
+  __device__ __forceinline__ v3 apply(v3 x) const {
+    return {m[0] * x.x + m[1] * x.y + m[2] * x.z, m[3] * x.x + m[4] * x.y + m[5] * x.z,
+            m[6] * x.x + m[7] * x.y + m[8] * x.z};
+  }
+};
+
+struct EndData {
+  P4 P;
+  v3 V, W;
+};
+
+// Hertz-Mindlin contact for the pair (i, j) (md_calculateforcesdem3.cu:199-336),
+// n from i to j, overlap ri + rj - L; O is i's tangential spring (in/out).
+// Returns the load on i and the contact moments on i and j.
+__device__ __forceinline__ void contact_law(const Mat &mt, v3 n, float L, float ri, float rj,
+                                            v3 Vi, v3 Vj, v3 Wi, v3 Wj, v3 &O, v3 &Fi, v3 &Mi,
+                                            v3 &Mj) {
+  const float Rs = ri * rj / (ri + rj);
+  const float mi = mt.rho43pi * ri * ri * ri, mj = mt.rho43pi * rj * rj * rj;
+  const float ms = mi * mj / (mi + mj);
+  const float q = sqrtf(Rs * (ri + rj - L));
+  const float Kn = 2.0f * mt.Estar * q;
+  const float Kt = 8.0f * mt.Gstar * q;
+  const v3 v = Vj - Vi + cross(n, Wi * ri + Wj * rj);
+  const float vn = dot(n, v);
+  const float fn = (L - (ri + rj)) * Kn * kTwoThirdsF;
+  const float dmp = -1.8257f * mt.pA * vn * sqrtf(Kn * ms);
+  Fi = n * (fn + dmp);
+  const v3 vt = v - n * vn;
+  v3 o = O - n * dot(n, O);  // rotate the old spring into the tangent plane
+  const float oo = dot(o, o);
+  if ((double)oo > 1e-18) o = o * sqrtf(dot(O, O) / oo);
+  o = o + vt * mt.dt;
+  v3 ft = o * Kt;
+  const float ftm = dot(ft, ft);
+  if (ftm > mt.mu * mt.mu * fn * fn) {  // Coulomb cap
+    ft = ft * (mt.mu * fabsf(fn) * rsqrt_rn(ftm));
+    o = ft * (1.0f / Kt);
+  } else {
+    ft = ft + vt * (-1.8257f * mt.pA * sqrtf(Kt * ms));
+  }
+  O = o;
+  Fi = Fi + ft;
+  const v3 nxf = cross(n, ft);
+  Mi = nxf * ri;
+  Mj = nxf * rj;
+  const float wi2 = dot(Wi, Wi), wj2 = dot(Wj, Wj);
+  if ((double)wi2 > 1e-18) Mi = Mi + Wi * (-mt.mur * fabsf(fn) * rsqrt_rn(wi2));
+  if ((double)wj2 > 1e-18) Mj = Mj + Wj * (-mt.mur * fabsf(fn) * rsqrt_rn(wj2));
+}
+
+struct BondOut {
+  v3 Fi;        // load on i (j gets -Fi)
+  v3 Mbi, Mbj;  // bond moments on i and j
+  v3 Mci, Mcj;  // contact moments on i and j
+  bool hasc;    // the contact law acted (counts for the rolling-wipe rule)
+  BS st;        // new state
+  uint8_t flag;
+};
+
+// evaluate bond record (i, j) from its old state
+__device__ __forceinline__ void eval_bond(const Mat &mt, const EndData &A, const EndData &B,
+                                          float4 bp, const BS &old, uint8_t flag,
+                                          BondOut &o) {
+  const double rx = B.P.x - A.P.x, ry = B.P.y - A.P.y, rz = B.P.z - A.P.z;
+  const double L64 = __builtin_sqrt(rx * rx + ry * ry + rz * rz);
+  o.Fi = o.Mbi = o.Mbj = o.Mci = o.Mcj = kZero;
+  o.hasc = false;
+  o.st = old;
+  o.flag = flag;
+  if (!(L64 > 1e-5)) return;  // coincident ends (both parked): no load
+  const double inv = 1.0 / L64;
+  const v3 n = {(float)(rx * inv), (float)(ry * inv), (float)(rz * inv)};
+  const v3 r = {(float)rx, (float)ry, (float)rz};
+  const float L = (float)L64;
+  const float ri = (float)A.P.r, rj = (float)B.P.r;
+  const float L0 = bp.x, br = bp.y;
+  v3 O = {old.a.w, old.b.x, old.b.y};
+  bool contact_now = flag == kBroken;
+  if (flag == kBond) {
+    const v3 np = xyz(old.a);
+    const v3 Mn0 = {old.b.z, old.b.w, old.c.x};
+    const v3 Mt0 = {old.c.y, old.c.z, old.c.w};
+    const float il = 1.0f / L0;
+    const float A2 = kPiF * br * br, J = kPiF * br * br * br * br;
+    const double eps = (L64 - (double)L0) / (double)L0;
+    const v3 Fn = n * ((float)(mt.E * eps) * A2);
+    const v3 ws = A.W + B.W;
+    Rot M;
+    M.build(cross(np, n), n * (dot(ws, n) * mt.dt * 0.5f));
+    const v3 dw = B.W - A.W;
+    const float dwn = dot(n, dw);
+    const v3 Mn = M.apply(Mn0) + n * (dwn * (mt.dt * 0.5f * mt.G * il));
+    const float cb = mt.dt * 0.25f * mt.E * il;
+    const v3 Mt = M.apply(Mt0) + (dw - n * dwn) * cb;
+    const v3 v = B.V - A.V - cross(ws * 0.5f, r);
+    const float vn = dot(n, v);
+    O = M.apply(O) + (v - n * vn) * mt.dt;
+    const float kt = mt.G * il * A2;
+    const v3 Ft = O * kt;
+    const float mtf = mt.refmode ? cb * J : J;
+    const v3 half = cross(r, Ft) * 0.5f;
+    o.Fi = Fn + Ft;
+    o.Mbi = Mn * J + Mt * mtf + half;
+    o.Mbj = half - Mn * J - Mt * mtf;
+    const double Sc = (double)mt.E * eps + dnorm(Mt) * 4.0 * (double)br;
+    const double Mc = (double)mt.G / (double)L0 * dnorm(O) + dnorm(Mn) * 2.0 * (double)br;
+    if (Sc > (double)bp.z || Mc > (double)bp.w) {
+      o.flag = kBroken;
+      O = kZero;
+      o.st.a = make_float4(n.x, n.y, n.z, 0.f);
+      o.st.b = make_float4(0.f, 0.f, 0.f, 0.f);
+      o.st.c = make_float4(0.f, 0.f, 0.f, 0.f);
+      contact_now = true;  // the freed pair meets the contact law this pass
+    } else {
+      o.st.a = make_float4(n.x, n.y, n.z, O.x);
+      o.st.b = make_float4(O.y, O.z, Mn.x, Mn.y);
+      o.st.c = make_float4(Mn.z, Mt.x, Mt.y, Mt.z);
+      return;
+    }
+  }
+  if (contact_now) {
+    if (L < ri + rj) {
+      v3 Fc;
+      contact_law(mt, n, L, ri, rj, A.V, B.V, A.W, B.W, O, Fc, o.Mci, o.Mcj);
+      o.Fi = o.Fi + Fc;
+      o.hasc = true;
+    } else {
+      O = kZero;
+    }
+    o.st.a.w = O.x;
+    o.st.b.x = O.y;
+    o.st.b.y = O.z;
+  }
+}
+
+struct SysDev {
+  uint32_t N, Kc;
+  const P4 *P;
+  const float4 *V, *W;
+  float4 *F, *M;
+  const uint32_t *boff, *ioff, *iq;
+  const uint2 *bij;
+  const float4 *bp;
+  const BS *st_old;
+  BS *st_new;
+  const uint8_t *fl_old;
+  uint8_t *fl_new;
+  const uint32_t *cj, *ccnt;
+  float4 *cO;
+};
+
+__device__ __forceinline__ EndData load_end(const SysDev &s, uint32_t i) {
+  EndData e;
+  e.P = s.P[i];
+  e.V = xyz(s.V[i]);
+  e.W = xyz(s.W[i]);
+  return e;
+}
+
+__global__ void __launch_bounds__(kFT) k_nat_forces(SysDev s, Mat mt) {
+  __shared__ float sF[3][kCH], sMb[3][kCH], sMc[3][kCH];
+  __shared__ uint8_t sC[kCH];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t p0 = blockIdx.x * kFP;
+  const uint32_t np = min((uint32_t)kFP, s.N - p0);
+  const uint32_t ob0 = s.boff[p0], ob1 = s.boff[p0 + np];
+  const uint32_t ib0 = s.ioff[p0], ib1 = s.ioff[p0 + np];
+  const uint32_t n_own = ob1 - ob0, n_end = n_own + (ib1 - ib0);
+  // per-particle accumulators live in the first np threads
+  v3 F = kZero, Mb = kZero, Mc = kZero;
+  bool anyc = false;
+  uint32_t o0 = 0, o1 = 0, q0 = 0, q1 = 0;
+  if (tid < np) {
+    const uint32_t p = p0 + tid;
+    o0 = s.boff[p] - ob0;
+    o1 = s.boff[p + 1] - ob0;
+    q0 = n_own + s.ioff[p] - ib0;
+    q1 = n_own + s.ioff[p + 1] - ib0;
+  }
+  for (uint32_t c0 = 0; c0 < n_end; c0 += kCH) {
+    const uint32_t c1 = min(c0 + (uint32_t)kCH, n_end);
+    for (uint32_t e = c0 + tid; e < c1; e += kFT) {
+      const bool owner = e < n_own;
+      const uint32_t b = owner ? ob0 + e : s.iq[ib0 + (e - n_own)];
+      const uint8_t fl = s.fl_old[b];
+      BondOut o;
+      if (fl == kRemoved) {
+        o.Fi = o.Mbi = o.Mbj = o.Mci = o.Mcj = kZero;
+        o.hasc = false;
+        if (owner) s.fl_new[b] = kRemoved;
+      } else {
+        const uint2 ij = s.bij[b];
+        eval_bond(mt, load_end(s, ij.x), load_end(s, ij.y), s.bp[b], s.st_old[b], fl, o);
+        if (owner) {
+          s.st_new[b] = o.st;
+          s.fl_new[b] = o.flag;
+        }
+      }
+      const uint32_t k = e - c0;
+      const v3 f = owner ? o.Fi : kZero - o.Fi;
+      const v3 mb = owner ? o.Mbi : o.Mbj;
+      const v3 mc = owner ? o.Mci : o.Mcj;
+      sF[0][k] = f.x;
+      sF[1][k] = f.y;
+      sF[2][k] = f.z;
+      sMb[0][k] = mb.x;
+      sMb[1][k] = mb.y;
+      sMb[2][k] = mb.z;
+      sMc[0][k] = mc.x;
+      sMc[1][k] = mc.y;
+      sMc[2][k] = mc.z;
+      sC[k] = o.hasc ? 1 : 0;
+    }
+    __syncthreads();
+    if (tid < np) {
+      // owner ends of this particle inside the chunk, then its partner ends
+      const uint32_t a0 = max(o0, c0), a1 = min(o1, c1);
+      for (uint32_t e = a0; e < a1; ++e) {
+        const uint32_t k = e - c0;
+        F = F + v3{sF[0][k], sF[1][k], sF[2][k]};
+        Mb = Mb + v3{sMb[0][k], sMb[1][k], sMb[2][k]};
+        Mc = Mc + v3{sMc[0][k], sMc[1][k], sMc[2][k]};
+        anyc = anyc || sC[k];
+      }
+      const uint32_t b0 = max(q0, c0), b1 = min(q1, c1);
+      for (uint32_t e = b0; e < b1; ++e) {
+        const uint32_t k = e - c0;
+        F = F + v3{sF[0][k], sF[1][k], sF[2][k]};
+        Mb = Mb + v3{sMb[0][k], sMb[1][k], sMb[2][k]};
+        Mc = Mc + v3{sMc[0][k], sMc[1][k], sMc[2][k]};
+        anyc = anyc || sC[k];
+      }
+    }
+    __syncthreads();
+  }
+  if (tid >= np) return;
+  // never-bonded contact rows of this particle (per-end spring, in place)
+  const uint32_t i = p0 + tid, N = s.N;
+  const EndData me = load_end(s, i);
+  const uint32_t nc = s.ccnt[i];
+  for (uint32_t k = 0; k < nc; ++k) {
+    const size_t sl = (size_t)k * N + i;
+    const uint32_t j = s.cj[sl];
+    const EndData other = load_end(s, j);
+    const double rx = other.P.x - me.P.x, ry = other.P.y - me.P.y, rz = other.P.z - me.P.z;
+    const double L64 = __builtin_sqrt(rx * rx + ry * ry + rz * rz);
+    const float ri = (float)me.P.r, rj = (float)other.P.r;
+    if ((float)L64 < ri + rj && L64 > 1e-5) {
+      const double inv = 1.0 / L64;
+      const v3 n = {(float)(rx * inv), (float)(ry * inv), (float)(rz * inv)};
+      v3 O = xyz(s.cO[sl]);
+      v3 Fi, Mi, Mj;
+      contact_law(mt, n, (float)L64, ri, rj, me.V, other.V, me.W, other.W, O, Fi, Mi, Mj);
+      F = F + Fi;
+      Mc = Mc + Mi;
+      anyc = true;
+      s.cO[sl] = f4(O);
+    } else {
+      s.cO[sl] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  }
+  // rolling-moment wipe of md_calculateforcesdem3.cu:330-335: with W_i ~ 0 the
+  // reference clears the whole contact moment accumulator of the particle
+  if (mt.refmode && anyc && !((double)dot(me.W, me.W) > 1e-18)) Mc = kZero;
+  s.F[i] = f4(F);
+  s.M[i] = f4(Mb + Mc);
+}
+
+// ---------------------------------------------------------- particle pass
+struct PassDev {
+  uint32_t N;
+  P4 *P;
+  float4 *V, *W, *F;
+  const float4 *M;
+  const float2 *inv;
+  float dt;
+  rfd_loading *ctl;
+  double *part;
+};
+
+// platen clamp + reaction sums (md_uniaxialcompression3_simple.cu:19-76,
+// md_braziltest_simple2.cu:19-92).  MODE 1: load axis z, 2: load axis y.
+template <int MODE>
+__device__ __forceinline__ void platen(const rfd_loading &L, P4 &R, v3 &V, v3 &F, double &sb,
+                                       double &stp) {
+  double ra, rb, h, zc;
+  if (MODE == 1) {
+    ra = R.x - L.cx;
+    rb = R.y - L.cy;
+    h = R.z;
+    zc = R.z;
+  } else {
+    ra = R.x - L.cx;
+    rb = R.z - L.cz;
+    h = R.y;
+    zc = R.z - L.cz;
+  }
+  if (!(ra * ra + rb * rb < L.RR && zc < L.zcut)) return;
+  float &Fa = MODE == 1 ? F.z : F.y;
+  float &Va = MODE == 1 ? V.z : V.y;
+  double &Ra = MODE == 1 ? R.z : R.y;
+  if (h - L.top_r > -1e-9) {
+    const float f = Fa;
+    if (f > 0) {
+      stp += f;
+      if (MODE == 2) {
+        float vv = V.x * V.x + V.z * V.z;
+        if ((double)vv > 1e-9) {
+          vv = rsqrt_rn(vv);
+          F.x = F.x - 0.45f * f * V.x * vv;
+          F.z = F.z - 0.45f * f * V.z * vv;
+        }
+      }
+    }
+    if (h - L.top > -1e-9) {
+      Ra = L.top;
+      Va = (float)L.v;
+      Fa = 0.f;
+    }
+  }
+  if (h - L.bottom_r < 1e-9) {
+    const float f = Fa;
+    if (f < 0) {
+      sb += f;
+      if (MODE == 2) {
+        float vv = V.x * V.x + V.z * V.z;
+        if ((double)vv > 1e-9) {
+          vv = rsqrt_rn(vv);
+          F.x = F.x + 0.45f * f * V.x * vv;
+          F.z = F.z + 0.45f * f * V.z * vv;
+        }
+      }
+    }
+    if (h - L.bottom < 1e-9) {
+      Ra = L.bottom;
+      Va = 0.f;
+      Fa = 0.f;
+    }
+  }
+}
+
+// platens, symplectic Euler with FP64 positions (md_calculateincrementsdem.cu:19-49),
+// cut box (md_parallelepipedrestriction.cu:73-158)
+template <int MODE>
+__global__ void __launch_bounds__(256) k_nat_pass(PassDev p) {
+  __shared__ double s_part[8];
+  const rfd_loading L = *p.ctl;
+  const uint32_t N = p.N;
+  double sb = 0.0, stp = 0.0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
+    P4 R = p.P[i];
+    v3 V = xyz(p.V[i]), F = xyz(p.F[i]);
+    if (MODE) platen<MODE>(L, R, V, F, sb, stp);
+    const float2 iv = p.inv[i];
+    V = V + F * iv.x;
+    R.x += (double)V.x * p.dt;
+    R.y += (double)V.y * p.dt;
+    R.z += (double)V.z * p.dt;
+    v3 W = xyz(p.W[i]) + xyz(p.M[i]) * iv.y;
+    if (L.cut) {
+      const double dx = R.x - L.box_c[0], dy = R.y - L.box_c[1], dz = R.z - L.box_c[2];
+      if (dx - L.box_h[0] > 0 || dx + L.box_h[0] < 0 || dy - L.box_h[1] > 0 ||
+          dy + L.box_h[1] < 0 || dz - L.box_h[2] > 0 || dz + L.box_h[2] < 0) {
+        R.x = L.hidden[0];
+        R.y = L.hidden[1];
+        R.z = L.hidden[2];
+        V = kZero;
+        F = kZero;
+      }
+    }
+    p.P[i] = R;
+    p.V[i] = f4(V);
+    p.W[i] = f4(W);
+    p.F[i] = f4(F);
+  }
+  if (MODE == 0) return;
+  sb = wave_sum(sb);
+  stp = wave_sum(stp);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    s_part[2 * wid] = sb;
+    s_part[2 * wid + 1] = stp;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, c = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+      a += s_part[2 * w];
+      c += s_part[2 * w + 1];
+    }
+    p.part[2 * blockIdx.x] = a;
+    p.part[2 * blockIdx.x + 1] = c;
+  }
+}
+
+// loading control, one thread: SumForcesULoading (md_results.cu:34-74) on the
+// platen sums, then CompressU / CompressB (md_data_types.h:223-244, 272-293)
+__global__ void k_nat_control(rfd_loading *ctl, const double *part, int nblocks, double *hist,
+                              double dt) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  rfd_loading L = *ctl;
+  L.steps += 1;
+  if (L.mode) {
+    double s0 = 0, s1 = 0;
+    for (int b = 0; b < nblocks; ++b) {
+      s0 += part[2 * b];
+      s1 += part[2 * b + 1];
+    }
+    L.sum_bottom = s0;
+    L.sum_top = s1;
+    if (L.hist_len < (uint32_t)kHistCap) {
+      hist[3 * (size_t)L.hist_len] = (L.size - L.size0) / L.size0;
+      hist[3 * (size_t)L.hist_len + 1] = s0 * L.inv_area;
+      hist[3 * (size_t)L.hist_len + 2] = s1 * L.inv_area;
+      L.hist_len += 1;
+    }
+    L.stress = 0.5 * (fabs(s0) + fabs(s1)) * L.inv_area;
+    if (!L.loading && L.stress > L.min_stress) L.loading = 1;
+    if (L.fracture_stress < L.stress) {
+      L.fracture_stress = L.stress;
+      L.fracture_strain = (L.size0 - L.size) / L.size0;
+    }
+    if (L.loading && L.stress < L.zero_stress) L.fracture = 1;
+    L.v = L.stress < 1e-9 ? 10.0 * L.v_load : L.v_load;
+    L.size += L.v * dt;
+    L.top += L.v * dt;
+    L.top_r = L.top - L.band;
+    L.strain = (L.size - L.size0) / L.size0;
+  }
+  *ctl = L;
+}
+
+// ----------------------------------------------------- neighbour search
+struct GridDev {
+  double ox, oy, oz, inv_cell;
+  uint32_t nx, ny, nz;
+};
+
+__device__ __forceinline__ uint32_t cell_clamped(const GridDev &g, const P4 &R, int c[3]) {
+  const double f[3] = {(R.x - g.ox) * g.inv_cell, (R.y - g.oy) * g.inv_cell,
+                       (R.z - g.oz) * g.inv_cell};
+  const uint32_t n[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double q = floor(f[d]);
+    if (!(q >= 0)) q = 0;
+    if (q > (double)(n[d] - 1)) q = (double)(n[d] - 1);
+    c[d] = (int)q;
+  }
+  return (uint32_t)c[0] + (uint32_t)c[1] * g.nx + (uint32_t)c[2] * g.nx * g.ny;
+}
+
+__global__ void k_nat_cell_index(const P4 *__restrict__ P, uint32_t N, GridDev g,
+                                 uint32_t *__restrict__ CI, uint32_t *__restrict__ cnt) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
+    int c[3];
+    const uint32_t ci = cell_clamped(g, P[i], c);
+    CI[i] = ci;
+    atomicAdd(&cnt[ci], 1u);
+  }
+}
+
+// exclusive scan of cnt[0..C) by one workgroup of 1024 threads; start[C] = total
+__global__ void __launch_bounds__(1024)
+k_nat_scan(const uint32_t *__restrict__ cnt, uint32_t C, uint32_t *__restrict__ start,
+           uint32_t *__restrict__ cursor) {
+  __shared__ uint32_t s_sum[16];
+  __shared__ uint32_t s_carry;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  if (tid == 0) s_carry = 0;
+  __syncthreads();
+  constexpr int IT = 8;
+  for (uint32_t base = 0; base < C; base += 1024 * IT) {
+    uint32_t v[IT], local = 0;
+    const uint32_t b0 = base + tid * IT;
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+      v[k] = (b0 + k < C) ? cnt[b0 + k] : 0u;
+      local += v[k];
+    }
+    uint32_t inc = local;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t t = __shfl_up(inc, off, 64);
+      if (lane >= off) inc += t;
+    }
+    if (lane == 63) s_sum[wid] = inc;
+    __syncthreads();
+    uint32_t wo = 0;
+    for (int w = 0; w < wid; ++w) wo += s_sum[w];
+    uint32_t run = s_carry + wo + inc - local;
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+      if (b0 + k < C) {
+        start[b0 + k] = run;
+        if (cursor) cursor[b0 + k] = run;
+      }
+      run += v[k];
+    }
+    __syncthreads();
+    if (tid == 1023) s_carry = run;
+    __syncthreads();
+  }
+  if (tid == 0) start[C] = s_carry;
+}
+
+__global__ void k_nat_scatter(const uint32_t *__restrict__ CI, uint32_t N,
+                              uint32_t *__restrict__ cursor, uint32_t *__restrict__ tmp) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride())
+    tmp[atomicAdd(&cursor[CI[i]], 1u)] = i;
+}
+
+// stable order inside each cell: rank members by id (one wave per cell)
+__global__ void k_nat_rank(const uint32_t *__restrict__ tmp, const uint32_t *__restrict__ start,
+                           uint32_t C, uint32_t *__restrict__ sorted) {
+  const uint32_t lane = threadIdx.x & 63, waves = blockDim.x >> 6;
+  for (uint32_t c = blockIdx.x * waves + (threadIdx.x >> 6); c < C; c += gridDim.x * waves) {
+    const uint32_t s0 = start[c], n = start[c + 1] - s0;
+    for (uint32_t e = lane; e < n; e += 64) {
+      const uint32_t id = tmp[s0 + e];
+      uint32_t rank = 0;
+      for (uint32_t t = 0; t < n; ++t) rank += (tmp[s0 + t] < id) ? 1u : 0u;
+      sorted[s0 + rank] = id;
+    }
+  }
+}
+
+// is there a bond record (not removed) between a < b ?
+__device__ __forceinline__ bool bonded(const uint32_t *__restrict__ boff,
+                                       const uint2 *__restrict__ bij,
+                                       const uint8_t *__restrict__ fl, uint32_t a, uint32_t b) {
+  for (uint32_t q = boff[a]; q < boff[a + 1]; ++q)
+    if (bij[q].y == b) return fl[q] != kRemoved;
+  return false;
+}
+
+struct NbrDev {
+  uint32_t N;
+  const P4 *P;
+  GridDev g;
+  const uint32_t *start, *sorted;
+  double extra;
+  uint32_t *count;
+  const uint32_t *pair_off;
+  uint32_t *pairs;
+  uint32_t Kc;
+  const uint32_t *boff;
+  const uint2 *bij;
+  const uint8_t *fl;
+  const uint32_t *cj_old, *ccnt_old;
+  const float4 *cO_old;
+  uint32_t *cj_new, *ccnt_new;
+  float4 *cO_new;
+  uint32_t *flags;
+};
+
+// Per particle, the 27 surrounding cells in fixed order, members by id.
+// MODE 0: count pairs j>i with |rij| < ri + rj + extra; MODE 1: write them;
+// MODE 2: rebuild the contact row (never-bonded j within ri + rj + extra),
+// carrying each spring O over from the previous row.
+template <int MODE>
+__global__ void __launch_bounds__(128) k_nat_neighbours(NbrDev s) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= s.N) return;
+  const uint32_t N = s.N;
+  const P4 Ri = s.P[i];
+  int c[3];
+  cell_clamped(s.g, Ri, c);
+  uint32_t nfound = 0;
+  uint32_t out = MODE == 1 ? s.pair_off[i] : 0;
+  for (int dz = -1; dz <= 1; ++dz) {
+    const int z = c[2] + dz;
+    if (z < 0 || z >= (int)s.g.nz) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int y = c[1] + dy;
+      if (y < 0 || y >= (int)s.g.ny) continue;
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int x = c[0] + dx;
+        if (x < 0 || x >= (int)s.g.nx) continue;
+        const uint32_t cc = (uint32_t)x + (uint32_t)y * s.g.nx + (uint32_t)z * s.g.nx * s.g.ny;
+        const uint32_t e1 = s.start[cc + 1];
+        for (uint32_t e = s.start[cc]; e < e1; ++e) {
+          const uint32_t j = s.sorted[e];
+          if (j == i) continue;
+          if (MODE != 2 && j < i) continue;
+          const P4 Rj = s.P[j];
+          const double rx = Rj.x - Ri.x, ry = Rj.y - Ri.y, rz = Rj.z - Ri.z;
+          const double d2 = rx * rx + ry * ry + rz * rz;
+          if (d2 < 1e-10) continue;
+          const double lim = Ri.r + Rj.r + s.extra;
+          if (d2 >= lim * lim) continue;
+          if (MODE == 0) {
+            ++nfound;
+          } else if (MODE == 1) {
+            s.pairs[2 * (size_t)out] = i;
+            s.pairs[2 * (size_t)out + 1] = j;
+            ++out;
+          } else {
+            const bool bd = i < j ? bonded(s.boff, s.bij, s.fl, i, j)
+                                  : bonded(s.boff, s.bij, s.fl, j, i);
+            if (bd) continue;
+            if (nfound < s.Kc) {
+              const size_t sl = (size_t)nfound * N + i;
+              s.cj_new[sl] = j;
+              float4 O = make_float4(0.f, 0.f, 0.f, 0.f);
+              const uint32_t no = s.ccnt_old ? s.ccnt_old[i] : 0;
+              for (uint32_t k = 0; k < no; ++k) {
+                const size_t so = (size_t)k * N + i;
+                if (s.cj_old[so] == j) {
+                  O = s.cO_old[so];
+                  break;
+                }
+              }
+              s.cO_new[sl] = O;
+            } else {
+              atomicOr(s.flags, RFD_FLAG_ROW_FULL);
+            }
+            ++nfound;
+          }
+        }
+      }
+    }
+  }
+  if (MODE == 0) s.count[i] = nfound;
+  if (MODE == 2) s.ccnt_new[i] = nfound < s.Kc ? nfound : s.Kc;
+}
+
+// far-link pruning of bond records (d_DeleteFarLinks,
+// md_interaction_list_reconstructbpm.cu:141-186): a record longer than
+// farcut*(ri+rj), or of coincident ends, is removed
+__global__ void k_nat_prune(const P4 *__restrict__ P, uint32_t N, const uint32_t *__restrict__ boff,
+                            const uint2 *__restrict__ bij, uint8_t *__restrict__ fl,
+                            double farcut) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
+    const P4 Ri = P[i];
+    for (uint32_t b = boff[i]; b < boff[i + 1]; ++b) {
+      if (fl[b] == kRemoved) continue;
+      const P4 Rj = P[bij[b].y];
+      const double rx = Rj.x - Ri.x, ry = Rj.y - Ri.y, rz = Rj.z - Ri.z;
+      const double lim = farcut * (Ri.r + Rj.r);
+      const double d2 = rx * rx + ry * ry + rz * rz;
+      if (d2 > lim * lim || d2 < 1e-10) fl[b] = kRemoved;
+    }
+  }
+}
+
+__global__ void k_set_rest_length(const P4 *__restrict__ P, uint32_t Nb,
+                                  const uint2 *__restrict__ bij, float4 *__restrict__ bp) {
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < Nb; b += grid_stride()) {
+    const P4 A = P[bij[b].x], B = P[bij[b].y];
+    const double rx = B.x - A.x, ry = B.y - A.y, rz = B.z - A.z;
+    bp[b].x = (float)__builtin_sqrt(rx * rx + ry * ry + rz * rz);
+  }
+}
+
+__global__ void k_fill_u32n(uint32_t *p, size_t n, uint32_t v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)blockDim.x * gridDim.x)
+    p[i] = v;
+}
+
+}  // namespace
+
+// ================================================================= host side
+struct rfd_system {
+  rfd_ctx *ctx = nullptr;
+  rfd_system_params p{};
+  uint32_t N = 0, Nb = 0;
+  P4 *P = nullptr;
+  float4 *V = nullptr, *W = nullptr, *F = nullptr, *M = nullptr;
+  float2 *inv = nullptr;
+  uint32_t *boff = nullptr, *ioff = nullptr, *iq = nullptr;
+  uint2 *bij = nullptr;
+  float4 *bp = nullptr;
+  BS *st[2] = {nullptr, nullptr};
+  uint8_t *fl[2] = {nullptr, nullptr};
+  int cur = 0;  // index of the current (old) bond state buffer
+  uint32_t *cj[2] = {nullptr, nullptr}, *ccnt[2] = {nullptr, nullptr};
+  float4 *cO[2] = {nullptr, nullptr};
+  int ccur = 0;
+  GridDev g{};
+  uint32_t C = 0;
+  uint32_t *CI = nullptr, *cstart = nullptr, *ccursor = nullptr, *ctmp = nullptr,
+           *csorted = nullptr, *ccnt_cells = nullptr;
+  rfd_loading *ctl = nullptr;
+  double *part = nullptr;
+  double *hist = nullptr;
+  uint32_t nblocks_pass = 0;
+  uint64_t step_count = 0;
+  int mode = 0;  // host copy of the loading mode (selects the pass kernel)
+  float rmax = 0.f;
+  bool have_particles = false, have_bonds = false, rows_valid = false;
+  bool timing = false;  // force-kernel events on the context stream
+  std::vector<hipEvent_t> ev;
+  uint32_t ev_used = 0;
+};
+
+namespace {
+
+template <class T>
+int dalloc(rfd_ctx *ctx, T **p, size_t n) {
+  if (*p) {
+    hipFree(*p);
+    *p = nullptr;
+  }
+  if (n == 0) n = 1;
+  RFD_HIP(ctx, hipMalloc((void **)p, sizeof(T) * n));
+  return RFD_OK;
+}
+
+template <class T>
+void dfree(T *&p) {
+  if (p) hipFree(p);
+  p = nullptr;
+}
+
+Mat make_mat(const rfd_system_params &p) {
+  Mat m{};
+  const double G = 0.5 * p.E / (1.0 + p.nu);
+  m.E = (float)p.E;
+  m.G = (float)G;
+  m.Estar = (float)(0.5 * p.E / (1.0 - p.nu * p.nu));
+  m.Gstar = (float)(0.5 * G / (2.0 - p.nu));
+  const double le = log(p.e);
+  m.pA = (float)(le / sqrt(M_PI * M_PI + le * le));
+  m.mu = (float)p.mu;
+  m.mur = (float)p.mur;
+  m.rho43pi = (float)(p.rho * 4.0 / 3.0 * M_PI);
+  m.dt = (float)p.dt;
+  m.refmode = p.refmode;
+  return m;
+}
+
+SysDev make_dev(rfd_system *s) {
+  SysDev d{};
+  d.N = s->N;
+  d.Kc = s->p.contact_capacity;
+  d.P = s->P;
+  d.V = s->V;
+  d.W = s->W;
+  d.F = s->F;
+  d.M = s->M;
+  d.boff = s->boff;
+  d.ioff = s->ioff;
+  d.iq = s->iq;
+  d.bij = s->bij;
+  d.bp = s->bp;
+  d.st_old = s->st[s->cur];
+  d.st_new = s->st[1 - s->cur];
+  d.fl_old = s->fl[s->cur];
+  d.fl_new = s->fl[1 - s->cur];
+  d.cj = s->cj[s->ccur];
+  d.ccnt = s->ccnt[s->ccur];
+  d.cO = s->cO[s->ccur];
+  return d;
+}
+
+int bin_particles(rfd_system *s) {
+  hipStream_t st = s->ctx->stream;
+  k_fill_u32n<<<rfd_blocks(s->C, 256, 4096), 256, 0, st>>>(s->ccnt_cells, s->C, 0u);
+  k_nat_cell_index<<<rfd_blocks(s->N, 256), 256, 0, st>>>(s->P, s->N, s->g, s->CI,
+                                                          s->ccnt_cells);
+  k_nat_scan<<<1, 1024, 0, st>>>(s->ccnt_cells, s->C, s->cstart, s->ccursor);
+  k_nat_scatter<<<rfd_blocks(s->N, 256), 256, 0, st>>>(s->CI, s->N, s->ccursor, s->ctmp);
+  k_nat_rank<<<rfd_blocks(s->C, 4), 256, 0, st>>>(s->ctmp, s->cstart, s->C, s->csorted);
+  RFD_HIP(s->ctx, hipGetLastError());
+  return RFD_OK;
+}
+
+NbrDev make_nbr(rfd_system *s, double extra) {
+  NbrDev d{};
+  d.N = s->N;
+  d.P = s->P;
+  d.g = s->g;
+  d.start = s->cstart;
+  d.sorted = s->csorted;
+  d.extra = extra;
+  d.Kc = s->p.contact_capacity;
+  d.boff = s->boff;
+  d.bij = s->bij;
+  d.fl = s->fl[s->cur];
+  d.flags = s->ctx->d_flags;
+  return d;
+}
+
+int rebuild_rows(rfd_system *s) {
+  int st = bin_particles(s);
+  if (st) return st;
+  NbrDev d = make_nbr(s, s->p.skin);
+  const int o = s->ccur, n = 1 - s->ccur;
+  d.cj_old = s->rows_valid ? s->cj[o] : nullptr;
+  d.ccnt_old = s->rows_valid ? s->ccnt[o] : nullptr;
+  d.cO_old = s->cO[o];
+  d.cj_new = s->cj[n];
+  d.ccnt_new = s->ccnt[n];
+  d.cO_new = s->cO[n];
+  k_nat_neighbours<2><<<rfd_blocks(s->N, 128), 128, 0, s->ctx->stream>>>(d);
+  k_nat_prune<<<rfd_blocks(s->N, 256), 256, 0, s->ctx->stream>>>(s->P, s->N, s->boff, s->bij,
+                                                                   s->fl[s->cur], s->p.farcut);
+  RFD_HIP(s->ctx, hipGetLastError());
+  s->ccur = n;
+  s->rows_valid = true;
+  return RFD_OK;
+}
+
+int forces(rfd_system *s) {
+  SysDev d = make_dev(s);
+  Mat m = make_mat(s->p);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (s->timing && s->ev_used + 2 <= s->ev.size()) {
+    e0 = s->ev[s->ev_used];
+    e1 = s->ev[s->ev_used + 1];
+    s->ev_used += 2;
+    RFD_HIP(s->ctx, hipEventRecord(e0, s->ctx->stream));
+  }
+  k_nat_forces<<<(s->N + kFP - 1) / kFP, kFT, 0, s->ctx->stream>>>(d, m);
+  RFD_HIP(s->ctx, hipGetLastError());
+  if (e1) RFD_HIP(s->ctx, hipEventRecord(e1, s->ctx->stream));
+  s->cur = 1 - s->cur;
+  return RFD_OK;
+}
+
+int pass_and_control(rfd_system *s) {
+  PassDev p{};
+  p.N = s->N;
+  p.P = s->P;
+  p.V = s->V;
+  p.W = s->W;
+  p.F = s->F;
+  p.M = s->M;
+  p.inv = s->inv;
+  p.dt = (float)s->p.dt;
+  p.ctl = s->ctl;
+  p.part = s->part;
+  hipStream_t st = s->ctx->stream;
+  if (s->mode == 1)
+    k_nat_pass<1><<<s->nblocks_pass, 256, 0, st>>>(p);
+  else if (s->mode == 2)
+    k_nat_pass<2><<<s->nblocks_pass, 256, 0, st>>>(p);
+  else
+    k_nat_pass<0><<<s->nblocks_pass, 256, 0, st>>>(p);
+  k_nat_control<<<1, 64, 0, st>>>(s->ctl, s->part, (int)s->nblocks_pass, s->hist, s->p.dt);
+  RFD_HIP(s->ctx, hipGetLastError());
+  return RFD_OK;
+}
+
+bool valid(rfd_system *s) { return s && s->ctx; }
+
+int upload_vec3(rfd_ctx *ctx, float4 *dst, const float *soa, uint32_t N) {
+  std::vector<float4> h(N);
+  for (uint32_t i = 0; i < N; ++i)
+    h[i] = make_float4(soa[i], soa[i + (size_t)N], soa[i + 2 * (size_t)N], 0.f);
+  RFD_HIP(ctx, hipMemcpy(dst, h.data(), sizeof(float4) * N, hipMemcpyHostToDevice));
+  return RFD_OK;
+}
+
+int download_vec3(rfd_ctx *ctx, float *soa, const float4 *src, uint32_t N) {
+  std::vector<float4> h(N);
+  RFD_HIP(ctx, hipMemcpy(h.data(), src, sizeof(float4) * N, hipMemcpyDeviceToHost));
+  for (uint32_t i = 0; i < N; ++i) {
+    soa[i] = h[i].x;
+    soa[i + (size_t)N] = h[i].y;
+    soa[i + 2 * (size_t)N] = h[i].z;
+  }
+  return RFD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rfd_system_create(rfd_ctx *ctx, const rfd_system_params *p, uint32_t N, rfd_system **out) {
+  if (!ctx) return RFD_E_ARG;
+  RFD_CHECK_ARG(ctx, p && out && N > 0, "rfd_system_create: bad arguments");
+  RFD_CHECK_ARG(ctx, p->dt > 0 && p->E > 0 && p->rho > 0 && p->e > 0 && p->e < 1,
+                "rfd_system_create: bad material parameters");
+  RFD_CHECK_ARG(ctx, p->contact_capacity > 0 && p->contact_capacity <= 64,
+                "rfd_system_create: contact_capacity must be 1..64");
+  *out = nullptr;
+  rfd_system *s = new rfd_system();
+  s->ctx = ctx;
+  s->p = *p;
+  if (s->p.rebuild_every == 0) s->p.rebuild_every = 20;
+  if (s->p.farcut <= 0) s->p.farcut = 1.45;
+  s->N = N;
+  int st = 0;
+  const size_t KN = (size_t)p->contact_capacity * N;
+  if (!st) st = dalloc(ctx, &s->P, N);
+  if (!st) st = dalloc(ctx, &s->V, N);
+  if (!st) st = dalloc(ctx, &s->W, N);
+  if (!st) st = dalloc(ctx, &s->F, N);
+  if (!st) st = dalloc(ctx, &s->M, N);
+  if (!st) st = dalloc(ctx, &s->inv, N);
+  if (!st) st = dalloc(ctx, &s->boff, (size_t)N + 1);
+  if (!st) st = dalloc(ctx, &s->ioff, (size_t)N + 1);
+  for (int q = 0; q < 2 && !st; ++q) {
+    st = dalloc(ctx, &s->cj[q], KN);
+    if (!st) st = dalloc(ctx, &s->ccnt[q], N);
+    if (!st) st = dalloc(ctx, &s->cO[q], KN);
+  }
+  if (!st) st = dalloc(ctx, &s->CI, N);
+  if (!st) st = dalloc(ctx, &s->ctmp, N);
+  if (!st) st = dalloc(ctx, &s->csorted, N);
+  if (!st) st = dalloc(ctx, &s->ctl, 1);
+  s->nblocks_pass = rfd_blocks(N, 256, 2048);
+  if (!st) st = dalloc(ctx, &s->part, 2 * (size_t)s->nblocks_pass);
+  if (!st) st = dalloc(ctx, &s->hist, 3 * (size_t)kHistCap);
+  if (!st) {
+    rfd_loading L{};
+    hipError_t e = hipMemcpy(s->ctl, &L, sizeof(L), hipMemcpyHostToDevice);
+    for (int q = 0; q < 2 && e == hipSuccess; ++q)
+      e = hipMemset(s->ccnt[q], 0, sizeof(uint32_t) * N);
+    if (e != hipSuccess) st = rfd_hip_status(ctx, e, "rfd_system_create");
+  }
+  if (st) {
+    rfd_system_destroy(s);
+    return st;
+  }
+  *out = s;
+  return RFD_OK;
+}
+
+int rfd_system_destroy(rfd_system *s) {
+  if (!s) return RFD_OK;
+  if (s->ctx) hipStreamSynchronize(s->ctx->stream);
+  dfree(s->P);
+  dfree(s->V);
+  dfree(s->W);
+  dfree(s->F);
+  dfree(s->M);
+  dfree(s->inv);
+  dfree(s->boff);
+  dfree(s->ioff);
+  dfree(s->iq);
+  dfree(s->bij);
+  dfree(s->bp);
+  for (int q = 0; q < 2; ++q) {
+    dfree(s->st[q]);
+    dfree(s->fl[q]);
+    dfree(s->cj[q]);
+    dfree(s->ccnt[q]);
+    dfree(s->cO[q]);
+  }
+  dfree(s->CI);
+  dfree(s->cstart);
+  dfree(s->ccursor);
+  dfree(s->ctmp);
+  dfree(s->csorted);
+  dfree(s->ccnt_cells);
+  dfree(s->ctl);
+  dfree(s->part);
+  dfree(s->hist);
+  for (hipEvent_t e : s->ev) hipEventDestroy(e);
+  delete s;
+  return RFD_OK;
+}
+
+int rfd_system_timing(rfd_system *s, int enable) {
+  if (!valid(s)) return RFD_E_ARG;
+  rfd_ctx *ctx = s->ctx;
+  RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (enable && s->ev.empty()) {
+    s->ev.resize(2 * 4096);
+    for (auto &e : s->ev) RFD_HIP(ctx, hipEventCreate(&e));
+  }
+  s->timing = enable != 0;
+  s->ev_used = 0;
+  return RFD_OK;
+}
+
+int rfd_system_timing_read(rfd_system *s, double *kernel_ms, uint32_t *launches) {
+  if (!valid(s)) return RFD_E_ARG;
+  rfd_ctx *ctx = s->ctx;
+  RFD_CHECK_ARG(ctx, kernel_ms && launches, "rfd_system_timing_read: null output");
+  RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  double tot = 0;
+  for (uint32_t k = 0; k + 1 < s->ev_used; k += 2) {
+    float ms = 0;
+    RFD_HIP(ctx, hipEventElapsedTime(&ms, s->ev[k], s->ev[k + 1]));
+    tot += ms;
+  }
+  *kernel_ms = tot;
+  *launches = s->ev_used / 2;
+  return RFD_OK;
+}
+
+int rfd_system_set_particles(rfd_system *s, const double *R, const float *radius) {
+  if (!valid(s)) return RFD_E_ARG;
+  rfd_ctx *ctx = s->ctx;
+  RFD_CHECK_ARG(ctx, R && radius, "rfd_system_set_particles: null input");
+  const uint32_t N = s->N;
+  std::vector<P4> hp(N);
+  std::vector<float2> hinv(N);
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  float rmax = 0.f;
+  for (uint32_t i = 0; i < N; ++i) {
+    const double r = radius[i];
+    RFD_CHECK_ARG(ctx, r > 0, "rfd_system_set_particles: radius[%u] <= 0", i);
+    const double m = s->p.rho * 4.0 / 3.0 * M_PI * r * r * r;
+    hinv[i] = make_float2((float)(s->p.dt / m), (float)(s->p.dt / (0.4 * m * r * r)));
+    rmax = fmaxf(rmax, radius[i]);
+    hp[i].x = R[i];
+    hp[i].y = R[i + (size_t)N];
+    hp[i].z = R[i + 2 * (size_t)N];
+    hp[i].r = r;
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = fmin(lo[d], R[i + (size_t)d * N]);
+      hi[d] = fmax(hi[d], R[i + (size_t)d * N]);
+    }
+  }
+  s->rmax = rmax;
+  // grid cell >= the largest search distance: contact rows (2 rmax + skin) and
+  // bond creation with a surface gap up to 0.5 rmax; margin for drift
+  const double cell = fmax(2.5 * rmax + fmax(s->p.skin, 0.0) + 1e-6, 1e-3);
+  const double margin = cell + s->p.farcut * 2.0 * rmax;
+  s->g.ox = lo[0] - margin;
+  s->g.oy = lo[1] - margin;
+  s->g.oz = lo[2] - margin;
+  s->g.inv_cell = 1.0 / cell;
+  s->g.nx = (uint32_t)((hi[0] - lo[0] + 2 * margin) / cell) + 1;
+  s->g.ny = (uint32_t)((hi[1] - lo[1] + 2 * margin) / cell) + 1;
+  s->g.nz = (uint32_t)((hi[2] - lo[2] + 2 * margin) / cell) + 1;
+  const uint64_t C = (uint64_t)s->g.nx * s->g.ny * s->g.nz;
+  RFD_CHECK_ARG(ctx, C < (1u << 30), "rfd_system_set_particles: grid too large (%llu cells)",
+                (unsigned long long)C);
+  s->C = (uint32_t)C;
+  int st = dalloc(ctx, &s->cstart, (size_t)s->C + 1);
+  if (!st) st = dalloc(ctx, &s->ccursor, s->C);
+  if (!st) st = dalloc(ctx, &s->ccnt_cells, s->C);
+  if (st) return st;
+  RFD_HIP(ctx, hipMemcpy(s->P, hp.data(), sizeof(P4) * N, hipMemcpyHostToDevice));
+  RFD_HIP(ctx, hipMemcpy(s->inv, hinv.data(), sizeof(float2) * N, hipMemcpyHostToDevice));
+  RFD_HIP(ctx, hipMemset(s->V, 0, sizeof(float4) * N));
+  RFD_HIP(ctx, hipMemset(s->W, 0, sizeof(float4) * N));
+  RFD_HIP(ctx, hipMemset(s->F, 0, sizeof(float4) * N));
+  RFD_HIP(ctx, hipMemset(s->M, 0, sizeof(float4) * N));
+  for (int q = 0; q < 2; ++q) RFD_HIP(ctx, hipMemset(s->ccnt[q], 0, sizeof(uint32_t) * N));
+  s->have_particles = true;
+  s->rows_valid = false;
+  return RFD_OK;
+}
+
+int rfd_system_find_pairs(rfd_system *s, double gap, uint32_t *pairs_host, uint32_t cap,
+                          uint32_t *count) {
+  if (!valid(s)) return RFD_E_ARG;
+  rfd_ctx *ctx = s->ctx;
+  RFD_CHECK_ARG(ctx, count && s->have_particles, "rfd_system_find_pairs: bad state");
+  RFD_CHECK_ARG(ctx, 2.0 * s->rmax + gap <= 1.0 / s->g.inv_cell,
+                "rfd_system_find_pairs: gap %g exceeds the grid's search distance", gap);
+  int st = bin_particles(s);
+  if (st) return st;
+  uint32_t *cnt = nullptr, *off = nullptr, *pairs = nullptr;
+  st = dalloc(ctx, &cnt, s->N);
+  if (!st) st = dalloc(ctx, &off, (size_t)s->N + 1);
+  if (st) {
+    dfree(cnt);
+    return st;
+  }
+  NbrDev d = make_nbr(s, gap);
+  d.count = cnt;
+  k_nat_neighbours<0><<<rfd_blocks(s->N, 128), 128, 0, ctx->stream>>>(d);
+  k_nat_scan<<<1, 1024, 0, ctx->stream>>>(cnt, s->N, off, nullptr);
+  uint32_t total = 0;
+  hipError_t e = hipMemcpyAsync(&total, off + s->N, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    dfree(cnt);
+    dfree(off);
+    return rfd_hip_status(ctx, e, "rfd_system_find_pairs");
+  }
+  *count = total;
+  if (pairs_host && total <= cap && total > 0) {
+    st = dalloc(ctx, &pairs, 2 * (size_t)total);
+    if (!st) {
+      d.pair_off = off;
+      d.pairs = pairs;
+      k_nat_neighbours<1><<<rfd_blocks(s->N, 128), 128, 0, ctx->stream>>>(d);
+      e = hipMemcpyAsync(pairs_host, pairs, sizeof(uint32_t) * 2 * total,
+                         hipMemcpyDeviceToHost, ctx->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+      if (e != hipSuccess) st = rfd_hip_status(ctx, e, "rfd_system_find_pairs copy");
+    }
+  }
+  dfree(cnt);
+  dfree(off);
+  dfree(pairs);
+  return st;
+}
+
+int rfd_system_set_bonds(rfd_system *s, uint32_t Nb, const uint32_t *bi, const uint32_t *bj,
+                         const float *br, const float *sc, const float *tc, const float *L0) {
+  if (!valid(s)) return RFD_E_ARG;
+  rfd_ctx *ctx = s->ctx;
+  RFD_CHECK_ARG(ctx, s->have_particles, "rfd_system_set_bonds: set particles first");
+  RFD_CHECK_ARG(ctx, Nb == 0 || (bi && bj && br && sc && tc), "rfd_system_set_bonds: null input");
+  const uint32_t N = s->N;
+  std::vector<uint32_t> boff(N + 1, 0), ioff(N + 1, 0), iq(Nb);
+  std::vector<uint2> hij(Nb);
+  std::vector<float4> hbp(Nb);
+  for (uint32_t b = 0; b < Nb; ++b) {
+    RFD_CHECK_ARG(ctx, bi[b] < bj[b] && bj[b] < N, "rfd_system_set_bonds: bond %u (%u,%u) invalid",
+                  b, bi[b], bj[b]);
+    RFD_CHECK_ARG(ctx, b == 0 || bi[b - 1] < bi[b] || (bi[b - 1] == bi[b] && bj[b - 1] < bj[b]),
+                  "rfd_system_set_bonds: bonds not sorted by (i, j) at %u", b);
+    boff[bi[b] + 1]++;
+    ioff[bj[b] + 1]++;
+    hij[b] = make_uint2(bi[b], bj[b]);
+    hbp[b] = make_float4(L0 ? L0[b] : 0.f, br[b], sc[b], tc[b]);
+  }
+  for (uint32_t i = 0; i < N; ++i) {
+    boff[i + 1] += boff[i];
+    ioff[i + 1] += ioff[i];
+  }
+  {
+    std::vector<uint32_t> cur(ioff.begin(), ioff.end() - 1);
+    for (uint32_t b = 0; b < Nb; ++b) iq[cur[bj[b]]++] = b;
+  }
+  s->Nb = Nb;
+  int st = dalloc(ctx, &s->bij, Nb);
+  if (!st) st = dalloc(ctx, &s->bp, Nb);
+  if (!st) st = dalloc(ctx, &s->iq, Nb);
+  for (int q = 0; q < 2 && !st; ++q) {
+    st = dalloc(ctx, &s->st[q], Nb);
+    if (!st) st = dalloc(ctx, &s->fl[q], Nb);
+  }
+  if (st) return st;
+  RFD_HIP(ctx, hipMemcpy(s->boff, boff.data(), sizeof(uint32_t) * (N + 1), hipMemcpyHostToDevice));
+  RFD_HIP(ctx, hipMemcpy(s->ioff, ioff.data(), sizeof(uint32_t) * (N + 1), hipMemcpyHostToDevice));
+  if (Nb) {
+    RFD_HIP(ctx, hipMemcpy(s->bij, hij.data(), sizeof(uint2) * Nb, hipMemcpyHostToDevice));
+    RFD_HIP(ctx, hipMemcpy(s->bp, hbp.data(), sizeof(float4) * Nb, hipMemcpyHostToDevice));
+    RFD_HIP(ctx, hipMemcpy(s->iq, iq.data(), sizeof(uint32_t) * Nb, hipMemcpyHostToDevice));
+    for (int q = 0; q < 2; ++q) {
+      RFD_HIP(ctx, hipMemset(s->st[q], 0, sizeof(BS) * Nb));
+      RFD_HIP(ctx, hipMemset(s->fl[q], kBond, Nb));
+    }
+    if (!L0) {
+      k_set_rest_length<<<rfd_blocks(Nb, 256), 256, 0, ctx->stream>>>(s->P, Nb, s->bij, s->bp);
+      RFD_HIP(ctx, hipGetLastError());
+    }
+  }
+  s->cur = 0;
+  s->have_bonds = true;
+  s->rows_valid = false;
+  RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return RFD_OK;
+}
+
+int rfd_system_set_bond_state(rfd_system *s, const float *state, const uint8_t *flag) {
+  if (!valid(s)) return RFD_E_ARG;
+  rfd_ctx *ctx = s->ctx;
+  RFD_CHECK_ARG(ctx, s->have_bonds && state && flag, "rfd_system_set_bond_state: bad state");
+  static_assert(sizeof(BS) == 12 * sizeof(float), "bond state record is 12 floats");
+  RFD_HIP(ctx, hipMemcpy(s->st[s->cur], state, sizeof(BS) * s->Nb, hipMemcpyHostToDevice));
+  RFD_HIP(ctx, hipMemcpy(s->fl[s->cur], flag, s->Nb, hipMemcpyHostToDevice));
+  return RFD_OK;
+}
+
+int rfd_system_get_bond_state(rfd_system *s, float *state, uint8_t *flag) {
+  if (!valid(s)) return RFD_E_ARG;
+  rfd_ctx *ctx = s->ctx;
+  RFD_CHECK_ARG(ctx, s->have_bonds, "rfd_system_get_bond_state: no bonds");
+  RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (state)
+    RFD_HIP(ctx, hipMemcpy(state, s->st[s->cur], sizeof(BS) * s->Nb, hipMemcpyDeviceToHost));
+  if (flag) RFD_HIP(ctx, hipMemcpy(flag, s->fl[s->cur], s->Nb, hipMemcpyDeviceToHost));
+  return RFD_OK;
+}
+
+int rfd_system_set_velocities(rfd_system *s, const float *V, const float *W) {
+  if (!valid(s)) return RFD_E_ARG;
+  int st = RFD_OK;
+  if (V) st = upload_vec3(s->ctx, s->V, V, s->N);
+  if (!st && W) st = upload_vec3(s->ctx, s->W, W, s->N);
+  return st;
+}
+
+int rfd_system_set_loading(rfd_system *s, const rfd_loading *L) {
+  if (!valid(s)) return RFD_E_ARG;
+  rfd_ctx *ctx = s->ctx;
+  RFD_CHECK_ARG(ctx, L && L->mode >= 0 && L->mode <= 2, "rfd_system_set_loading: bad loading");
+  RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  RFD_HIP(ctx, hipMemcpy(s->ctl, L, sizeof(*L), hipMemcpyHostToDevice));
+  s->mode = L->mode;
+  return RFD_OK;
+}
+
+int rfd_system_get_loading(rfd_system *s, rfd_loading *L) {
+  if (!valid(s)) return RFD_E_ARG;
+  rfd_ctx *ctx = s->ctx;
+  RFD_CHECK_ARG(ctx, L, "rfd_system_get_loading: null output");
+  RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  RFD_HIP(ctx, hipMemcpy(L, s->ctl, sizeof(*L), hipMemcpyDeviceToHost));
+  return RFD_OK;
+}
+
+int rfd_system_get_state(rfd_system *s, double *R, float *V, float *W, float *F, float *M) {
+  if (!valid(s)) return RFD_E_ARG;
+  rfd_ctx *ctx = s->ctx;
+  const uint32_t N = s->N;
+  RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (R) {
+    std::vector<P4> h(N);
+    RFD_HIP(ctx, hipMemcpy(h.data(), s->P, sizeof(P4) * N, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < N; ++i) {
+      R[i] = h[i].x;
+      R[i + (size_t)N] = h[i].y;
+      R[i + 2 * (size_t)N] = h[i].z;
+    }
+  }
+  int st = RFD_OK;
+  if (V) st = download_vec3(ctx, V, s->V, N);
+  if (!st && W) st = download_vec3(ctx, W, s->W, N);
+  if (!st && F) st = download_vec3(ctx, F, s->F, N);
+  if (!st && M) st = download_vec3(ctx, M, s->M, N);
+  return st;
+}
+
+int rfd_system_compute_forces(rfd_system *s) {
+  if (!valid(s)) return RFD_E_ARG;
+  rfd_ctx *ctx = s->ctx;
+  RFD_CHECK_ARG(ctx, s->have_bonds, "rfd_system_compute_forces: set bonds first");
+  if (!s->rows_valid) {
+    int st = rebuild_rows(s);
+    if (st) return st;
+  }
+  return forces(s);
+}
+
+int rfd_system_step(rfd_system *s, uint32_t nsteps) {
+  if (!valid(s)) return RFD_E_ARG;
+  rfd_ctx *ctx = s->ctx;
+  RFD_CHECK_ARG(ctx, s->have_bonds, "rfd_system_step: set bonds first");
+  for (uint32_t k = 0; k < nsteps; ++k) {
+    if (!s->rows_valid || s->step_count % s->p.rebuild_every == 0) {
+      int st = rebuild_rows(s);
+      if (st) return st;
+    }
+    int st = forces(s);
+    if (!st) st = pass_and_control(s);
+    if (st) return st;
+    s->step_count++;
+  }
+  return RFD_OK;
+}
+
+int rfd_system_get_history(rfd_system *s, double *hist, uint32_t cap, uint32_t *n) {
+  if (!valid(s)) return RFD_E_ARG;
+  rfd_ctx *ctx = s->ctx;
+  RFD_CHECK_ARG(ctx, n, "rfd_system_get_history: null count");
+  rfd_loading L;
+  int st = rfd_system_get_loading(s, &L);
+  if (st) return st;
+  *n = L.hist_len;
+  const uint32_t m = L.hist_len < cap ? L.hist_len : cap;
+  if (hist && m)
+    RFD_HIP(ctx, hipMemcpy(hist, s->hist, sizeof(double) * 3 * m, hipMemcpyDeviceToHost));
+  return RFD_OK;
+}
+
+int rfd_system_bond_counts(rfd_system *s, uint32_t out[3]) {
+  if (!valid(s)) return RFD_E_ARG;
+  rfd_ctx *ctx = s->ctx;
+  RFD_CHECK_ARG(ctx, out, "rfd_system_bond_counts: null output");
+  std::vector<uint8_t> f(s->Nb);
+  int st = rfd_system_get_bond_state(s, nullptr, f.data());
+  if (st) return st;
+  out[0] = out[1] = out[2] = 0;
+  for (uint8_t v : f) out[v == kBond ? 0 : (v == kBroken ? 1 : 2)]++;
+  return RFD_OK;
+}
+
+int rfd_system_sizes(rfd_system *s, uint32_t *N, uint32_t *Nb) {
+  if (!valid(s)) return RFD_E_ARG;
+  if (N) *N = s->N;
+  if (Nb) *Nb = s->Nb;
+  return RFD_OK;
+}
+
+int rfd_system_device_arrays(rfd_system *s, double **P, float **V, float **W, float **F,
+                             float **M) {
+  if (!valid(s)) return RFD_E_ARG;
+  if (P) *P = (double *)s->P;
+  if (V) *V = (float *)s->V;
+  if (W) *W = (float *)s->W;
+  if (F) *F = (float *)s->F;
+  if (M) *M = (float *)s->M;
+  return RFD_OK;
+}
+
+}  // extern "C"

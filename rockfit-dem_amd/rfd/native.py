@@ -1,0 +1,312 @@
+"""Native tier: an rfd_system (include/rfd.h, tier 2) driven from Python.
+
+The system owns SoA particle and edge-list bond storage in HBM and runs whole
+explicit steps on the device (forces, platens, integration, cut box, loading
+control), so a step needs no host synchronisation.  This module builds
+specimens into it, maps the reference slot layout onto it (for parity tests)
+and reads results back.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import RfdError, lib
+from .params import Compression
+
+_u32p = C.POINTER(C.c_uint32)
+
+
+class SystemParams(C.Structure):
+    _fields_ = [("E", C.c_double), ("nu", C.c_double), ("rho", C.c_double),
+                ("mu", C.c_double), ("mur", C.c_double), ("e", C.c_double),
+                ("dt", C.c_double), ("farcut", C.c_double), ("skin", C.c_double),
+                ("rebuild_every", C.c_uint32), ("contact_capacity", C.c_uint32),
+                ("refmode", C.c_int)]
+
+
+class Loading(C.Structure):
+    _fields_ = [("mode", C.c_int), ("cut", C.c_int),
+                ("cx", C.c_double), ("cy", C.c_double), ("cz", C.c_double),
+                ("RR", C.c_double), ("zcut", C.c_double),
+                ("top", C.c_double), ("bottom", C.c_double), ("top_r", C.c_double),
+                ("bottom_r", C.c_double), ("band", C.c_double),
+                ("v", C.c_double), ("v_load", C.c_double),
+                ("size", C.c_double), ("size0", C.c_double),
+                ("inv_area", C.c_double), ("min_stress", C.c_double),
+                ("zero_stress", C.c_double), ("max_strain", C.c_double),
+                ("stress", C.c_double), ("strain", C.c_double),
+                ("fracture_stress", C.c_double), ("fracture_strain", C.c_double),
+                ("sum_bottom", C.c_double), ("sum_top", C.c_double),
+                ("loading", C.c_int), ("fracture", C.c_int),
+                ("steps", C.c_uint32), ("hist_len", C.c_uint32),
+                ("box_c", C.c_double * 3), ("box_h", C.c_double * 3),
+                ("hidden", C.c_double * 3)]
+
+
+_vp, _u32, _dbl = C.c_void_p, C.c_uint32, C.c_double
+NATIVE_SIGNATURES = {
+    "rfd_system_create": [_vp, C.POINTER(SystemParams), _u32, C.POINTER(_vp)],
+    "rfd_system_destroy": [_vp],
+    "rfd_system_set_particles": [_vp, _vp, _vp],
+    "rfd_system_find_pairs": [_vp, _dbl, _vp, _u32, _u32p],
+    "rfd_system_set_bonds": [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp],
+    "rfd_system_set_bond_state": [_vp, _vp, _vp],
+    "rfd_system_get_bond_state": [_vp, _vp, _vp],
+    "rfd_system_set_velocities": [_vp, _vp, _vp],
+    "rfd_system_set_loading": [_vp, C.POINTER(Loading)],
+    "rfd_system_get_loading": [_vp, C.POINTER(Loading)],
+    "rfd_system_get_state": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "rfd_system_compute_forces": [_vp],
+    "rfd_system_step": [_vp, _u32],
+    "rfd_system_get_history": [_vp, _vp, _u32, _u32p],
+    "rfd_system_bond_counts": [_vp, _u32p],
+    "rfd_system_sizes": [_vp, _u32p, _u32p],
+    "rfd_system_device_arrays": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "rfd_system_timing": [_vp, C.c_int],
+    "rfd_system_timing_read": [_vp, C.POINTER(C.c_double), _u32p],
+}
+
+
+def _bind():
+    L = lib()
+    if not getattr(L, "_native_bound", False):
+        for name, args in NATIVE_SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        L._native_bound = True
+    return L
+
+
+def _np(a, dtype):
+    a = np.ascontiguousarray(a, dtype=dtype)
+    return a, a.ctypes.data_as(C.c_void_p)
+
+
+def morton_order(R: np.ndarray, cell: float) -> np.ndarray:
+    """Permutation sorting points by the Morton (Z-order) code of their cell,
+    so that particles close in space are close in memory."""
+    q = np.floor((R - R.min(0)) / cell).astype(np.uint64)
+    q = np.minimum(q, (1 << 21) - 1)
+
+    def spread(x):
+        x = x & np.uint64(0x1FFFFF)
+        x = (x | (x << np.uint64(32))) & np.uint64(0x1F00000000FFFF)
+        x = (x | (x << np.uint64(16))) & np.uint64(0x1F0000FF0000FF)
+        x = (x | (x << np.uint64(8))) & np.uint64(0x100F00F00F00F00F)
+        x = (x | (x << np.uint64(4))) & np.uint64(0x10C30C30C30C30C3)
+        x = (x | (x << np.uint64(2))) & np.uint64(0x1249249249249249)
+        return x
+
+    code = spread(q[:, 0]) | (spread(q[:, 1]) << np.uint64(1)) | (spread(q[:, 2]) << np.uint64(2))
+    return np.argsort(code, kind="stable")
+
+
+@dataclass
+class Bonds:
+    bi: np.ndarray
+    bj: np.ndarray
+    br: np.ndarray
+    sc: np.ndarray
+    tc: np.ndarray
+    L0: np.ndarray | None = None  # rest lengths (None: current distances)
+
+    @property
+    def n(self) -> int:
+        return int(self.bi.size)
+
+
+class NativeSystem:
+    """rfd_system handle.  Positions are kept in the order given (callers that
+    want spatial locality reorder first, see ``morton_order``)."""
+
+    def __init__(self, ctx, params: SystemParams, R: np.ndarray, radius: np.ndarray):
+        self.L = _bind()
+        self.ctx = ctx
+        self.params = params
+        R = np.asarray(R, np.float64)
+        self.N = N = R.shape[0]
+        h = C.c_void_p()
+        self._call("rfd_system_create", ctx.h, C.byref(params), N, C.byref(h), sys=False)
+        self.h = h
+        self.radius = np.ascontiguousarray(radius, np.float32)
+        soa, p = _np(R.T.reshape(-1), np.float64)
+        _, pr = _np(self.radius, np.float32)
+        self._call("rfd_system_set_particles", self.h, p, pr)
+        self.bonds: Bonds | None = None
+
+    def _call(self, name, *args, sys=True):
+        st = getattr(self.L, name)(*args)
+        if st != 0:
+            msg = self.L.rfd_last_error(self.ctx.h)
+            raise RfdError(name, st, msg.decode() if msg else "")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.rfd_system_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- setup ---------------------------------------------------------
+    def find_pairs(self, gap: float) -> np.ndarray:
+        cnt = C.c_uint32()
+        self._call("rfd_system_find_pairs", self.h, gap, None, 0, C.byref(cnt))
+        n = cnt.value
+        out = np.zeros(2 * max(n, 1), np.uint32)
+        self._call("rfd_system_find_pairs", self.h, gap, out.ctypes.data_as(C.c_void_p), n,
+                   C.byref(cnt))
+        return out[: 2 * n].reshape(-1, 2)
+
+    def set_bonds(self, b: Bonds):
+        order = np.lexsort((b.bj, b.bi))
+        L0 = None if b.L0 is None else np.ascontiguousarray(b.L0[order], np.float32)
+        b = Bonds(*(np.ascontiguousarray(x[order]) for x in (b.bi, b.bj, b.br, b.sc, b.tc)), L0)
+        arrs = [_np(b.bi, np.uint32), _np(b.bj, np.uint32), _np(b.br, np.float32),
+                _np(b.sc, np.float32), _np(b.tc, np.float32)]
+        pl = L0.ctypes.data_as(C.c_void_p) if L0 is not None else None
+        self._call("rfd_system_set_bonds", self.h, b.n, *[p for _, p in arrs], pl)
+        self.bonds = b
+
+    def set_bond_state(self, state: np.ndarray, flag: np.ndarray):
+        _, ps = _np(state, np.float32)
+        _, pf = _np(flag, np.uint8)
+        self._call("rfd_system_set_bond_state", self.h, ps, pf)
+
+    def bond_state(self):
+        nb = self.bonds.n
+        st = np.zeros(12 * nb, np.float32)
+        fl = np.zeros(nb, np.uint8)
+        self._call("rfd_system_get_bond_state", self.h, st.ctypes.data_as(C.c_void_p),
+                   fl.ctypes.data_as(C.c_void_p))
+        return st.reshape(nb, 12), fl
+
+    def set_velocities(self, V=None, W=None):
+        a = _np(V, np.float32) if V is not None else (None, None)
+        b = _np(W, np.float32) if W is not None else (None, None)
+        self._call("rfd_system_set_velocities", self.h, a[1], b[1])
+
+    def set_loading(self, L: Loading):
+        self._call("rfd_system_set_loading", self.h, C.byref(L))
+
+    def loading(self) -> Loading:
+        L = Loading()
+        self._call("rfd_system_get_loading", self.h, C.byref(L))
+        return L
+
+    # -- run -----------------------------------------------------------
+    def compute_forces(self):
+        self._call("rfd_system_compute_forces", self.h)
+
+    def step(self, n: int = 1):
+        self._call("rfd_system_step", self.h, n)
+
+    def state(self):
+        n3 = 3 * self.N
+        R = np.zeros(n3, np.float64)
+        V, W, F, M = (np.zeros(n3, np.float32) for _ in range(4))
+        self._call("rfd_system_get_state", self.h, *(a.ctypes.data_as(C.c_void_p)
+                                                      for a in (R, V, W, F, M)))
+        return {"R": R, "V": V, "W": W, "F": F, "M": M}
+
+    def history(self) -> np.ndarray:
+        n = C.c_uint32()
+        self._call("rfd_system_get_history", self.h, None, 0, C.byref(n))
+        h = np.zeros(3 * max(n.value, 1), np.float64)
+        self._call("rfd_system_get_history", self.h, h.ctypes.data_as(C.c_void_p), n.value,
+                   C.byref(n))
+        return h[: 3 * n.value].reshape(-1, 3)
+
+    def timing(self, enable: bool = True):
+        self._call("rfd_system_timing", self.h, int(enable))
+
+    def timing_read(self) -> tuple[float, int]:
+        ms, n = C.c_double(), C.c_uint32()
+        self._call("rfd_system_timing_read", self.h, C.byref(ms), C.byref(n))
+        return ms.value, n.value
+
+    def bond_counts(self) -> tuple[int, int, int]:
+        out = (C.c_uint32 * 3)()
+        self._call("rfd_system_bond_counts", self.h, out)
+        return tuple(out)
+
+
+# ------------------------------------------------------------- loading
+def loading_from_compression(cmp: Compression, zcut: float, box=None) -> Loading:
+    """Device loading control from the host compression_data mirror."""
+    L = Loading()
+    L.mode = 1 if cmp.mode == "uniaxial" else 2
+    L.cx, L.cy, L.cz = cmp.center_d
+    L.RR = cmp.RR
+    L.zcut = zcut
+    L.top, L.bottom, L.top_r, L.bottom_r = cmp.top, cmp.bottom, cmp.top_r, cmp.bottom_r
+    L.band = 2.0 * cmp.p_r
+    L.v, L.v_load = cmp.V, cmp.v_load
+    L.size = L.size0 = cmp.size0
+    L.inv_area = cmp.inv_area
+    L.min_stress, L.zero_stress, L.max_strain = cmp.min_stress, cmp.zero_stress, cmp.max_strain
+    if box is not None:
+        L.cut = 1
+        for d in range(3):
+            L.box_c[d] = box.center[d]
+            L.box_h[d] = box.spacesized2[d]
+            L.hidden[d] = box.hidden_point[d]
+    return L
+
+
+def params_from_material(mat, refmode: int = 1, skin: float = 0.05, rebuild_every: int = 20,
+                         contact_capacity: int = 16) -> SystemParams:
+    p = SystemParams()
+    p.E, p.nu, p.rho = mat.E, mat.nu, mat.rho
+    p.mu, p.mur, p.e = mat.p_mu, mat.p_mur, mat.p_e
+    p.dt = mat.dt
+    p.farcut = mat.a_farcut / (2.0 * mat.p_r)
+    p.skin = skin
+    p.rebuild_every = rebuild_every
+    p.contact_capacity = contact_capacity
+    p.refmode = refmode
+    return p
+
+
+# ----------------------------------------------- reference slot layout map
+def bonds_from_slots(h: dict, N: int, K: int, mat) -> tuple[Bonds, np.ndarray, np.ndarray]:
+    """Bond records (i<j) from the reference slot arrays: a pair is a bond
+    record when slot i->j has ILtype 1; its state is i's slot state
+    (rij -> n_prev, Oijt, Mijn, Mijt)."""
+    IL = h["IL"].reshape(N, K)
+    ty = h["ILtype"].reshape(N, K)
+    ii, kk = np.nonzero((IL != 0xFFFFFFFF) & (ty == 1))
+    jj = IL[ii, kk].astype(np.int64)
+    m = ii < jj
+    ii, kk, jj = ii[m], kk[m], jj[m]
+    s = ii * K + kk
+    st = np.concatenate([h[k].reshape(-1, 3)[s] for k in ("rij", "Oijt", "Mijn", "Mijt")], 1)
+    n = ii.size
+    # the bond law uses 1/iL as the rest length (md_calculateforcesdem3.cu:96)
+    L0 = (np.float32(1.0) / h["iL"][s].astype(np.float32)).astype(np.float32)
+    b = Bonds(ii.astype(np.uint32), jj.astype(np.uint32), np.full(n, mat.b_r, np.float32),
+              np.full(n, mat.sc, np.float32), np.full(n, mat.tc, np.float32), L0)
+    return b, st.astype(np.float32), np.ones(n, np.uint8)
+
+
+def strengths(g: np.random.Generator, n: int, mean_t=20e6, mean_s=40e6, cv=0.2):
+    """Bond strengths N(mu, cv*mu) truncated > 0, in stress units (1e9 Pa)."""
+    from .params import STRESS
+    from .specimen import truncated_normal
+
+    sc = truncated_normal(g, mean_t / STRESS, cv * mean_t / STRESS, n)
+    tc = truncated_normal(g, mean_s / STRESS, cv * mean_s / STRESS, n)
+    return sc.astype(np.float32), tc.astype(np.float32)
+
+
+def lattice_rest_gap(d_nn: float, r_lo: float) -> float:
+    """Gap below which every nearest lattice neighbour of an FCC lattice of
+    spacing d_nn is joined (radii >= r_lo)."""
+    return d_nn - 2.0 * r_lo + 1e-3 * d_nn
