@@ -39,7 +39,43 @@ def parse():
     p.add_argument("--H", type=float, default=200.0)
     p.add_argument("--cpu-seconds", type=float, default=10.0)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--copy-calib", action="store_true",
+                   help="only run a plain device copy of 1 GiB (counter calibration and "
+                        "stream-copy ceiling)")
     return p.parse_args()
+
+
+COPY_BYTES = 1 << 30
+
+
+def stream_copy(reps: int = 5) -> float:
+    """Plain device-to-device copy of 1 GiB; returns the copy rate in B/s
+    counting bytes read + written."""
+    import torch
+
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    b.record()
+    torch.cuda.synchronize()
+    return 2 * COPY_BYTES * reps / (a.elapsed_time(b) * 1e-3)
+
+
+def load_traffic(kernel: str, particles: int, bonds: int):
+    """HBM bytes per launch of `kernel` measured by the FETCH_SIZE / WRITE_SIZE
+    counter runs recorded under profiles/ for this workload, or None."""
+    f = ROOT / "profiles" / "traffic.json"
+    if not f.exists():
+        return None
+    for rec in json.loads(f.read_text()):
+        if rec["kernel"] == kernel and rec["particles"] == particles and rec["bonds"] == bonds:
+            return rec["bytes_per_launch"]
+    return None
 
 
 def cpu_baseline_ref(seconds: float) -> dict:
@@ -169,7 +205,15 @@ def main():
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU runs are not part of this build", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calib:
+        print(json.dumps({"copy_bytes_read": COPY_BYTES, "copy_bytes_written": COPY_BYTES,
+                          "stream_copy_GBps": stream_copy() / 1e9}))
+        return
     res = bench_ref(args) if args.tier == "ref" else bench_native(args)
+    res["roofline"]["traffic"] = load_traffic(res["roofline"]["kernel"],
+                                              res["config"]["particles"],
+                                              res["config"]["bonds"])
+    res["roofline"]["stream_copy_GBps"] = stream_copy() / 1e9
     out = {
         "metric": "bond-updates/s",
         "value": res["value"],

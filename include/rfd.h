@@ -262,6 +262,52 @@ int rfd_system_sizes(rfd_system *sys, uint32_t *N, uint32_t *Nb);
  * clears), then read the summed kernel time in ms and the launch count */
 int rfd_system_timing(rfd_system *sys, int enable);
 int rfd_system_timing_read(rfd_system *sys, double *kernel_ms, uint32_t *launches);
+/* ---- quasi-static load steps: axial strain increment + FIRE relaxation ----
+ * Standard FIRE (velocity mixing, adaptive dt, half-step back-off on uphill
+ * steps) over translations and rotations, run on the device; the reference's
+ * FIRE (md_firerelaxation.cu:18-84,263-304) is translation-only and packing
+ * only.  Platen particles are held by the displacement boundary condition of
+ * the explicit platens with zero platen speed. */
+typedef struct rfd_fire_params {
+  double ftol;          /* converged when max|F_i| and max|M_i|/r_i < ftol   */
+  double dt0, dt_max;   /* initial / maximum FIRE time step                   */
+  double alpha0, f_inc, f_dec, f_alpha;
+  uint32_t n_delay;     /* positive steps before dt may grow                  */
+  uint32_t max_iter;    /* iteration cap per load step                        */
+  uint32_t check_every; /* host convergence poll period (iterations)          */
+} rfd_fire_params;
+
+typedef struct rfd_fire_result {
+  uint32_t iterations;
+  int converged;
+  double fmax0, fmax;   /* residual max|F_i| before / after relaxation        */
+  double mmax;          /* final max|M_i|/r_i                                 */
+  double sum_bottom, sum_top; /* platen reactions after relaxation            */
+  double strain;        /* specimen strain after the step                     */
+  double seconds;       /* wall time of the load step                         */
+} rfd_fire_result;
+
+/* One quasi-static load step: move the top platen by dstrain * size0 (load
+ * axis of the loading mode; dstrain < 0 compresses), optionally apply a
+ * predicted increment (warm start: dU, dTh are device arrays of N x 3 floats,
+ * row-major, or NULL) as one kinematic step so the incremental spring laws
+ * see it, then relax with FIRE to ftol.  The displacement of the step is kept
+ * on the device (rfd_system_fire_buffers). */
+int rfd_system_fire_load_step(rfd_system *sys, double dstrain, const rfd_fire_params *fp,
+                              const float *dU, const float *dTh, rfd_fire_result *out);
+/* residual of the current configuration (one force pass + constraint):
+ * out[0] = max|F_i|, out[1] = max|M_i|/r_i over free DOFs */
+int rfd_system_residual(rfd_system *sys, double out[2]);
+/* device arrays (N x 3 floats, row-major) of the last two load-step
+ * displacements and rotations, newest first; valid after a load step */
+int rfd_system_fire_buffers(rfd_system *sys, float **du_last, float **du_prev,
+                            float **dth_last, float **dth_prev);
+/* half-bond rows on the device: hoff[N+1]; rec[h] = {j, L0, br, sc} (16 B);
+ * flag[h] (1 bond, 0 broken, 2 removed) -- the bond graph for callers that
+ * build per-bond features in HBM */
+int rfd_system_bond_graph(rfd_system *sys, const uint32_t **hoff, const void **rec,
+                          const uint8_t **flag, uint32_t *H);
+
 /* device pointers of the particle arrays (for callers that keep data in HBM):
  * P is N records of 4 doubles {x, y, z, radius}; V, W, F, M are N records of
  * 4 floats {x, y, z, 0} */

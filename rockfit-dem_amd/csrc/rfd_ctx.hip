@@ -64,13 +64,14 @@ int rfd_ctx_create(rfd_ctx **out, int device, void *stream) {
 
 int rfd_ctx_destroy(rfd_ctx *ctx) {
   if (!ctx) return RFD_OK;
-  hipSetDevice(ctx->device);
-  if (ctx->stream) hipStreamSynchronize(ctx->stream);
-  else hipDeviceSynchronize();
-  if (ctx->d_flags) hipFree(ctx->d_flags);
-  if (ctx->d_red) hipFree(ctx->d_red);
-  if (ctx->h_red) hipHostFree(ctx->h_red);
-  if (ctx->scratch) hipFree(ctx->scratch);
+  // teardown: errors here have nowhere to go, the context is released anyway
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  else (void)hipDeviceSynchronize();
+  if (ctx->d_flags) (void)hipFree(ctx->d_flags);
+  if (ctx->d_red) (void)hipFree(ctx->d_red);
+  if (ctx->h_red) (void)hipHostFree(ctx->h_red);
+  if (ctx->scratch) (void)hipFree(ctx->scratch);
   delete ctx;
   return RFD_OK;
 }

@@ -1,26 +1,27 @@
 // rfd_native.hip -- native tier: AoS particle records (FP64 position +
-// radius), undirected edge-list bonds with owner/partner CSR, slot-major
-// contact rows, fused platen + integration pass, device-side loading control.
+// radius), bonds stored as two mirrored half-bonds in per-particle rows,
+// slot-major contact rows, fused platen + integration pass, device-side
+// loading control.
 //
 // Memory layout (all in HBM, sized for one GPU's share of a specimen):
 //   particle  P4 {x, y, z, r} (double, 32 B), V4 / W4 / F4 / M4 (float4),
 //             inv2 {dt/m, dt/I} (float2)
-//   bond      bij {i, j} (uint2), bp {L0, br, sc, tc} (float4),
+//   half-bond rows hoff[N+1]; per half h: HB {j, L0, br, sc} (16 B), tc[h],
 //             state {n_prev, O, Mn, Mt} (3 x float4 = 48 B, double-buffered),
 //             flag (uint8, double-buffered: 1 bond, 0 broken, 2 removed)
-//   owner CSR boff[N+1] over bonds sorted by (i, j); partner CSR ioff[N+1],
-//             iq[] = bond ids grouped by j
 //   contacts  slot-major rows cj[k*N+i], cO[k*N+i] (float4), ccnt[i]
-// A particle record or a bond record is one contiguous segment, so a gather
-// of a neighbour costs one or two cache lines.
+// A bond (i, j) is held twice: in row i from i's side and in row j from j's
+// side, as the reference holds one directed slot per end.  The two halves
+// evaluate exact mirror images (every operation of the law is odd or even
+// under i<->j and IEEE rounding is sign-symmetric), so their loads are exact
+// opposites and they break together; each half reads and writes only its own
+// contiguous record, so the force pass streams the bond arrays once with
+// coalesced loads and gathers only neighbour particle records.
 //
-// Force pass: one workgroup per 64 particles.  Its bond ends (owner ends, a
-// contiguous range of bonds, then partner ends) are evaluated one per lane in
-// chunks of 512; each lane writes its end's load to LDS, then each particle's
-// thread adds its ends in a fixed order (owner ends, then partner ends).  The
-// owner lane writes the new bond state; the partner lane evaluates the same
-// bond from the same old state (bit-identical) and writes nothing.  No
-// atomics: the pass is deterministic.
+// Force pass: one workgroup per 64 particles; their half-bonds form one
+// contiguous range, evaluated one per lane in chunks of 512; each lane writes
+// its load to LDS and each particle's thread then adds its own halves in row
+// order.  No atomics: the pass is deterministic.
 //
 // Bond law: md_calculateforcesdem3.cu:38-197 with per-bond radius; stretch in
 // FP64 from FP64 positions, the rest in FP32, break criterion (:20-36) in
@@ -30,6 +31,8 @@
 // records act through the Hertz-Mindlin law of :199-336 with R* = ri rj /
 // (ri + rj) and m* = mi mj / (mi + mj), which is the reference's law for
 // equal radii.
+#include <chrono>
+#include <cstddef>
 #include <vector>
 
 #include "rfd_internal.h"
@@ -43,12 +46,23 @@ constexpr int kFT = 256;   // threads per force workgroup
 constexpr int kCH = 512;   // bond ends per LDS chunk
 constexpr uint8_t kBond = 1, kBroken = 0, kRemoved = 2;
 constexpr int kHistCap = 1 << 20;
+// gfx950 places workgroup b on XCD b % 8; tiles are remapped so that each XCD
+// (own L2) works through one contiguous run of Morton-ordered particle tiles,
+// which keeps the neighbour gathers of adjacent tiles in the same L2
+constexpr uint32_t kXcd = 8;
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t per) {
+  return (blockIdx.x % kXcd) * per + blockIdx.x / kXcd;
+}
 
 struct alignas(32) P4 {
   double x, y, z, r;
 };
 struct alignas(16) BS {
   float4 a, b, c;  // (npx,npy,npz,Ox) (Oy,Oz,Mnx,Mny) (Mnz,Mtx,Mty,Mtz)
+};
+struct alignas(16) HB {
+  uint32_t j;  // the other end
+  float L0, br, sc;
 };
 
 struct v3 {
@@ -71,6 +85,7 @@ __device__ constexpr v3 kZero = {0.f, 0.f, 0.f};
 
 struct Mat {
   float E, G, Estar, Gstar, pA, mu, mur, rho43pi, dt;
+  double farcut;  // far-link distance as a multiple of ri + rj
   int refmode;
 };
 
@@ -151,21 +166,27 @@ struct BondOut {
 
 // evaluate bond record (i, j) from its old state
 __device__ __forceinline__ void eval_bond(const Mat &mt, const EndData &A, const EndData &B,
-                                          float4 bp, const BS &old, uint8_t flag,
-                                          BondOut &o) {
+                                          float L0, float br, float scrit, float tcrit,
+                                          const BS &old, uint8_t flag, BondOut &o) {
   const double rx = B.P.x - A.P.x, ry = B.P.y - A.P.y, rz = B.P.z - A.P.z;
   const double L64 = __builtin_sqrt(rx * rx + ry * ry + rz * rz);
   o.Fi = o.Mbi = o.Mbj = o.Mci = o.Mcj = kZero;
   o.hasc = false;
   o.st = old;
   o.flag = flag;
-  if (!(L64 > 1e-5)) return;  // coincident ends (both parked): no load
-  const double inv = 1.0 / L64;
-  const v3 n = {(float)(rx * inv), (float)(ry * inv), (float)(rz * inv)};
+  // far-link removal (d_DeleteFarLinks, md_interaction_list_reconstructbpm.cu:
+  // 141-186): a record longer than farcut*(ri+rj), or of coincident ends, is
+  // removed; both halves see the same distance and go together
+  const double lim = mt.farcut * (A.P.r + B.P.r);
+  if (L64 > lim || !(L64 > 1e-5)) {
+    o.flag = kRemoved;
+    return;
+  }
   const v3 r = {(float)rx, (float)ry, (float)rz};
   const float L = (float)L64;
+  const float invL = 1.0f / L;
+  const v3 n = r * invL;
   const float ri = (float)A.P.r, rj = (float)B.P.r;
-  const float L0 = bp.x, br = bp.y;
   v3 O = {old.a.w, old.b.x, old.b.y};
   bool contact_now = flag == kBroken;
   if (flag == kBond) {
@@ -174,7 +195,9 @@ __device__ __forceinline__ void eval_bond(const Mat &mt, const EndData &A, const
     const v3 Mt0 = {old.c.y, old.c.z, old.c.w};
     const float il = 1.0f / L0;
     const float A2 = kPiF * br * br, J = kPiF * br * br * br * br;
-    const double eps = (L64 - (double)L0) / (double)L0;
+    // the stretch is formed in FP64 (no cancellation); 1/L0 carries FP32
+    // relative precision, which is what the strain needs
+    const double eps = (L64 - (double)L0) * (double)il;
     const v3 Fn = n * ((float)(mt.E * eps) * A2);
     const v3 ws = A.W + B.W;
     Rot M;
@@ -194,9 +217,19 @@ __device__ __forceinline__ void eval_bond(const Mat &mt, const EndData &A, const
     o.Fi = Fn + Ft;
     o.Mbi = Mn * J + Mt * mtf + half;
     o.Mbj = half - Mn * J - Mt * mtf;
-    const double Sc = (double)mt.E * eps + dnorm(Mt) * 4.0 * (double)br;
-    const double Mc = (double)mt.G / (double)L0 * dnorm(O) + dnorm(Mn) * 2.0 * (double)br;
-    if (Sc > (double)bp.z || Mc > (double)bp.w) {
+    // break criterion (dd_CheckBreakConditionDEM_2, :20-36): evaluated in FP32,
+    // and again in FP64 when FP32 lands within 1e-4 of a threshold, so the
+    // decision at the threshold is the FP64 one
+    const float sig32 = (float)(mt.E * eps);
+    const float Sc32 = sig32 + sqrtf(dot(Mt, Mt)) * 4.0f * br;
+    const float Mc32 = mt.G * il * sqrtf(dot(O, O)) + sqrtf(dot(Mn, Mn)) * 2.0f * br;
+    bool broken = Sc32 > scrit || Mc32 > tcrit;
+    if (fabsf(Sc32 - scrit) <= 1e-4f * scrit || fabsf(Mc32 - tcrit) <= 1e-4f * tcrit) {
+      const double Sc = (double)mt.E * eps + dnorm(Mt) * 4.0 * (double)br;
+      const double Mc = (double)mt.G / (double)L0 * dnorm(O) + dnorm(Mn) * 2.0 * (double)br;
+      broken = Sc > (double)scrit || Mc > (double)tcrit;
+    }
+    if (broken) {
       o.flag = kBroken;
       O = kZero;
       o.st.a = make_float4(n.x, n.y, n.z, 0.f);
@@ -230,15 +263,14 @@ struct SysDev {
   const P4 *P;
   const float4 *V, *W;
   float4 *F, *M;
-  const uint32_t *boff, *ioff, *iq;
-  const uint2 *bij;
-  const float4 *bp;
-  const BS *st_old;
-  BS *st_new;
-  const uint8_t *fl_old;
-  uint8_t *fl_new;
+  const uint32_t *hoff;
+  const HB *hb;
+  const float *tc;
+  BS *st;       // updated in place: each half has one reader and writer
+  uint8_t *fl;
   const uint32_t *cj, *ccnt;
   float4 *cO;
+  const float *dtp;  // when set, the time step of the last move (FIRE) replaces mt.dt
 };
 
 __device__ __forceinline__ EndData load_end(const SysDev &s, uint32_t i) {
@@ -249,81 +281,91 @@ __device__ __forceinline__ EndData load_end(const SysDev &s, uint32_t i) {
   return e;
 }
 
-__global__ void __launch_bounds__(kFT) k_nat_forces(SysDev s, Mat mt) {
+__global__ void __launch_bounds__(kFT) k_nat_forces(SysDev s, Mat mt, uint32_t ntiles,
+                                                     uint32_t per) {
   __shared__ float sF[3][kCH], sMb[3][kCH], sMc[3][kCH];
   __shared__ uint8_t sC[kCH];
+  __shared__ uint32_t sOff[kFP + 1];
+  __shared__ float sAcc[3][3][kFP];  // F, Mb, Mc running sums per particle
+  __shared__ uint8_t sAnyc[kFP];
+  const uint32_t tile = xcd_tile(per);
+  if (tile >= ntiles) return;  // padding tiles of the last XCD run
+  if (s.dtp) mt.dt = *s.dtp;
   const uint32_t tid = threadIdx.x;
-  const uint32_t p0 = blockIdx.x * kFP;
+  const uint32_t p0 = tile * kFP;
   const uint32_t np = min((uint32_t)kFP, s.N - p0);
-  const uint32_t ob0 = s.boff[p0], ob1 = s.boff[p0 + np];
-  const uint32_t ib0 = s.ioff[p0], ib1 = s.ioff[p0 + np];
-  const uint32_t n_own = ob1 - ob0, n_end = n_own + (ib1 - ib0);
-  // per-particle accumulators live in the first np threads
+  if (tid <= np) sOff[tid] = s.hoff[p0 + tid];
+  for (uint32_t q = tid; q < 9 * kFP; q += kFT) (&sAcc[0][0][0])[q] = 0.f;
+  if (tid < kFP) sAnyc[tid] = 0;
+  __syncthreads();
+  const uint32_t h0 = sOff[0], n_end = sOff[np] - h0;
   v3 F = kZero, Mb = kZero, Mc = kZero;
   bool anyc = false;
-  uint32_t o0 = 0, o1 = 0, q0 = 0, q1 = 0;
-  if (tid < np) {
-    const uint32_t p = p0 + tid;
-    o0 = s.boff[p] - ob0;
-    o1 = s.boff[p + 1] - ob0;
-    q0 = n_own + s.ioff[p] - ib0;
-    q1 = n_own + s.ioff[p + 1] - ib0;
-  }
   for (uint32_t c0 = 0; c0 < n_end; c0 += kCH) {
     const uint32_t c1 = min(c0 + (uint32_t)kCH, n_end);
     for (uint32_t e = c0 + tid; e < c1; e += kFT) {
-      const bool owner = e < n_own;
-      const uint32_t b = owner ? ob0 + e : s.iq[ib0 + (e - n_own)];
-      const uint8_t fl = s.fl_old[b];
+      const uint32_t h = h0 + e;
+      // row particle of half h: last row whose start is <= h
+      uint32_t lo = 0, hi = np;
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (sOff[mid] <= h) lo = mid; else hi = mid;
+      }
+      const uint8_t fl = s.fl[h];
       BondOut o;
       if (fl == kRemoved) {
-        o.Fi = o.Mbi = o.Mbj = o.Mci = o.Mcj = kZero;
+        o.Fi = o.Mbi = o.Mci = kZero;
         o.hasc = false;
-        if (owner) s.fl_new[b] = kRemoved;
       } else {
-        const uint2 ij = s.bij[b];
-        eval_bond(mt, load_end(s, ij.x), load_end(s, ij.y), s.bp[b], s.st_old[b], fl, o);
-        if (owner) {
-          s.st_new[b] = o.st;
-          s.fl_new[b] = o.flag;
-        }
+        const HB hb = s.hb[h];
+        eval_bond(mt, load_end(s, p0 + lo), load_end(s, hb.j), hb.L0, hb.br, hb.sc, s.tc[h],
+                  s.st[h], fl, o);
+        if (o.flag != kRemoved) s.st[h] = o.st;
+        if (o.flag != fl) s.fl[h] = o.flag;
       }
       const uint32_t k = e - c0;
-      const v3 f = owner ? o.Fi : kZero - o.Fi;
-      const v3 mb = owner ? o.Mbi : o.Mbj;
-      const v3 mc = owner ? o.Mci : o.Mcj;
-      sF[0][k] = f.x;
-      sF[1][k] = f.y;
-      sF[2][k] = f.z;
-      sMb[0][k] = mb.x;
-      sMb[1][k] = mb.y;
-      sMb[2][k] = mb.z;
-      sMc[0][k] = mc.x;
-      sMc[1][k] = mc.y;
-      sMc[2][k] = mc.z;
+      sF[0][k] = o.Fi.x;
+      sF[1][k] = o.Fi.y;
+      sF[2][k] = o.Fi.z;
+      sMb[0][k] = o.Mbi.x;
+      sMb[1][k] = o.Mbi.y;
+      sMb[2][k] = o.Mbi.z;
+      sMc[0][k] = o.Mci.x;
+      sMc[1][k] = o.Mci.y;
+      sMc[2][k] = o.Mci.z;
       sC[k] = o.hasc ? 1 : 0;
     }
     __syncthreads();
-    if (tid < np) {
-      // owner ends of this particle inside the chunk, then its partner ends
-      const uint32_t a0 = max(o0, c0), a1 = min(o1, c1);
-      for (uint32_t e = a0; e < a1; ++e) {
-        const uint32_t k = e - c0;
-        F = F + v3{sF[0][k], sF[1][k], sF[2][k]};
-        Mb = Mb + v3{sMb[0][k], sMb[1][k], sMb[2][k]};
-        Mc = Mc + v3{sMc[0][k], sMc[1][k], sMc[2][k]};
-        anyc = anyc || sC[k];
-      }
-      const uint32_t b0 = max(q0, c0), b1 = min(q1, c1);
-      for (uint32_t e = b0; e < b1; ++e) {
-        const uint32_t k = e - c0;
-        F = F + v3{sF[0][k], sF[1][k], sF[2][k]};
-        Mb = Mb + v3{sMb[0][k], sMb[1][k], sMb[2][k]};
-        Mc = Mc + v3{sMc[0][k], sMc[1][k], sMc[2][k]};
-        anyc = anyc || sC[k];
+    // per-particle sums of this chunk: task (component group g, particle p),
+    // each adds its particle's entries in row order
+    for (uint32_t task = tid; task < 4 * np; task += kFT) {
+      const uint32_t p = task % np, g = task / np;
+      const uint32_t a0 = max(sOff[p] - h0, c0), a1 = min(sOff[p + 1] - h0, c1);
+      if (g < 3) {
+        float (*src)[kCH] = g == 0 ? sF : (g == 1 ? sMb : sMc);
+        float x = 0.f, y = 0.f, z = 0.f;
+        for (uint32_t e = a0; e < a1; ++e) {
+          const uint32_t k = e - c0;
+          x = x + src[0][k];
+          y = y + src[1][k];
+          z = z + src[2][k];
+        }
+        sAcc[g][0][p] += x;
+        sAcc[g][1][p] += y;
+        sAcc[g][2][p] += z;
+      } else {
+        uint8_t c = 0;
+        for (uint32_t e = a0; e < a1; ++e) c |= sC[e - c0];
+        sAnyc[p] |= c;
       }
     }
     __syncthreads();
+  }
+  if (tid < np) {
+    F = v3{sAcc[0][0][tid], sAcc[0][1][tid], sAcc[0][2][tid]};
+    Mb = v3{sAcc[1][0][tid], sAcc[1][1][tid], sAcc[1][2][tid]};
+    Mc = v3{sAcc[2][0][tid], sAcc[2][1][tid], sAcc[2][2][tid]};
+    anyc = sAnyc[tid] != 0;
   }
   if (tid >= np) return;
   // never-bonded contact rows of this particle (per-end spring, in place)
@@ -485,18 +527,33 @@ __global__ void __launch_bounds__(256) k_nat_pass(PassDev p) {
   }
 }
 
-// loading control, one thread: SumForcesULoading (md_results.cu:34-74) on the
-// platen sums, then CompressU / CompressB (md_data_types.h:223-244, 272-293)
-__global__ void k_nat_control(rfd_loading *ctl, const double *part, int nblocks, double *hist,
-                              double dt) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// loading control: the platen partial sums are reduced by one workgroup in a
+// fixed order, then one thread applies SumForcesULoading (md_results.cu:34-74)
+// and CompressU / CompressB (md_data_types.h:223-244, 272-293)
+__global__ void __launch_bounds__(256) k_nat_control(rfd_loading *ctl, const double *part,
+                                                     int nblocks, double *hist, double dt) {
+  __shared__ double s_part[8];
+  double a = 0, c = 0;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+    a += part[2 * b];
+    c += part[2 * b + 1];
+  }
+  a = wave_sum(a);
+  c = wave_sum(c);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    s_part[2 * wid] = a;
+    s_part[2 * wid + 1] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
   rfd_loading L = *ctl;
   L.steps += 1;
   if (L.mode) {
     double s0 = 0, s1 = 0;
-    for (int b = 0; b < nblocks; ++b) {
-      s0 += part[2 * b];
-      s1 += part[2 * b + 1];
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+      s0 += s_part[2 * w];
+      s1 += s_part[2 * w + 1];
     }
     L.sum_bottom = s0;
     L.sum_top = s1;
@@ -595,6 +652,62 @@ k_nat_scan(const uint32_t *__restrict__ cnt, uint32_t C, uint32_t *__restrict__ 
   if (tid == 0) start[C] = s_carry;
 }
 
+// three-phase exclusive scan for long arrays: per-chunk totals, a scan of the
+// totals (k_nat_scan, one workgroup), then per-chunk scans with their offsets
+constexpr uint32_t kScanChunk = 1024 * 8;
+
+__global__ void __launch_bounds__(1024)
+k_scan_totals(const uint32_t *__restrict__ in, uint32_t n, uint32_t *__restrict__ tot) {
+  __shared__ uint32_t s_w[16];
+  const uint32_t base = blockIdx.x * kScanChunk, tid = threadIdx.x;
+  uint32_t v = 0;
+  for (uint32_t k = 0; k < 8; ++k) {
+    const uint32_t i = base + tid * 8 + k;
+    v += i < n ? in[i] : 0u;
+  }
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((tid & 63) == 0) s_w[tid >> 6] = v;
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t a = 0;
+    for (int w = 0; w < 16; ++w) a += s_w[w];
+    tot[blockIdx.x] = a;
+  }
+}
+
+__global__ void __launch_bounds__(1024)
+k_scan_apply(const uint32_t *__restrict__ in, uint32_t n, const uint32_t *__restrict__ off,
+             uint32_t nchunks, uint32_t *__restrict__ start, uint32_t *__restrict__ cursor) {
+  __shared__ uint32_t s_sum[16];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const uint32_t b0 = blockIdx.x * kScanChunk + tid * 8;
+  uint32_t v[8], local = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    v[k] = (b0 + k < n) ? in[b0 + k] : 0u;
+    local += v[k];
+  }
+  uint32_t inc = local;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += t;
+  }
+  if (lane == 63) s_sum[wid] = inc;
+  __syncthreads();
+  uint32_t wo = 0;
+  for (int w = 0; w < wid; ++w) wo += s_sum[w];
+  uint32_t run = off[blockIdx.x] + wo + inc - local;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if (b0 + k < n) {
+      start[b0 + k] = run;
+      if (cursor) cursor[b0 + k] = run;
+    }
+    run += v[k];
+  }
+  if (blockIdx.x == nchunks - 1 && tid == 0) start[n] = off[nchunks];
+}
+
 __global__ void k_nat_scatter(const uint32_t *__restrict__ CI, uint32_t N,
                               uint32_t *__restrict__ cursor, uint32_t *__restrict__ tmp) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride())
@@ -616,12 +729,12 @@ __global__ void k_nat_rank(const uint32_t *__restrict__ tmp, const uint32_t *__r
   }
 }
 
-// is there a bond record (not removed) between a < b ?
-__device__ __forceinline__ bool bonded(const uint32_t *__restrict__ boff,
-                                       const uint2 *__restrict__ bij,
+// is there a bond record (not removed) between a and b ?  (row a holds b)
+__device__ __forceinline__ bool bonded(const uint32_t *__restrict__ hoff,
+                                       const HB *__restrict__ hb,
                                        const uint8_t *__restrict__ fl, uint32_t a, uint32_t b) {
-  for (uint32_t q = boff[a]; q < boff[a + 1]; ++q)
-    if (bij[q].y == b) return fl[q] != kRemoved;
+  for (uint32_t q = hoff[a]; q < hoff[a + 1]; ++q)
+    if (hb[q].j == b) return fl[q] != kRemoved;
   return false;
 }
 
@@ -635,8 +748,8 @@ struct NbrDev {
   const uint32_t *pair_off;
   uint32_t *pairs;
   uint32_t Kc;
-  const uint32_t *boff;
-  const uint2 *bij;
+  const uint32_t *hoff;
+  const HB *hb;
   const uint8_t *fl;
   const uint32_t *cj_old, *ccnt_old;
   const float4 *cO_old;
@@ -687,9 +800,7 @@ __global__ void __launch_bounds__(128) k_nat_neighbours(NbrDev s) {
             s.pairs[2 * (size_t)out + 1] = j;
             ++out;
           } else {
-            const bool bd = i < j ? bonded(s.boff, s.bij, s.fl, i, j)
-                                  : bonded(s.boff, s.bij, s.fl, j, i);
-            if (bd) continue;
+            if (bonded(s.hoff, s.hb, s.fl, i, j)) continue;
             if (nfound < s.Kc) {
               const size_t sl = (size_t)nfound * N + i;
               s.cj_new[sl] = j;
@@ -716,31 +827,16 @@ __global__ void __launch_bounds__(128) k_nat_neighbours(NbrDev s) {
   if (MODE == 2) s.ccnt_new[i] = nfound < s.Kc ? nfound : s.Kc;
 }
 
-// far-link pruning of bond records (d_DeleteFarLinks,
-// md_interaction_list_reconstructbpm.cu:141-186): a record longer than
-// farcut*(ri+rj), or of coincident ends, is removed
-__global__ void k_nat_prune(const P4 *__restrict__ P, uint32_t N, const uint32_t *__restrict__ boff,
-                            const uint2 *__restrict__ bij, uint8_t *__restrict__ fl,
-                            double farcut) {
+// rest length = current distance (both halves compute the same value)
+__global__ void k_set_rest_length(const P4 *__restrict__ P, uint32_t N,
+                                  const uint32_t *__restrict__ hoff, HB *__restrict__ hb) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
-    const P4 Ri = P[i];
-    for (uint32_t b = boff[i]; b < boff[i + 1]; ++b) {
-      if (fl[b] == kRemoved) continue;
-      const P4 Rj = P[bij[b].y];
-      const double rx = Rj.x - Ri.x, ry = Rj.y - Ri.y, rz = Rj.z - Ri.z;
-      const double lim = farcut * (Ri.r + Rj.r);
-      const double d2 = rx * rx + ry * ry + rz * rz;
-      if (d2 > lim * lim || d2 < 1e-10) fl[b] = kRemoved;
+    const P4 A = P[i];
+    for (uint32_t h = hoff[i]; h < hoff[i + 1]; ++h) {
+      const P4 B = P[hb[h].j];
+      const double rx = B.x - A.x, ry = B.y - A.y, rz = B.z - A.z;
+      hb[h].L0 = (float)__builtin_sqrt(rx * rx + ry * ry + rz * rz);
     }
-  }
-}
-
-__global__ void k_set_rest_length(const P4 *__restrict__ P, uint32_t Nb,
-                                  const uint2 *__restrict__ bij, float4 *__restrict__ bp) {
-  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < Nb; b += grid_stride()) {
-    const P4 A = P[bij[b].x], B = P[bij[b].y];
-    const double rx = B.x - A.x, ry = B.y - A.y, rz = B.z - A.z;
-    bp[b].x = (float)__builtin_sqrt(rx * rx + ry * ry + rz * rz);
   }
 }
 
@@ -752,6 +848,262 @@ __global__ void k_fill_u32n(uint32_t *p, size_t n, uint32_t v) {
 
 }  // namespace
 
+// ------------------------------------------------------------------- FIRE
+// Standard FIRE (Bitzek et al. 2006, with the half-step back-off of FIRE 2.0)
+// over translations and rotations.  The reference's variant
+// (md_firerelaxation.cu:18-84, 263-304) recomputes V from F alone and runs on
+// translations of unbonded packings only.
+struct FireDev {
+  double dt, dt_last, alpha;
+  double cV, cW;  // mixing coefficients alpha*|V|/|F|, alpha*|W|/|M|
+  double fmax, mmax, fmax0, sum_bottom, sum_top;
+  double dt_max, dt_min, alpha0, f_inc, f_dec, f_alpha, ftol;
+  uint32_t n_pos, iter, max_iter, n_delay;
+  int done, converged, backoff, first, measure_only;
+  float dt_f;  // time step of the last move, read by the force pass
+};
+
+constexpr int kFireVals = 9;  // P, |F|^2, |V|^2, |M|^2, |W|^2, max|F|^2, max(|M|/r)^2, sb, st
+
+struct FireParticles {
+  uint32_t N;
+  P4 *P;
+  float4 *V, *W, *F, *Th;
+  const float4 *M;
+  const float2 *inv;
+  float dt_phys;
+  rfd_loading *ctl;
+  FireDev *fire;
+  double *part;
+};
+
+// platen constraint with the platens at rest, reaction sums, power and norms
+template <int MODE>
+__global__ void __launch_bounds__(256) k_fire_measure(FireParticles p) {
+  __shared__ double s_part[4 * kFireVals];
+  if (p.fire->done) return;
+  rfd_loading L = *p.ctl;
+  L.v = 0.0;
+  double v[kFireVals] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.N; i += grid_stride()) {
+    P4 R = p.P[i];
+    v3 V = xyz(p.V[i]), F = xyz(p.F[i]);
+    if (MODE) {
+      const v3 V0 = V, F0 = F;
+      const double z0 = R.z, y0 = R.y;
+      platen<MODE>(L, R, V, F, v[7], v[8]);
+      if (R.z != z0 || R.y != y0) p.P[i] = R;
+      if (V.x != V0.x || V.y != V0.y || V.z != V0.z) p.V[i] = f4(V);
+      if (F.x != F0.x || F.y != F0.y || F.z != F0.z) p.F[i] = f4(F);
+    }
+    const v3 W = xyz(p.W[i]), M = xyz(p.M[i]);
+    v[0] += (double)dot(F, V) + (double)dot(M, W);
+    const double f2 = dot(F, F), m2 = dot(M, M);
+    v[1] += f2;
+    v[2] += (double)dot(V, V);
+    v[3] += m2;
+    v[4] += (double)dot(W, W);
+    v[5] = fmax(v[5], f2);
+    v[6] = fmax(v[6], m2 / (R.r * R.r));
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < kFireVals; ++q) v[q] = (q == 5 || q == 6) ? wave_max(v[q]) : wave_sum(v[q]);
+  if (lane == 0)
+    for (int q = 0; q < kFireVals; ++q) s_part[wid * kFireVals + q] = v[q];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < kFireVals; ++q) {
+      double a = 0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
+        a = (q == 5 || q == 6) ? fmax(a, s_part[w * kFireVals + q]) : a + s_part[w * kFireVals + q];
+      p.part[blockIdx.x * kFireVals + q] = a;
+    }
+  }
+}
+
+// reduce the partials in a fixed order, then one thread runs the FIRE logic
+__global__ void __launch_bounds__(256) k_fire_control(FireDev *fire, const double *part,
+                                                      int nblocks, rfd_loading *ctl) {
+  __shared__ double s_part[4 * kFireVals];
+  if (fire->done) return;
+  double v[kFireVals] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+    for (int q = 0; q < kFireVals; ++q) {
+      const double x = part[b * kFireVals + q];
+      v[q] = (q == 5 || q == 6) ? fmax(v[q], x) : v[q] + x;
+    }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int q = 0; q < kFireVals; ++q) v[q] = (q == 5 || q == 6) ? wave_max(v[q]) : wave_sum(v[q]);
+  if (lane == 0)
+    for (int q = 0; q < kFireVals; ++q) s_part[wid * kFireVals + q] = v[q];
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int q = 0; q < kFireVals; ++q) {
+    double a = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
+      a = (q == 5 || q == 6) ? fmax(a, s_part[w * kFireVals + q]) : a + s_part[w * kFireVals + q];
+    v[q] = a;
+  }
+  FireDev f = *fire;
+  f.fmax = __builtin_sqrt(v[5]);
+  f.mmax = __builtin_sqrt(v[6]);
+  f.sum_bottom = v[7];
+  f.sum_top = v[8];
+  if (f.measure_only) {
+    f.done = 1;
+    *fire = f;
+    return;
+  }
+  if (f.first) {
+    f.fmax0 = f.fmax;
+    f.first = 0;
+  }
+  f.converged = f.fmax < f.ftol && f.mmax < f.ftol;
+  if (f.converged || f.iter >= f.max_iter) {
+    f.done = 1;
+    ctl->sum_bottom = v[7];
+    ctl->sum_top = v[8];
+    *fire = f;
+    return;
+  }
+  f.iter += 1;
+  f.dt_last = f.dt;
+  if (v[0] > 0.0) {
+    f.backoff = 0;
+    f.cV = v[1] > 0 ? f.alpha * __builtin_sqrt(v[2] / v[1]) : 0.0;
+    f.cW = v[3] > 0 ? f.alpha * __builtin_sqrt(v[4] / v[3]) : 0.0;
+    f.n_pos += 1;
+    if (f.n_pos > f.n_delay) {
+      f.dt = fmin(f.dt * f.f_inc, f.dt_max);
+      f.alpha *= f.f_alpha;
+    }
+  } else {
+    f.backoff = 1;
+    f.n_pos = 0;
+    f.dt = fmax(f.dt * f.f_dec, f.dt_min);
+    f.alpha = f.alpha0;
+  }
+  f.dt_f = (float)f.dt;
+  *fire = f;
+}
+
+// velocity mixing (or back-off), semi-implicit Euler kick and move
+__global__ void __launch_bounds__(256) k_fire_update(FireParticles p) {
+  const FireDev f = *p.fire;
+  if (f.done) return;
+  const float a1 = (float)(1.0 - f.alpha), cV = (float)f.cV, cW = (float)f.cW;
+  const float scale = (float)(f.dt / p.dt_phys);  // inv2 holds dt_phys/m, dt_phys/I
+  const float dtf = (float)f.dt;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.N; i += grid_stride()) {
+    P4 R = p.P[i];
+    v3 V = xyz(p.V[i]), W = xyz(p.W[i]);
+    const v3 F = xyz(p.F[i]), M = xyz(p.M[i]);
+    if (f.backoff) {
+      const double h = 0.5 * f.dt_last;
+      R.x -= (double)V.x * h;
+      R.y -= (double)V.y * h;
+      R.z -= (double)V.z * h;
+      V = kZero;
+      W = kZero;
+    } else {
+      V = V * a1 + F * cV;
+      W = W * a1 + M * cW;
+    }
+    const float2 iv = p.inv[i];
+    V = V + F * (iv.x * scale);
+    W = W + M * (iv.y * scale);
+    R.x += (double)V.x * f.dt;
+    R.y += (double)V.y * f.dt;
+    R.z += (double)V.z * f.dt;
+    p.P[i] = R;
+    p.V[i] = f4(V);
+    p.W[i] = f4(W);
+    p.Th[i] = f4(xyz(p.Th[i]) + W * dtf);
+  }
+}
+
+// displacement boundary condition of the moved platens, applied to the
+// positions before the first force pass of a load step (the force pass must
+// see the constrained configuration for the residual to be meaningful).  A
+// snapped particle gets the axial velocity jump/dt0 so that the first force
+// pass (run with dt0) carries the jump into the incremental spring states.
+template <int MODE>
+__global__ void k_fire_constrain(FireParticles p) {
+  rfd_loading L = *p.ctl;
+  L.v = 0.0;
+  const double inv_dt = 1.0 / p.fire->dt;
+  double sb = 0, stp = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.N; i += grid_stride()) {
+    P4 R = p.P[i];
+    v3 V = xyz(p.V[i]), F = kZero;
+    const double a0 = MODE == 1 ? R.z : R.y;
+    platen<MODE>(L, R, V, F, sb, stp);
+    const double a1 = MODE == 1 ? R.z : R.y;
+    if (a1 != a0) {
+      if (MODE == 1)
+        V.z = (float)((a1 - a0) * inv_dt);
+      else
+        V.y = (float)((a1 - a0) * inv_dt);
+      p.P[i] = R;
+      p.V[i] = f4(V);
+    }
+  }
+}
+
+// warm start as one kinematic step: V = dU/tau, W = dTh/tau
+__global__ void k_fire_kinematic_set(const float *__restrict__ dU, const float *__restrict__ dTh,
+                                     uint32_t N, float inv_tau, float4 *__restrict__ V,
+                                     float4 *__restrict__ W) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
+    V[i] = dU ? make_float4(dU[3 * i] * inv_tau, dU[3 * i + 1] * inv_tau,
+                            dU[3 * i + 2] * inv_tau, 0.f)
+              : make_float4(0.f, 0.f, 0.f, 0.f);
+    W[i] = dTh ? make_float4(dTh[3 * i] * inv_tau, dTh[3 * i + 1] * inv_tau,
+                             dTh[3 * i + 2] * inv_tau, 0.f)
+               : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
+__global__ void k_fire_kinematic_move(P4 *__restrict__ P, float4 *__restrict__ V,
+                                      float4 *__restrict__ W, float4 *__restrict__ Th,
+                                      uint32_t N, float tau) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
+    P4 R = P[i];
+    const v3 v = xyz(V[i]), w = xyz(W[i]);
+    R.x += (double)v.x * tau;
+    R.y += (double)v.y * tau;
+    R.z += (double)v.z * tau;
+    P[i] = R;
+    Th[i] = f4(xyz(Th[i]) + w * tau);
+    V[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    W[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
+// displacement of the load step: du = R - R0, dth = Th (N x 3 floats)
+__global__ void k_fire_step_disp(const P4 *__restrict__ P, const P4 *__restrict__ P0,
+                                 const float4 *__restrict__ Th, uint32_t N,
+                                 float *__restrict__ du, float *__restrict__ dth) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
+    const P4 a = P[i], b = P0[i];
+    du[3 * i] = (float)(a.x - b.x);
+    du[3 * i + 1] = (float)(a.y - b.y);
+    du[3 * i + 2] = (float)(a.z - b.z);
+    const float4 t = Th[i];
+    dth[3 * i] = t.x;
+    dth[3 * i + 1] = t.y;
+    dth[3 * i + 2] = t.z;
+  }
+}
+
+__global__ void k_zero_f4(float4 *__restrict__ a, float4 *__restrict__ b, uint32_t N) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
+    a[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (b) b[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
 // ================================================================= host side
 struct rfd_system {
   rfd_ctx *ctx = nullptr;
@@ -760,12 +1112,16 @@ struct rfd_system {
   P4 *P = nullptr;
   float4 *V = nullptr, *W = nullptr, *F = nullptr, *M = nullptr;
   float2 *inv = nullptr;
-  uint32_t *boff = nullptr, *ioff = nullptr, *iq = nullptr;
-  uint2 *bij = nullptr;
-  float4 *bp = nullptr;
+  uint32_t *hoff = nullptr;
+  HB *hb = nullptr;
+  float *tc = nullptr;
+  uint32_t H = 0;                     // half-bonds (2 per bond)
+  std::vector<uint32_t> hown, hmir;   // per bond: its half in row i and in row j
   BS *st[2] = {nullptr, nullptr};
   uint8_t *fl[2] = {nullptr, nullptr};
-  int cur = 0;  // index of the current (old) bond state buffer
+  int cur = 0;  // bond state buffer in use (one buffer, updated in place)
+  uint32_t *scan_tot = nullptr, *scan_off = nullptr;  // chunk totals / offsets
+  uint32_t scan_cap = 0;
   uint32_t *cj[2] = {nullptr, nullptr}, *ccnt[2] = {nullptr, nullptr};
   float4 *cO[2] = {nullptr, nullptr};
   int ccur = 0;
@@ -781,6 +1137,14 @@ struct rfd_system {
   int mode = 0;  // host copy of the loading mode (selects the pass kernel)
   float rmax = 0.f;
   bool have_particles = false, have_bonds = false, rows_valid = false;
+  // quasi-static load steps
+  FireDev *fire = nullptr;
+  float *fire_dt = nullptr;  // &fire->dt_f, handed to the force pass
+  double *fire_part = nullptr;
+  P4 *P0 = nullptr;          // positions at the start of the load step
+  float4 *Th = nullptr;      // rotation accumulated during the load step
+  float *du[2] = {nullptr, nullptr}, *dth[2] = {nullptr, nullptr};
+  int dcur = 0;              // index of the newest step displacement
   bool timing = false;  // force-kernel events on the context stream
   std::vector<hipEvent_t> ev;
   uint32_t ev_used = 0;
@@ -791,7 +1155,7 @@ namespace {
 template <class T>
 int dalloc(rfd_ctx *ctx, T **p, size_t n) {
   if (*p) {
-    hipFree(*p);
+    (void)hipFree(*p);  // hipFree waits for the device before releasing
     *p = nullptr;
   }
   if (n == 0) n = 1;
@@ -801,7 +1165,7 @@ int dalloc(rfd_ctx *ctx, T **p, size_t n) {
 
 template <class T>
 void dfree(T *&p) {
-  if (p) hipFree(p);
+  if (p) (void)hipFree(p);
   p = nullptr;
 }
 
@@ -818,6 +1182,7 @@ Mat make_mat(const rfd_system_params &p) {
   m.mur = (float)p.mur;
   m.rho43pi = (float)(p.rho * 4.0 / 3.0 * M_PI);
   m.dt = (float)p.dt;
+  m.farcut = p.farcut;
   m.refmode = p.refmode;
   return m;
 }
@@ -831,19 +1196,32 @@ SysDev make_dev(rfd_system *s) {
   d.W = s->W;
   d.F = s->F;
   d.M = s->M;
-  d.boff = s->boff;
-  d.ioff = s->ioff;
-  d.iq = s->iq;
-  d.bij = s->bij;
-  d.bp = s->bp;
-  d.st_old = s->st[s->cur];
-  d.st_new = s->st[1 - s->cur];
-  d.fl_old = s->fl[s->cur];
-  d.fl_new = s->fl[1 - s->cur];
+  d.hoff = s->hoff;
+  d.hb = s->hb;
+  d.tc = s->tc;
+  d.st = s->st[0];
+  d.fl = s->fl[0];
   d.cj = s->cj[s->ccur];
   d.ccnt = s->ccnt[s->ccur];
   d.cO = s->cO[s->ccur];
   return d;
+}
+
+// exclusive scan of in[0..n) into start[0..n] (start[n] = total)
+int scan_u32(rfd_system *s, const uint32_t *in, uint32_t n, uint32_t *start, uint32_t *cursor) {
+  hipStream_t st = s->ctx->stream;
+  const uint32_t nc = (n + kScanChunk - 1) / kScanChunk;
+  if (nc + 1 > s->scan_cap) {
+    int e = dalloc(s->ctx, &s->scan_tot, (size_t)nc + 1);
+    if (!e) e = dalloc(s->ctx, &s->scan_off, (size_t)nc + 1);
+    if (e) return e;
+    s->scan_cap = nc + 1;
+  }
+  k_scan_totals<<<nc, 1024, 0, st>>>(in, n, s->scan_tot);
+  k_nat_scan<<<1, 1024, 0, st>>>(s->scan_tot, nc, s->scan_off, nullptr);
+  k_scan_apply<<<nc, 1024, 0, st>>>(in, n, s->scan_off, nc, start, cursor);
+  RFD_HIP(s->ctx, hipGetLastError());
+  return RFD_OK;
 }
 
 int bin_particles(rfd_system *s) {
@@ -851,7 +1229,10 @@ int bin_particles(rfd_system *s) {
   k_fill_u32n<<<rfd_blocks(s->C, 256, 4096), 256, 0, st>>>(s->ccnt_cells, s->C, 0u);
   k_nat_cell_index<<<rfd_blocks(s->N, 256), 256, 0, st>>>(s->P, s->N, s->g, s->CI,
                                                           s->ccnt_cells);
-  k_nat_scan<<<1, 1024, 0, st>>>(s->ccnt_cells, s->C, s->cstart, s->ccursor);
+  {
+    int e = scan_u32(s, s->ccnt_cells, s->C, s->cstart, s->ccursor);
+    if (e) return e;
+  }
   k_nat_scatter<<<rfd_blocks(s->N, 256), 256, 0, st>>>(s->CI, s->N, s->ccursor, s->ctmp);
   k_nat_rank<<<rfd_blocks(s->C, 4), 256, 0, st>>>(s->ctmp, s->cstart, s->C, s->csorted);
   RFD_HIP(s->ctx, hipGetLastError());
@@ -867,8 +1248,8 @@ NbrDev make_nbr(rfd_system *s, double extra) {
   d.sorted = s->csorted;
   d.extra = extra;
   d.Kc = s->p.contact_capacity;
-  d.boff = s->boff;
-  d.bij = s->bij;
+  d.hoff = s->hoff;
+  d.hb = s->hb;
   d.fl = s->fl[s->cur];
   d.flags = s->ctx->d_flags;
   return d;
@@ -886,16 +1267,15 @@ int rebuild_rows(rfd_system *s) {
   d.ccnt_new = s->ccnt[n];
   d.cO_new = s->cO[n];
   k_nat_neighbours<2><<<rfd_blocks(s->N, 128), 128, 0, s->ctx->stream>>>(d);
-  k_nat_prune<<<rfd_blocks(s->N, 256), 256, 0, s->ctx->stream>>>(s->P, s->N, s->boff, s->bij,
-                                                                   s->fl[s->cur], s->p.farcut);
   RFD_HIP(s->ctx, hipGetLastError());
   s->ccur = n;
   s->rows_valid = true;
   return RFD_OK;
 }
 
-int forces(rfd_system *s) {
+int forces(rfd_system *s, const float *dtp = nullptr) {
   SysDev d = make_dev(s);
+  d.dtp = dtp;
   Mat m = make_mat(s->p);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (s->timing && s->ev_used + 2 <= s->ev.size()) {
@@ -904,10 +1284,11 @@ int forces(rfd_system *s) {
     s->ev_used += 2;
     RFD_HIP(s->ctx, hipEventRecord(e0, s->ctx->stream));
   }
-  k_nat_forces<<<(s->N + kFP - 1) / kFP, kFT, 0, s->ctx->stream>>>(d, m);
+  const uint32_t nb = (s->N + kFP - 1) / kFP;
+  const uint32_t per = (nb + kXcd - 1) / kXcd;
+  k_nat_forces<<<per * kXcd, kFT, 0, s->ctx->stream>>>(d, m, nb, per);
   RFD_HIP(s->ctx, hipGetLastError());
   if (e1) RFD_HIP(s->ctx, hipEventRecord(e1, s->ctx->stream));
-  s->cur = 1 - s->cur;
   return RFD_OK;
 }
 
@@ -930,12 +1311,73 @@ int pass_and_control(rfd_system *s) {
     k_nat_pass<2><<<s->nblocks_pass, 256, 0, st>>>(p);
   else
     k_nat_pass<0><<<s->nblocks_pass, 256, 0, st>>>(p);
-  k_nat_control<<<1, 64, 0, st>>>(s->ctl, s->part, (int)s->nblocks_pass, s->hist, s->p.dt);
+  k_nat_control<<<1, 256, 0, st>>>(s->ctl, s->part, (int)s->nblocks_pass, s->hist, s->p.dt);
   RFD_HIP(s->ctx, hipGetLastError());
   return RFD_OK;
 }
 
 bool valid(rfd_system *s) { return s && s->ctx; }
+
+int fire_alloc(rfd_system *s) {
+  if (s->fire) return RFD_OK;
+  rfd_ctx *ctx = s->ctx;
+  int st = dalloc(ctx, &s->fire, 1);
+  if (!st) st = dalloc(ctx, &s->fire_part, (size_t)kFireVals * s->nblocks_pass);
+  if (!st) st = dalloc(ctx, &s->P0, s->N);
+  if (!st) st = dalloc(ctx, &s->Th, s->N);
+  for (int q = 0; q < 2 && !st; ++q) {
+    st = dalloc(ctx, &s->du[q], 3 * (size_t)s->N);
+    if (!st) st = dalloc(ctx, &s->dth[q], 3 * (size_t)s->N);
+  }
+  if (st) return st;
+  for (int q = 0; q < 2; ++q) {
+    RFD_HIP(ctx, hipMemset(s->du[q], 0, sizeof(float) * 3 * s->N));
+    RFD_HIP(ctx, hipMemset(s->dth[q], 0, sizeof(float) * 3 * s->N));
+  }
+  s->fire_dt = (float *)((char *)s->fire + offsetof(FireDev, dt_f));
+  return RFD_OK;
+}
+
+FireParticles fire_particles(rfd_system *s) {
+  FireParticles p{};
+  p.N = s->N;
+  p.P = s->P;
+  p.V = s->V;
+  p.W = s->W;
+  p.F = s->F;
+  p.Th = s->Th;
+  p.M = s->M;
+  p.inv = s->inv;
+  p.dt_phys = (float)s->p.dt;
+  p.ctl = s->ctl;
+  p.fire = s->fire;
+  p.part = s->fire_part;
+  return p;
+}
+
+// one FIRE iteration: force pass at the current configuration, constraint +
+// reductions, control, update
+int fire_iteration(rfd_system *s) {
+  if (!s->rows_valid || s->step_count % s->p.rebuild_every == 0) {
+    int st = rebuild_rows(s);
+    if (st) return st;
+  }
+  int st = forces(s, s->fire_dt);
+  if (st) return st;
+  FireParticles fp = fire_particles(s);
+  hipStream_t q = s->ctx->stream;
+  if (s->mode == 1)
+    k_fire_measure<1><<<s->nblocks_pass, 256, 0, q>>>(fp);
+  else if (s->mode == 2)
+    k_fire_measure<2><<<s->nblocks_pass, 256, 0, q>>>(fp);
+  else
+    k_fire_measure<0><<<s->nblocks_pass, 256, 0, q>>>(fp);
+  k_fire_control<<<1, 256, 0, q>>>(s->fire, s->fire_part, (int)s->nblocks_pass, s->ctl);
+  k_fire_update<<<s->nblocks_pass, 256, 0, q>>>(fp);
+  RFD_HIP(s->ctx, hipGetLastError());
+  s->step_count++;
+  return RFD_OK;
+}
 
 int upload_vec3(rfd_ctx *ctx, float4 *dst, const float *soa, uint32_t N) {
   std::vector<float4> h(N);
@@ -982,8 +1424,7 @@ int rfd_system_create(rfd_ctx *ctx, const rfd_system_params *p, uint32_t N, rfd_
   if (!st) st = dalloc(ctx, &s->F, N);
   if (!st) st = dalloc(ctx, &s->M, N);
   if (!st) st = dalloc(ctx, &s->inv, N);
-  if (!st) st = dalloc(ctx, &s->boff, (size_t)N + 1);
-  if (!st) st = dalloc(ctx, &s->ioff, (size_t)N + 1);
+  if (!st) st = dalloc(ctx, &s->hoff, (size_t)N + 1);
   for (int q = 0; q < 2 && !st; ++q) {
     st = dalloc(ctx, &s->cj[q], KN);
     if (!st) st = dalloc(ctx, &s->ccnt[q], N);
@@ -1013,18 +1454,16 @@ int rfd_system_create(rfd_ctx *ctx, const rfd_system_params *p, uint32_t N, rfd_
 
 int rfd_system_destroy(rfd_system *s) {
   if (!s) return RFD_OK;
-  if (s->ctx) hipStreamSynchronize(s->ctx->stream);
+  if (s->ctx) (void)hipStreamSynchronize(s->ctx->stream);
   dfree(s->P);
   dfree(s->V);
   dfree(s->W);
   dfree(s->F);
   dfree(s->M);
   dfree(s->inv);
-  dfree(s->boff);
-  dfree(s->ioff);
-  dfree(s->iq);
-  dfree(s->bij);
-  dfree(s->bp);
+  dfree(s->hoff);
+  dfree(s->hb);
+  dfree(s->tc);
   for (int q = 0; q < 2; ++q) {
     dfree(s->st[q]);
     dfree(s->fl[q]);
@@ -1041,7 +1480,17 @@ int rfd_system_destroy(rfd_system *s) {
   dfree(s->ctl);
   dfree(s->part);
   dfree(s->hist);
-  for (hipEvent_t e : s->ev) hipEventDestroy(e);
+  dfree(s->scan_tot);
+  dfree(s->scan_off);
+  dfree(s->fire);
+  dfree(s->fire_part);
+  dfree(s->P0);
+  dfree(s->Th);
+  for (int q = 0; q < 2; ++q) {
+    dfree(s->du[q]);
+    dfree(s->dth[q]);
+  }
+  for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
   delete s;
   return RFD_OK;
 }
@@ -1150,7 +1599,12 @@ int rfd_system_find_pairs(rfd_system *s, double gap, uint32_t *pairs_host, uint3
   NbrDev d = make_nbr(s, gap);
   d.count = cnt;
   k_nat_neighbours<0><<<rfd_blocks(s->N, 128), 128, 0, ctx->stream>>>(d);
-  k_nat_scan<<<1, 1024, 0, ctx->stream>>>(cnt, s->N, off, nullptr);
+  st = scan_u32(s, cnt, s->N, off, nullptr);
+  if (st) {
+    dfree(cnt);
+    dfree(off);
+    return st;
+  }
   uint32_t total = 0;
   hipError_t e = hipMemcpyAsync(&total, off + s->N, sizeof(uint32_t), hipMemcpyDeviceToHost,
                                 ctx->stream);
@@ -1185,49 +1639,62 @@ int rfd_system_set_bonds(rfd_system *s, uint32_t Nb, const uint32_t *bi, const u
   rfd_ctx *ctx = s->ctx;
   RFD_CHECK_ARG(ctx, s->have_particles, "rfd_system_set_bonds: set particles first");
   RFD_CHECK_ARG(ctx, Nb == 0 || (bi && bj && br && sc && tc), "rfd_system_set_bonds: null input");
+  RFD_CHECK_ARG(ctx, Nb < 0x7FFFFFFFu, "rfd_system_set_bonds: too many bonds");
   const uint32_t N = s->N;
-  std::vector<uint32_t> boff(N + 1, 0), ioff(N + 1, 0), iq(Nb);
-  std::vector<uint2> hij(Nb);
-  std::vector<float4> hbp(Nb);
+  // rows: particle p holds one half per incident bond, sorted by partner id
+  std::vector<uint32_t> hoff(N + 1, 0);
   for (uint32_t b = 0; b < Nb; ++b) {
     RFD_CHECK_ARG(ctx, bi[b] < bj[b] && bj[b] < N, "rfd_system_set_bonds: bond %u (%u,%u) invalid",
                   b, bi[b], bj[b]);
     RFD_CHECK_ARG(ctx, b == 0 || bi[b - 1] < bi[b] || (bi[b - 1] == bi[b] && bj[b - 1] < bj[b]),
                   "rfd_system_set_bonds: bonds not sorted by (i, j) at %u", b);
-    boff[bi[b] + 1]++;
-    ioff[bj[b] + 1]++;
-    hij[b] = make_uint2(bi[b], bj[b]);
-    hbp[b] = make_float4(L0 ? L0[b] : 0.f, br[b], sc[b], tc[b]);
+    hoff[bi[b] + 1]++;
+    hoff[bj[b] + 1]++;
   }
-  for (uint32_t i = 0; i < N; ++i) {
-    boff[i + 1] += boff[i];
-    ioff[i + 1] += ioff[i];
-  }
+  for (uint32_t i = 0; i < N; ++i) hoff[i + 1] += hoff[i];
+  const uint32_t H = 2 * Nb;
+  std::vector<HB> hhb(H);
+  std::vector<float> htc(H);
+  std::vector<uint32_t> partner(H);
+  s->hown.assign(Nb, 0);
+  s->hmir.assign(Nb, 0);
   {
-    std::vector<uint32_t> cur(ioff.begin(), ioff.end() - 1);
-    for (uint32_t b = 0; b < Nb; ++b) iq[cur[bj[b]]++] = b;
+    // lower partners first (bonds where p is j, in bond order = ascending i),
+    // then higher partners (bonds where p is i, ascending j): rows sorted by id
+    std::vector<uint32_t> cur(hoff.begin(), hoff.end() - 1);
+    for (uint32_t b = 0; b < Nb; ++b) {
+      const uint32_t h = cur[bj[b]]++;
+      s->hmir[b] = h;
+      partner[h] = bi[b];
+    }
+    for (uint32_t b = 0; b < Nb; ++b) {
+      const uint32_t h = cur[bi[b]]++;
+      s->hown[b] = h;
+      partner[h] = bj[b];
+    }
+    for (uint32_t b = 0; b < Nb; ++b) {
+      const float l0 = L0 ? L0[b] : 0.f;
+      hhb[s->hown[b]] = HB{bj[b], l0, br[b], sc[b]};
+      hhb[s->hmir[b]] = HB{bi[b], l0, br[b], sc[b]};
+      htc[s->hown[b]] = tc[b];
+      htc[s->hmir[b]] = tc[b];
+    }
   }
   s->Nb = Nb;
-  int st = dalloc(ctx, &s->bij, Nb);
-  if (!st) st = dalloc(ctx, &s->bp, Nb);
-  if (!st) st = dalloc(ctx, &s->iq, Nb);
-  for (int q = 0; q < 2 && !st; ++q) {
-    st = dalloc(ctx, &s->st[q], Nb);
-    if (!st) st = dalloc(ctx, &s->fl[q], Nb);
-  }
+  s->H = H;
+  int st = dalloc(ctx, &s->hb, H);
+  if (!st) st = dalloc(ctx, &s->tc, H);
+  if (!st) st = dalloc(ctx, &s->st[0], H);
+  if (!st) st = dalloc(ctx, &s->fl[0], H);
   if (st) return st;
-  RFD_HIP(ctx, hipMemcpy(s->boff, boff.data(), sizeof(uint32_t) * (N + 1), hipMemcpyHostToDevice));
-  RFD_HIP(ctx, hipMemcpy(s->ioff, ioff.data(), sizeof(uint32_t) * (N + 1), hipMemcpyHostToDevice));
-  if (Nb) {
-    RFD_HIP(ctx, hipMemcpy(s->bij, hij.data(), sizeof(uint2) * Nb, hipMemcpyHostToDevice));
-    RFD_HIP(ctx, hipMemcpy(s->bp, hbp.data(), sizeof(float4) * Nb, hipMemcpyHostToDevice));
-    RFD_HIP(ctx, hipMemcpy(s->iq, iq.data(), sizeof(uint32_t) * Nb, hipMemcpyHostToDevice));
-    for (int q = 0; q < 2; ++q) {
-      RFD_HIP(ctx, hipMemset(s->st[q], 0, sizeof(BS) * Nb));
-      RFD_HIP(ctx, hipMemset(s->fl[q], kBond, Nb));
-    }
+  RFD_HIP(ctx, hipMemcpy(s->hoff, hoff.data(), sizeof(uint32_t) * (N + 1), hipMemcpyHostToDevice));
+  if (H) {
+    RFD_HIP(ctx, hipMemcpy(s->hb, hhb.data(), sizeof(HB) * H, hipMemcpyHostToDevice));
+    RFD_HIP(ctx, hipMemcpy(s->tc, htc.data(), sizeof(float) * H, hipMemcpyHostToDevice));
+    RFD_HIP(ctx, hipMemset(s->st[0], 0, sizeof(BS) * H));
+    RFD_HIP(ctx, hipMemset(s->fl[0], kBond, H));
     if (!L0) {
-      k_set_rest_length<<<rfd_blocks(Nb, 256), 256, 0, ctx->stream>>>(s->P, Nb, s->bij, s->bp);
+      k_set_rest_length<<<rfd_blocks(N, 256), 256, 0, ctx->stream>>>(s->P, N, s->hoff, s->hb);
       RFD_HIP(ctx, hipGetLastError());
     }
   }
@@ -1238,13 +1705,26 @@ int rfd_system_set_bonds(rfd_system *s, uint32_t Nb, const uint32_t *bi, const u
   return RFD_OK;
 }
 
+// bond-level state API: the owner half (row i) carries the state as seen from
+// i; the mirror half holds its negation (n_prev, O, Mn, Mt all change sign
+// under i <-> j)
 int rfd_system_set_bond_state(rfd_system *s, const float *state, const uint8_t *flag) {
   if (!valid(s)) return RFD_E_ARG;
   rfd_ctx *ctx = s->ctx;
   RFD_CHECK_ARG(ctx, s->have_bonds && state && flag, "rfd_system_set_bond_state: bad state");
   static_assert(sizeof(BS) == 12 * sizeof(float), "bond state record is 12 floats");
-  RFD_HIP(ctx, hipMemcpy(s->st[s->cur], state, sizeof(BS) * s->Nb, hipMemcpyHostToDevice));
-  RFD_HIP(ctx, hipMemcpy(s->fl[s->cur], flag, s->Nb, hipMemcpyHostToDevice));
+  const uint32_t Nb = s->Nb;
+  std::vector<float> hs(12 * (size_t)s->H);
+  std::vector<uint8_t> hf(s->H);
+  for (uint32_t b = 0; b < Nb; ++b) {
+    for (int c = 0; c < 12; ++c) {
+      hs[12 * (size_t)s->hown[b] + c] = state[12 * (size_t)b + c];
+      hs[12 * (size_t)s->hmir[b] + c] = -state[12 * (size_t)b + c];
+    }
+    hf[s->hown[b]] = hf[s->hmir[b]] = flag[b];
+  }
+  RFD_HIP(ctx, hipMemcpy(s->st[s->cur], hs.data(), sizeof(BS) * s->H, hipMemcpyHostToDevice));
+  RFD_HIP(ctx, hipMemcpy(s->fl[s->cur], hf.data(), s->H, hipMemcpyHostToDevice));
   return RFD_OK;
 }
 
@@ -1253,9 +1733,18 @@ int rfd_system_get_bond_state(rfd_system *s, float *state, uint8_t *flag) {
   rfd_ctx *ctx = s->ctx;
   RFD_CHECK_ARG(ctx, s->have_bonds, "rfd_system_get_bond_state: no bonds");
   RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  if (state)
-    RFD_HIP(ctx, hipMemcpy(state, s->st[s->cur], sizeof(BS) * s->Nb, hipMemcpyDeviceToHost));
-  if (flag) RFD_HIP(ctx, hipMemcpy(flag, s->fl[s->cur], s->Nb, hipMemcpyDeviceToHost));
+  const uint32_t Nb = s->Nb;
+  if (state) {
+    std::vector<float> hs(12 * (size_t)s->H);
+    RFD_HIP(ctx, hipMemcpy(hs.data(), s->st[s->cur], sizeof(BS) * s->H, hipMemcpyDeviceToHost));
+    for (uint32_t b = 0; b < Nb; ++b)
+      for (int c = 0; c < 12; ++c) state[12 * (size_t)b + c] = hs[12 * (size_t)s->hown[b] + c];
+  }
+  if (flag) {
+    std::vector<uint8_t> hf(s->H);
+    RFD_HIP(ctx, hipMemcpy(hf.data(), s->fl[s->cur], s->H, hipMemcpyDeviceToHost));
+    for (uint32_t b = 0; b < Nb; ++b) flag[b] = hf[s->hown[b]];
+  }
   return RFD_OK;
 }
 
@@ -1380,4 +1869,150 @@ int rfd_system_device_arrays(rfd_system *s, double **P, float **V, float **W, fl
   return RFD_OK;
 }
 
+int rfd_system_fire_load_step(rfd_system *s, double dstrain, const rfd_fire_params *fp,
+                              const float *dU, const float *dTh, rfd_fire_result *out) {
+  if (!valid(s)) return RFD_E_ARG;
+  rfd_ctx *ctx = s->ctx;
+  RFD_CHECK_ARG(ctx, fp && out && s->have_bonds, "rfd_system_fire_load_step: bad arguments");
+  RFD_CHECK_ARG(ctx, fp->dt0 > 0 && fp->dt_max >= fp->dt0 && fp->ftol > 0 && fp->max_iter > 0,
+                "rfd_system_fire_load_step: bad FIRE parameters");
+  const auto t0 = std::chrono::steady_clock::now();
+  int st = fire_alloc(s);
+  if (st) return st;
+  hipStream_t q = ctx->stream;
+  // platen move (displacement control) on the host copy of the loading state
+  rfd_loading L;
+  st = rfd_system_get_loading(s, &L);
+  if (st) return st;
+  if (L.mode) {
+    const double d = dstrain * L.size0;
+    L.size += d;
+    L.top += d;
+    L.top_r = L.top - L.band;
+    L.strain = (L.size - L.size0) / L.size0;
+    L.v = 0.0;
+    RFD_HIP(ctx, hipMemcpyAsync(s->ctl, &L, sizeof(L), hipMemcpyHostToDevice, q));
+  }
+  RFD_HIP(ctx, hipMemcpyAsync(s->P0, s->P, sizeof(P4) * s->N, hipMemcpyDeviceToDevice, q));
+  k_zero_f4<<<rfd_blocks(s->N, 256), 256, 0, q>>>(s->Th, nullptr, s->N);
+  FireDev f{};
+  f.dt = f.dt_last = fp->dt0;
+  f.alpha = f.alpha0 = fp->alpha0;
+  f.dt_max = fp->dt_max;
+  f.dt_min = 1e-3 * fp->dt0;
+  f.f_inc = fp->f_inc;
+  f.f_dec = fp->f_dec;
+  f.f_alpha = fp->f_alpha;
+  f.ftol = fp->ftol;
+  f.n_delay = fp->n_delay;
+  f.max_iter = fp->max_iter;
+  f.first = 1;
+  f.dt_f = (float)fp->dt0;
+  RFD_HIP(ctx, hipMemcpyAsync(s->fire, &f, sizeof(f), hipMemcpyHostToDevice, q));
+  if (dU || dTh) {
+    // warm start: one kinematic step of length dt0 carrying the increment
+    const float tau = (float)fp->dt0;
+    k_fire_kinematic_set<<<rfd_blocks(s->N, 256), 256, 0, q>>>(dU, dTh, s->N, 1.0f / tau, s->V,
+                                                                s->W);
+    st = forces(s, s->fire_dt);
+    if (st) return st;
+    k_fire_kinematic_move<<<rfd_blocks(s->N, 256), 256, 0, q>>>(s->P, s->V, s->W, s->Th, s->N,
+                                                                 tau);
+  } else {
+    k_zero_f4<<<rfd_blocks(s->N, 256), 256, 0, q>>>(s->V, s->W, s->N);
+  }
+  {
+    FireParticles fpp = fire_particles(s);
+    if (s->mode == 1)
+      k_fire_constrain<1><<<s->nblocks_pass, 256, 0, q>>>(fpp);
+    else if (s->mode == 2)
+      k_fire_constrain<2><<<s->nblocks_pass, 256, 0, q>>>(fpp);
+  }
+  RFD_HIP(ctx, hipGetLastError());
+  const uint32_t chunk = fp->check_every ? fp->check_every : 50;
+  for (;;) {
+    for (uint32_t k = 0; k < chunk; ++k) {
+      st = fire_iteration(s);
+      if (st) return st;
+    }
+    RFD_HIP(ctx, hipMemcpyAsync(&f, s->fire, sizeof(f), hipMemcpyDeviceToHost, q));
+    RFD_HIP(ctx, hipStreamSynchronize(q));
+    if (f.done) break;
+  }
+  // the step's displacement and rotation, newest first
+  s->dcur = 1 - s->dcur;
+  k_fire_step_disp<<<rfd_blocks(s->N, 256), 256, 0, q>>>(s->P, s->P0, s->Th, s->N,
+                                                          s->du[s->dcur], s->dth[s->dcur]);
+  RFD_HIP(ctx, hipGetLastError());
+  st = rfd_system_get_loading(s, &L);
+  if (st) return st;
+  out->iterations = f.iter;
+  out->converged = f.converged;
+  out->fmax0 = f.fmax0;
+  out->fmax = f.fmax;
+  out->mmax = f.mmax;
+  out->sum_bottom = f.sum_bottom;
+  out->sum_top = f.sum_top;
+  out->strain = L.strain;
+  out->seconds =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return RFD_OK;
+}
+
+int rfd_system_residual(rfd_system *s, double out[2]) {
+  if (!valid(s)) return RFD_E_ARG;
+  rfd_ctx *ctx = s->ctx;
+  RFD_CHECK_ARG(ctx, out && s->have_bonds, "rfd_system_residual: bad arguments");
+  int st = fire_alloc(s);
+  if (st) return st;
+  hipStream_t q = ctx->stream;
+  FireDev f{};
+  f.measure_only = 1;
+  f.dt_f = 0.0f;  // no spring increments: the state is only re-expressed
+  RFD_HIP(ctx, hipMemcpyAsync(s->fire, &f, sizeof(f), hipMemcpyHostToDevice, q));
+  if (!s->rows_valid) {
+    st = rebuild_rows(s);
+    if (st) return st;
+  }
+  st = forces(s, s->fire_dt);
+  if (st) return st;
+  FireParticles fp = fire_particles(s);
+  if (s->mode == 1)
+    k_fire_measure<1><<<s->nblocks_pass, 256, 0, q>>>(fp);
+  else if (s->mode == 2)
+    k_fire_measure<2><<<s->nblocks_pass, 256, 0, q>>>(fp);
+  else
+    k_fire_measure<0><<<s->nblocks_pass, 256, 0, q>>>(fp);
+  k_fire_control<<<1, 256, 0, q>>>(s->fire, s->fire_part, (int)s->nblocks_pass, s->ctl);
+  RFD_HIP(ctx, hipGetLastError());
+  RFD_HIP(ctx, hipMemcpyAsync(&f, s->fire, sizeof(f), hipMemcpyDeviceToHost, q));
+  RFD_HIP(ctx, hipStreamSynchronize(q));
+  out[0] = f.fmax;
+  out[1] = f.mmax;
+  return RFD_OK;
+}
+
+int rfd_system_fire_buffers(rfd_system *s, float **du_last, float **du_prev, float **dth_last,
+                            float **dth_prev) {
+  if (!valid(s)) return RFD_E_ARG;
+  int st = fire_alloc(s);
+  if (st) return st;
+  if (du_last) *du_last = s->du[s->dcur];
+  if (du_prev) *du_prev = s->du[1 - s->dcur];
+  if (dth_last) *dth_last = s->dth[s->dcur];
+  if (dth_prev) *dth_prev = s->dth[1 - s->dcur];
+  return RFD_OK;
+}
+
+int rfd_system_bond_graph(rfd_system *s, const uint32_t **hoff, const void **rec,
+                          const uint8_t **flag, uint32_t *H) {
+  if (!valid(s)) return RFD_E_ARG;
+  if (hoff) *hoff = s->hoff;
+  if (rec) *rec = s->hb;
+  if (flag) *flag = s->fl[0];
+  if (H) *H = s->H;
+  return RFD_OK;
+}
+
 }  // extern "C"
+

@@ -46,6 +46,22 @@ class Loading(C.Structure):
                 ("hidden", C.c_double * 3)]
 
 
+class FireParams(C.Structure):
+    _fields_ = [("ftol", C.c_double), ("dt0", C.c_double), ("dt_max", C.c_double),
+                ("alpha0", C.c_double), ("f_inc", C.c_double), ("f_dec", C.c_double),
+                ("f_alpha", C.c_double), ("n_delay", C.c_uint32), ("max_iter", C.c_uint32),
+                ("check_every", C.c_uint32)]
+
+
+class FireResult(C.Structure):
+    _fields_ = [("iterations", C.c_uint32), ("converged", C.c_int), ("fmax0", C.c_double),
+                ("fmax", C.c_double), ("mmax", C.c_double), ("sum_bottom", C.c_double),
+                ("sum_top", C.c_double), ("strain", C.c_double), ("seconds", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _vp, _u32, _dbl = C.c_void_p, C.c_uint32, C.c_double
 NATIVE_SIGNATURES = {
     "rfd_system_create": [_vp, C.POINTER(SystemParams), _u32, C.POINTER(_vp)],
@@ -66,6 +82,12 @@ NATIVE_SIGNATURES = {
     "rfd_system_sizes": [_vp, _u32p, _u32p],
     "rfd_system_device_arrays": [_vp, _vp, _vp, _vp, _vp, _vp],
     "rfd_system_timing": [_vp, C.c_int],
+    "rfd_system_fire_load_step": [_vp, _dbl, C.POINTER(FireParams), _vp, _vp,
+                                  C.POINTER(FireResult)],
+    "rfd_system_residual": [_vp, C.POINTER(C.c_double)],
+    "rfd_system_fire_buffers": [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp),
+                                C.POINTER(_vp)],
+    "rfd_system_bond_graph": [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), _u32p],
     "rfd_system_timing_read": [_vp, C.POINTER(C.c_double), _u32p],
 }
 
@@ -176,6 +198,7 @@ class NativeSystem:
         self.bonds = b
 
     def set_bond_state(self, state: np.ndarray, flag: np.ndarray):
+        """state[Nb, 12], flag[Nb] in the system's bond order (sorted by (i, j))."""
         _, ps = _np(state, np.float32)
         _, pf = _np(flag, np.uint8)
         self._call("rfd_system_set_bond_state", self.h, ps, pf)
@@ -224,6 +247,59 @@ class NativeSystem:
                    C.byref(n))
         return h[: 3 * n.value].reshape(-1, 3)
 
+    # -- quasi-static load steps -----------------------------------------
+    def fire_load_step(self, dstrain: float, params: FireParams, dU=None, dTh=None) -> FireResult:
+        """One load step; dU / dTh are device tensors (N, 3) float32 or None."""
+        r = FireResult()
+        pu = C.c_void_p(dU.data_ptr()) if dU is not None else None
+        pt = C.c_void_p(dTh.data_ptr()) if dTh is not None else None
+        self._call("rfd_system_fire_load_step", self.h, dstrain, C.byref(params), pu, pt,
+                   C.byref(r))
+        return r
+
+    def residual(self) -> tuple[float, float]:
+        out = (C.c_double * 2)()
+        self._call("rfd_system_residual", self.h, out)
+        return out[0], out[1]
+
+    def fire_buffers(self):
+        """(du_last, du_prev, dth_last, dth_prev) as torch views (N, 3) of the
+        device arrays."""
+        ptrs = [C.c_void_p() for _ in range(4)]
+        self._call("rfd_system_fire_buffers", self.h, *[C.byref(q) for q in ptrs])
+        return tuple(device_view(q.value, (self.N, 3), "<f4") for q in ptrs)
+
+    def bond_graph(self):
+        """(hoff[N+1] int32, rec[H, 4] (j, L0, br, sc as int32 bits), flag[H]) views."""
+        a, b, c, H = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_uint32()
+        self._call("rfd_system_bond_graph", self.h, C.byref(a), C.byref(b), C.byref(c),
+                   C.byref(H))
+        return (device_view(a.value, (self.N + 1,), "<i4"),
+                device_view(b.value, (H.value, 4), "<i4"),
+                device_view(c.value, (H.value,), "|u1"))
+
+    def seat_platens(self, preload: float = 0.0, L: Loading | None = None):
+        """Move the platens onto the specimen: top at the highest and bottom at
+        the lowest particle centre inside the platen footprint (load axis of
+        the loading mode), each pushed `preload` further in so that the whole
+        end layer is held; size0 becomes their distance."""
+        L = L or self.loading()
+        R = self.state()["R"].reshape(3, -1)
+        ax = 2 if L.mode == 1 else 1
+        if L.mode == 1:
+            inside = (R[0] - L.cx) ** 2 + (R[1] - L.cy) ** 2 < L.RR
+        else:
+            inside = (R[0] - L.cx) ** 2 + (R[2] - L.cz) ** 2 < L.RR
+        h = R[ax][inside]
+        L.top, L.bottom = float(h.max()) - preload, float(h.min()) + preload
+        L.top_r = L.top - L.band
+        L.bottom_r = L.bottom + 0.5 * L.band
+        L.size = L.size0 = L.top - L.bottom
+        L.strain = 0.0
+        L.v = 0.0
+        self.set_loading(L)
+        return L
+
     def timing(self, enable: bool = True):
         self._call("rfd_system_timing", self.h, int(enable))
 
@@ -239,6 +315,36 @@ class NativeSystem:
 
 
 # ------------------------------------------------------------- loading
+class _DevArray:
+    """Minimal __cuda_array_interface__ wrapper so torch can view device memory
+    owned by the library (lifetime: the owning rfd_system)."""
+
+    def __init__(self, ptr: int, shape, typestr: str):
+        self.__cuda_array_interface__ = {"shape": tuple(shape), "typestr": typestr,
+                                         "data": (ptr, False), "version": 3, "strides": None}
+
+
+def device_view(ptr: int, shape, typestr: str):
+    import torch
+
+    return torch.as_tensor(_DevArray(ptr, shape, typestr), device="cuda")
+
+
+def fire_params(mat_E: float, r_mean: float, dt: float, rel_tol: float = 1e-6,
+                max_iter: int = 20000, check_every: int = 100) -> FireParams:
+    """FIRE settings; the tolerance is rel_tol * F_ref with F_ref = E*pi*r^2,
+    the bond force at unit strain for a bond of the mean radius."""
+    p = FireParams()
+    p.ftol = rel_tol * mat_E * 3.141592653589793 * r_mean * r_mean
+    p.dt0 = dt
+    p.dt_max = 10.0 * dt
+    p.alpha0, p.f_inc, p.f_dec, p.f_alpha = 0.1, 1.1, 0.5, 0.99
+    p.n_delay = 5
+    p.max_iter = max_iter
+    p.check_every = check_every
+    return p
+
+
 def loading_from_compression(cmp: Compression, zcut: float, box=None) -> Loading:
     """Device loading control from the host compression_data mirror."""
     L = Loading()
@@ -286,6 +392,8 @@ def bonds_from_slots(h: dict, N: int, K: int, mat) -> tuple[Bonds, np.ndarray, n
     jj = IL[ii, kk].astype(np.int64)
     m = ii < jj
     ii, kk, jj = ii[m], kk[m], jj[m]
+    order = np.lexsort((jj, ii))  # the system's bond order: sorted by (i, j)
+    ii, kk, jj = ii[order], kk[order], jj[order]
     s = ii * K + kk
     st = np.concatenate([h[k].reshape(-1, 3)[s] for k in ("rij", "Oijt", "Mijn", "Mijt")], 1)
     n = ii.size
