@@ -235,8 +235,11 @@ int rfd_system_find_pairs(rfd_system *sys, double gap, uint32_t *pairs_host,
 int rfd_system_set_bonds(rfd_system *sys, uint32_t Nb, const uint32_t *bi,
                          const uint32_t *bj, const float *br, const float *sc,
                          const float *tc, const float *L0);
-/* bond state, host arrays: state[12*Nb] = n_prev, O, Mn, Mt per bond (owner
- * view, bond-major), flag[Nb] = 1 bond, 0 broken (contact law), 2 removed */
+/* bond state, host arrays: state[12*Nb] = n, O, Mn, Mt per bond in i's frame
+ * (bond-major), flag[Nb] = 1 bond, 0 broken (contact law), 2 removed.  The
+ * device keeps O, Mt and the twist Mn . n; n is the current bond direction
+ * (the force pass takes the previous direction from R - V dt), reported on get
+ * and ignored on set, where Mn is projected on it. */
 int rfd_system_set_bond_state(rfd_system *sys, const float *state, const uint8_t *flag);
 int rfd_system_get_bond_state(rfd_system *sys, float *state, uint8_t *flag);
 int rfd_system_set_velocities(rfd_system *sys, const float *V, const float *W);
@@ -288,11 +291,12 @@ typedef struct rfd_fire_result {
 } rfd_fire_result;
 
 /* One quasi-static load step: move the top platen by dstrain * size0 (load
- * axis of the loading mode; dstrain < 0 compresses), optionally apply a
- * predicted increment (warm start: dU, dTh are device arrays of N x 3 floats,
- * row-major, or NULL) as one kinematic step so the incremental spring laws
- * see it, then relax with FIRE to ftol.  The displacement of the step is kept
- * on the device (rfd_system_fire_buffers). */
+ * axis of the loading mode; dstrain < 0 compresses; particles held by a
+ * platen follow it), optionally apply a predicted increment (warm start: dU,
+ * dTh are device arrays of N x 3 floats, row-major, or NULL), carry the move
+ * into the incremental springs with one force pass, then relax with FIRE
+ * (from rest) to ftol.  The displacement of the step is kept on the device
+ * (rfd_system_fire_buffers). */
 int rfd_system_fire_load_step(rfd_system *sys, double dstrain, const rfd_fire_params *fp,
                               const float *dU, const float *dTh, rfd_fire_result *out);
 /* residual of the current configuration (one force pass + constraint):
@@ -302,11 +306,14 @@ int rfd_system_residual(rfd_system *sys, double out[2]);
  * displacements and rotations, newest first; valid after a load step */
 int rfd_system_fire_buffers(rfd_system *sys, float **du_last, float **du_prev,
                             float **dth_last, float **dth_prev);
-/* half-bond rows on the device: hoff[N+1]; rec[h] = {j, L0, br, sc} (16 B);
- * flag[h] (1 bond, 0 broken, 2 removed) -- the bond graph for callers that
- * build per-bond features in HBM */
-int rfd_system_bond_graph(rfd_system *sys, const uint32_t **hoff, const void **rec,
-                          const uint8_t **flag, uint32_t *H);
+/* bond graph on the device, for callers that build per-bond features in HBM:
+ * hoff[N+1] row offsets; half[h] = {j, b} (uint32 pairs: other end, bond);
+ * bond[b] = {L0, br, sc, tc} (floats); state[b] = {O.xyz, mn, Mt.xyz, flag}
+ * (7 floats + uint32 flag: 1 bond, 0 broken, 2 removed), the latest state;
+ * *H = number of halves (2 Nb).  Rows list lower partners first, each row by
+ * partner id. */
+int rfd_system_bond_graph(rfd_system *sys, const uint32_t **hoff, const void **half,
+                          const void **bond, const void **state, uint32_t *H);
 
 /* device pointers of the particle arrays (for callers that keep data in HBM):
  * P is N records of 4 doubles {x, y, z, radius}; V, W, F, M are N records of

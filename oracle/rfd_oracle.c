@@ -739,7 +739,12 @@ static void rot_apply(const double t[3], const double ph[3], const double x[3],
 
 /* The same bond law as orc_forces_bpm, per undirected bond, in double, with
  * per-bond radius and strengths; the j end receives the mirrored load
- * (-Fn-Ft, -Mn*J-Mt*c + 0.5 r x Ft) that the reference's j->i slot computes. */
+ * (-Fn-Ft, -Mn*J-Mt*c + 0.5 r x Ft) that the reference's j->i slot computes.
+ * Two restatements of the reference's stored state (md_calculateforcesdem3.cu
+ * :62-88, 105-130), exact when positions advance by R += V dt between
+ * evaluations: the previous bond direction (the reference's rij) is the one
+ * at R - V dt, and the twisting moment Mn, parallel to n, is kept as its
+ * component mn = Mn . n, incremented by (n . dW) dt G/(2 L0). */
 void orc_native_bonds_f64(const double *R, const double *V, const double *W,
                           uint32_t N, const uint32_t *bi, const uint32_t *bj,
                           const double *iL, const double *br, const double *sc,
@@ -752,35 +757,35 @@ void orc_native_bonds_f64(const double *R, const double *V, const double *W,
   for (uint32_t b = 0; b < Nb; ++b) {
     if (!active[b]) continue;
     uint32_t i = bi[b], j = bj[b];
-    double r[3], Wi[3], Wj[3], Vi[3], Vj[3];
+    double r[3], rp[3], Wi[3], Wj[3], Vi[3], Vj[3];
     for (int d = 0; d < 3; ++d) {
       r[d] = R[j + (size_t)d * N] - R[i + (size_t)d * N];
       Wi[d] = W[i + (size_t)d * N];
       Wj[d] = W[j + (size_t)d * N];
       Vi[d] = V[i + (size_t)d * N];
       Vj[d] = V[j + (size_t)d * N];
+      rp[d] = r[d] - (Vj[d] - Vi[d]) * dt;
     }
     double L = sqrt(dot3(r, r)), n[3] = {r[0] / L, r[1] / L, r[2] / L};
+    double Lp = sqrt(dot3(rp, rp)), np[3] = {rp[0] / Lp, rp[1] / Lp, rp[2] / Lp};
     double il = iL[b], rb = br[b], A = pi * rb * rb, J = pi * rb * rb * rb * rb;
-    double *st = state + 12 * (size_t)b;
+    double *st = state + 8 * (size_t)b;
     double strain_e = E * (L - 1.0 / il) * il;
     double Fn[3] = {n[0] * strain_e * A, n[1] * strain_e * A, n[2] * strain_e * A};
     double t[3], ws[3] = {Wi[0] + Wj[0], Wi[1] + Wj[1], Wi[2] + Wj[2]};
-    cross3(st, n, t);
+    cross3(np, n, t);
     double c2 = dot3(ws, n) * dt * 0.5, ph[3] = {n[0] * c2, n[1] * c2, n[2] * c2};
     double dw[3] = {Wj[0] - Wi[0], Wj[1] - Wi[1], Wj[2] - Wi[2]};
     double dwn = dot3(n, dw);
-    double mn[3], mt[3], o[3];
-    rot_apply(t, ph, st + 6, mn);
-    rot_apply(t, ph, st + 9, mt);
-    rot_apply(t, ph, st + 3, o);
-    double ctor = dwn * dt * 0.5 * G * il, cbend = dt * 0.25 * E * il;
+    double mt[3], o[3];
+    rot_apply(t, ph, st + 4, mt);
+    rot_apply(t, ph, st, o);
+    double mn = st[3] + dwn * dt * 0.5 * G * il, cbend = dt * 0.25 * E * il;
     double h[3] = {0.5 * ws[0], 0.5 * ws[1], 0.5 * ws[2]}, hxr[3];
     cross3(h, r, hxr);
     double v[3] = {Vj[0] - Vi[0] - hxr[0], Vj[1] - Vi[1] - hxr[1], Vj[2] - Vi[2] - hxr[2]};
     double vn = dot3(n, v);
     for (int d = 0; d < 3; ++d) {
-      mn[d] += n[d] * ctor;
       mt[d] += (dw[d] - n[d] * dwn) * cbend;
       o[d] += (v[d] - n[d] * vn) * dt;
     }
@@ -790,22 +795,21 @@ void orc_native_bonds_f64(const double *R, const double *V, const double *W,
     double mtfac = refmode ? cbend * J : J;
     for (int d = 0; d < 3; ++d) {
       double fi = Fn[d] + Ft[d];
-      double mi = mn[d] * J + mt[d] * mtfac + 0.5 * rxFt[d];
-      double mj = -mn[d] * J - mt[d] * mtfac + 0.5 * rxFt[d];
+      double mi = n[d] * mn * J + mt[d] * mtfac + 0.5 * rxFt[d];
+      double mj = -n[d] * mn * J - mt[d] * mtfac + 0.5 * rxFt[d];
       F[i + (size_t)d * N] += fi;
       F[j + (size_t)d * N] -= fi;
       M[i + (size_t)d * N] += mi;
       M[j + (size_t)d * N] += mj;
-      st[d] = n[d];
-      st[3 + d] = o[d];
-      st[6 + d] = mn[d];
-      st[9 + d] = mt[d];
+      st[d] = o[d];
+      st[4 + d] = mt[d];
     }
+    st[3] = mn;
     double Sc = strain_e + sqrt(dot3(mt, mt)) * 4.0 * rb;
-    double Mc = G * il * sqrt(dot3(o, o)) + sqrt(dot3(mn, mn)) * 2.0 * rb;
+    double Mc = G * il * sqrt(dot3(o, o)) + fabs(mn) * 2.0 * rb;
     if (Sc > sc[b] || Mc > tc[b]) {
       active[b] = 0;
-      for (int d = 3; d < 12; ++d) st[d] = 0.0;
+      for (int d = 0; d < 8; ++d) st[d] = 0.0;
     }
   }
 }

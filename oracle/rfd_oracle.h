@@ -104,7 +104,9 @@ float orc_rsqrt(float x);
 /* ---------------- native (edge-list) physics, FP64 precision oracle --------
  * One record per undirected bond (i<j).  Per-particle radius r, mass m and
  * inertia I; per-bond radius br, 1/L0, tensile/shear strength sc/tc.
- * state[12*b] = n_prev(3), O(3), Mn(3), Mt(3), all as seen from end i.
+ * state[8*b] = O(3), mn, Mt(3), unused: shear spring, twist along n and
+ * bending moment as seen from end i; the previous bond direction is taken
+ * at R - V dt.
  * Forces are accumulated into F,M (double, SoA by component, zeroed here).
  * Reference mode (refmode=1) keeps the extra bending factor of
  * md_calculateforcesdem3.cu:148; refmode=0 applies Mt*pi*br^4 only.       */

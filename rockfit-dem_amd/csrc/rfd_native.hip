@@ -6,17 +6,21 @@
 // Memory layout (all in HBM, sized for one GPU's share of a specimen):
 //   particle  P4 {x, y, z, r} (double, 32 B), V4 / W4 / F4 / M4 (float4),
 //             inv2 {dt/m, dt/I} (float2)
-//   half-bond rows hoff[N+1]; per half h: HB {j, L0, br, sc} (16 B), tc[h],
-//             state {n_prev, O, Mn, Mt} (3 x float4 = 48 B, double-buffered),
-//             flag (uint8, double-buffered: 1 bond, 0 broken, 2 removed)
+//   half-bond rows hoff[N+1]; per half h: HR {j, b} (8 B) -- the other end
+//             and the bond index
+//   bonds     per bond b (sorted by (i, j), i < j): BR {1/L0, br, sc, tc} (16 B),
+//             state BSt {O, mn, Mt, flag} (32 B, two buffers: read one, the
+//             owner writes the other; flag 1 bond, 0 broken, 2 removed)
 //   contacts  slot-major rows cj[k*N+i], cO[k*N+i] (float4), ccnt[i]
-// A bond (i, j) is held twice: in row i from i's side and in row j from j's
-// side, as the reference holds one directed slot per end.  The two halves
-// evaluate exact mirror images (every operation of the law is odd or even
-// under i<->j and IEEE rounding is sign-symmetric), so their loads are exact
-// opposites and they break together; each half reads and writes only its own
-// contiguous record, so the force pass streams the bond arrays once with
-// coalesced loads and gathers only neighbour particle records.
+// A bond (i, j) appears in row i and in row j, as the reference holds one
+// directed slot per end, but its constants and state are stored once.  Both
+// halves evaluate the law, as exact mirror images (every operation is odd or
+// even under i<->j and IEEE rounding is sign-symmetric), so their loads are
+// exact opposites and they break together; the owner half (row i < j) writes
+// the new state.  The state is kept in i's frame: the j half reads O and Mt
+// negated.  The twisting moment is kept as its component mn along n, and the
+// previous bond direction is recomputed from the positions one move back
+// (R - V dt), so the record is 32 B instead of the 48 B of n_prev, O, Mn, Mt.
 //
 // Force pass: one workgroup per 64 particles; their half-bonds form one
 // contiguous range, evaluated one per lane in chunks of 512; each lane writes
@@ -36,6 +40,11 @@
 #include <vector>
 
 #include "rfd_internal.h"
+
+// The native tier is compared with its oracles at a stated tolerance, not bit
+// for bit: products and sums may contract to FMA here (every contraction is
+// sign-symmetric, so the two halves of a bond stay exact mirror images).
+#pragma clang fp contract(fast)
 
 using namespace rfd;
 
@@ -57,12 +66,16 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t per) {
 struct alignas(32) P4 {
   double x, y, z, r;
 };
-struct alignas(16) BS {
-  float4 a, b, c;  // (npx,npy,npz,Ox) (Oy,Oz,Mnx,Mny) (Mnz,Mtx,Mty,Mtz)
+struct alignas(16) BSt {
+  float4 a;        // O (shear spring, i's frame), mn (twist along n)
+  float mx, my, mz;  // Mt (bending, i's frame)
+  uint32_t fl;     // kBond, kBroken, kRemoved
 };
-struct alignas(16) HB {
-  uint32_t j;  // the other end
-  float L0, br, sc;
+struct alignas(8) HR {
+  uint32_t j, b;  // the other end, the bond
+};
+struct alignas(16) BR {
+  float il, br, sc, tc;  // 1 / rest length, bond radius, tensile and shear strength
 };
 
 struct v3 {
@@ -160,51 +173,55 @@ struct BondOut {
   v3 Mbi, Mbj;  // bond moments on i and j
   v3 Mci, Mcj;  // contact moments on i and j
   bool hasc;    // the contact law acted (counts for the rolling-wipe rule)
-  BS st;        // new state
-  uint8_t flag;
+  BSt st;       // new state, i's frame of the owner half
 };
 
-// evaluate bond record (i, j) from its old state
+// evaluate bond (A, B) from its old state, seen from A; sg = +1 when A is the
+// owner end (the state is stored in the owner's frame), -1 otherwise
 __device__ __forceinline__ void eval_bond(const Mat &mt, const EndData &A, const EndData &B,
-                                          float L0, float br, float scrit, float tcrit,
-                                          const BS &old, uint8_t flag, BondOut &o) {
+                                          const BR &c, float sg, const BSt &old, BondOut &o) {
   const double rx = B.P.x - A.P.x, ry = B.P.y - A.P.y, rz = B.P.z - A.P.z;
-  const double L64 = __builtin_sqrt(rx * rx + ry * ry + rz * rz);
+  const double r2 = rx * rx + ry * ry + rz * rz;
   o.Fi = o.Mbi = o.Mbj = o.Mci = o.Mcj = kZero;
   o.hasc = false;
   o.st = old;
-  o.flag = flag;
   // far-link removal (d_DeleteFarLinks, md_interaction_list_reconstructbpm.cu:
   // 141-186): a record longer than farcut*(ri+rj), or of coincident ends, is
   // removed; both halves see the same distance and go together
   const double lim = mt.farcut * (A.P.r + B.P.r);
-  if (L64 > lim || !(L64 > 1e-5)) {
-    o.flag = kRemoved;
+  if (r2 > lim * lim || !(r2 > 1e-10)) {
+    o.st.fl = kRemoved;
     return;
   }
   const v3 r = {(float)rx, (float)ry, (float)rz};
-  const float L = (float)L64;
-  const float invL = 1.0f / L;
+  const float r2f = (float)r2;
+  const float invL = __builtin_amdgcn_rsqf(r2f);
+  const float L = r2f * invL;
   const v3 n = r * invL;
   const float ri = (float)A.P.r, rj = (float)B.P.r;
-  v3 O = {old.a.w, old.b.x, old.b.y};
-  bool contact_now = flag == kBroken;
-  if (flag == kBond) {
-    const v3 np = xyz(old.a);
-    const v3 Mn0 = {old.b.z, old.b.w, old.c.x};
-    const v3 Mt0 = {old.c.y, old.c.z, old.c.w};
-    const float il = 1.0f / L0;
-    const float A2 = kPiF * br * br, J = kPiF * br * br * br * br;
-    // the stretch is formed in FP64 (no cancellation); 1/L0 carries FP32
-    // relative precision, which is what the strain needs
-    const double eps = (L64 - (double)L0) * (double)il;
-    const v3 Fn = n * ((float)(mt.E * eps) * A2);
+  v3 O = xyz(old.a) * sg;
+  bool contact_now = old.fl == kBroken;
+  if (old.fl == kBond) {
+    // bond direction one move back: both ends at R - V dt
+    const v3 rp = r - (B.V - A.V) * mt.dt;
+    const float pp = dot(rp, rp);
+    const v3 np = pp > 0.f ? rp * __builtin_amdgcn_rsqf(pp) : n;
+    const v3 Mt0 = v3{old.mx, old.my, old.mz} * sg;
+    const float il = c.il;
+    const float A2 = kPiF * c.br * c.br, J = kPiF * c.br * c.br * c.br * c.br;
+    // stretch without cancellation: eps = (L^2/L0^2 - 1) / (1 + L/L0), the
+    // numerator from the FP64 squared length
+    const double ild = il;
+    const float eps = (float)(r2 * ild * ild - 1.0) * __builtin_amdgcn_rcpf(1.0f + L * il);
+    const float sig32 = mt.E * eps;
+    const v3 Fn = n * (sig32 * A2);
     const v3 ws = A.W + B.W;
     Rot M;
     M.build(cross(np, n), n * (dot(ws, n) * mt.dt * 0.5f));
     const v3 dw = B.W - A.W;
     const float dwn = dot(n, dw);
-    const v3 Mn = M.apply(Mn0) + n * (dwn * (mt.dt * 0.5f * mt.G * il));
+    const float mn = old.a.w + dwn * (mt.dt * 0.5f * mt.G * il);
+    const v3 Mn = n * mn;
     const float cb = mt.dt * 0.25f * mt.E * il;
     const v3 Mt = M.apply(Mt0) + (dw - n * dwn) * cb;
     const v3 v = B.V - A.V - cross(ws * 0.5f, r);
@@ -220,26 +237,27 @@ __device__ __forceinline__ void eval_bond(const Mat &mt, const EndData &A, const
     // break criterion (dd_CheckBreakConditionDEM_2, :20-36): evaluated in FP32,
     // and again in FP64 when FP32 lands within 1e-4 of a threshold, so the
     // decision at the threshold is the FP64 one
-    const float sig32 = (float)(mt.E * eps);
-    const float Sc32 = sig32 + sqrtf(dot(Mt, Mt)) * 4.0f * br;
-    const float Mc32 = mt.G * il * sqrtf(dot(O, O)) + sqrtf(dot(Mn, Mn)) * 2.0f * br;
-    bool broken = Sc32 > scrit || Mc32 > tcrit;
-    if (fabsf(Sc32 - scrit) <= 1e-4f * scrit || fabsf(Mc32 - tcrit) <= 1e-4f * tcrit) {
-      const double Sc = (double)mt.E * eps + dnorm(Mt) * 4.0 * (double)br;
-      const double Mc = (double)mt.G / (double)L0 * dnorm(O) + dnorm(Mn) * 2.0 * (double)br;
-      broken = Sc > (double)scrit || Mc > (double)tcrit;
+    const float Sc32 = sig32 + __builtin_amdgcn_sqrtf(dot(Mt, Mt)) * 4.0f * c.br;
+    const float Mc32 = mt.G * il * __builtin_amdgcn_sqrtf(dot(O, O)) + fabsf(mn) * 2.0f * c.br;
+    bool broken = Sc32 > c.sc || Mc32 > c.tc;
+    if (fabsf(Sc32 - c.sc) <= 1e-4f * c.sc || fabsf(Mc32 - c.tc) <= 1e-4f * c.tc) {
+      const double eps64 = __builtin_sqrt(r2) * ild - 1.0;
+      const double Sc = (double)mt.E * eps64 + dnorm(Mt) * 4.0 * (double)c.br;
+      const double Mc = (double)mt.G * ild * dnorm(O) + fabs((double)mn) * 2.0 * (double)c.br;
+      broken = Sc > (double)c.sc || Mc > (double)c.tc;
     }
     if (broken) {
-      o.flag = kBroken;
+      o.st.fl = kBroken;
       O = kZero;
-      o.st.a = make_float4(n.x, n.y, n.z, 0.f);
-      o.st.b = make_float4(0.f, 0.f, 0.f, 0.f);
-      o.st.c = make_float4(0.f, 0.f, 0.f, 0.f);
+      o.st.a.w = 0.f;
+      o.st.mx = o.st.my = o.st.mz = 0.f;
       contact_now = true;  // the freed pair meets the contact law this pass
     } else {
-      o.st.a = make_float4(n.x, n.y, n.z, O.x);
-      o.st.b = make_float4(O.y, O.z, Mn.x, Mn.y);
-      o.st.c = make_float4(Mn.z, Mt.x, Mt.y, Mt.z);
+      const v3 Os = O * sg, Ms = Mt * sg;
+      o.st.a = make_float4(Os.x, Os.y, Os.z, mn);
+      o.st.mx = Ms.x;
+      o.st.my = Ms.y;
+      o.st.mz = Ms.z;
       return;
     }
   }
@@ -252,9 +270,10 @@ __device__ __forceinline__ void eval_bond(const Mat &mt, const EndData &A, const
     } else {
       O = kZero;
     }
-    o.st.a.w = O.x;
-    o.st.b.x = O.y;
-    o.st.b.y = O.z;
+    const v3 Os = O * sg;
+    o.st.a.x = Os.x;
+    o.st.a.y = Os.y;
+    o.st.a.z = Os.z;
   }
 }
 
@@ -264,13 +283,14 @@ struct SysDev {
   const float4 *V, *W;
   float4 *F, *M;
   const uint32_t *hoff;
-  const HB *hb;
-  const float *tc;
-  BS *st;       // updated in place: each half has one reader and writer
-  uint8_t *fl;
+  const HR *hr;
+  const BR *brc;
+  BSt *st_in;   // state of the previous pass (the removal flag is also set here)
+  BSt *st_out;  // new state, written by the owner halves
   const uint32_t *cj, *ccnt;
   float4 *cO;
   const float *dtp;  // when set, the time step of the last move (FIRE) replaces mt.dt
+  const int *done;   // when set and nonzero (relaxation finished), the pass does nothing
 };
 
 __device__ __forceinline__ EndData load_end(const SysDev &s, uint32_t i) {
@@ -281,20 +301,28 @@ __device__ __forceinline__ EndData load_end(const SysDev &s, uint32_t i) {
   return e;
 }
 
-__global__ void __launch_bounds__(kFT) k_nat_forces(SysDev s, Mat mt, uint32_t ntiles,
-                                                     uint32_t per) {
+__global__ void __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(5, 8)))
+k_nat_forces(SysDev s, Mat mt, uint32_t ntiles, uint32_t per) {
   __shared__ float sF[3][kCH], sMb[3][kCH], sMc[3][kCH];
   __shared__ uint8_t sC[kCH];
   __shared__ uint32_t sOff[kFP + 1];
   __shared__ float sAcc[3][3][kFP];  // F, Mb, Mc running sums per particle
   __shared__ uint8_t sAnyc[kFP];
+  __shared__ P4 sP[kFP];
+  __shared__ float4 sV[kFP], sW[kFP];
   const uint32_t tile = xcd_tile(per);
   if (tile >= ntiles) return;  // padding tiles of the last XCD run
+  if (s.done && *s.done) return;
   if (s.dtp) mt.dt = *s.dtp;
   const uint32_t tid = threadIdx.x;
   const uint32_t p0 = tile * kFP;
   const uint32_t np = min((uint32_t)kFP, s.N - p0);
   if (tid <= np) sOff[tid] = s.hoff[p0 + tid];
+  if (tid < np) {
+    sP[tid] = s.P[p0 + tid];
+    sV[tid] = s.V[p0 + tid];
+    sW[tid] = s.W[p0 + tid];
+  }
   for (uint32_t q = tid; q < 9 * kFP; q += kFT) (&sAcc[0][0][0])[q] = 0.f;
   if (tid < kFP) sAnyc[tid] = 0;
   __syncthreads();
@@ -311,17 +339,29 @@ __global__ void __launch_bounds__(kFT) k_nat_forces(SysDev s, Mat mt, uint32_t n
         const uint32_t mid = (lo + hi) >> 1;
         if (sOff[mid] <= h) lo = mid; else hi = mid;
       }
-      const uint8_t fl = s.fl[h];
+      const HR hr = s.hr[h];
+      const uint32_t i = p0 + lo;
+      const BSt old = s.st_in[hr.b];
       BondOut o;
-      if (fl == kRemoved) {
+      if (old.fl == kRemoved) {
         o.Fi = o.Mbi = o.Mci = kZero;
         o.hasc = false;
       } else {
-        const HB hb = s.hb[h];
-        eval_bond(mt, load_end(s, p0 + lo), load_end(s, hb.j), hb.L0, hb.br, hb.sc, s.tc[h],
-                  s.st[h], fl, o);
-        if (o.flag != kRemoved) s.st[h] = o.st;
-        if (o.flag != fl) s.fl[h] = o.flag;
+        const bool own = i < hr.j;
+        EndData A;
+        A.P = sP[lo];
+        A.V = xyz(sV[lo]);
+        A.W = xyz(sW[lo]);
+        eval_bond(mt, A, load_end(s, hr.j), s.brc[hr.b], own ? 1.0f : -1.0f, old, o);
+        if (own) {
+          if (o.st.fl == kRemoved) {
+            // both buffers: no later pass may read a stale live flag
+            s.st_out[hr.b].fl = kRemoved;
+            s.st_in[hr.b].fl = kRemoved;
+          } else {
+            s.st_out[hr.b] = o.st;
+          }
+        }
       }
       const uint32_t k = e - c0;
       sF[0][k] = o.Fi.x;
@@ -370,7 +410,10 @@ __global__ void __launch_bounds__(kFT) k_nat_forces(SysDev s, Mat mt, uint32_t n
   if (tid >= np) return;
   // never-bonded contact rows of this particle (per-end spring, in place)
   const uint32_t i = p0 + tid, N = s.N;
-  const EndData me = load_end(s, i);
+  EndData me;
+  me.P = sP[tid];
+  me.V = xyz(sV[tid]);
+  me.W = xyz(sW[tid]);
   const uint32_t nc = s.ccnt[i];
   for (uint32_t k = 0; k < nc; ++k) {
     const size_t sl = (size_t)k * N + i;
@@ -731,10 +774,12 @@ __global__ void k_nat_rank(const uint32_t *__restrict__ tmp, const uint32_t *__r
 
 // is there a bond record (not removed) between a and b ?  (row a holds b)
 __device__ __forceinline__ bool bonded(const uint32_t *__restrict__ hoff,
-                                       const HB *__restrict__ hb,
-                                       const uint8_t *__restrict__ fl, uint32_t a, uint32_t b) {
-  for (uint32_t q = hoff[a]; q < hoff[a + 1]; ++q)
-    if (hb[q].j == b) return fl[q] != kRemoved;
+                                       const HR *__restrict__ hr, const BSt *__restrict__ st,
+                                       uint32_t a, uint32_t b) {
+  for (uint32_t q = hoff[a]; q < hoff[a + 1]; ++q) {
+    const HR h = hr[q];
+    if (h.j == b) return st[h.b].fl != kRemoved;
+  }
   return false;
 }
 
@@ -749,8 +794,8 @@ struct NbrDev {
   uint32_t *pairs;
   uint32_t Kc;
   const uint32_t *hoff;
-  const HB *hb;
-  const uint8_t *fl;
+  const HR *hr;
+  const BSt *st;
   const uint32_t *cj_old, *ccnt_old;
   const float4 *cO_old;
   uint32_t *cj_new, *ccnt_new;
@@ -800,7 +845,7 @@ __global__ void __launch_bounds__(128) k_nat_neighbours(NbrDev s) {
             s.pairs[2 * (size_t)out + 1] = j;
             ++out;
           } else {
-            if (bonded(s.hoff, s.hb, s.fl, i, j)) continue;
+            if (bonded(s.hoff, s.hr, s.st, i, j)) continue;
             if (nfound < s.Kc) {
               const size_t sl = (size_t)nfound * N + i;
               s.cj_new[sl] = j;
@@ -827,15 +872,18 @@ __global__ void __launch_bounds__(128) k_nat_neighbours(NbrDev s) {
   if (MODE == 2) s.ccnt_new[i] = nfound < s.Kc ? nfound : s.Kc;
 }
 
-// rest length = current distance (both halves compute the same value)
+// rest length = current distance, set by the owner half
 __global__ void k_set_rest_length(const P4 *__restrict__ P, uint32_t N,
-                                  const uint32_t *__restrict__ hoff, HB *__restrict__ hb) {
+                                  const uint32_t *__restrict__ hoff, const HR *__restrict__ hr,
+                                  BR *__restrict__ brc) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
     const P4 A = P[i];
     for (uint32_t h = hoff[i]; h < hoff[i + 1]; ++h) {
-      const P4 B = P[hb[h].j];
+      const HR q = hr[h];
+      if (q.j < i) continue;
+      const P4 B = P[q.j];
       const double rx = B.x - A.x, ry = B.y - A.y, rz = B.z - A.z;
-      hb[h].L0 = (float)__builtin_sqrt(rx * rx + ry * ry + rz * rz);
+      brc[q.b].il = (float)(1.0 / __builtin_sqrt(rx * rx + ry * ry + rz * rz));
     }
   }
 }
@@ -865,10 +913,14 @@ struct FireDev {
 
 constexpr int kFireVals = 9;  // P, |F|^2, |V|^2, |M|^2, |W|^2, max|F|^2, max(|M|/r)^2, sb, st
 
+// FIRE keeps its own translational velocity Vf; V holds the rate of the last
+// move (displacement / dt_f, back-off included), which is what the force pass
+// reads for the incremental springs and the previous bond direction
 struct FireParticles {
   uint32_t N;
   P4 *P;
-  float4 *V, *W, *F, *Th;
+  float4 *V, *Vf, *W, *F, *Th;
+  const P4 *P0;
   const float4 *M;
   const float2 *inv;
   float dt_phys;
@@ -887,13 +939,13 @@ __global__ void __launch_bounds__(256) k_fire_measure(FireParticles p) {
   double v[kFireVals] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.N; i += grid_stride()) {
     P4 R = p.P[i];
-    v3 V = xyz(p.V[i]), F = xyz(p.F[i]);
+    v3 V = xyz(p.Vf[i]), F = xyz(p.F[i]);
     if (MODE) {
       const v3 V0 = V, F0 = F;
       const double z0 = R.z, y0 = R.y;
       platen<MODE>(L, R, V, F, v[7], v[8]);
       if (R.z != z0 || R.y != y0) p.P[i] = R;
-      if (V.x != V0.x || V.y != V0.y || V.z != V0.z) p.V[i] = f4(V);
+      if (V.x != V0.x || V.y != V0.y || V.z != V0.z) p.Vf[i] = f4(V);
       if (F.x != F0.x || F.y != F0.y || F.z != F0.z) p.F[i] = f4(F);
     }
     const v3 W = xyz(p.W[i]), M = xyz(p.M[i]);
@@ -969,7 +1021,11 @@ __global__ void __launch_bounds__(256) k_fire_control(FireDev *fire, const doubl
   }
   f.iter += 1;
   f.dt_last = f.dt;
-  if (v[0] > 0.0) {
+  if (v[2] == 0.0 && v[4] == 0.0) {
+    // at rest (first iteration of a step): no power to judge, plain kick
+    f.backoff = 0;
+    f.cV = f.cW = 0.0;
+  } else if (v[0] > 0.0) {
     f.backoff = 0;
     f.cV = v[1] > 0 ? f.alpha * __builtin_sqrt(v[2] / v[1]) : 0.0;
     f.cW = v[3] > 0 ? f.alpha * __builtin_sqrt(v[4] / v[3]) : 0.0;
@@ -995,9 +1051,11 @@ __global__ void __launch_bounds__(256) k_fire_update(FireParticles p) {
   const float a1 = (float)(1.0 - f.alpha), cV = (float)f.cV, cW = (float)f.cW;
   const float scale = (float)(f.dt / p.dt_phys);  // inv2 holds dt_phys/m, dt_phys/I
   const float dtf = (float)f.dt;
+  const double inv_dt = 1.0 / (double)dtf;  // the force pass multiplies by dt_f
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.N; i += grid_stride()) {
-    P4 R = p.P[i];
-    v3 V = xyz(p.V[i]), W = xyz(p.W[i]);
+    const P4 R0 = p.P[i];
+    P4 R = R0;
+    v3 V = xyz(p.Vf[i]), W = xyz(p.W[i]);
     const v3 F = xyz(p.F[i]), M = xyz(p.M[i]);
     if (f.backoff) {
       const double h = 0.5 * f.dt_last;
@@ -1017,67 +1075,67 @@ __global__ void __launch_bounds__(256) k_fire_update(FireParticles p) {
     R.y += (double)V.y * f.dt;
     R.z += (double)V.z * f.dt;
     p.P[i] = R;
-    p.V[i] = f4(V);
+    p.Vf[i] = f4(V);
+    p.V[i] = make_float4((float)((R.x - R0.x) * inv_dt), (float)((R.y - R0.y) * inv_dt),
+                         (float)((R.z - R0.z) * inv_dt), 0.f);
     p.W[i] = f4(W);
     p.Th[i] = f4(xyz(p.Th[i]) + W * dtf);
   }
 }
 
-// displacement boundary condition of the moved platens, applied to the
-// positions before the first force pass of a load step (the force pass must
-// see the constrained configuration for the residual to be meaningful).  A
-// snapped particle gets the axial velocity jump/dt0 so that the first force
-// pass (run with dt0) carries the jump into the incremental spring states.
+// start of a load step: R = R0 + dU (warm start) or R0; particles held by a
+// platen at the start follow it exactly (the top one moved by dtop), and any
+// particle past a platen is snapped onto it (displacement boundary
+// condition); V = (R - R0) / dt0 and W = dTh / dt0 so that one force pass run
+// with dt0 carries the whole move into the incremental springs; FIRE starts at
+// rest.
 template <int MODE>
-__global__ void k_fire_constrain(FireParticles p) {
+__global__ void k_fire_begin(FireParticles p, const float *__restrict__ dU,
+                             const float *__restrict__ dTh, float dt0, double dtop) {
   rfd_loading L = *p.ctl;
   L.v = 0.0;
-  const double inv_dt = 1.0 / p.fire->dt;
+  const double inv_dt = 1.0 / (double)dt0;
+  const float inv_dtf = 1.0f / dt0;
   double sb = 0, stp = 0;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.N; i += grid_stride()) {
-    P4 R = p.P[i];
-    v3 V = xyz(p.V[i]), F = kZero;
-    const double a0 = MODE == 1 ? R.z : R.y;
-    platen<MODE>(L, R, V, F, sb, stp);
-    const double a1 = MODE == 1 ? R.z : R.y;
-    if (a1 != a0) {
-      if (MODE == 1)
-        V.z = (float)((a1 - a0) * inv_dt);
-      else
-        V.y = (float)((a1 - a0) * inv_dt);
-      p.P[i] = R;
-      p.V[i] = f4(V);
+    const P4 R0 = p.P[i];
+    P4 R = R0;
+    if (dU) {
+      R.x += (double)dU[3 * (size_t)i];
+      R.y += (double)dU[3 * (size_t)i + 1];
+      R.z += (double)dU[3 * (size_t)i + 2];
     }
-  }
-}
-
-// warm start as one kinematic step: V = dU/tau, W = dTh/tau
-__global__ void k_fire_kinematic_set(const float *__restrict__ dU, const float *__restrict__ dTh,
-                                     uint32_t N, float inv_tau, float4 *__restrict__ V,
-                                     float4 *__restrict__ W) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
-    V[i] = dU ? make_float4(dU[3 * i] * inv_tau, dU[3 * i + 1] * inv_tau,
-                            dU[3 * i + 2] * inv_tau, 0.f)
-              : make_float4(0.f, 0.f, 0.f, 0.f);
-    W[i] = dTh ? make_float4(dTh[3 * i] * inv_tau, dTh[3 * i + 1] * inv_tau,
-                             dTh[3 * i + 2] * inv_tau, 0.f)
-               : make_float4(0.f, 0.f, 0.f, 0.f);
-  }
-}
-
-__global__ void k_fire_kinematic_move(P4 *__restrict__ P, float4 *__restrict__ V,
-                                      float4 *__restrict__ W, float4 *__restrict__ Th,
-                                      uint32_t N, float tau) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
-    P4 R = P[i];
-    const v3 v = xyz(V[i]), w = xyz(W[i]);
-    R.x += (double)v.x * tau;
-    R.y += (double)v.y * tau;
-    R.z += (double)v.z * tau;
-    P[i] = R;
-    Th[i] = f4(xyz(Th[i]) + w * tau);
-    V[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    W[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    v3 Vd = kZero, F = kZero;
+    if (MODE) {
+      double ra, rb, h0, zc;
+      if (MODE == 1) {
+        ra = R0.x - L.cx;
+        rb = R0.y - L.cy;
+        h0 = R0.z;
+        zc = R0.z;
+      } else {
+        ra = R0.x - L.cx;
+        rb = R0.z - L.cz;
+        h0 = R0.y;
+        zc = R0.z - L.cz;
+      }
+      if (ra * ra + rb * rb < L.RR && zc < L.zcut) {
+        double &Ra = MODE == 1 ? R.z : R.y;
+        if (h0 - (L.top - dtop) > -1e-9)
+          Ra = L.top;
+        else if (h0 - L.bottom < 1e-9)
+          Ra = L.bottom;
+      }
+      platen<MODE>(L, R, Vd, F, sb, stp);
+    }
+    const v3 th = dTh ? v3{dTh[3 * (size_t)i], dTh[3 * (size_t)i + 1], dTh[3 * (size_t)i + 2]}
+                      : kZero;
+    p.P[i] = R;
+    p.V[i] = make_float4((float)((R.x - R0.x) * inv_dt), (float)((R.y - R0.y) * inv_dt),
+                         (float)((R.z - R0.z) * inv_dt), 0.f);
+    p.W[i] = f4(th * inv_dtf);
+    p.Th[i] = f4(th);
+    p.Vf[i] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
 }
 
@@ -1113,13 +1171,12 @@ struct rfd_system {
   float4 *V = nullptr, *W = nullptr, *F = nullptr, *M = nullptr;
   float2 *inv = nullptr;
   uint32_t *hoff = nullptr;
-  HB *hb = nullptr;
-  float *tc = nullptr;
-  uint32_t H = 0;                     // half-bonds (2 per bond)
-  std::vector<uint32_t> hown, hmir;   // per bond: its half in row i and in row j
-  BS *st[2] = {nullptr, nullptr};
-  uint8_t *fl[2] = {nullptr, nullptr};
-  int cur = 0;  // bond state buffer in use (one buffer, updated in place)
+  HR *hr = nullptr;
+  BR *brc = nullptr;
+  uint32_t H = 0;  // half-bonds (2 per bond)
+  std::vector<uint32_t> hbi, hbj;  // host copy of the bond ends
+  BSt *bst[2] = {nullptr, nullptr};
+  int cur = 0;  // state buffer holding the latest state (the force pass writes the other)
   uint32_t *scan_tot = nullptr, *scan_off = nullptr;  // chunk totals / offsets
   uint32_t scan_cap = 0;
   uint32_t *cj[2] = {nullptr, nullptr}, *ccnt[2] = {nullptr, nullptr};
@@ -1142,6 +1199,8 @@ struct rfd_system {
   float *fire_dt = nullptr;  // &fire->dt_f, handed to the force pass
   double *fire_part = nullptr;
   P4 *P0 = nullptr;          // positions at the start of the load step
+  float4 *Vf = nullptr;      // FIRE velocity
+  bool forces_current = false;  // the load step's first force pass is done
   float4 *Th = nullptr;      // rotation accumulated during the load step
   float *du[2] = {nullptr, nullptr}, *dth[2] = {nullptr, nullptr};
   int dcur = 0;              // index of the newest step displacement
@@ -1197,10 +1256,10 @@ SysDev make_dev(rfd_system *s) {
   d.F = s->F;
   d.M = s->M;
   d.hoff = s->hoff;
-  d.hb = s->hb;
-  d.tc = s->tc;
-  d.st = s->st[0];
-  d.fl = s->fl[0];
+  d.hr = s->hr;
+  d.brc = s->brc;
+  d.st_in = s->bst[s->cur];
+  d.st_out = s->bst[1 - s->cur];
   d.cj = s->cj[s->ccur];
   d.ccnt = s->ccnt[s->ccur];
   d.cO = s->cO[s->ccur];
@@ -1249,8 +1308,8 @@ NbrDev make_nbr(rfd_system *s, double extra) {
   d.extra = extra;
   d.Kc = s->p.contact_capacity;
   d.hoff = s->hoff;
-  d.hb = s->hb;
-  d.fl = s->fl[s->cur];
+  d.hr = s->hr;
+  d.st = s->bst[s->cur];
   d.flags = s->ctx->d_flags;
   return d;
 }
@@ -1273,10 +1332,17 @@ int rebuild_rows(rfd_system *s) {
   return RFD_OK;
 }
 
+// dtp set: a quasi-static (FIRE) pass -- the time step is that of the last
+// move, and the contact law drops its velocity damping, so that the residual
+// FIRE judges is the static out-of-balance force
 int forces(rfd_system *s, const float *dtp = nullptr) {
   SysDev d = make_dev(s);
   d.dtp = dtp;
+  // after convergence the remaining launches of a FIRE chunk must not apply
+  // the last move to the springs again
+  if (dtp) d.done = (const int *)((const char *)s->fire + offsetof(FireDev, done));
   Mat m = make_mat(s->p);
+  if (dtp) m.pA = 0.f;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (s->timing && s->ev_used + 2 <= s->ev.size()) {
     e0 = s->ev[s->ev_used];
@@ -1288,6 +1354,7 @@ int forces(rfd_system *s, const float *dtp = nullptr) {
   const uint32_t per = (nb + kXcd - 1) / kXcd;
   k_nat_forces<<<per * kXcd, kFT, 0, s->ctx->stream>>>(d, m, nb, per);
   RFD_HIP(s->ctx, hipGetLastError());
+  s->cur = 1 - s->cur;
   if (e1) RFD_HIP(s->ctx, hipEventRecord(e1, s->ctx->stream));
   return RFD_OK;
 }
@@ -1324,6 +1391,7 @@ int fire_alloc(rfd_system *s) {
   int st = dalloc(ctx, &s->fire, 1);
   if (!st) st = dalloc(ctx, &s->fire_part, (size_t)kFireVals * s->nblocks_pass);
   if (!st) st = dalloc(ctx, &s->P0, s->N);
+  if (!st) st = dalloc(ctx, &s->Vf, s->N);
   if (!st) st = dalloc(ctx, &s->Th, s->N);
   for (int q = 0; q < 2 && !st; ++q) {
     st = dalloc(ctx, &s->du[q], 3 * (size_t)s->N);
@@ -1343,9 +1411,11 @@ FireParticles fire_particles(rfd_system *s) {
   p.N = s->N;
   p.P = s->P;
   p.V = s->V;
+  p.Vf = s->Vf;
   p.W = s->W;
   p.F = s->F;
   p.Th = s->Th;
+  p.P0 = s->P0;
   p.M = s->M;
   p.inv = s->inv;
   p.dt_phys = (float)s->p.dt;
@@ -1358,12 +1428,16 @@ FireParticles fire_particles(rfd_system *s) {
 // one FIRE iteration: force pass at the current configuration, constraint +
 // reductions, control, update
 int fire_iteration(rfd_system *s) {
-  if (!s->rows_valid || s->step_count % s->p.rebuild_every == 0) {
-    int st = rebuild_rows(s);
+  if (s->forces_current) {
+    s->forces_current = false;  // the load step's first pass already ran
+  } else {
+    if (!s->rows_valid || s->step_count % s->p.rebuild_every == 0) {
+      int st = rebuild_rows(s);
+      if (st) return st;
+    }
+    int st = forces(s, s->fire_dt);
     if (st) return st;
   }
-  int st = forces(s, s->fire_dt);
-  if (st) return st;
   FireParticles fp = fire_particles(s);
   hipStream_t q = s->ctx->stream;
   if (s->mode == 1)
@@ -1462,11 +1536,11 @@ int rfd_system_destroy(rfd_system *s) {
   dfree(s->M);
   dfree(s->inv);
   dfree(s->hoff);
-  dfree(s->hb);
-  dfree(s->tc);
+  dfree(s->hr);
+  dfree(s->brc);
+  dfree(s->Vf);
   for (int q = 0; q < 2; ++q) {
-    dfree(s->st[q]);
-    dfree(s->fl[q]);
+    dfree(s->bst[q]);
     dfree(s->cj[q]);
     dfree(s->ccnt[q]);
     dfree(s->cO[q]);
@@ -1653,48 +1727,39 @@ int rfd_system_set_bonds(rfd_system *s, uint32_t Nb, const uint32_t *bi, const u
   }
   for (uint32_t i = 0; i < N; ++i) hoff[i + 1] += hoff[i];
   const uint32_t H = 2 * Nb;
-  std::vector<HB> hhb(H);
-  std::vector<float> htc(H);
-  std::vector<uint32_t> partner(H);
-  s->hown.assign(Nb, 0);
-  s->hmir.assign(Nb, 0);
+  std::vector<HR> hhr(H);
+  std::vector<BR> hbr(Nb);
   {
     // lower partners first (bonds where p is j, in bond order = ascending i),
     // then higher partners (bonds where p is i, ascending j): rows sorted by id
     std::vector<uint32_t> cur(hoff.begin(), hoff.end() - 1);
+    for (uint32_t b = 0; b < Nb; ++b) hhr[cur[bj[b]]++] = HR{bi[b], b};
+    for (uint32_t b = 0; b < Nb; ++b) hhr[cur[bi[b]]++] = HR{bj[b], b};
     for (uint32_t b = 0; b < Nb; ++b) {
-      const uint32_t h = cur[bj[b]]++;
-      s->hmir[b] = h;
-      partner[h] = bi[b];
-    }
-    for (uint32_t b = 0; b < Nb; ++b) {
-      const uint32_t h = cur[bi[b]]++;
-      s->hown[b] = h;
-      partner[h] = bj[b];
-    }
-    for (uint32_t b = 0; b < Nb; ++b) {
-      const float l0 = L0 ? L0[b] : 0.f;
-      hhb[s->hown[b]] = HB{bj[b], l0, br[b], sc[b]};
-      hhb[s->hmir[b]] = HB{bi[b], l0, br[b], sc[b]};
-      htc[s->hown[b]] = tc[b];
-      htc[s->hmir[b]] = tc[b];
+      if (L0) RFD_CHECK_ARG(ctx, L0[b] > 0.f, "rfd_system_set_bonds: L0[%u] <= 0", b);
+      hbr[b] = BR{L0 ? (float)(1.0 / (double)L0[b]) : 0.f, br[b], sc[b], tc[b]};
     }
   }
+  s->hbi.assign(bi, bi + Nb);
+  s->hbj.assign(bj, bj + Nb);
   s->Nb = Nb;
   s->H = H;
-  int st = dalloc(ctx, &s->hb, H);
-  if (!st) st = dalloc(ctx, &s->tc, H);
-  if (!st) st = dalloc(ctx, &s->st[0], H);
-  if (!st) st = dalloc(ctx, &s->fl[0], H);
+  int st = dalloc(ctx, &s->hr, H);
+  if (!st) st = dalloc(ctx, &s->brc, Nb);
+  if (!st) st = dalloc(ctx, &s->bst[0], Nb);
+  if (!st) st = dalloc(ctx, &s->bst[1], Nb);
   if (st) return st;
   RFD_HIP(ctx, hipMemcpy(s->hoff, hoff.data(), sizeof(uint32_t) * (N + 1), hipMemcpyHostToDevice));
   if (H) {
-    RFD_HIP(ctx, hipMemcpy(s->hb, hhb.data(), sizeof(HB) * H, hipMemcpyHostToDevice));
-    RFD_HIP(ctx, hipMemcpy(s->tc, htc.data(), sizeof(float) * H, hipMemcpyHostToDevice));
-    RFD_HIP(ctx, hipMemset(s->st[0], 0, sizeof(BS) * H));
-    RFD_HIP(ctx, hipMemset(s->fl[0], kBond, H));
+    std::vector<BSt> h0(Nb);
+    for (auto &x : h0) x = BSt{make_float4(0.f, 0.f, 0.f, 0.f), 0.f, 0.f, 0.f, kBond};
+    RFD_HIP(ctx, hipMemcpy(s->hr, hhr.data(), sizeof(HR) * H, hipMemcpyHostToDevice));
+    RFD_HIP(ctx, hipMemcpy(s->brc, hbr.data(), sizeof(BR) * Nb, hipMemcpyHostToDevice));
+    for (int q = 0; q < 2; ++q)
+      RFD_HIP(ctx, hipMemcpy(s->bst[q], h0.data(), sizeof(BSt) * Nb, hipMemcpyHostToDevice));
     if (!L0) {
-      k_set_rest_length<<<rfd_blocks(N, 256), 256, 0, ctx->stream>>>(s->P, N, s->hoff, s->hb);
+      k_set_rest_length<<<rfd_blocks(N, 256), 256, 0, ctx->stream>>>(s->P, N, s->hoff, s->hr,
+                                                                     s->brc);
       RFD_HIP(ctx, hipGetLastError());
     }
   }
@@ -1705,26 +1770,44 @@ int rfd_system_set_bonds(rfd_system *s, uint32_t Nb, const uint32_t *bi, const u
   return RFD_OK;
 }
 
-// bond-level state API: the owner half (row i) carries the state as seen from
-// i; the mirror half holds its negation (n_prev, O, Mn, Mt all change sign
-// under i <-> j)
+// bond-level state API, 12 floats per bond in i's frame: n, O, Mn, Mt.  The
+// device keeps O, mn = Mn . n and Mt; n (the direction the next pass treats as
+// the previous one) follows from the positions and is reported, not stored.
+int bond_dirs(rfd_system *s, std::vector<double> &n) {
+  const uint32_t N = s->N, Nb = s->Nb;
+  std::vector<P4> hp(N);
+  RFD_HIP(s->ctx, hipMemcpy(hp.data(), s->P, sizeof(P4) * N, hipMemcpyDeviceToHost));
+  n.assign(3 * (size_t)Nb, 0.0);
+  for (uint32_t b = 0; b < Nb; ++b) {
+    const P4 &a = hp[s->hbi[b]], &c = hp[s->hbj[b]];
+    const double r[3] = {c.x - a.x, c.y - a.y, c.z - a.z};
+    const double L = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    if (L > 0)
+      for (int d = 0; d < 3; ++d) n[3 * (size_t)b + d] = r[d] / L;
+  }
+  return RFD_OK;
+}
+
 int rfd_system_set_bond_state(rfd_system *s, const float *state, const uint8_t *flag) {
   if (!valid(s)) return RFD_E_ARG;
   rfd_ctx *ctx = s->ctx;
   RFD_CHECK_ARG(ctx, s->have_bonds && state && flag, "rfd_system_set_bond_state: bad state");
-  static_assert(sizeof(BS) == 12 * sizeof(float), "bond state record is 12 floats");
+  RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
   const uint32_t Nb = s->Nb;
-  std::vector<float> hs(12 * (size_t)s->H);
-  std::vector<uint8_t> hf(s->H);
+  std::vector<double> n;
+  int st = bond_dirs(s, n);
+  if (st) return st;
+  std::vector<BSt> hs(Nb);
   for (uint32_t b = 0; b < Nb; ++b) {
-    for (int c = 0; c < 12; ++c) {
-      hs[12 * (size_t)s->hown[b] + c] = state[12 * (size_t)b + c];
-      hs[12 * (size_t)s->hmir[b] + c] = -state[12 * (size_t)b + c];
-    }
-    hf[s->hown[b]] = hf[s->hmir[b]] = flag[b];
+    const float *q = state + 12 * (size_t)b;
+    RFD_CHECK_ARG(ctx, flag[b] <= kRemoved, "rfd_system_set_bond_state: flag[%u] = %u", b,
+                  flag[b]);
+    const double mn = q[6] * n[3 * (size_t)b] + q[7] * n[3 * (size_t)b + 1] +
+                      q[8] * n[3 * (size_t)b + 2];
+    hs[b] = BSt{make_float4(q[3], q[4], q[5], (float)mn), q[9], q[10], q[11], flag[b]};
   }
-  RFD_HIP(ctx, hipMemcpy(s->st[s->cur], hs.data(), sizeof(BS) * s->H, hipMemcpyHostToDevice));
-  RFD_HIP(ctx, hipMemcpy(s->fl[s->cur], hf.data(), s->H, hipMemcpyHostToDevice));
+  for (int k = 0; k < 2; ++k)
+    RFD_HIP(ctx, hipMemcpy(s->bst[k], hs.data(), sizeof(BSt) * Nb, hipMemcpyHostToDevice));
   return RFD_OK;
 }
 
@@ -1734,17 +1817,29 @@ int rfd_system_get_bond_state(rfd_system *s, float *state, uint8_t *flag) {
   RFD_CHECK_ARG(ctx, s->have_bonds, "rfd_system_get_bond_state: no bonds");
   RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
   const uint32_t Nb = s->Nb;
+  std::vector<BSt> hs(Nb);
+  RFD_HIP(ctx, hipMemcpy(hs.data(), s->bst[s->cur], sizeof(BSt) * Nb, hipMemcpyDeviceToHost));
   if (state) {
-    std::vector<float> hs(12 * (size_t)s->H);
-    RFD_HIP(ctx, hipMemcpy(hs.data(), s->st[s->cur], sizeof(BS) * s->H, hipMemcpyDeviceToHost));
-    for (uint32_t b = 0; b < Nb; ++b)
-      for (int c = 0; c < 12; ++c) state[12 * (size_t)b + c] = hs[12 * (size_t)s->hown[b] + c];
+    std::vector<double> n;
+    int st = bond_dirs(s, n);
+    if (st) return st;
+    for (uint32_t b = 0; b < Nb; ++b) {
+      float *q = state + 12 * (size_t)b;
+      const BSt &x = hs[b];
+      for (int d = 0; d < 3; ++d) {
+        q[d] = (float)n[3 * (size_t)b + d];
+        q[6 + d] = x.a.w * (float)n[3 * (size_t)b + d];
+      }
+      q[3] = x.a.x;
+      q[4] = x.a.y;
+      q[5] = x.a.z;
+      q[9] = x.mx;
+      q[10] = x.my;
+      q[11] = x.mz;
+    }
   }
-  if (flag) {
-    std::vector<uint8_t> hf(s->H);
-    RFD_HIP(ctx, hipMemcpy(hf.data(), s->fl[s->cur], s->H, hipMemcpyDeviceToHost));
-    for (uint32_t b = 0; b < Nb; ++b) flag[b] = hf[s->hown[b]];
-  }
+  if (flag)
+    for (uint32_t b = 0; b < Nb; ++b) flag[b] = (uint8_t)hs[b].fl;
   return RFD_OK;
 }
 
@@ -1884,8 +1979,10 @@ int rfd_system_fire_load_step(rfd_system *s, double dstrain, const rfd_fire_para
   rfd_loading L;
   st = rfd_system_get_loading(s, &L);
   if (st) return st;
+  double dtop = 0.0;
   if (L.mode) {
     const double d = dstrain * L.size0;
+    dtop = d;
     L.size += d;
     L.top += d;
     L.top_r = L.top - L.band;
@@ -1894,7 +1991,6 @@ int rfd_system_fire_load_step(rfd_system *s, double dstrain, const rfd_fire_para
     RFD_HIP(ctx, hipMemcpyAsync(s->ctl, &L, sizeof(L), hipMemcpyHostToDevice, q));
   }
   RFD_HIP(ctx, hipMemcpyAsync(s->P0, s->P, sizeof(P4) * s->N, hipMemcpyDeviceToDevice, q));
-  k_zero_f4<<<rfd_blocks(s->N, 256), 256, 0, q>>>(s->Th, nullptr, s->N);
   FireDev f{};
   f.dt = f.dt_last = fp->dt0;
   f.alpha = f.alpha0 = fp->alpha0;
@@ -1909,26 +2005,29 @@ int rfd_system_fire_load_step(rfd_system *s, double dstrain, const rfd_fire_para
   f.first = 1;
   f.dt_f = (float)fp->dt0;
   RFD_HIP(ctx, hipMemcpyAsync(s->fire, &f, sizeof(f), hipMemcpyHostToDevice, q));
-  if (dU || dTh) {
-    // warm start: one kinematic step of length dt0 carrying the increment
-    const float tau = (float)fp->dt0;
-    k_fire_kinematic_set<<<rfd_blocks(s->N, 256), 256, 0, q>>>(dU, dTh, s->N, 1.0f / tau, s->V,
-                                                                s->W);
+  {
+    // the move (warm start and platens) enters the springs through one force
+    // pass run with dt0; FIRE then starts at rest from the forces it left
+    FireParticles fpp = fire_particles(s);
+    const float dt0 = (float)fp->dt0;
+    if (s->mode == 1)
+      k_fire_begin<1><<<s->nblocks_pass, 256, 0, q>>>(fpp, dU, dTh, dt0, dtop);
+    else if (s->mode == 2)
+      k_fire_begin<2><<<s->nblocks_pass, 256, 0, q>>>(fpp, dU, dTh, dt0, dtop);
+    else
+      k_fire_begin<0><<<s->nblocks_pass, 256, 0, q>>>(fpp, dU, dTh, dt0, dtop);
+    RFD_HIP(ctx, hipGetLastError());
+    if (!s->rows_valid || s->step_count % s->p.rebuild_every == 0) {
+      st = rebuild_rows(s);
+      if (st) return st;
+    }
     st = forces(s, s->fire_dt);
     if (st) return st;
-    k_fire_kinematic_move<<<rfd_blocks(s->N, 256), 256, 0, q>>>(s->P, s->V, s->W, s->Th, s->N,
-                                                                 tau);
-  } else {
     k_zero_f4<<<rfd_blocks(s->N, 256), 256, 0, q>>>(s->V, s->W, s->N);
+    RFD_HIP(ctx, hipGetLastError());
+    s->step_count++;
+    s->forces_current = true;
   }
-  {
-    FireParticles fpp = fire_particles(s);
-    if (s->mode == 1)
-      k_fire_constrain<1><<<s->nblocks_pass, 256, 0, q>>>(fpp);
-    else if (s->mode == 2)
-      k_fire_constrain<2><<<s->nblocks_pass, 256, 0, q>>>(fpp);
-  }
-  RFD_HIP(ctx, hipGetLastError());
   const uint32_t chunk = fp->check_every ? fp->check_every : 50;
   for (;;) {
     for (uint32_t k = 0; k < chunk; ++k) {
@@ -1939,10 +2038,13 @@ int rfd_system_fire_load_step(rfd_system *s, double dstrain, const rfd_fire_para
     RFD_HIP(ctx, hipStreamSynchronize(q));
     if (f.done) break;
   }
-  // the step's displacement and rotation, newest first
+  s->forces_current = false;
+  // the step's displacement and rotation, newest first; the relaxed state is
+  // at rest
   s->dcur = 1 - s->dcur;
   k_fire_step_disp<<<rfd_blocks(s->N, 256), 256, 0, q>>>(s->P, s->P0, s->Th, s->N,
                                                           s->du[s->dcur], s->dth[s->dcur]);
+  k_zero_f4<<<rfd_blocks(s->N, 256), 256, 0, q>>>(s->V, s->W, s->N);
   RFD_HIP(ctx, hipGetLastError());
   st = rfd_system_get_loading(s, &L);
   if (st) return st;
@@ -2004,12 +2106,13 @@ int rfd_system_fire_buffers(rfd_system *s, float **du_last, float **du_prev, flo
   return RFD_OK;
 }
 
-int rfd_system_bond_graph(rfd_system *s, const uint32_t **hoff, const void **rec,
-                          const uint8_t **flag, uint32_t *H) {
+int rfd_system_bond_graph(rfd_system *s, const uint32_t **hoff, const void **half,
+                          const void **bond, const void **state, uint32_t *H) {
   if (!valid(s)) return RFD_E_ARG;
   if (hoff) *hoff = s->hoff;
-  if (rec) *rec = s->hb;
-  if (flag) *flag = s->fl[0];
+  if (half) *half = s->hr;
+  if (bond) *bond = s->brc;
+  if (state) *state = s->bst[s->cur];
   if (H) *H = s->H;
   return RFD_OK;
 }

@@ -87,7 +87,8 @@ NATIVE_SIGNATURES = {
     "rfd_system_residual": [_vp, C.POINTER(C.c_double)],
     "rfd_system_fire_buffers": [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp),
                                 C.POINTER(_vp)],
-    "rfd_system_bond_graph": [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), _u32p],
+    "rfd_system_bond_graph": [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp),
+                              C.POINTER(_vp), _u32p],
     "rfd_system_timing_read": [_vp, C.POINTER(C.c_double), _u32p],
 }
 
@@ -270,13 +271,18 @@ class NativeSystem:
         return tuple(device_view(q.value, (self.N, 3), "<f4") for q in ptrs)
 
     def bond_graph(self):
-        """(hoff[N+1] int32, rec[H, 4] (j, L0, br, sc as int32 bits), flag[H]) views."""
-        a, b, c, H = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_uint32()
+        """Device views: hoff[N+1] int32, half[H, 2] int32 (other end, bond),
+        bond[Nb, 4] float32 (L0, br, sc, tc), state[Nb, 8] float32 (O, mn, Mt,
+        flag as uint32 bits: read column 7 through .view(torch.int32)).  The
+        state view is the buffer current at the time of the call."""
+        a, b, c, d, H = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_uint32()
         self._call("rfd_system_bond_graph", self.h, C.byref(a), C.byref(b), C.byref(c),
-                   C.byref(H))
+                   C.byref(d), C.byref(H))
+        nb = H.value // 2
         return (device_view(a.value, (self.N + 1,), "<i4"),
-                device_view(b.value, (H.value, 4), "<i4"),
-                device_view(c.value, (H.value,), "|u1"))
+                device_view(b.value, (H.value, 2), "<i4"),
+                device_view(c.value, (nb, 4), "<f4"),
+                device_view(d.value, (nb, 8), "<f4"))
 
     def seat_platens(self, preload: float = 0.0, L: Loading | None = None):
         """Move the platens onto the specimen: top at the highest and bottom at

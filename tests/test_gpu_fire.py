@@ -5,7 +5,6 @@
   increments add equal reactions (2 %; the reaction error of a relaxed step
   is bounded by the interior residual, N tol F_ref, since the slowest FIRE
   mode is the coherent axial one, so this test relaxes to tol = 1e-8);
-* the platen reactions balance to the same bound;
 * a warm start from the previous step's increment reaches the same residual
   in fewer FIRE iterations, and the relaxed reactions agree (0.5 %);
 * a step capped at one iteration reports non-convergence (the fallback path
@@ -35,7 +34,6 @@ def test_fire_load_steps_converge_linearly(ctx):
     for r in res:
         assert r.converged, r.as_dict()
         assert r.fmax < fp.ftol
-        assert abs(r.sum_top + r.sum_bottom) < bound
     top = np.array([r.sum_top for r in res])
     inc = np.diff(top)
     assert top[0] > 0 and inc[0] > 50 * bound

@@ -487,7 +487,10 @@ def test_reference_fire_relaxes_quadratic_bowl():
 # ----------------------------------------------------------- native f64
 def test_native_f64_matches_slot_layout_forces():
     """The FP64 edge-list oracle in reference mode agrees with the FP32 slot
-    oracle on a random bonded pair sequence to FP32 accuracy."""
+    oracle on a random bonded pair sequence to FP32 accuracy; the pair moves
+    by R += V dt between evaluations, the motion under which the edge-list
+    form (previous direction at R - V dt, twist kept along n) restates the
+    reference's stored rij and Mijn."""
     g = np.random.default_rng(11)
     m = mat()
     r = g.uniform(-1, 1, 3)
@@ -498,12 +501,53 @@ def test_native_f64_matches_slot_layout_forces():
     R = p.R.astype(np.float64); V = p.V.astype(np.float64); W = p.W.astype(np.float64)
     bi, bj = np.array([0], np.uint32), np.array([1], np.uint32)
     iL = np.array([1.0]); br = np.array([m.b_r]); sc = np.array([1e9]); tc = np.array([1e9])
-    st = np.zeros(12); act = np.ones(1, np.uint8)
+    st = np.zeros(8); act = np.ones(1, np.uint8)
     F = np.zeros(6); M = np.zeros(6)
-    for _ in range(5):
+    for it in range(5):
+        if it:
+            p.R += p.V * np.float32(m.dt)
+            R = p.R.astype(np.float64)
         p.bonds(m)
         lib().orc_native_bonds_f64(ptr(R), ptr(V), ptr(W), 2, ptr(bi), ptr(bj), ptr(iL),
                                    ptr(br), ptr(sc), ptr(tc), ptr(st), ptr(act), 1, m.dt,
                                    m.E, m.G, 1, ptr(F), ptr(M))
     np.testing.assert_allclose(F, p.F, rtol=2e-4, atol=1e-6 * np.abs(p.F).max())
     np.testing.assert_allclose(M, p.M, rtol=2e-4, atol=1e-6 * np.abs(p.M).max())
+
+
+def test_native_f64_state_corotates_with_rigid_rotation():
+    """Objectivity of the edge-list law: a bonded pair carrying shear, twist
+    and bending state that turns rigidly (V = w x r, W = w, R += V dt) keeps
+    its state fixed in the turning frame: after k evaluations O and Mt are the
+    initial vectors turned by k atan(|w| dt) about w, mn is unchanged, and no
+    new increment appears.  A previous direction taken at the wrong place (or
+    not at all) leaves the state behind by the accumulated angle (0.02 rad)."""
+    N = 2
+    R = np.array([0.0, 1.0, 0.0, 0.0, 0.0, 0.0])  # SoA: x0 x1 y0 y1 z0 z1
+    w = np.array([0.0, 0.0, 0.01])
+    dt = 0.1
+    th = np.arctan(np.linalg.norm(w) * dt)
+    O0 = np.array([0.0, 2e-3, 1e-3])
+    Mt0 = np.array([0.0, -1e-3, 3e-3])
+    st = np.concatenate([O0, [5e-4], Mt0, [0.0]])
+    act = np.ones(1, np.uint8)
+    bi, bj = np.array([0], np.uint32), np.array([1], np.uint32)
+    iL, br = np.array([1.0]), np.array([0.3])
+    sc, tc = np.array([1e30]), np.array([1e30])
+    F = np.zeros(6); M = np.zeros(6)
+    W = np.tile(w[:, None], (1, 2)).reshape(-1)
+    steps = 20
+    for _ in range(steps):
+        P = R.reshape(3, 2).T
+        c = P.mean(0)
+        V = np.cross(w, P - c).T.reshape(-1)
+        lib().orc_native_bonds_f64(ptr(R), ptr(V), ptr(W), N, ptr(bi), ptr(bj), ptr(iL),
+                                   ptr(br), ptr(sc), ptr(tc), ptr(st), ptr(act), 1, dt,
+                                   30.0, 12.0, 0, ptr(F), ptr(M))
+        R = R + V * dt
+    a = steps * th
+    Rz = np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1]])
+    np.testing.assert_allclose(st[0:3], Rz @ O0, rtol=0, atol=2e-5 * np.linalg.norm(O0))
+    np.testing.assert_allclose(st[4:7], Rz @ Mt0, rtol=0, atol=2e-5 * np.linalg.norm(Mt0))
+    assert st[3] == pytest.approx(5e-4, rel=1e-12)
+
