@@ -53,6 +53,8 @@ namespace {
 constexpr int kFP = 64;    // particles per force workgroup
 constexpr int kFT = 256;   // threads per force workgroup
 constexpr int kCH = 512;   // bond ends per LDS chunk
+// the row offsets of a tile (kFP + 1 of them) are loaded one per thread
+static_assert(kFP < kFT, "a force workgroup needs more threads than particles");
 constexpr uint8_t kBond = 1, kBroken = 0, kRemoved = 2;
 constexpr int kHistCap = 1 << 20;
 // gfx950 places workgroup b on XCD b % 8; tiles are remapped so that each XCD
@@ -287,6 +289,7 @@ struct SysDev {
   const BR *brc;
   BSt *st_in;   // state of the previous pass (the removal flag is also set here)
   BSt *st_out;  // new state, written by the owner halves
+  uint8_t *fl8; // per-bond flag copy, written by the owner half when it changes
   const uint32_t *cj, *ccnt;
   float4 *cO;
   const float *dtp;  // when set, the time step of the last move (FIRE) replaces mt.dt
@@ -361,6 +364,7 @@ k_nat_forces(SysDev s, Mat mt, uint32_t ntiles, uint32_t per) {
           } else {
             s.st_out[hr.b] = o.st;
           }
+          if (o.st.fl != old.fl) s.fl8[hr.b] = (uint8_t)o.st.fl;
         }
       }
       const uint32_t k = e - c0;
@@ -758,8 +762,11 @@ __global__ void k_nat_scatter(const uint32_t *__restrict__ CI, uint32_t N,
 }
 
 // stable order inside each cell: rank members by id (one wave per cell)
+// also writes the members' positions relative to the grid origin (FP32) and
+// radii in cell order, for the contact-row search
 __global__ void k_nat_rank(const uint32_t *__restrict__ tmp, const uint32_t *__restrict__ start,
-                           uint32_t C, uint32_t *__restrict__ sorted) {
+                           uint32_t C, uint32_t *__restrict__ sorted, const P4 *__restrict__ P,
+                           GridDev g, float4 *__restrict__ cpos) {
   const uint32_t lane = threadIdx.x & 63, waves = blockDim.x >> 6;
   for (uint32_t c = blockIdx.x * waves + (threadIdx.x >> 6); c < C; c += gridDim.x * waves) {
     const uint32_t s0 = start[c], n = start[c + 1] - s0;
@@ -768,17 +775,20 @@ __global__ void k_nat_rank(const uint32_t *__restrict__ tmp, const uint32_t *__r
       uint32_t rank = 0;
       for (uint32_t t = 0; t < n; ++t) rank += (tmp[s0 + t] < id) ? 1u : 0u;
       sorted[s0 + rank] = id;
+      const P4 R = P[id];
+      cpos[s0 + rank] = make_float4((float)(R.x - g.ox), (float)(R.y - g.oy),
+                                    (float)(R.z - g.oz), (float)R.r);
     }
   }
 }
 
 // is there a bond record (not removed) between a and b ?  (row a holds b)
 __device__ __forceinline__ bool bonded(const uint32_t *__restrict__ hoff,
-                                       const HR *__restrict__ hr, const BSt *__restrict__ st,
+                                       const HR *__restrict__ hr, const uint8_t *__restrict__ fl8,
                                        uint32_t a, uint32_t b) {
   for (uint32_t q = hoff[a]; q < hoff[a + 1]; ++q) {
     const HR h = hr[q];
-    if (h.j == b) return st[h.b].fl != kRemoved;
+    if (h.j == b) return fl8[h.b] != kRemoved;
   }
   return false;
 }
@@ -788,6 +798,7 @@ struct NbrDev {
   const P4 *P;
   GridDev g;
   const uint32_t *start, *sorted;
+  const float4 *cpos;  // cell-ordered positions relative to the grid origin, radius
   double extra;
   uint32_t *count;
   const uint32_t *pair_off;
@@ -795,7 +806,7 @@ struct NbrDev {
   uint32_t Kc;
   const uint32_t *hoff;
   const HR *hr;
-  const BSt *st;
+  const uint8_t *fl8;
   const uint32_t *cj_old, *ccnt_old;
   const float4 *cO_old;
   uint32_t *cj_new, *ccnt_new;
@@ -808,13 +819,19 @@ struct NbrDev {
 // MODE 2: rebuild the contact row (never-bonded j within ri + rj + extra),
 // carrying each spring O over from the previous row.
 template <int MODE>
-__global__ void __launch_bounds__(128) k_nat_neighbours(NbrDev s) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(128) k_nat_neighbours(NbrDev s, uint32_t per) {
+  // XCD-aware: each XCD scans one contiguous run of Morton-ordered particles,
+  // so the cells around a run are fetched into one L2 rather than into all
+  const uint32_t i = xcd_tile(per) * 128 + threadIdx.x;
   if (i >= s.N) return;
   const uint32_t N = s.N;
   const P4 Ri = s.P[i];
   int c[3];
   cell_clamped(s.g, Ri, c);
+  // MODE 2 tests candidates in FP32 relative coordinates from the cell-ordered
+  // copy (contiguous per cell); its skin absorbs the rounding
+  const float pix = (float)(Ri.x - s.g.ox), piy = (float)(Ri.y - s.g.oy),
+              piz = (float)(Ri.z - s.g.oz), rif = (float)Ri.r, exf = (float)s.extra;
   uint32_t nfound = 0;
   uint32_t out = MODE == 1 ? s.pair_off[i] : 0;
   for (int dz = -1; dz <= 1; ++dz) {
@@ -829,15 +846,25 @@ __global__ void __launch_bounds__(128) k_nat_neighbours(NbrDev s) {
         const uint32_t cc = (uint32_t)x + (uint32_t)y * s.g.nx + (uint32_t)z * s.g.nx * s.g.ny;
         const uint32_t e1 = s.start[cc + 1];
         for (uint32_t e = s.start[cc]; e < e1; ++e) {
-          const uint32_t j = s.sorted[e];
-          if (j == i) continue;
-          if (MODE != 2 && j < i) continue;
-          const P4 Rj = s.P[j];
-          const double rx = Rj.x - Ri.x, ry = Rj.y - Ri.y, rz = Rj.z - Ri.z;
-          const double d2 = rx * rx + ry * ry + rz * rz;
-          if (d2 < 1e-10) continue;
-          const double lim = Ri.r + Rj.r + s.extra;
-          if (d2 >= lim * lim) continue;
+          uint32_t j;
+          if (MODE == 2) {
+            const float4 q = s.cpos[e];
+            const float dx = q.x - pix, dy2 = q.y - piy, dz2 = q.z - piz;
+            const float d2 = dx * dx + dy2 * dy2 + dz2 * dz2;
+            const float lim = rif + q.w + exf;
+            if (!(d2 >= 1e-10f) || d2 >= lim * lim) continue;
+            j = s.sorted[e];
+            if (j == i) continue;
+          } else {
+            j = s.sorted[e];
+            if (j <= i) continue;
+            const P4 Rj = s.P[j];
+            const double rx = Rj.x - Ri.x, ry = Rj.y - Ri.y, rz = Rj.z - Ri.z;
+            const double d2 = rx * rx + ry * ry + rz * rz;
+            if (d2 < 1e-10) continue;
+            const double lim = Ri.r + Rj.r + s.extra;
+            if (d2 >= lim * lim) continue;
+          }
           if (MODE == 0) {
             ++nfound;
           } else if (MODE == 1) {
@@ -845,7 +872,7 @@ __global__ void __launch_bounds__(128) k_nat_neighbours(NbrDev s) {
             s.pairs[2 * (size_t)out + 1] = j;
             ++out;
           } else {
-            if (bonded(s.hoff, s.hr, s.st, i, j)) continue;
+            if (bonded(s.hoff, s.hr, s.fl8, i, j)) continue;
             if (nfound < s.Kc) {
               const size_t sl = (size_t)nfound * N + i;
               s.cj_new[sl] = j;
@@ -1186,6 +1213,8 @@ struct rfd_system {
   uint32_t C = 0;
   uint32_t *CI = nullptr, *cstart = nullptr, *ccursor = nullptr, *ctmp = nullptr,
            *csorted = nullptr, *ccnt_cells = nullptr;
+  float4 *cpos = nullptr;    // cell-ordered relative positions (contact search)
+  uint8_t *fl8 = nullptr;    // per-bond flag, kept beside the state (contact search)
   rfd_loading *ctl = nullptr;
   double *part = nullptr;
   double *hist = nullptr;
@@ -1260,6 +1289,7 @@ SysDev make_dev(rfd_system *s) {
   d.brc = s->brc;
   d.st_in = s->bst[s->cur];
   d.st_out = s->bst[1 - s->cur];
+  d.fl8 = s->fl8;
   d.cj = s->cj[s->ccur];
   d.ccnt = s->ccnt[s->ccur];
   d.cO = s->cO[s->ccur];
@@ -1293,7 +1323,8 @@ int bin_particles(rfd_system *s) {
     if (e) return e;
   }
   k_nat_scatter<<<rfd_blocks(s->N, 256), 256, 0, st>>>(s->CI, s->N, s->ccursor, s->ctmp);
-  k_nat_rank<<<rfd_blocks(s->C, 4), 256, 0, st>>>(s->ctmp, s->cstart, s->C, s->csorted);
+  k_nat_rank<<<rfd_blocks(s->C, 4), 256, 0, st>>>(s->ctmp, s->cstart, s->C, s->csorted, s->P,
+                                                  s->g, s->cpos);
   RFD_HIP(s->ctx, hipGetLastError());
   return RFD_OK;
 }
@@ -1305,14 +1336,19 @@ NbrDev make_nbr(rfd_system *s, double extra) {
   d.g = s->g;
   d.start = s->cstart;
   d.sorted = s->csorted;
+  d.cpos = s->cpos;
   d.extra = extra;
   d.Kc = s->p.contact_capacity;
   d.hoff = s->hoff;
   d.hr = s->hr;
-  d.st = s->bst[s->cur];
+  d.fl8 = s->fl8;
   d.flags = s->ctx->d_flags;
   return d;
 }
+
+// neighbour-search launch: ceil(N/128) tiles spread over kXcd runs
+uint32_t nbr_per(uint32_t N) { return ((N + 127) / 128 + kXcd - 1) / kXcd; }
+uint32_t nbr_grid(uint32_t N) { return nbr_per(N) * kXcd; }
 
 int rebuild_rows(rfd_system *s) {
   int st = bin_particles(s);
@@ -1325,7 +1361,7 @@ int rebuild_rows(rfd_system *s) {
   d.cj_new = s->cj[n];
   d.ccnt_new = s->ccnt[n];
   d.cO_new = s->cO[n];
-  k_nat_neighbours<2><<<rfd_blocks(s->N, 128), 128, 0, s->ctx->stream>>>(d);
+  k_nat_neighbours<2><<<nbr_grid(s->N), 128, 0, s->ctx->stream>>>(d, nbr_per(s->N));
   RFD_HIP(s->ctx, hipGetLastError());
   s->ccur = n;
   s->rows_valid = true;
@@ -1507,6 +1543,7 @@ int rfd_system_create(rfd_ctx *ctx, const rfd_system_params *p, uint32_t N, rfd_
   if (!st) st = dalloc(ctx, &s->CI, N);
   if (!st) st = dalloc(ctx, &s->ctmp, N);
   if (!st) st = dalloc(ctx, &s->csorted, N);
+  if (!st) st = dalloc(ctx, &s->cpos, N);
   if (!st) st = dalloc(ctx, &s->ctl, 1);
   s->nblocks_pass = rfd_blocks(N, 256, 2048);
   if (!st) st = dalloc(ctx, &s->part, 2 * (size_t)s->nblocks_pass);
@@ -1550,6 +1587,8 @@ int rfd_system_destroy(rfd_system *s) {
   dfree(s->ccursor);
   dfree(s->ctmp);
   dfree(s->csorted);
+  dfree(s->cpos);
+  dfree(s->fl8);
   dfree(s->ccnt_cells);
   dfree(s->ctl);
   dfree(s->part);
@@ -1672,7 +1711,7 @@ int rfd_system_find_pairs(rfd_system *s, double gap, uint32_t *pairs_host, uint3
   }
   NbrDev d = make_nbr(s, gap);
   d.count = cnt;
-  k_nat_neighbours<0><<<rfd_blocks(s->N, 128), 128, 0, ctx->stream>>>(d);
+  k_nat_neighbours<0><<<nbr_grid(s->N), 128, 0, ctx->stream>>>(d, nbr_per(s->N));
   st = scan_u32(s, cnt, s->N, off, nullptr);
   if (st) {
     dfree(cnt);
@@ -1694,7 +1733,7 @@ int rfd_system_find_pairs(rfd_system *s, double gap, uint32_t *pairs_host, uint3
     if (!st) {
       d.pair_off = off;
       d.pairs = pairs;
-      k_nat_neighbours<1><<<rfd_blocks(s->N, 128), 128, 0, ctx->stream>>>(d);
+      k_nat_neighbours<1><<<nbr_grid(s->N), 128, 0, ctx->stream>>>(d, nbr_per(s->N));
       e = hipMemcpyAsync(pairs_host, pairs, sizeof(uint32_t) * 2 * total,
                          hipMemcpyDeviceToHost, ctx->stream);
       if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -1748,6 +1787,7 @@ int rfd_system_set_bonds(rfd_system *s, uint32_t Nb, const uint32_t *bi, const u
   if (!st) st = dalloc(ctx, &s->brc, Nb);
   if (!st) st = dalloc(ctx, &s->bst[0], Nb);
   if (!st) st = dalloc(ctx, &s->bst[1], Nb);
+  if (!st) st = dalloc(ctx, &s->fl8, Nb);
   if (st) return st;
   RFD_HIP(ctx, hipMemcpy(s->hoff, hoff.data(), sizeof(uint32_t) * (N + 1), hipMemcpyHostToDevice));
   if (H) {
@@ -1757,6 +1797,7 @@ int rfd_system_set_bonds(rfd_system *s, uint32_t Nb, const uint32_t *bi, const u
     RFD_HIP(ctx, hipMemcpy(s->brc, hbr.data(), sizeof(BR) * Nb, hipMemcpyHostToDevice));
     for (int q = 0; q < 2; ++q)
       RFD_HIP(ctx, hipMemcpy(s->bst[q], h0.data(), sizeof(BSt) * Nb, hipMemcpyHostToDevice));
+    RFD_HIP(ctx, hipMemset(s->fl8, kBond, Nb));
     if (!L0) {
       k_set_rest_length<<<rfd_blocks(N, 256), 256, 0, ctx->stream>>>(s->P, N, s->hoff, s->hr,
                                                                      s->brc);
@@ -1808,6 +1849,7 @@ int rfd_system_set_bond_state(rfd_system *s, const float *state, const uint8_t *
   }
   for (int k = 0; k < 2; ++k)
     RFD_HIP(ctx, hipMemcpy(s->bst[k], hs.data(), sizeof(BSt) * Nb, hipMemcpyHostToDevice));
+  RFD_HIP(ctx, hipMemcpy(s->fl8, flag, Nb, hipMemcpyHostToDevice));
   return RFD_OK;
 }
 
