@@ -150,6 +150,14 @@ class OracleLayout:
                                  ptr(self.M), self.N, self.K, m.dt, m.hm_E, m.hm_G, m.p_A,
                                  m.p_mu, m.p_mur, m.p_m, m.p_r)
 
+    def sumup_forces(self):
+        lib().orc_sumup_forces(*self._s("Fijn", "Fijt", "Mijn", "Mijt", "Mijadd"), ptr(self.F),
+                               ptr(self.M), self.N, self.K)
+
+    def check_break(self, sc, tc, b_r):
+        lib().orc_check_break(*self._s("rij", "Oijt", "Fijn", "Fijt", "Mijn", "Mijt"),
+                              ptr(self.ILtype), self.N * self.K, sc, tc, b_r)
+
     def increments(self, m):
         lib().orc_increments(ptr(self.F), ptr(self.V), ptr(self.R), ptr(self.M), ptr(self.W),
                              self.N, m.dt_d_m, m.dt, m.dt_d_I)

@@ -149,6 +149,17 @@ class RefLayout:
                       _p(self.ILtype), _p(self.slot["Oijt"]), _p(self.F), _p(self.M), self.N,
                       self.K, m.dt, m.hm_E, m.hm_G, m.p_A, m.p_mu, m.p_mur, m.p_m, m.p_r)
 
+    def sumup_forces(self):
+        """Stand-alone sum-up (d_SumUpForcesDEM): F, M from the slot loads."""
+        self.ctx.call("rfd_sumup_forces", *self._slots("Fijn", "Fijt", "Mijn", "Mijt", "Mijadd"),
+                      _p(self.F), _p(self.M), self.N, self.K)
+
+    def check_break(self, sc: float, tc: float, b_r: float):
+        """Stand-alone break check (d_CheckBreakConditionDEM) over all slots."""
+        self.ctx.call("rfd_check_break", *self._slots("rij", "Oijt", "Fijn", "Fijt", "Mijn",
+                                                      "Mijt"), _p(self.ILtype),
+                      self.N * self.K, sc, tc, b_r)
+
     def increments(self, m: Material):
         self.ctx.call("rfd_increments", _p(self.F), _p(self.V), _p(self.R), _p(self.M),
                       _p(self.W), self.N, m.dt_d_m, m.dt, m.dt_d_I)

@@ -192,3 +192,71 @@ def test_cell_range_flag(ctx):
     g.cell_distribution()
     torch.cuda.synchronize()
     assert ctx.flags() & FLAG_CELL_RANGE
+
+
+def test_standalone_sumup_and_break_bitexact(ctx):
+    """The stand-alone sum-up (md_sumupforcesdem.cu:19) and break check
+    (md_checkbreakconditiondem.cu:20) on the slot arrays of a bonded specimen
+    with random slot loads: F, M, ILtype and Oijt bit for bit."""
+    import torch
+
+    case = cases.reference_uniaxial(6.0, 8.0, seed=5)
+    g, o = twin(ctx, case)
+    m = case.mat
+    for s in (g, o):
+        s.cell_distribution()
+        s.il_construct(float(np.float32(m.a_farcut) ** 2))
+        s.bonds_construct(m.b_dd)
+    assert_same(g.host(), o.host(), where="setup")
+    rng = np.random.default_rng(17)
+    S = case.N * case.K
+    scale = {"rij": 1.0, "Oijt": 1e-3, "Fijn": 2e-3, "Fijt": 1e-3, "Mijn": 1e-4,
+             "Mijt": 1e-4, "Mijadd": 1e-5}
+    for k, sc in scale.items():
+        a = (rng.standard_normal(3 * S) * sc).astype(np.float32)
+        o.slot[k][:] = a
+        g.slot[k].copy_(torch.from_numpy(a).cuda())
+    for s in (g, o):
+        s.sumup_forces()
+    # thresholds near the median stress so that a good share of bonds break
+    A = float(np.float32(3.14159265358979)) * m.b_r ** 2
+    sc_, tc_ = 2e-3 / A, 1.5e-3 / A
+    for s in (g, o):
+        s.check_break(sc_, tc_, m.b_r)
+    hg, ho = g.host(), o.host()
+    assert_same(hg, ho, keys=("F", "M", "ILtype", "Oijt"), where="sumup/break")
+    alive = int((ho["ILtype"] == 1).sum())
+    assert 0 < alive < int((o.IL != 0xFFFFFFFF).sum())  # some, not all, slots broke
+
+
+def test_brazil_platen_and_cut_bitexact(ctx):
+    """Brazilian platens (md_braziltest_simple2.cu:19) with in-plane friction,
+    then the cut box (md_parallelepipedrestriction.cu:73), on a disc with
+    random velocities and forces: R, V, F bit for bit, reaction sums to 1e-12."""
+    import torch
+
+    case = cases.reference_brazil(10.0, 3.0, seed=6)
+    g, o = twin(ctx, case)
+    rng = np.random.default_rng(23)
+    f = (rng.standard_normal(3 * case.N) * 1e-2).astype(np.float32)
+    v = (rng.standard_normal(3 * case.N) * 1e-3).astype(np.float32)
+    for s in (g, o):
+        if s is g:
+            g.F.copy_(torch.from_numpy(f).cuda())
+            g.V.copy_(torch.from_numpy(v).cuda())
+        else:
+            o.F[:] = f
+            o.V[:] = v
+    cmp = case.cmp
+    zcut = 1e3
+    sg, so = g.platen_brazil(cmp, zcut), o.platen_brazil(cmp, zcut)
+    np.testing.assert_allclose(sg, so, rtol=1e-12)
+    assert sg[0] < 0 < sg[1]
+    c = case.box.center
+    half = [3.0, 3.0, 3.0]  # cuts the outer part of the 10 mm disc
+    hp = case.box.hidden_point
+    for s in (g, o):
+        s.cut_restriction(c, half, hp)
+    hg, ho = g.host(), o.host()
+    assert_same(hg, ho, keys=("R", "V", "F"), where="brazil/cut")
+    assert np.any(ho["R"] == np.float32(hp[0]))

@@ -551,3 +551,116 @@ def test_native_f64_state_corotates_with_rigid_rotation():
     np.testing.assert_allclose(st[4:7], Rz @ Mt0, rtol=0, atol=2e-5 * np.linalg.norm(Mt0))
     assert st[3] == pytest.approx(5e-4, rel=1e-12)
 
+
+
+# ------------------------------------------- stand-alone sum-up and break
+def _bare_layout(N, K):
+    from oracle.oracle import OracleLayout
+    from rfd.reflayout import Grid
+
+    return OracleLayout(np.zeros(3 * N, np.float32), K, Grid(1, 1, 1, 1, 1))
+
+
+def test_sumup_known_answer():
+    """md_sumupforcesdem.cu:19-60 on 2 particles x 2 slots with exactly
+    representable loads: particle p sums its own row (slots p*K..p*K+K-1) of
+    Fijn+Fijt into F and Mijn+Mijt+Mijadd into M, written SoA by component."""
+    o = _bare_layout(2, 2)
+    # slot s, component d: value 2^s * (d + 1) in Fijn, 3x that in Fijt, etc.
+    for s in range(4):
+        for d in range(3):
+            o.slot["Fijn"][3 * s + d] = 2.0 ** s * (d + 1)
+            o.slot["Fijt"][3 * s + d] = 3 * 2.0 ** s * (d + 1)
+            o.slot["Mijn"][3 * s + d] = -(2.0 ** s)
+            o.slot["Mijt"][3 * s + d] = 0.5 * d
+            o.slot["Mijadd"][3 * s + d] = 100.0 * s
+    o.sumup_forces()
+    # particle 0: slots 0, 1 -> sum 2^s = 3; particle 1: slots 2, 3 -> 12
+    for p, w, ss in ((0, 3.0, (0, 1)), (1, 12.0, (2, 3))):
+        for d in range(3):
+            assert o.F[p + 2 * d] == 4 * w * (d + 1)
+            assert o.M[p + 2 * d] == -w + 2 * 0.5 * d + 100.0 * sum(ss)
+
+
+def test_standalone_break_thresholds():
+    """md_checkbreakconditiondem.cu:20-55: sigma = sign(Fn.n) |Fn| / A +
+    |Mt| b / I, tau = |Ft| / A + |Mn| b / (2 I), I = pi b^4 / 4.  Loads just
+    below and just above each threshold, and a compressive load whose stress
+    counts negative."""
+    b = 0.25
+    A = float(PI_F) * b * b
+    I = float(PI_F) * b ** 4 * 0.25
+    sc, tc = 10.0, 20.0
+    cases_ = [  # (Fn, Mt, Ft, Mn, broken)
+        (0.999 * sc * A, 0, 0, 0, False),
+        (1.001 * sc * A, 0, 0, 0, True),
+        (0.5 * sc * A, 0.499 * sc * I / b, 0, 0, False),
+        (0.5 * sc * A, 0.501 * sc * I / b, 0, 0, True),
+        (-5 * sc * A, 5.9 * sc * I / b, 0, 0, False),   # compression offsets bending
+        (0, 0, 0.999 * tc * A, 0, False),
+        (0, 0, 1.001 * tc * A, 0, True),
+        (0, 0, 0, 1.998 * tc * I / b, False),
+        (0, 0, 0, 2.002 * tc * I / b, True),
+    ]
+    o = _bare_layout(1, len(cases_))
+    n = np.array([0.6, 0.0, 0.8], np.float32)
+    t = np.array([0.8, 0.0, -0.6], np.float32)
+    for s, (fn, mt, ft, mn, _) in enumerate(cases_):
+        o.ILtype[s] = 1
+        o.slot["rij"][3 * s:3 * s + 3] = n
+        o.slot["Fijn"][3 * s:3 * s + 3] = n * np.float32(fn)
+        o.slot["Mijt"][3 * s:3 * s + 3] = t * np.float32(mt)
+        o.slot["Fijt"][3 * s:3 * s + 3] = t * np.float32(ft)
+        o.slot["Mijn"][3 * s:3 * s + 3] = n * np.float32(mn)
+        o.slot["Oijt"][3 * s:3 * s + 3] = 1.0
+    o.check_break(sc, tc, b)
+    for s, c in enumerate(cases_):
+        assert o.ILtype[s] == (0 if c[4] else 1), (s, c)
+        assert np.all(o.slot["Oijt"][3 * s:3 * s + 3] == (0.0 if c[4] else 1.0))
+
+
+def test_platen_brazil_hand_sums():
+    """md_braziltest_simple2.cu:19-92: load axis y; inside the radius (x, z)
+    and below Zcut, the top band sums F_y > 0 and brakes the in-plane force by
+    0.45 F_y along the in-plane velocity; past Yt the particle is clamped."""
+    P = np.array([[0, 9.9, 0], [0.1, 10.2, 0], [0, -0.1, 0], [0, 0.3, 0], [9, 9.9, 9]],
+                 np.float32)
+    N = len(P)
+    R = np.ascontiguousarray(P.T).reshape(-1).copy()
+    V = np.zeros(3 * N, np.float32)
+    F = np.zeros(3 * N, np.float32)
+    F[N + np.arange(N)] = [2.0, 3.0, -4.0, -1.5, 5.0]
+    V[0], V[2 * N] = 3.0, 4.0          # particle 0 moves in-plane, |v| = 5
+    F[0], F[2 * N] = 1.0, 1.0
+    sums = (C.c_double * 2)()
+    lib().orc_platen_brazil(ptr(R), ptr(V), ptr(F), N, 0.0, 0.0, 1.0, 10.0, 0.0, 9.5, 0.5,
+                            -1e-3, 100.0, sums)
+    assert (sums[0], sums[1]) == (-5.5, 5.0)
+    # friction on particle 0: F -= 0.45 * 2 * v / |v|
+    assert F[0] == pytest.approx(1.0 - 0.45 * 2 * 0.6, rel=1e-6)
+    assert F[2 * N] == pytest.approx(1.0 - 0.45 * 2 * 0.8, rel=1e-6)
+    assert R[N + 1] == 10.0 and V[N + 1] == np.float32(-1e-3) and F[N + 1] == 0
+    assert R[N + 2] == 0.0 and F[N + 2] == 0
+    assert R[N + 3] == np.float32(0.3) and F[N + 3] == np.float32(-1.5)
+    assert F[N + 4] == 5.0  # outside the radius
+
+
+def test_cut_restriction_moves_outside_particles():
+    """md_parallelepipedrestriction.cu:73-158: a particle outside the box on
+    any axis goes to the hidden point with V = F = 0; inside ones stay."""
+    P = np.array([[0, 0, 0], [1.01, 0, 0], [0, -2.5, 0], [0, 0, 3.2], [0.99, -1.99, 2.99]],
+                 np.float32)
+    N = len(P)
+    R = np.ascontiguousarray(P.T).reshape(-1).copy()
+    V = np.ones(3 * N, np.float32)
+    F = np.full(3 * N, 2.0, np.float32)
+    c = np.zeros(3, np.float32); h = np.array([1, 2, 3], np.float32)
+    hp = np.array([100, 200, 300], np.float32)
+    lib().orc_cut_restriction(ptr(R), ptr(V), ptr(F), N, ptr(c), ptr(h), ptr(hp))
+    out = [False, True, True, True, False]
+    for i, o in enumerate(out):
+        got = np.array([R[i], R[N + i], R[2 * N + i]])
+        if o:
+            assert np.all(got == hp) and V[i::N].max() == 0 and F[i::N].max() == 0
+        else:
+            assert np.all(got == P[i]) and V[i::N].min() == 1 and F[i::N].min() == 2
