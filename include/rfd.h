@@ -288,6 +288,9 @@ typedef struct rfd_fire_result {
   double sum_bottom, sum_top; /* platen reactions after relaxation            */
   double strain;        /* specimen strain after the step                     */
   double seconds;       /* wall time of the load step                         */
+  int warm_used;        /* 1: the predicted start was taken                   */
+  double fmax0_plain, fmax0_pred; /* residuals of the two candidate starts
+                                     (-1 when no prediction was given)        */
 } rfd_fire_result;
 
 /* One quasi-static load step: move the top platen by dstrain * size0 (load
@@ -295,8 +298,10 @@ typedef struct rfd_fire_result {
  * platen follow it), optionally apply a predicted increment (warm start: dU,
  * dTh are device arrays of N x 3 floats, row-major, or NULL), carry the move
  * into the incremental springs with one force pass, then relax with FIRE
- * (from rest) to ftol.  The displacement of the step is kept on the device
- * (rfd_system_fire_buffers). */
+ * (from rest) to ftol.  A prediction is judged by FIRE's residual: the start
+ * it gives is taken only if its max|F| is not above the plain start's, else
+ * the step runs from the plain start (fallback).  The displacement of the
+ * step is kept on the device (rfd_system_fire_buffers). */
 int rfd_system_fire_load_step(rfd_system *sys, double dstrain, const rfd_fire_params *fp,
                               const float *dU, const float *dTh, rfd_fire_result *out);
 /* residual of the current configuration (one force pass + constraint):
@@ -320,6 +325,60 @@ int rfd_system_bond_graph(rfd_system *sys, const uint32_t **hoff, const void **h
  * 4 floats {x, y, z, 0} */
 int rfd_system_device_arrays(rfd_system *sys, double **P, float **V, float **W,
                              float **F, float **M);
+
+/* ============ learned warm start: message-passing network kernels ========
+ * No counterpart in the reference (the warm start is new); these are the
+ * device kernels a per-particle message-passing network over the bond graph
+ * needs for inference and training.  All arrays are FP32 device memory,
+ * row-major with the given leading dimensions; dense products round their
+ * operands to BF16 and accumulate in FP32 on the matrix cores; every
+ * reduction has a fixed order (deterministic). */
+
+/* mirror half of each half-bond (hm[h] = the half of the same bond in the
+ * other end's row), H entries */
+int rfd_system_half_mirror(rfd_system *sys, const uint32_t **hm);
+/* network inputs at the start of a load step: X[N x 32] = last two step
+ * displacements / su and rotations / sth (12 used, rest 0); E[H x 4] per half
+ * = (1e3 * bond strain if intact else 0, unit direction to the other end) */
+int rfd_system_mpnn_inputs(rfd_system *sys, double su, double sth, float *X, float *E);
+
+/* Y[M x N] = X[M x K] B[K x N] + bias (+ Y if accumulate), B(k, c) = W[k*wsk + c*wsc]
+ * (wsk = N, wsc = 1 for row-major W; wsk = 1, wsc = ldw for its transpose);
+ * K a multiple of 32 up to 128, N a multiple of 16 up to 128; bias may be NULL */
+int rfd_mm_bf16(rfd_ctx *ctx, const float *X, uint32_t ldx, const float *W, uint32_t wsk,
+                uint32_t wsc, const float *bias, float *Y, uint32_t ldy, uint32_t M,
+                uint32_t K, uint32_t N, int accumulate);
+/* G[K x N] (+)= X[M x K]^T D[M x N]; K a multiple of 16 up to 128, N of 16 up to 128 */
+int rfd_mm_tn_bf16(rfd_ctx *ctx, const float *X, uint32_t ldx, const float *D, uint32_t ldd,
+                   float *G, uint32_t M, uint32_t K, uint32_t N, int accumulate);
+/* g[c] (+)= sum over rows of D[m][c], c < ncols */
+int rfd_colsum(rfd_ctx *ctx, const float *D, uint32_t ldd, float *g, uint32_t M,
+               uint32_t ncols, int accumulate);
+/* elementwise over M x ncols: op 0 out = relu(a); 1 out = b + relu(a);
+ * 2 out = b > 0 ? a : 0; 3 out = a + b; 4 out = a; 5 out = alpha (a + b);
+ * 6 out = alpha a - b; 7 out = alpha a (out may alias a or b) */
+int rfd_mpnn_ewise(rfd_ctx *ctx, int op, uint32_t M, uint32_t ncols, float alpha,
+                   const float *a, uint32_t lda, const float *b, uint32_t ldb, float *out,
+                   uint32_t ldo);
+/* message step over half-bond rows (hoff[N+1], partner of half h at
+ * hj[h * hj_stride]): q_h = Z[j] + E[h] We + bm (hidden width 64, We 4 x 64),
+ * A[i] = sum over row i of relu(q_h) (row stride lda); Q[H x 64] keeps q when
+ * not NULL (training) */
+int rfd_mpnn_edge_fwd(rfd_ctx *ctx, uint32_t N, const uint32_t *hoff, const uint32_t *hj,
+                      uint32_t hj_stride, const float *E, const float *We, const float *bm,
+                      const float *Z, float *Q, float *A, uint32_t lda);
+/* its backward: dZ[N x 64], dWe[4 x 64] and dbm[64] (+= if accumulate) from dA */
+int rfd_mpnn_edge_bwd(rfd_ctx *ctx, uint32_t N, const uint32_t *hoff, const uint32_t *hj,
+                      uint32_t hj_stride, const uint32_t *hm, const float *E, const float *Q,
+                      const float *dA, uint32_t ldda, float *dZ, float *dWe, float *dbm,
+                      int accumulate);
+/* mean of (Y - T)^2 over N x ncols (FP64 sum, synchronises); dY = its gradient
+ * (may be NULL) */
+int rfd_mse_loss(rfd_ctx *ctx, const float *Y, uint32_t ldy, const float *T, uint32_t ldt,
+                 uint32_t N, uint32_t ncols, float *dY, uint32_t lddy, double *loss);
+/* one Adam update of n parameters at step t >= 1 */
+int rfd_adam(rfd_ctx *ctx, float *p, const float *g, float *m, float *v, uint32_t n, float lr,
+             float b1, float b2, float eps, uint32_t t);
 
 #ifdef __cplusplus
 }

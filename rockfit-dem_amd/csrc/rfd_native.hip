@@ -294,6 +294,7 @@ struct SysDev {
   float4 *cO;
   const float *dtp;  // when set, the time step of the last move (FIRE) replaces mt.dt
   const int *done;   // when set and nonzero (relaxation finished), the pass does nothing
+  int trial;         // trial pass: writes only F, M and st_out (nothing the next pass reads)
 };
 
 __device__ __forceinline__ EndData load_end(const SysDev &s, uint32_t i) {
@@ -360,11 +361,11 @@ k_nat_forces(SysDev s, Mat mt, uint32_t ntiles, uint32_t per) {
           if (o.st.fl == kRemoved) {
             // both buffers: no later pass may read a stale live flag
             s.st_out[hr.b].fl = kRemoved;
-            s.st_in[hr.b].fl = kRemoved;
+            if (!s.trial) s.st_in[hr.b].fl = kRemoved;
           } else {
             s.st_out[hr.b] = o.st;
           }
-          if (o.st.fl != old.fl) s.fl8[hr.b] = (uint8_t)o.st.fl;
+          if (!s.trial && o.st.fl != old.fl) s.fl8[hr.b] = (uint8_t)o.st.fl;
         }
       }
       const uint32_t k = e - c0;
@@ -435,8 +436,8 @@ k_nat_forces(SysDev s, Mat mt, uint32_t ntiles, uint32_t per) {
       F = F + Fi;
       Mc = Mc + Mi;
       anyc = true;
-      s.cO[sl] = f4(O);
-    } else {
+      if (!s.trial) s.cO[sl] = f4(O);
+    } else if (!s.trial) {
       s.cO[sl] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
   }
@@ -1189,6 +1190,40 @@ __global__ void k_zero_f4(float4 *__restrict__ a, float4 *__restrict__ b, uint32
   }
 }
 
+// inputs of the learned warm start: per particle the last two step
+// displacements and rotations (scaled), per half the bond strain (x 1e3, zero
+// unless the bond is intact) and the unit direction to the other end
+__global__ void k_mpnn_inputs(const P4 *__restrict__ P, uint32_t N,
+                              const uint32_t *__restrict__ hoff, const HR *__restrict__ hr,
+                              const BR *__restrict__ brc, const uint8_t *__restrict__ fl8,
+                              const float *__restrict__ du0, const float *__restrict__ du1,
+                              const float *__restrict__ dth0, const float *__restrict__ dth1,
+                              float isu, float isth, float *__restrict__ X,
+                              float *__restrict__ E) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
+    float *x = X + 32 * (size_t)i;
+    for (int d = 0; d < 3; ++d) {
+      x[d] = du0[3 * (size_t)i + d] * isu;
+      x[3 + d] = du1[3 * (size_t)i + d] * isu;
+      x[6 + d] = dth0[3 * (size_t)i + d] * isth;
+      x[9 + d] = dth1[3 * (size_t)i + d] * isth;
+    }
+    for (int d = 12; d < 32; ++d) x[d] = 0.f;
+    const P4 A = P[i];
+    for (uint32_t h = hoff[i]; h < hoff[i + 1]; ++h) {
+      const HR q = hr[h];
+      const P4 B = P[q.j];
+      const double rx = B.x - A.x, ry = B.y - A.y, rz = B.z - A.z;
+      const double L = __builtin_sqrt(rx * rx + ry * ry + rz * rz);
+      const double il = L > 0 ? 1.0 / L : 0.0;
+      const float eps =
+          fl8[q.b] == kBond ? (float)((L * (double)brc[q.b].il - 1.0) * 1e3) : 0.f;
+      *reinterpret_cast<float4 *>(E + 4 * (size_t)h) =
+          make_float4(eps, (float)(rx * il), (float)(ry * il), (float)(rz * il));
+    }
+  }
+}
+
 // ================================================================= host side
 struct rfd_system {
   rfd_ctx *ctx = nullptr;
@@ -1199,6 +1234,7 @@ struct rfd_system {
   float2 *inv = nullptr;
   uint32_t *hoff = nullptr;
   HR *hr = nullptr;
+  uint32_t *hm = nullptr;  // mirror half of each half (graph consumers)
   BR *brc = nullptr;
   uint32_t H = 0;  // half-bonds (2 per bond)
   std::vector<uint32_t> hbi, hbj;  // host copy of the bond ends
@@ -1371,9 +1407,10 @@ int rebuild_rows(rfd_system *s) {
 // dtp set: a quasi-static (FIRE) pass -- the time step is that of the last
 // move, and the contact law drops its velocity damping, so that the residual
 // FIRE judges is the static out-of-balance force
-int forces(rfd_system *s, const float *dtp = nullptr) {
+int forces(rfd_system *s, const float *dtp = nullptr, bool trial = false) {
   SysDev d = make_dev(s);
   d.dtp = dtp;
+  d.trial = trial ? 1 : 0;
   // after convergence the remaining launches of a FIRE chunk must not apply
   // the last move to the springs again
   if (dtp) d.done = (const int *)((const char *)s->fire + offsetof(FireDev, done));
@@ -1489,6 +1526,51 @@ int fire_iteration(rfd_system *s) {
   return RFD_OK;
 }
 
+// start of a load step from the positions saved in P0: the move (warm start
+// dU, dTh and the platens) enters the springs through one force pass run with
+// dt0 (FireDev f written fresh), FIRE then starts at rest.  A trial pass
+// reports the residual it leaves and leaves the relaxation state as it was.
+int begin_pass(rfd_system *s, const FireDev &f, const float *dU, const float *dTh, double dtop,
+               bool trial, double *fmax_out) {
+  rfd_ctx *ctx = s->ctx;
+  hipStream_t q = ctx->stream;
+  RFD_HIP(ctx, hipMemcpyAsync(s->fire, &f, sizeof(f), hipMemcpyHostToDevice, q));
+  RFD_HIP(ctx, hipMemcpyAsync(s->P, s->P0, sizeof(P4) * s->N, hipMemcpyDeviceToDevice, q));
+  FireParticles fpp = fire_particles(s);
+  const float dt0 = (float)f.dt;
+  if (s->mode == 1)
+    k_fire_begin<1><<<s->nblocks_pass, 256, 0, q>>>(fpp, dU, dTh, dt0, dtop);
+  else if (s->mode == 2)
+    k_fire_begin<2><<<s->nblocks_pass, 256, 0, q>>>(fpp, dU, dTh, dt0, dtop);
+  else
+    k_fire_begin<0><<<s->nblocks_pass, 256, 0, q>>>(fpp, dU, dTh, dt0, dtop);
+  RFD_HIP(ctx, hipGetLastError());
+  const int cur0 = s->cur;
+  int st = forces(s, s->fire_dt, trial);
+  if (st) return st;
+  k_zero_f4<<<rfd_blocks(s->N, 256), 256, 0, q>>>(s->V, s->W, s->N);
+  RFD_HIP(ctx, hipGetLastError());
+  if (trial) {
+    s->cur = cur0;  // the new state went to the other buffer: drop it
+    FireDev fm = f;
+    fm.measure_only = 1;
+    RFD_HIP(ctx, hipMemcpyAsync(s->fire, &fm, sizeof(fm), hipMemcpyHostToDevice, q));
+    if (s->mode == 1)
+      k_fire_measure<1><<<s->nblocks_pass, 256, 0, q>>>(fpp);
+    else if (s->mode == 2)
+      k_fire_measure<2><<<s->nblocks_pass, 256, 0, q>>>(fpp);
+    else
+      k_fire_measure<0><<<s->nblocks_pass, 256, 0, q>>>(fpp);
+    k_fire_control<<<1, 256, 0, q>>>(s->fire, s->fire_part, (int)s->nblocks_pass, s->ctl);
+    RFD_HIP(ctx, hipGetLastError());
+    FireDev r;
+    RFD_HIP(ctx, hipMemcpyAsync(&r, s->fire, sizeof(r), hipMemcpyDeviceToHost, q));
+    RFD_HIP(ctx, hipStreamSynchronize(q));
+    *fmax_out = r.fmax;
+  }
+  return RFD_OK;
+}
+
 int upload_vec3(rfd_ctx *ctx, float4 *dst, const float *soa, uint32_t N) {
   std::vector<float4> h(N);
   for (uint32_t i = 0; i < N; ++i)
@@ -1574,6 +1656,7 @@ int rfd_system_destroy(rfd_system *s) {
   dfree(s->inv);
   dfree(s->hoff);
   dfree(s->hr);
+  dfree(s->hm);
   dfree(s->brc);
   dfree(s->Vf);
   for (int q = 0; q < 2; ++q) {
@@ -1768,12 +1851,22 @@ int rfd_system_set_bonds(rfd_system *s, uint32_t Nb, const uint32_t *bi, const u
   const uint32_t H = 2 * Nb;
   std::vector<HR> hhr(H);
   std::vector<BR> hbr(Nb);
+  std::vector<uint32_t> hm;  // the other half of each half's bond
   {
     // lower partners first (bonds where p is j, in bond order = ascending i),
     // then higher partners (bonds where p is i, ascending j): rows sorted by id
-    std::vector<uint32_t> cur(hoff.begin(), hoff.end() - 1);
-    for (uint32_t b = 0; b < Nb; ++b) hhr[cur[bj[b]]++] = HR{bi[b], b};
-    for (uint32_t b = 0; b < Nb; ++b) hhr[cur[bi[b]]++] = HR{bj[b], b};
+    std::vector<uint32_t> cur(hoff.begin(), hoff.end() - 1), hmir(Nb);
+    hm.assign(H, 0);
+    for (uint32_t b = 0; b < Nb; ++b) {
+      hmir[b] = cur[bj[b]]++;
+      hhr[hmir[b]] = HR{bi[b], b};
+    }
+    for (uint32_t b = 0; b < Nb; ++b) {
+      const uint32_t ho = cur[bi[b]]++;
+      hhr[ho] = HR{bj[b], b};
+      hm[ho] = hmir[b];
+      hm[hmir[b]] = ho;
+    }
     for (uint32_t b = 0; b < Nb; ++b) {
       if (L0) RFD_CHECK_ARG(ctx, L0[b] > 0.f, "rfd_system_set_bonds: L0[%u] <= 0", b);
       hbr[b] = BR{L0 ? (float)(1.0 / (double)L0[b]) : 0.f, br[b], sc[b], tc[b]};
@@ -1784,6 +1877,7 @@ int rfd_system_set_bonds(rfd_system *s, uint32_t Nb, const uint32_t *bi, const u
   s->Nb = Nb;
   s->H = H;
   int st = dalloc(ctx, &s->hr, H);
+  if (!st) st = dalloc(ctx, &s->hm, H);
   if (!st) st = dalloc(ctx, &s->brc, Nb);
   if (!st) st = dalloc(ctx, &s->bst[0], Nb);
   if (!st) st = dalloc(ctx, &s->bst[1], Nb);
@@ -1794,6 +1888,7 @@ int rfd_system_set_bonds(rfd_system *s, uint32_t Nb, const uint32_t *bi, const u
     std::vector<BSt> h0(Nb);
     for (auto &x : h0) x = BSt{make_float4(0.f, 0.f, 0.f, 0.f), 0.f, 0.f, 0.f, kBond};
     RFD_HIP(ctx, hipMemcpy(s->hr, hhr.data(), sizeof(HR) * H, hipMemcpyHostToDevice));
+    RFD_HIP(ctx, hipMemcpy(s->hm, hm.data(), sizeof(uint32_t) * H, hipMemcpyHostToDevice));
     RFD_HIP(ctx, hipMemcpy(s->brc, hbr.data(), sizeof(BR) * Nb, hipMemcpyHostToDevice));
     for (int q = 0; q < 2; ++q)
       RFD_HIP(ctx, hipMemcpy(s->bst[q], h0.data(), sizeof(BSt) * Nb, hipMemcpyHostToDevice));
@@ -2046,30 +2141,25 @@ int rfd_system_fire_load_step(rfd_system *s, double dstrain, const rfd_fire_para
   f.max_iter = fp->max_iter;
   f.first = 1;
   f.dt_f = (float)fp->dt0;
-  RFD_HIP(ctx, hipMemcpyAsync(s->fire, &f, sizeof(f), hipMemcpyHostToDevice, q));
-  {
-    // the move (warm start and platens) enters the springs through one force
-    // pass run with dt0; FIRE then starts at rest from the forces it left
-    FireParticles fpp = fire_particles(s);
-    const float dt0 = (float)fp->dt0;
-    if (s->mode == 1)
-      k_fire_begin<1><<<s->nblocks_pass, 256, 0, q>>>(fpp, dU, dTh, dt0, dtop);
-    else if (s->mode == 2)
-      k_fire_begin<2><<<s->nblocks_pass, 256, 0, q>>>(fpp, dU, dTh, dt0, dtop);
-    else
-      k_fire_begin<0><<<s->nblocks_pass, 256, 0, q>>>(fpp, dU, dTh, dt0, dtop);
-    RFD_HIP(ctx, hipGetLastError());
-    if (!s->rows_valid || s->step_count % s->p.rebuild_every == 0) {
-      st = rebuild_rows(s);
-      if (st) return st;
-    }
-    st = forces(s, s->fire_dt);
+  if (!s->rows_valid || s->step_count % s->p.rebuild_every == 0) {
+    st = rebuild_rows(s);
     if (st) return st;
-    k_zero_f4<<<rfd_blocks(s->N, 256), 256, 0, q>>>(s->V, s->W, s->N);
-    RFD_HIP(ctx, hipGetLastError());
-    s->step_count++;
-    s->forces_current = true;
   }
+  // a prediction is judged by the residual it leaves against the plain start's
+  // (two trial passes that change nothing the relaxation reads); the better
+  // start is then taken for real
+  double fmax_plain = -1.0, fmax_pred = -1.0;
+  bool use_pred = false;
+  if (dU || dTh) {
+    st = begin_pass(s, f, nullptr, nullptr, dtop, true, &fmax_plain);
+    if (!st) st = begin_pass(s, f, dU, dTh, dtop, true, &fmax_pred);
+    if (st) return st;
+    use_pred = fmax_pred <= fmax_plain;
+  }
+  st = begin_pass(s, f, use_pred ? dU : nullptr, use_pred ? dTh : nullptr, dtop, false, nullptr);
+  if (st) return st;
+  s->step_count++;
+  s->forces_current = true;
   const uint32_t chunk = fp->check_every ? fp->check_every : 50;
   for (;;) {
     for (uint32_t k = 0; k < chunk; ++k) {
@@ -2100,6 +2190,9 @@ int rfd_system_fire_load_step(rfd_system *s, double dstrain, const rfd_fire_para
   out->strain = L.strain;
   out->seconds =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  out->warm_used = use_pred ? 1 : 0;
+  out->fmax0_plain = fmax_plain;
+  out->fmax0_pred = fmax_pred;
   return RFD_OK;
 }
 
@@ -2145,6 +2238,28 @@ int rfd_system_fire_buffers(rfd_system *s, float **du_last, float **du_prev, flo
   if (du_prev) *du_prev = s->du[1 - s->dcur];
   if (dth_last) *dth_last = s->dth[s->dcur];
   if (dth_prev) *dth_prev = s->dth[1 - s->dcur];
+  return RFD_OK;
+}
+
+int rfd_system_half_mirror(rfd_system *s, const uint32_t **hm) {
+  if (!valid(s)) return RFD_E_ARG;
+  RFD_CHECK_ARG(s->ctx, hm && s->have_bonds, "rfd_system_half_mirror: no bonds");
+  *hm = s->hm;
+  return RFD_OK;
+}
+
+int rfd_system_mpnn_inputs(rfd_system *s, double su, double sth, float *X, float *E) {
+  if (!valid(s)) return RFD_E_ARG;
+  rfd_ctx *ctx = s->ctx;
+  RFD_CHECK_ARG(ctx, X && E && su > 0 && sth > 0 && s->have_bonds,
+                "rfd_system_mpnn_inputs: bad arguments");
+  int st = fire_alloc(s);
+  if (st) return st;
+  const int a = s->dcur, b = 1 - s->dcur;
+  k_mpnn_inputs<<<rfd_blocks(s->N, 256), 256, 0, ctx->stream>>>(
+      s->P, s->N, s->hoff, s->hr, s->brc, s->fl8, s->du[a], s->du[b], s->dth[a], s->dth[b],
+      (float)(1.0 / su), (float)(1.0 / sth), X, E);
+  RFD_HIP(ctx, hipGetLastError());
   return RFD_OK;
 }
 

@@ -56,7 +56,8 @@ class FireParams(C.Structure):
 class FireResult(C.Structure):
     _fields_ = [("iterations", C.c_uint32), ("converged", C.c_int), ("fmax0", C.c_double),
                 ("fmax", C.c_double), ("mmax", C.c_double), ("sum_bottom", C.c_double),
-                ("sum_top", C.c_double), ("strain", C.c_double), ("seconds", C.c_double)]
+                ("sum_top", C.c_double), ("strain", C.c_double), ("seconds", C.c_double),
+                ("warm_used", C.c_int), ("fmax0_plain", C.c_double), ("fmax0_pred", C.c_double)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -85,6 +86,8 @@ NATIVE_SIGNATURES = {
     "rfd_system_fire_load_step": [_vp, _dbl, C.POINTER(FireParams), _vp, _vp,
                                   C.POINTER(FireResult)],
     "rfd_system_residual": [_vp, C.POINTER(C.c_double)],
+    "rfd_system_half_mirror": [_vp, C.POINTER(_vp)],
+    "rfd_system_mpnn_inputs": [_vp, _dbl, _dbl, _vp, _vp],
     "rfd_system_fire_buffers": [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp),
                                 C.POINTER(_vp)],
     "rfd_system_bond_graph": [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp),
@@ -258,6 +261,21 @@ class NativeSystem:
                    C.byref(r))
         return r
 
+    def half_mirror(self):
+        """Device view hm[H] int32: the other half of each half's bond."""
+        a = C.c_void_p()
+        self._call("rfd_system_half_mirror", self.h, C.byref(a))
+        return device_view(a.value, (2 * self.bonds.n,), "<i4", self)
+
+    def mpnn_inputs(self, su: float, sth: float, X, E):
+        """Fill the warm-start network inputs X [N x 32], E [H x 4] (device)."""
+        assert X.shape == (self.N, 32) and E.shape == (2 * self.bonds.n, 4)
+        self._call("rfd_system_mpnn_inputs", self.h, su, sth, C.c_void_p(X.data_ptr()),
+                   C.c_void_p(E.data_ptr()))
+
+    def mean_radius(self) -> float:
+        return float(self.radius.astype(np.float64).mean())
+
     def residual(self) -> tuple[float, float]:
         out = (C.c_double * 2)()
         self._call("rfd_system_residual", self.h, out)
@@ -268,7 +286,7 @@ class NativeSystem:
         device arrays."""
         ptrs = [C.c_void_p() for _ in range(4)]
         self._call("rfd_system_fire_buffers", self.h, *[C.byref(q) for q in ptrs])
-        return tuple(device_view(q.value, (self.N, 3), "<f4") for q in ptrs)
+        return tuple(device_view(q.value, (self.N, 3), "<f4", self) for q in ptrs)
 
     def bond_graph(self):
         """Device views: hoff[N+1] int32, half[H, 2] int32 (other end, bond),
@@ -279,10 +297,10 @@ class NativeSystem:
         self._call("rfd_system_bond_graph", self.h, C.byref(a), C.byref(b), C.byref(c),
                    C.byref(d), C.byref(H))
         nb = H.value // 2
-        return (device_view(a.value, (self.N + 1,), "<i4"),
-                device_view(b.value, (H.value, 2), "<i4"),
-                device_view(c.value, (nb, 4), "<f4"),
-                device_view(d.value, (nb, 8), "<f4"))
+        return (device_view(a.value, (self.N + 1,), "<i4", self),
+                device_view(b.value, (H.value, 2), "<i4", self),
+                device_view(c.value, (nb, 4), "<f4", self),
+                device_view(d.value, (nb, 8), "<f4", self))
 
     def seat_platens(self, preload: float = 0.0, L: Loading | None = None):
         """Move the platens onto the specimen: top at the highest and bottom at
@@ -323,17 +341,20 @@ class NativeSystem:
 # ------------------------------------------------------------- loading
 class _DevArray:
     """Minimal __cuda_array_interface__ wrapper so torch can view device memory
-    owned by the library (lifetime: the owning rfd_system)."""
+    owned by the library.  It holds a reference to the owning object (the
+    NativeSystem), and the tensor torch makes from it holds this wrapper, so
+    the memory outlives every view of it."""
 
-    def __init__(self, ptr: int, shape, typestr: str):
+    def __init__(self, ptr: int, shape, typestr: str, owner=None):
+        self.owner = owner
         self.__cuda_array_interface__ = {"shape": tuple(shape), "typestr": typestr,
                                          "data": (ptr, False), "version": 3, "strides": None}
 
 
-def device_view(ptr: int, shape, typestr: str):
+def device_view(ptr: int, shape, typestr: str, owner=None):
     import torch
 
-    return torch.as_tensor(_DevArray(ptr, shape, typestr), device="cuda")
+    return torch.as_tensor(_DevArray(ptr, shape, typestr, owner), device="cuda")
 
 
 def fire_params(mat_E: float, r_mean: float, dt: float, rel_tol: float = 1e-6,
