@@ -39,6 +39,10 @@ def parse():
     p.add_argument("--H", type=float, default=200.0)
     p.add_argument("--cpu-seconds", type=float, default=10.0)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-warmstart", action="store_true",
+                   help="skip the learned-warm-start load-step measurement")
+    p.add_argument("--warmstart-only", action="store_true",
+                   help="only the learned-warm-start load-step measurement")
     p.add_argument("--copy-calib", action="store_true",
                    help="only run a plain device copy of 1 GiB (counter calibration and "
                         "stream-copy ceiling)")
@@ -199,6 +203,78 @@ def bench_native(args) -> dict:
     }
 
 
+def bench_warmstart(D: float = 61.0, H: float = 122.0, steps: int = 6) -> dict:
+    """Quasi-static load steps (strain increment -1e-5, FIRE to |F|max <
+    1e-6 F_ref) on a ~500k-particle cylinder, plain GPU FIRE against FIRE
+    warm-started by the message-passing network, which is first trained on
+    the device on plain-FIRE relaxations of two smaller cylinders (~20k and
+    ~50k particles) and validated on a third.  Both runs start from the same
+    seeded specimen; the first two steps (seating, first increment) are plain
+    in both and not timed."""
+    from rfd import native, native_cases, warmstart as ws
+    from rfd.reflayout import Context
+
+    ctx = Context(0)
+    dstrain = -1e-5
+    t0 = time.perf_counter()
+
+    def spec(d, h, seed):
+        nc = native_cases.uniaxial_cylinder(ctx, d, h, seed=seed, flat_ends=True)
+        nc.sys.seat_platens(preload=0.0)
+        return nc
+
+    def log(tag):
+        def f(k, r):
+            print(f"[bench] {tag} step {k}: {r.iterations} FIRE iterations, {r.seconds:.2f}s, "
+                  f"converged {r.converged}, warm {r.warm_used}", file=sys.stderr, flush=True)
+        return f
+
+    tr = [spec(21.0, 42.0, 101), spec(28.0, 56.0, 102)]
+    fp = native.fire_params(tr[0].mat.E, 0.5, tr[0].mat.dt, rel_tol=1e-6, max_iter=100000,
+                            check_every=200)
+    samples = []
+    for i, c in enumerate(tr):
+        s_, _ = ws.collect_samples(c.sys, dstrain, fp, 10, log=log(f"train{i} N={c.sys.N}"))
+        samples += s_
+    vs = spec(21.0, 42.0, 103)
+    val, _ = ws.collect_samples(vs.sys, dstrain, fp, 6, log=log("val"))
+    net = ws.WarmStartNet(ctx, seed=7)
+    t1 = time.perf_counter()
+    hist = ws.train(net, samples, epochs=30, lr=1e-3, val=val)
+    ctx.sync()
+    t_train = time.perf_counter() - t1
+    print(f"[bench] warm start: {len(samples)} samples, trained in {t_train:.1f}s, "
+          f"head {'on' if hist['head'] else 'off'}", file=sys.stderr, flush=True)
+    out = {}
+    for name in ("plain", "learned"):
+        c = spec(D, H, 104)
+        warm = (lambda s: ws.predict(net, s, dstrain)) if name == "learned" else None
+        _, res = ws.collect_samples(c.sys, dstrain, fp, 2 + steps, warm=warm,
+                                    log=log(f"{name} N={c.sys.N}"))
+        timed = res[2:]
+        sec = sum(r.seconds for r in timed)
+        out[name] = {"particles": c.sys.N, "bonds": c.sys.bonds.n,
+                     "load_steps_per_s": len(timed) / sec, "s_per_step": sec / len(timed),
+                     "fire_iterations_per_step": sum(r.iterations for r in timed) / len(timed),
+                     "all_converged": all(bool(r.converged) for r in timed),
+                     "max_final_residual": max(r.fmax for r in timed),
+                     "warm_starts_taken": sum(r.warm_used for r in timed)}
+        print(f"[bench] warm start {name}: {out[name]}", file=sys.stderr, flush=True)
+        del c
+    return {"metric": "converged load-steps/s", "unit": "load-steps/s",
+            "workload": f"uniaxial cylinder D{D:g}xH{H:g} mm, strain step {dstrain:g}, "
+                        "FIRE to |F|max < 1e-6 E pi r_mean^2",
+            "ftol": fp.ftol, "plain": out["plain"], "learned": out["learned"],
+            "speedup_wall": out["plain"]["s_per_step"] / out["learned"]["s_per_step"],
+            "training": {"specimens": "D21xH42 (seed 101), D28xH56 (seed 102); "
+                                      "validation D21xH42 (seed 103)",
+                         "samples": len(samples), "epochs": 30, "seconds": t_train,
+                         "correction_head": hist["head"],
+                         "val_loss": min(hist["val"]) if hist["val"] else None,
+                         "val_loss_zero_correction": hist["val_zero"]},
+            "total_seconds": time.perf_counter() - t0}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -208,6 +284,9 @@ def main():
     if args.copy_calib:
         print(json.dumps({"copy_bytes_read": COPY_BYTES, "copy_bytes_written": COPY_BYTES,
                           "stream_copy_GBps": stream_copy() / 1e9}))
+        return
+    if args.warmstart_only:
+        print(json.dumps({"warm_start": bench_warmstart()}))
         return
     res = bench_ref(args) if args.tier == "ref" else bench_native(args)
     res["roofline"]["traffic"] = load_traffic(res["roofline"]["kernel"],
@@ -233,6 +312,11 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_ref(args.cpu_seconds)
         out["cpu_baseline"]["host_cores_available"] = os.cpu_count()
+    if not args.no_warmstart and args.tier == "native":
+        try:
+            out["warm_start"] = bench_warmstart()
+        except Exception as e:  # reported, never hides the main line
+            out["warm_start"] = {"error": f"{type(e).__name__}: {e}"}
     print(json.dumps(out))
 
 

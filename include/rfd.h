@@ -291,6 +291,8 @@ typedef struct rfd_fire_result {
   int warm_used;        /* 1: the predicted start was taken                   */
   double fmax0_plain, fmax0_pred; /* residuals of the two candidate starts
                                      (-1 when no prediction was given)        */
+  uint32_t breaks;      /* bonds broken at equilibrium during the step        */
+  uint32_t break_rounds;/* relax-and-break rounds after the first relaxation  */
 } rfd_fire_result;
 
 /* One quasi-static load step: move the top platen by dstrain * size0 (load
@@ -298,7 +300,10 @@ typedef struct rfd_fire_result {
  * platen follow it), optionally apply a predicted increment (warm start: dU,
  * dTh are device arrays of N x 3 floats, row-major, or NULL), carry the move
  * into the incremental springs with one force pass, then relax with FIRE
- * (from rest) to ftol.  A prediction is judged by FIRE's residual: the start
+ * (from rest) to ftol with bonds held intact; then bonds are judged at the
+ * relaxed state (break criterion of md_calculateforcesdem3.cu:20-36), and
+ * while any breaks the step relaxes again.  A prediction is judged by FIRE's
+ * residual: the start
  * it gives is taken only if its max|F| is not above the plain start's, else
  * the step runs from the plain start (fallback).  The displacement of the
  * step is kept on the device (rfd_system_fire_buffers). */

@@ -57,6 +57,7 @@ constexpr int kCH = 512;   // bond ends per LDS chunk
 static_assert(kFP < kFT, "a force workgroup needs more threads than particles");
 constexpr uint8_t kBond = 1, kBroken = 0, kRemoved = 2;
 constexpr int kHistCap = 1 << 20;
+constexpr uint32_t kMaxBreakRounds = 200;  // relax-and-break rounds per load step
 // gfx950 places workgroup b on XCD b % 8; tiles are remapped so that each XCD
 // (own L2) works through one contiguous run of Morton-ordered particle tiles,
 // which keeps the neighbour gathers of adjacent tiles in the same L2
@@ -102,6 +103,7 @@ struct Mat {
   float E, G, Estar, Gstar, pA, mu, mur, rho43pi, dt;
   double farcut;  // far-link distance as a multiple of ri + rj
   int refmode;
+  int nobreak;    // quasi-static relaxation: bonds are judged at equilibrium only
 };
 
 // first-order rotation of md_calculateforcesdem3.cu:80-88 built from
@@ -248,6 +250,7 @@ __device__ __forceinline__ void eval_bond(const Mat &mt, const EndData &A, const
       const double Mc = (double)mt.G * ild * dnorm(O) + fabs((double)mn) * 2.0 * (double)c.br;
       broken = Sc > (double)c.sc || Mc > (double)c.tc;
     }
+    if (mt.nobreak) broken = false;
     if (broken) {
       o.st.fl = kBroken;
       O = kZero;
@@ -287,14 +290,15 @@ struct SysDev {
   const uint32_t *hoff;
   const HR *hr;
   const BR *brc;
-  BSt *st_in;   // state of the previous pass (the removal flag is also set here)
-  BSt *st_out;  // new state, written by the owner halves
+  BSt *bst0, *bst1;   // the two state buffers
+  const int *curp;    // device index of the buffer holding the latest state
   uint8_t *fl8; // per-bond flag copy, written by the owner half when it changes
   const uint32_t *cj, *ccnt;
   float4 *cO;
   const float *dtp;  // when set, the time step of the last move (FIRE) replaces mt.dt
   const int *done;   // when set and nonzero (relaxation finished), the pass does nothing
   int trial;         // trial pass: writes only F, M and st_out (nothing the next pass reads)
+  uint32_t *nbreak;  // when set, counts the bonds this pass breaks
 };
 
 __device__ __forceinline__ EndData load_end(const SysDev &s, uint32_t i) {
@@ -318,6 +322,10 @@ k_nat_forces(SysDev s, Mat mt, uint32_t ntiles, uint32_t per) {
   if (tile >= ntiles) return;  // padding tiles of the last XCD run
   if (s.done && *s.done) return;
   if (s.dtp) mt.dt = *s.dtp;
+  // the pass reads the latest state and writes the other buffer (the index
+  // flips on the device after the pass, only when the pass ran)
+  BSt *const st_in = *s.curp ? s.bst1 : s.bst0;
+  BSt *const st_out = *s.curp ? s.bst0 : s.bst1;
   const uint32_t tid = threadIdx.x;
   const uint32_t p0 = tile * kFP;
   const uint32_t np = min((uint32_t)kFP, s.N - p0);
@@ -345,7 +353,7 @@ k_nat_forces(SysDev s, Mat mt, uint32_t ntiles, uint32_t per) {
       }
       const HR hr = s.hr[h];
       const uint32_t i = p0 + lo;
-      const BSt old = s.st_in[hr.b];
+      const BSt old = st_in[hr.b];
       BondOut o;
       if (old.fl == kRemoved) {
         o.Fi = o.Mbi = o.Mci = kZero;
@@ -360,12 +368,15 @@ k_nat_forces(SysDev s, Mat mt, uint32_t ntiles, uint32_t per) {
         if (own) {
           if (o.st.fl == kRemoved) {
             // both buffers: no later pass may read a stale live flag
-            s.st_out[hr.b].fl = kRemoved;
-            if (!s.trial) s.st_in[hr.b].fl = kRemoved;
+            st_out[hr.b].fl = kRemoved;
+            if (!s.trial) st_in[hr.b].fl = kRemoved;
           } else {
-            s.st_out[hr.b] = o.st;
+            st_out[hr.b] = o.st;
           }
-          if (!s.trial && o.st.fl != old.fl) s.fl8[hr.b] = (uint8_t)o.st.fl;
+          if (!s.trial && o.st.fl != old.fl) {
+            s.fl8[hr.b] = (uint8_t)o.st.fl;
+            if (s.nbreak && o.st.fl == kBroken) atomicAdd(s.nbreak, 1u);
+          }
         }
       }
       const uint32_t k = e - c0;
@@ -446,6 +457,11 @@ k_nat_forces(SysDev s, Mat mt, uint32_t ntiles, uint32_t per) {
   if (mt.refmode && anyc && !((double)dot(me.W, me.W) > 1e-18)) Mc = kZero;
   s.F[i] = f4(F);
   s.M[i] = f4(Mb + Mc);
+}
+
+// flips the state-buffer index after a force pass that ran
+__global__ void k_flip_state(int *curp, const int *done) {
+  if (threadIdx.x == 0 && blockIdx.x == 0 && !(done && *done)) curp[0] = 1 - curp[0];
 }
 
 // ---------------------------------------------------------- particle pass
@@ -1239,7 +1255,7 @@ struct rfd_system {
   uint32_t H = 0;  // half-bonds (2 per bond)
   std::vector<uint32_t> hbi, hbj;  // host copy of the bond ends
   BSt *bst[2] = {nullptr, nullptr};
-  int cur = 0;  // state buffer holding the latest state (the force pass writes the other)
+  int *d_cur = nullptr;  // device index of the buffer holding the latest state
   uint32_t *scan_tot = nullptr, *scan_off = nullptr;  // chunk totals / offsets
   uint32_t scan_cap = 0;
   uint32_t *cj[2] = {nullptr, nullptr}, *ccnt[2] = {nullptr, nullptr};
@@ -1265,6 +1281,7 @@ struct rfd_system {
   double *fire_part = nullptr;
   P4 *P0 = nullptr;          // positions at the start of the load step
   float4 *Vf = nullptr;      // FIRE velocity
+  uint32_t *d_nbreak = nullptr;  // bonds broken by the last break-check pass
   bool forces_current = false;  // the load step's first force pass is done
   float4 *Th = nullptr;      // rotation accumulated during the load step
   float *du[2] = {nullptr, nullptr}, *dth[2] = {nullptr, nullptr};
@@ -1323,8 +1340,9 @@ SysDev make_dev(rfd_system *s) {
   d.hoff = s->hoff;
   d.hr = s->hr;
   d.brc = s->brc;
-  d.st_in = s->bst[s->cur];
-  d.st_out = s->bst[1 - s->cur];
+  d.bst0 = s->bst[0];
+  d.bst1 = s->bst[1];
+  d.curp = s->d_cur;
   d.fl8 = s->fl8;
   d.cj = s->cj[s->ccur];
   d.ccnt = s->ccnt[s->ccur];
@@ -1405,17 +1423,24 @@ int rebuild_rows(rfd_system *s) {
 }
 
 // dtp set: a quasi-static (FIRE) pass -- the time step is that of the last
-// move, and the contact law drops its velocity damping, so that the residual
-// FIRE judges is the static out-of-balance force
-int forces(rfd_system *s, const float *dtp = nullptr, bool trial = false) {
+// move, and the contact law drops its velocity-dependent terms (normal and
+// tangential damping, rolling resistance against the spin direction), so
+// that the residual FIRE judges is the static out-of-balance force and moment
+int forces(rfd_system *s, const float *dtp = nullptr, bool trial = false,
+           bool nobreak = false) {
   SysDev d = make_dev(s);
   d.dtp = dtp;
   d.trial = trial ? 1 : 0;
+  d.nbreak = s->d_nbreak;
   // after convergence the remaining launches of a FIRE chunk must not apply
   // the last move to the springs again
   if (dtp) d.done = (const int *)((const char *)s->fire + offsetof(FireDev, done));
   Mat m = make_mat(s->p);
-  if (dtp) m.pA = 0.f;
+  if (dtp) {
+    m.pA = 0.f;
+    m.mur = 0.f;
+  }
+  m.nobreak = nobreak ? 1 : 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (s->timing && s->ev_used + 2 <= s->ev.size()) {
     e0 = s->ev[s->ev_used];
@@ -1427,7 +1452,8 @@ int forces(rfd_system *s, const float *dtp = nullptr, bool trial = false) {
   const uint32_t per = (nb + kXcd - 1) / kXcd;
   k_nat_forces<<<per * kXcd, kFT, 0, s->ctx->stream>>>(d, m, nb, per);
   RFD_HIP(s->ctx, hipGetLastError());
-  s->cur = 1 - s->cur;
+  k_flip_state<<<1, 64, 0, s->ctx->stream>>>(s->d_cur, d.done);
+  RFD_HIP(s->ctx, hipGetLastError());
   if (e1) RFD_HIP(s->ctx, hipEventRecord(e1, s->ctx->stream));
   return RFD_OK;
 }
@@ -1465,6 +1491,7 @@ int fire_alloc(rfd_system *s) {
   if (!st) st = dalloc(ctx, &s->fire_part, (size_t)kFireVals * s->nblocks_pass);
   if (!st) st = dalloc(ctx, &s->P0, s->N);
   if (!st) st = dalloc(ctx, &s->Vf, s->N);
+  if (!st) st = dalloc(ctx, &s->d_nbreak, 1);
   if (!st) st = dalloc(ctx, &s->Th, s->N);
   for (int q = 0; q < 2 && !st; ++q) {
     st = dalloc(ctx, &s->du[q], 3 * (size_t)s->N);
@@ -1508,7 +1535,7 @@ int fire_iteration(rfd_system *s) {
       int st = rebuild_rows(s);
       if (st) return st;
     }
-    int st = forces(s, s->fire_dt);
+    int st = forces(s, s->fire_dt, false, true);
     if (st) return st;
   }
   FireParticles fp = fire_particles(s);
@@ -1545,13 +1572,15 @@ int begin_pass(rfd_system *s, const FireDev &f, const float *dU, const float *dT
   else
     k_fire_begin<0><<<s->nblocks_pass, 256, 0, q>>>(fpp, dU, dTh, dt0, dtop);
   RFD_HIP(ctx, hipGetLastError());
-  const int cur0 = s->cur;
-  int st = forces(s, s->fire_dt, trial);
+  int cur0 = 0;
+  if (trial) RFD_HIP(ctx, hipMemcpy(&cur0, s->d_cur, sizeof(int), hipMemcpyDeviceToHost));
+  int st = forces(s, s->fire_dt, trial, true);
   if (st) return st;
   k_zero_f4<<<rfd_blocks(s->N, 256), 256, 0, q>>>(s->V, s->W, s->N);
   RFD_HIP(ctx, hipGetLastError());
   if (trial) {
-    s->cur = cur0;  // the new state went to the other buffer: drop it
+    // the new state went to the other buffer: drop it
+    RFD_HIP(ctx, hipMemcpy(s->d_cur, &cur0, sizeof(int), hipMemcpyHostToDevice));
     FireDev fm = f;
     fm.measure_only = 1;
     RFD_HIP(ctx, hipMemcpyAsync(s->fire, &fm, sizeof(fm), hipMemcpyHostToDevice, q));
@@ -1627,6 +1656,7 @@ int rfd_system_create(rfd_ctx *ctx, const rfd_system_params *p, uint32_t N, rfd_
   if (!st) st = dalloc(ctx, &s->csorted, N);
   if (!st) st = dalloc(ctx, &s->cpos, N);
   if (!st) st = dalloc(ctx, &s->ctl, 1);
+  if (!st) st = dalloc(ctx, &s->d_cur, 1);
   s->nblocks_pass = rfd_blocks(N, 256, 2048);
   if (!st) st = dalloc(ctx, &s->part, 2 * (size_t)s->nblocks_pass);
   if (!st) st = dalloc(ctx, &s->hist, 3 * (size_t)kHistCap);
@@ -1659,6 +1689,7 @@ int rfd_system_destroy(rfd_system *s) {
   dfree(s->hm);
   dfree(s->brc);
   dfree(s->Vf);
+  dfree(s->d_nbreak);
   for (int q = 0; q < 2; ++q) {
     dfree(s->bst[q]);
     dfree(s->cj[q]);
@@ -1674,6 +1705,7 @@ int rfd_system_destroy(rfd_system *s) {
   dfree(s->fl8);
   dfree(s->ccnt_cells);
   dfree(s->ctl);
+  dfree(s->d_cur);
   dfree(s->part);
   dfree(s->hist);
   dfree(s->scan_tot);
@@ -1899,7 +1931,10 @@ int rfd_system_set_bonds(rfd_system *s, uint32_t Nb, const uint32_t *bi, const u
       RFD_HIP(ctx, hipGetLastError());
     }
   }
-  s->cur = 0;
+  {
+    const int zero = 0;
+    RFD_HIP(ctx, hipMemcpy(s->d_cur, &zero, sizeof(int), hipMemcpyHostToDevice));
+  }
   s->have_bonds = true;
   s->rows_valid = false;
   RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1955,7 +1990,9 @@ int rfd_system_get_bond_state(rfd_system *s, float *state, uint8_t *flag) {
   RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
   const uint32_t Nb = s->Nb;
   std::vector<BSt> hs(Nb);
-  RFD_HIP(ctx, hipMemcpy(hs.data(), s->bst[s->cur], sizeof(BSt) * Nb, hipMemcpyDeviceToHost));
+  int cur = 0;
+  RFD_HIP(ctx, hipMemcpy(&cur, s->d_cur, sizeof(int), hipMemcpyDeviceToHost));
+  RFD_HIP(ctx, hipMemcpy(hs.data(), s->bst[cur], sizeof(BSt) * Nb, hipMemcpyDeviceToHost));
   if (state) {
     std::vector<double> n;
     int st = bond_dirs(s, n);
@@ -2161,14 +2198,46 @@ int rfd_system_fire_load_step(rfd_system *s, double dstrain, const rfd_fire_para
   s->step_count++;
   s->forces_current = true;
   const uint32_t chunk = fp->check_every ? fp->check_every : 50;
-  for (;;) {
-    for (uint32_t k = 0; k < chunk; ++k) {
-      st = fire_iteration(s);
-      if (st) return st;
+  const FireDev f_start = f;
+  auto relax = [&]() -> int {
+    for (;;) {
+      for (uint32_t k = 0; k < chunk; ++k) {
+        int e = fire_iteration(s);
+        if (e) return e;
+      }
+      RFD_HIP(ctx, hipMemcpyAsync(&f, s->fire, sizeof(f), hipMemcpyDeviceToHost, q));
+      RFD_HIP(ctx, hipStreamSynchronize(q));
+      if (f.done) return RFD_OK;
     }
-    RFD_HIP(ctx, hipMemcpyAsync(&f, s->fire, sizeof(f), hipMemcpyDeviceToHost, q));
+  };
+  st = relax();
+  if (st) return st;
+  const FireDev f_first = f;
+  uint32_t iters = f.iter, breaks = 0, rounds = 0;
+  // bonds are judged at equilibrium: a pass without increments breaks the
+  // over-stressed ones, and the step relaxes again until none breaks (a
+  // relaxation that did not converge is not judged)
+  while (f.converged) {
+    FireDev fc = f_start;
+    fc.dt_f = 0.0f;
+    RFD_HIP(ctx, hipMemcpyAsync(s->fire, &fc, sizeof(fc), hipMemcpyHostToDevice, q));
+    RFD_HIP(ctx, hipMemsetAsync(s->d_nbreak, 0, sizeof(uint32_t), q));
+    st = forces(s, s->fire_dt, false, false);
+    if (st) return st;
+    uint32_t nb = 0;
+    RFD_HIP(ctx, hipMemcpyAsync(&nb, s->d_nbreak, sizeof(nb), hipMemcpyDeviceToHost, q));
     RFD_HIP(ctx, hipStreamSynchronize(q));
-    if (f.done) break;
+    if (nb == 0 || rounds >= kMaxBreakRounds) break;
+    breaks += nb;
+    ++rounds;
+    RFD_HIP(ctx, hipMemcpyAsync(s->fire, &f_start, sizeof(f_start), hipMemcpyHostToDevice, q));
+    k_zero_f4<<<rfd_blocks(s->N, 256), 256, 0, q>>>(s->V, s->W, s->N);
+    k_zero_f4<<<rfd_blocks(s->N, 256), 256, 0, q>>>(s->Vf, nullptr, s->N);
+    RFD_HIP(ctx, hipGetLastError());
+    s->forces_current = true;  // the check pass left the forces of this state
+    st = relax();
+    if (st) return st;
+    iters += f.iter;
   }
   s->forces_current = false;
   // the step's displacement and rotation, newest first; the relaxed state is
@@ -2180,9 +2249,11 @@ int rfd_system_fire_load_step(rfd_system *s, double dstrain, const rfd_fire_para
   RFD_HIP(ctx, hipGetLastError());
   st = rfd_system_get_loading(s, &L);
   if (st) return st;
-  out->iterations = f.iter;
+  out->iterations = iters;
   out->converged = f.converged;
-  out->fmax0 = f.fmax0;
+  out->breaks = breaks;
+  out->break_rounds = rounds;
+  out->fmax0 = f_first.fmax0;
   out->fmax = f.fmax;
   out->mmax = f.mmax;
   out->sum_bottom = f.sum_bottom;
@@ -2211,7 +2282,7 @@ int rfd_system_residual(rfd_system *s, double out[2]) {
     st = rebuild_rows(s);
     if (st) return st;
   }
-  st = forces(s, s->fire_dt);
+  st = forces(s, s->fire_dt, false, true);
   if (st) return st;
   FireParticles fp = fire_particles(s);
   if (s->mode == 1)
@@ -2269,7 +2340,11 @@ int rfd_system_bond_graph(rfd_system *s, const uint32_t **hoff, const void **hal
   if (hoff) *hoff = s->hoff;
   if (half) *half = s->hr;
   if (bond) *bond = s->brc;
-  if (state) *state = s->bst[s->cur];
+  if (state) {
+    int cur = 0;
+    RFD_HIP(s->ctx, hipMemcpy(&cur, s->d_cur, sizeof(int), hipMemcpyDeviceToHost));
+    *state = s->bst[cur];
+  }
   if (H) *H = s->H;
   return RFD_OK;
 }

@@ -57,7 +57,8 @@ class FireResult(C.Structure):
     _fields_ = [("iterations", C.c_uint32), ("converged", C.c_int), ("fmax0", C.c_double),
                 ("fmax", C.c_double), ("mmax", C.c_double), ("sum_bottom", C.c_double),
                 ("sum_top", C.c_double), ("strain", C.c_double), ("seconds", C.c_double),
-                ("warm_used", C.c_int), ("fmax0_plain", C.c_double), ("fmax0_pred", C.c_double)]
+                ("warm_used", C.c_int), ("fmax0_plain", C.c_double), ("fmax0_pred", C.c_double),
+                ("breaks", C.c_uint32), ("break_rounds", C.c_uint32)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -31,7 +31,8 @@ class NativeCase:
 def uniaxial_cylinder(ctx, D: float, H: float, seed: int = specimen.BASE_SEED,
                       bond_gap: float | None = None, refmode: int = 0,
                       strength=(20e6, 40e6), morton: bool = True, rebuild_every: int = 20,
-                      v_load: float = -10.0 * 0.02 / VELOCITY) -> NativeCase:
+                      v_load: float = -10.0 * 0.02 / VELOCITY,
+                      flat_ends: bool = False) -> NativeCase:
     """Polydisperse cylinder (r ~ U[0.4, 0.6] mm) on a jittered FCC lattice of
     spacing 1 mm.  bond_gap=None joins every nearest lattice neighbour (Nb/N
     close to 6); otherwise pairs with surface gap < bond_gap are joined."""
@@ -39,7 +40,7 @@ def uniaxial_cylinder(ctx, D: float, H: float, seed: int = specimen.BASE_SEED,
     g = specimen.rng(seed + 1)
     a = 2.0 * 2.9 * 0.6
     box = SampleBox(D, D, H, space=(D + 16.0, D + 16.0, H + 12.0), cell_a=a, p_r=0.5)
-    sp = specimen.cylinder(D, H, center=box.center, d_nn=1.0, seed=seed)
+    sp = specimen.cylinder(D, H, center=box.center, d_nn=1.0, seed=seed, flat_ends=flat_ends)
     R, rad = sp.R, sp.radius
     perm = morton_order(R, 2.0) if morton else np.arange(len(R))
     R, rad = R[perm], rad[perm]

@@ -53,10 +53,14 @@ def fcc_points(extent: tuple[float, float, float], d_nn: float) -> np.ndarray:
 
 def cylinder(D: float, H: float, center=(0.0, 0.0, 0.0), d_nn: float = 1.0,
              r_lo: float = 0.4, r_hi: float = 0.6, jitter: float = 0.02,
-             seed: int = BASE_SEED, monodisperse: float | None = None) -> Specimen:
+             seed: int = BASE_SEED, monodisperse: float | None = None,
+             flat_ends: bool = False) -> Specimen:
     """Cylinder of diameter D and height H (axis z).  Particles are kept when
     the whole sphere of the mean radius lies inside.  ``monodisperse`` sets
-    every radius to that value instead of drawing U[r_lo, r_hi]."""
+    every radius to that value instead of drawing U[r_lo, r_hi].
+    ``flat_ends`` keeps the lowest and highest lattice layers unjittered in z
+    (capped ends), so that a platen placed at the end layer holds the whole
+    layer from the first step instead of a few high particles."""
     g = rng(seed)
     pts = fcc_points((D, D, H), d_nn)
     rm = monodisperse if monodisperse is not None else 0.5 * (r_lo + r_hi)
@@ -66,7 +70,11 @@ def cylinder(D: float, H: float, center=(0.0, 0.0, 0.0), d_nn: float = 1.0,
     # deterministic order: sort by (z, y, x) lattice position
     order = np.lexsort((pts[:, 0], pts[:, 1], pts[:, 2]))
     pts = pts[order]
-    pts = pts + g.uniform(-jitter, jitter, size=pts.shape)
+    jit = g.uniform(-jitter, jitter, size=pts.shape)
+    if flat_ends:
+        z = pts[:, 2]
+        jit[(z == z.max()) | (z == z.min()), 2] = 0.0
+    pts = pts + jit
     if monodisperse is not None:
         rad = np.full(len(pts), monodisperse)
     else:

@@ -312,11 +312,13 @@ def predict(net: WarmStartNet, sys, dstrain: float):
     return dU, dTh
 
 
-def collect_samples(sys, dstrain: float, fp, nsteps: int, warm=None) -> tuple[list, list]:
+def collect_samples(sys, dstrain: float, fp, nsteps: int, warm=None,
+                    log=None) -> tuple[list, list]:
     """Run nsteps load steps of dstrain (plain FIRE, or warm-started by
     ``warm(sys) -> (dU, dTh)`` when given) and keep, for every step that has
     two previous increments, the network inputs at its start and the
-    corrections it needed.  Returns (samples, results)."""
+    corrections it needed.  ``log(k, result)`` is called after each step.
+    Returns (samples, results)."""
     samples, results = [], []
     g = graph_of(sys)
     for k in range(nsteps):
@@ -328,6 +330,8 @@ def collect_samples(sys, dstrain: float, fp, nsteps: int, warm=None) -> tuple[li
             dU, dTh = warm(sys)
         r = sys.fire_load_step(dstrain, fp, dU, dTh)
         results.append(r)
+        if log is not None:
+            log(k, r)
         if X is not None:
             samples.append(Sample(g, X, E, targets_after_step(sys, dstrain, X)))
     return samples, results
