@@ -196,6 +196,9 @@ typedef struct rfd_system_params {
   int refmode;           /* 1: reference bending-moment factor
                             (md_calculateforcesdem3.cu:148) and rolling-moment
                             wipe (:330-335); 0: physical law                */
+  int fp64;              /* 1: every quantity of the step in double (the GPU
+                            precision oracle of the FP32 path; explicit steps
+                            only, no FIRE)                                  */
 } rfd_system_params;
 
 /* loading control, device resident; mirrors compression_data

@@ -476,11 +476,17 @@ struct PassDev {
   double *part;
 };
 
+// 1/sqrt for the platen friction: correctly rounded in FP32, plain in FP64
+__device__ __forceinline__ float platen_rsqrt(float x) { return rsqrt_rn(x); }
+__device__ __forceinline__ double platen_rsqrt(double x) { return 1.0 / __builtin_sqrt(x); }
+
 // platen clamp + reaction sums (md_uniaxialcompression3_simple.cu:19-76,
 // md_braziltest_simple2.cu:19-92).  MODE 1: load axis z, 2: load axis y.
-template <int MODE>
-__device__ __forceinline__ void platen(const rfd_loading &L, P4 &R, v3 &V, v3 &F, double &sb,
+// VT is the velocity/force vector type (FP32 v3 or FP64 d3).
+template <int MODE, class VT>
+__device__ __forceinline__ void platen(const rfd_loading &L, P4 &R, VT &V, VT &F, double &sb,
                                        double &stp) {
+  using T = decltype(V.x);
   double ra, rb, h, zc;
   if (MODE == 1) {
     ra = R.x - L.cx;
@@ -494,45 +500,46 @@ __device__ __forceinline__ void platen(const rfd_loading &L, P4 &R, v3 &V, v3 &F
     zc = R.z - L.cz;
   }
   if (!(ra * ra + rb * rb < L.RR && zc < L.zcut)) return;
-  float &Fa = MODE == 1 ? F.z : F.y;
-  float &Va = MODE == 1 ? V.z : V.y;
+  T &Fa = MODE == 1 ? F.z : F.y;
+  T &Va = MODE == 1 ? V.z : V.y;
   double &Ra = MODE == 1 ? R.z : R.y;
+  const T fr = (T)0.45f;
   if (h - L.top_r > -1e-9) {
-    const float f = Fa;
+    const T f = Fa;
     if (f > 0) {
       stp += f;
       if (MODE == 2) {
-        float vv = V.x * V.x + V.z * V.z;
+        T vv = V.x * V.x + V.z * V.z;
         if ((double)vv > 1e-9) {
-          vv = rsqrt_rn(vv);
-          F.x = F.x - 0.45f * f * V.x * vv;
-          F.z = F.z - 0.45f * f * V.z * vv;
+          vv = platen_rsqrt(vv);
+          F.x = F.x - fr * f * V.x * vv;
+          F.z = F.z - fr * f * V.z * vv;
         }
       }
     }
     if (h - L.top > -1e-9) {
       Ra = L.top;
-      Va = (float)L.v;
-      Fa = 0.f;
+      Va = (T)L.v;
+      Fa = (T)0;
     }
   }
   if (h - L.bottom_r < 1e-9) {
-    const float f = Fa;
+    const T f = Fa;
     if (f < 0) {
       sb += f;
       if (MODE == 2) {
-        float vv = V.x * V.x + V.z * V.z;
+        T vv = V.x * V.x + V.z * V.z;
         if ((double)vv > 1e-9) {
-          vv = rsqrt_rn(vv);
-          F.x = F.x + 0.45f * f * V.x * vv;
-          F.z = F.z + 0.45f * f * V.z * vv;
+          vv = platen_rsqrt(vv);
+          F.x = F.x + fr * f * V.x * vv;
+          F.z = F.z + fr * f * V.z * vv;
         }
       }
     }
     if (h - L.bottom < 1e-9) {
       Ra = L.bottom;
-      Va = 0.f;
-      Fa = 0.f;
+      Va = (T)0;
+      Fa = (T)0;
     }
   }
 }
@@ -826,6 +833,8 @@ struct NbrDev {
   const uint8_t *fl8;
   const uint32_t *cj_old, *ccnt_old;
   const float4 *cO_old;
+  const double4 *cO64_old;  // FP64 path springs (carried over beside cO when set)
+  double4 *cO64_new;
   uint32_t *cj_new, *ccnt_new;
   float4 *cO_new;
   uint32_t *flags;
@@ -894,15 +903,18 @@ __global__ void __launch_bounds__(128) k_nat_neighbours(NbrDev s, uint32_t per) 
               const size_t sl = (size_t)nfound * N + i;
               s.cj_new[sl] = j;
               float4 O = make_float4(0.f, 0.f, 0.f, 0.f);
+              double4 O64 = make_double4(0.0, 0.0, 0.0, 0.0);
               const uint32_t no = s.ccnt_old ? s.ccnt_old[i] : 0;
               for (uint32_t k = 0; k < no; ++k) {
                 const size_t so = (size_t)k * N + i;
                 if (s.cj_old[so] == j) {
                   O = s.cO_old[so];
+                  if (s.cO64_new) O64 = s.cO64_old[so];
                   break;
                 }
               }
               s.cO_new[sl] = O;
+              if (s.cO64_new) s.cO64_new[sl] = O64;
             } else {
               atomicOr(s.flags, RFD_FLAG_ROW_FULL);
             }
@@ -1240,6 +1252,296 @@ __global__ void k_mpnn_inputs(const P4 *__restrict__ P, uint32_t N,
   }
 }
 
+// =============================================================== FP64 path
+// The same native-tier step (bond law with break check, contact law, platens,
+// integration, loading control) with every quantity in double: the GPU
+// precision oracle of the FP32 path.  One thread per particle walks its
+// half-bond row and contact row in order (deterministic, no LDS staging: this
+// path is for checking, not for speed).  Contact rows, cells, positions and
+// the loading control are shared with the FP32 path.
+struct d3 {
+  double x, y, z;
+};
+__device__ __forceinline__ d3 operator+(d3 a, d3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ d3 operator-(d3 a, d3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ d3 operator*(d3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ double dot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ d3 cross(d3 a, d3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ d3 xyz(double4 q) { return {q.x, q.y, q.z}; }
+__device__ __forceinline__ double4 d4(d3 v) { return make_double4(v.x, v.y, v.z, 0.0); }
+__device__ constexpr d3 kZero64 = {0.0, 0.0, 0.0};
+
+struct alignas(16) BSt64 {
+  double ox, oy, oz, mn, mx, my, mz;
+  uint32_t fl, pad;
+};
+
+struct Mat64 {
+  double E, G, Estar, Gstar, pA, mu, mur, rho43pi, dt, farcut;
+  int refmode;
+};
+
+struct Rot64 {
+  double m[9];
+  __device__ __forceinline__ void build(d3 t, d3 ph) {
+    m[0] = 1.0 + t.z * ph.z + t.y * ph.y;
+    m[1] = ph.z - t.z - t.y * ph.x;
+    m[2] = -ph.y - t.z * ph.x + t.y;
+    m[3] = t.z - ph.z - t.x * ph.y;
+    m[4] = t.z * ph.z + 1.0 + t.x * ph.x;
+    m[5] = -t.z * ph.y + ph.x - t.x;
+    m[6] = -t.y - t.x * ph.z + ph.y;
+    m[7] = -t.y * ph.z + t.x - ph.x;
+    m[8] = t.y * ph.y + t.x * ph.x + 1.0;
+  }
+  __device__ __forceinline__ d3 apply(d3 x) const {
+    return {m[0] * x.x + m[1] * x.y + m[2] * x.z, m[3] * x.x + m[4] * x.y + m[5] * x.z,
+            m[6] * x.x + m[7] * x.y + m[8] * x.z};
+  }
+};
+
+__device__ __forceinline__ void contact_law64(const Mat64 &mt, d3 n, double L, double ri,
+                                              double rj, d3 Vi, d3 Vj, d3 Wi, d3 Wj, d3 &O,
+                                              d3 &Fi, d3 &Mi, d3 &Mj) {
+  const double Rs = ri * rj / (ri + rj);
+  const double mi = mt.rho43pi * ri * ri * ri, mj = mt.rho43pi * rj * rj * rj;
+  const double ms = mi * mj / (mi + mj);
+  const double q = __builtin_sqrt(Rs * (ri + rj - L));
+  const double Kn = 2.0 * mt.Estar * q, Kt = 8.0 * mt.Gstar * q;
+  const d3 v = Vj - Vi + cross(n, Wi * ri + Wj * rj);
+  const double vn = dot(n, v);
+  const double fn = (L - (ri + rj)) * Kn * (2.0 / 3.0);
+  const double dmp = -1.8257 * mt.pA * vn * __builtin_sqrt(Kn * ms);
+  Fi = n * (fn + dmp);
+  const d3 vt = v - n * vn;
+  d3 o = O - n * dot(n, O);
+  const double oo = dot(o, o);
+  if (oo > 1e-18) o = o * __builtin_sqrt(dot(O, O) / oo);
+  o = o + vt * mt.dt;
+  d3 ft = o * Kt;
+  const double ftm = dot(ft, ft);
+  if (ftm > mt.mu * mt.mu * fn * fn) {
+    ft = ft * (mt.mu * fabs(fn) / __builtin_sqrt(ftm));
+    o = ft * (1.0 / Kt);
+  } else {
+    ft = ft + vt * (-1.8257 * mt.pA * __builtin_sqrt(Kt * ms));
+  }
+  O = o;
+  Fi = Fi + ft;
+  const d3 nxf = cross(n, ft);
+  Mi = nxf * ri;
+  Mj = nxf * rj;
+  const double wi2 = dot(Wi, Wi), wj2 = dot(Wj, Wj);
+  if (wi2 > 1e-18) Mi = Mi + Wi * (-mt.mur * fabs(fn) / __builtin_sqrt(wi2));
+  if (wj2 > 1e-18) Mj = Mj + Wj * (-mt.mur * fabs(fn) / __builtin_sqrt(wj2));
+}
+
+struct Sys64 {
+  uint32_t N, Kc;
+  const P4 *P;
+  const double4 *V, *W;
+  double4 *F, *M;
+  const uint32_t *hoff;
+  const HR *hr;
+  const BR *brc;
+  BSt64 *b0, *b1;
+  const int *curp;
+  uint8_t *fl8;
+  const uint32_t *cj, *ccnt;
+  double4 *cO;
+};
+
+__global__ void __launch_bounds__(128) k_f64_forces(Sys64 s, Mat64 mt) {
+  BSt64 *const st_in = *s.curp ? s.b1 : s.b0;
+  BSt64 *const st_out = *s.curp ? s.b0 : s.b1;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < s.N; i += grid_stride()) {
+    const P4 A = s.P[i];
+    const d3 Vi = xyz(s.V[i]), Wi = xyz(s.W[i]);
+    d3 F = kZero64, Mb = kZero64, Mc = kZero64;
+    bool anyc = false;
+    for (uint32_t h = s.hoff[i]; h < s.hoff[i + 1]; ++h) {
+      const HR q = s.hr[h];
+      const BSt64 old = st_in[q.b];
+      if (old.fl == kRemoved) continue;
+      const bool own = i < q.j;
+      const double sg = own ? 1.0 : -1.0;
+      const P4 B = s.P[q.j];
+      const d3 Vj = xyz(s.V[q.j]), Wj = xyz(s.W[q.j]);
+      const BR c = s.brc[q.b];
+      const d3 r = {B.x - A.x, B.y - A.y, B.z - A.z};
+      const double r2 = dot(r, r);
+      const double lim = mt.farcut * (A.r + B.r);
+      BSt64 nw = old;
+      if (r2 > lim * lim || !(r2 > 1e-10)) {
+        if (own) {
+          st_out[q.b].fl = kRemoved;
+          st_in[q.b].fl = kRemoved;
+          s.fl8[q.b] = kRemoved;
+        }
+        continue;
+      }
+      const double L = __builtin_sqrt(r2);
+      const d3 n = r * (1.0 / L);
+      d3 O = d3{old.ox, old.oy, old.oz} * sg;
+      bool contact_now = old.fl == kBroken;
+      if (old.fl == kBond) {
+        const d3 rp = r - (Vj - Vi) * mt.dt;
+        const double pp = dot(rp, rp);
+        const d3 np = pp > 0.0 ? rp * (1.0 / __builtin_sqrt(pp)) : n;
+        const d3 Mt0 = d3{old.mx, old.my, old.mz} * sg;
+        const double il = (double)c.il, br = (double)c.br;
+        const double A2 = 3.14159265358979323846 * br * br, J = A2 * br * br;
+        const double eps = L * il - 1.0;
+        const double sig = mt.E * eps;
+        const d3 Fn = n * (sig * A2);
+        const d3 ws = Wi + Wj;
+        Rot64 M;
+        M.build(cross(np, n), n * (dot(ws, n) * mt.dt * 0.5));
+        const d3 dw = Wj - Wi;
+        const double dwn = dot(n, dw);
+        const double mn = old.mn + dwn * (mt.dt * 0.5 * mt.G * il);
+        const double cb = mt.dt * 0.25 * mt.E * il;
+        const d3 Mt = M.apply(Mt0) + (dw - n * dwn) * cb;
+        const d3 v = Vj - Vi - cross(ws * 0.5, r);
+        const double vn = dot(n, v);
+        O = M.apply(O) + (v - n * vn) * mt.dt;
+        const d3 Ft = O * (mt.G * il * A2);
+        const double mtf = mt.refmode ? cb * J : J;
+        F = F + Fn + Ft;
+        Mb = Mb + n * (mn * J) + Mt * mtf + cross(r, Ft) * 0.5;
+        const double Sc = sig + __builtin_sqrt(dot(Mt, Mt)) * 4.0 * br;
+        const double Mcr = mt.G * il * __builtin_sqrt(dot(O, O)) + fabs(mn) * 2.0 * br;
+        if (Sc > (double)c.sc || Mcr > (double)c.tc) {
+          nw.fl = kBroken;
+          O = kZero64;
+          nw.mn = nw.mx = nw.my = nw.mz = 0.0;
+          contact_now = true;
+        } else {
+          const d3 Os = O * sg, Ms = Mt * sg;
+          nw.ox = Os.x;
+          nw.oy = Os.y;
+          nw.oz = Os.z;
+          nw.mn = mn;
+          nw.mx = Ms.x;
+          nw.my = Ms.y;
+          nw.mz = Ms.z;
+        }
+      }
+      if (contact_now) {
+        if (L < A.r + B.r) {
+          d3 Fc, Mci, Mcj;
+          contact_law64(mt, n, L, A.r, B.r, Vi, Vj, Wi, Wj, O, Fc, Mci, Mcj);
+          F = F + Fc;
+          Mc = Mc + Mci;
+          anyc = true;
+        } else {
+          O = kZero64;
+        }
+        const d3 Os = O * sg;
+        nw.ox = Os.x;
+        nw.oy = Os.y;
+        nw.oz = Os.z;
+      }
+      if (own) {
+        st_out[q.b] = nw;
+        if (nw.fl != old.fl) s.fl8[q.b] = (uint8_t)nw.fl;
+      }
+    }
+    const uint32_t nc = s.ccnt[i];
+    for (uint32_t k = 0; k < nc; ++k) {
+      const size_t sl = (size_t)k * s.N + i;
+      const uint32_t j = s.cj[sl];
+      const P4 B = s.P[j];
+      const d3 r = {B.x - A.x, B.y - A.y, B.z - A.z};
+      const double L = __builtin_sqrt(dot(r, r));
+      if (L < A.r + B.r && L > 1e-5) {
+        d3 O = xyz(s.cO[sl]);
+        d3 Fi, Mi, Mj;
+        contact_law64(mt, r * (1.0 / L), L, A.r, B.r, Vi, xyz(s.V[j]), Wi, xyz(s.W[j]), O, Fi,
+                      Mi, Mj);
+        F = F + Fi;
+        Mc = Mc + Mi;
+        anyc = true;
+        s.cO[sl] = d4(O);
+      } else {
+        s.cO[sl] = make_double4(0.0, 0.0, 0.0, 0.0);
+      }
+    }
+    if (mt.refmode && anyc && !(dot(Wi, Wi) > 1e-18)) Mc = kZero64;
+    s.F[i] = d4(F);
+    s.M[i] = d4(Mb + Mc);
+  }
+}
+
+struct Pass64 {
+  uint32_t N;
+  P4 *P;
+  double4 *V, *W, *F;
+  const double4 *M;
+  double dt, rho43pi;
+  rfd_loading *ctl;
+  double *part;
+};
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_f64_pass(Pass64 p) {
+  __shared__ double s_part[8];
+  const rfd_loading L = *p.ctl;
+  double sb = 0.0, stp = 0.0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.N; i += grid_stride()) {
+    P4 R = p.P[i];
+    d3 V = xyz(p.V[i]), F = xyz(p.F[i]);
+    if (MODE) platen<MODE>(L, R, V, F, sb, stp);
+    const double m = p.rho43pi * R.r * R.r * R.r, I = 0.4 * m * R.r * R.r;
+    V = V + F * (p.dt / m);
+    R.x += V.x * p.dt;
+    R.y += V.y * p.dt;
+    R.z += V.z * p.dt;
+    const d3 W = xyz(p.W[i]) + xyz(p.M[i]) * (p.dt / I);
+    if (L.cut) {
+      const double dx = R.x - L.box_c[0], dy = R.y - L.box_c[1], dz = R.z - L.box_c[2];
+      if (dx - L.box_h[0] > 0 || dx + L.box_h[0] < 0 || dy - L.box_h[1] > 0 ||
+          dy + L.box_h[1] < 0 || dz - L.box_h[2] > 0 || dz + L.box_h[2] < 0) {
+        R.x = L.hidden[0];
+        R.y = L.hidden[1];
+        R.z = L.hidden[2];
+        V = kZero64;
+        F = kZero64;
+      }
+    }
+    p.P[i] = R;
+    p.V[i] = d4(V);
+    p.W[i] = d4(W);
+    p.F[i] = d4(F);
+  }
+  if (MODE == 0) return;
+  sb = wave_sum(sb);
+  stp = wave_sum(stp);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    s_part[2 * wid] = sb;
+    s_part[2 * wid + 1] = stp;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, c = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+      a += s_part[2 * w];
+      c += s_part[2 * w + 1];
+    }
+    p.part[2 * blockIdx.x] = a;
+    p.part[2 * blockIdx.x + 1] = c;
+  }
+}
+
+__global__ void k_f32_to_f64(const float *__restrict__ soa, uint32_t N,
+                             double4 *__restrict__ out) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride())
+    out[i] = make_double4(soa[i], soa[i + (size_t)N], soa[i + 2 * (size_t)N], 0.0);
+}
+
 // ================================================================= host side
 struct rfd_system {
   rfd_ctx *ctx = nullptr;
@@ -1266,6 +1568,11 @@ struct rfd_system {
   uint32_t *CI = nullptr, *cstart = nullptr, *ccursor = nullptr, *ctmp = nullptr,
            *csorted = nullptr, *ccnt_cells = nullptr;
   float4 *cpos = nullptr;    // cell-ordered relative positions (contact search)
+  // FP64 path (params.fp64): velocities, loads, bond state and contact springs
+  bool fp64 = false;
+  double4 *V64 = nullptr, *W64 = nullptr, *F64 = nullptr, *M64 = nullptr;
+  BSt64 *b64[2] = {nullptr, nullptr};
+  double4 *cO64[2] = {nullptr, nullptr};
   uint8_t *fl8 = nullptr;    // per-bond flag, kept beside the state (contact search)
   rfd_loading *ctl = nullptr;
   double *part = nullptr;
@@ -1412,6 +1719,10 @@ int rebuild_rows(rfd_system *s) {
   d.cj_old = s->rows_valid ? s->cj[o] : nullptr;
   d.ccnt_old = s->rows_valid ? s->ccnt[o] : nullptr;
   d.cO_old = s->cO[o];
+  if (s->fp64) {
+    d.cO64_old = s->cO64[o];
+    d.cO64_new = s->cO64[n];
+  }
   d.cj_new = s->cj[n];
   d.ccnt_new = s->ccnt[n];
   d.cO_new = s->cO[n];
@@ -1426,8 +1737,62 @@ int rebuild_rows(rfd_system *s) {
 // move, and the contact law drops its velocity-dependent terms (normal and
 // tangential damping, rolling resistance against the spin direction), so
 // that the residual FIRE judges is the static out-of-balance force and moment
+Mat64 make_mat64(const rfd_system_params &p) {
+  Mat64 m{};
+  const double G = 0.5 * p.E / (1.0 + p.nu);
+  m.E = p.E;
+  m.G = G;
+  m.Estar = 0.5 * p.E / (1.0 - p.nu * p.nu);
+  m.Gstar = 0.5 * G / (2.0 - p.nu);
+  const double le = log(p.e);
+  m.pA = le / sqrt(M_PI * M_PI + le * le);
+  m.mu = p.mu;
+  m.mur = p.mur;
+  m.rho43pi = p.rho * 4.0 / 3.0 * M_PI;
+  m.dt = p.dt;
+  m.farcut = p.farcut;
+  m.refmode = p.refmode;
+  return m;
+}
+
+// FP64 force pass (explicit steps only)
+int forces64(rfd_system *s) {
+  Sys64 d{};
+  d.N = s->N;
+  d.Kc = s->p.contact_capacity;
+  d.P = s->P;
+  d.V = s->V64;
+  d.W = s->W64;
+  d.F = s->F64;
+  d.M = s->M64;
+  d.hoff = s->hoff;
+  d.hr = s->hr;
+  d.brc = s->brc;
+  d.b0 = s->b64[0];
+  d.b1 = s->b64[1];
+  d.curp = s->d_cur;
+  d.fl8 = s->fl8;
+  d.cj = s->cj[s->ccur];
+  d.ccnt = s->ccnt[s->ccur];
+  d.cO = s->cO64[s->ccur];
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (s->timing && s->ev_used + 2 <= s->ev.size()) {
+    e0 = s->ev[s->ev_used];
+    e1 = s->ev[s->ev_used + 1];
+    s->ev_used += 2;
+    RFD_HIP(s->ctx, hipEventRecord(e0, s->ctx->stream));
+  }
+  k_f64_forces<<<rfd_blocks(s->N, 128), 128, 0, s->ctx->stream>>>(d, make_mat64(s->p));
+  RFD_HIP(s->ctx, hipGetLastError());
+  if (e1) RFD_HIP(s->ctx, hipEventRecord(e1, s->ctx->stream));
+  k_flip_state<<<1, 64, 0, s->ctx->stream>>>(s->d_cur, nullptr);
+  RFD_HIP(s->ctx, hipGetLastError());
+  return RFD_OK;
+}
+
 int forces(rfd_system *s, const float *dtp = nullptr, bool trial = false,
            bool nobreak = false) {
+  if (s->fp64) return forces64(s);
   SysDev d = make_dev(s);
   d.dtp = dtp;
   d.trial = trial ? 1 : 0;
@@ -1459,6 +1824,29 @@ int forces(rfd_system *s, const float *dtp = nullptr, bool trial = false,
 }
 
 int pass_and_control(rfd_system *s) {
+  hipStream_t st64 = s->ctx->stream;
+  if (s->fp64) {
+    Pass64 q{};
+    q.N = s->N;
+    q.P = s->P;
+    q.V = s->V64;
+    q.W = s->W64;
+    q.F = s->F64;
+    q.M = s->M64;
+    q.dt = s->p.dt;
+    q.rho43pi = s->p.rho * 4.0 / 3.0 * M_PI;
+    q.ctl = s->ctl;
+    q.part = s->part;
+    if (s->mode == 1)
+      k_f64_pass<1><<<s->nblocks_pass, 256, 0, st64>>>(q);
+    else if (s->mode == 2)
+      k_f64_pass<2><<<s->nblocks_pass, 256, 0, st64>>>(q);
+    else
+      k_f64_pass<0><<<s->nblocks_pass, 256, 0, st64>>>(q);
+    k_nat_control<<<1, 256, 0, st64>>>(s->ctl, s->part, (int)s->nblocks_pass, s->hist, s->p.dt);
+    RFD_HIP(s->ctx, hipGetLastError());
+    return RFD_OK;
+  }
   PassDev p{};
   p.N = s->N;
   p.P = s->P;
@@ -1657,6 +2045,18 @@ int rfd_system_create(rfd_ctx *ctx, const rfd_system_params *p, uint32_t N, rfd_
   if (!st) st = dalloc(ctx, &s->cpos, N);
   if (!st) st = dalloc(ctx, &s->ctl, 1);
   if (!st) st = dalloc(ctx, &s->d_cur, 1);
+  s->fp64 = p->fp64 != 0;
+  if (s->fp64) {
+    if (!st) st = dalloc(ctx, &s->V64, N);
+    if (!st) st = dalloc(ctx, &s->W64, N);
+    if (!st) st = dalloc(ctx, &s->F64, N);
+    if (!st) st = dalloc(ctx, &s->M64, N);
+    for (int q = 0; q < 2 && !st; ++q) {
+      st = dalloc(ctx, &s->cO64[q], KN);
+      if (!st && hipMemset(s->cO64[q], 0, sizeof(double4) * KN) != hipSuccess)
+        st = rfd_set_error(ctx, RFD_E_STATE, "rfd_system_create: memset");
+    }
+  }
   s->nblocks_pass = rfd_blocks(N, 256, 2048);
   if (!st) st = dalloc(ctx, &s->part, 2 * (size_t)s->nblocks_pass);
   if (!st) st = dalloc(ctx, &s->hist, 3 * (size_t)kHistCap);
@@ -1706,6 +2106,14 @@ int rfd_system_destroy(rfd_system *s) {
   dfree(s->ccnt_cells);
   dfree(s->ctl);
   dfree(s->d_cur);
+  dfree(s->V64);
+  dfree(s->W64);
+  dfree(s->F64);
+  dfree(s->M64);
+  for (int q = 0; q < 2; ++q) {
+    dfree(s->b64[q]);
+    dfree(s->cO64[q]);
+  }
   dfree(s->part);
   dfree(s->hist);
   dfree(s->scan_tot);
@@ -1802,6 +2210,12 @@ int rfd_system_set_particles(rfd_system *s, const double *R, const float *radius
   RFD_HIP(ctx, hipMemset(s->W, 0, sizeof(float4) * N));
   RFD_HIP(ctx, hipMemset(s->F, 0, sizeof(float4) * N));
   RFD_HIP(ctx, hipMemset(s->M, 0, sizeof(float4) * N));
+  if (s->fp64) {
+    RFD_HIP(ctx, hipMemset(s->V64, 0, sizeof(double4) * N));
+    RFD_HIP(ctx, hipMemset(s->W64, 0, sizeof(double4) * N));
+    RFD_HIP(ctx, hipMemset(s->F64, 0, sizeof(double4) * N));
+    RFD_HIP(ctx, hipMemset(s->M64, 0, sizeof(double4) * N));
+  }
   for (int q = 0; q < 2; ++q) RFD_HIP(ctx, hipMemset(s->ccnt[q], 0, sizeof(uint32_t) * N));
   s->have_particles = true;
   s->rows_valid = false;
@@ -1914,6 +2328,10 @@ int rfd_system_set_bonds(rfd_system *s, uint32_t Nb, const uint32_t *bi, const u
   if (!st) st = dalloc(ctx, &s->bst[0], Nb);
   if (!st) st = dalloc(ctx, &s->bst[1], Nb);
   if (!st) st = dalloc(ctx, &s->fl8, Nb);
+  if (s->fp64) {
+    if (!st) st = dalloc(ctx, &s->b64[0], Nb);
+    if (!st) st = dalloc(ctx, &s->b64[1], Nb);
+  }
   if (st) return st;
   RFD_HIP(ctx, hipMemcpy(s->hoff, hoff.data(), sizeof(uint32_t) * (N + 1), hipMemcpyHostToDevice));
   if (H) {
@@ -1925,6 +2343,12 @@ int rfd_system_set_bonds(rfd_system *s, uint32_t Nb, const uint32_t *bi, const u
     for (int q = 0; q < 2; ++q)
       RFD_HIP(ctx, hipMemcpy(s->bst[q], h0.data(), sizeof(BSt) * Nb, hipMemcpyHostToDevice));
     RFD_HIP(ctx, hipMemset(s->fl8, kBond, Nb));
+    if (s->fp64) {
+      std::vector<BSt64> z(Nb);
+      for (auto &x : z) x = BSt64{0, 0, 0, 0, 0, 0, 0, kBond, 0};
+      for (int q = 0; q < 2; ++q)
+        RFD_HIP(ctx, hipMemcpy(s->b64[q], z.data(), sizeof(BSt64) * Nb, hipMemcpyHostToDevice));
+    }
     if (!L0) {
       k_set_rest_length<<<rfd_blocks(N, 256), 256, 0, ctx->stream>>>(s->P, N, s->hoff, s->hr,
                                                                      s->brc);
@@ -1980,6 +2404,15 @@ int rfd_system_set_bond_state(rfd_system *s, const float *state, const uint8_t *
   for (int k = 0; k < 2; ++k)
     RFD_HIP(ctx, hipMemcpy(s->bst[k], hs.data(), sizeof(BSt) * Nb, hipMemcpyHostToDevice));
   RFD_HIP(ctx, hipMemcpy(s->fl8, flag, Nb, hipMemcpyHostToDevice));
+  if (s->fp64) {
+    std::vector<BSt64> h64(Nb);
+    for (uint32_t b = 0; b < Nb; ++b) {
+      const BSt &x = hs[b];
+      h64[b] = BSt64{x.a.x, x.a.y, x.a.z, x.a.w, x.mx, x.my, x.mz, x.fl, 0};
+    }
+    for (int k = 0; k < 2; ++k)
+      RFD_HIP(ctx, hipMemcpy(s->b64[k], h64.data(), sizeof(BSt64) * Nb, hipMemcpyHostToDevice));
+  }
   return RFD_OK;
 }
 
@@ -1992,7 +2425,17 @@ int rfd_system_get_bond_state(rfd_system *s, float *state, uint8_t *flag) {
   std::vector<BSt> hs(Nb);
   int cur = 0;
   RFD_HIP(ctx, hipMemcpy(&cur, s->d_cur, sizeof(int), hipMemcpyDeviceToHost));
-  RFD_HIP(ctx, hipMemcpy(hs.data(), s->bst[cur], sizeof(BSt) * Nb, hipMemcpyDeviceToHost));
+  if (s->fp64) {
+    std::vector<BSt64> h64(Nb);
+    RFD_HIP(ctx, hipMemcpy(h64.data(), s->b64[cur], sizeof(BSt64) * Nb, hipMemcpyDeviceToHost));
+    for (uint32_t b = 0; b < Nb; ++b) {
+      const BSt64 &x = h64[b];
+      hs[b] = BSt{make_float4((float)x.ox, (float)x.oy, (float)x.oz, (float)x.mn), (float)x.mx,
+                  (float)x.my, (float)x.mz, x.fl};
+    }
+  } else {
+    RFD_HIP(ctx, hipMemcpy(hs.data(), s->bst[cur], sizeof(BSt) * Nb, hipMemcpyDeviceToHost));
+  }
   if (state) {
     std::vector<double> n;
     int st = bond_dirs(s, n);
@@ -2017,9 +2460,33 @@ int rfd_system_get_bond_state(rfd_system *s, float *state, uint8_t *flag) {
   return RFD_OK;
 }
 
+int upload_vec3_64(rfd_ctx *ctx, double4 *dst, const float *soa, uint32_t N) {
+  std::vector<double4> h(N);
+  for (uint32_t i = 0; i < N; ++i)
+    h[i] = make_double4(soa[i], soa[i + (size_t)N], soa[i + 2 * (size_t)N], 0.0);
+  RFD_HIP(ctx, hipMemcpy(dst, h.data(), sizeof(double4) * N, hipMemcpyHostToDevice));
+  return RFD_OK;
+}
+
+int download_vec3_64(rfd_ctx *ctx, float *soa, const double4 *src, uint32_t N) {
+  std::vector<double4> h(N);
+  RFD_HIP(ctx, hipMemcpy(h.data(), src, sizeof(double4) * N, hipMemcpyDeviceToHost));
+  for (uint32_t i = 0; i < N; ++i) {
+    soa[i] = (float)h[i].x;
+    soa[i + (size_t)N] = (float)h[i].y;
+    soa[i + 2 * (size_t)N] = (float)h[i].z;
+  }
+  return RFD_OK;
+}
+
 int rfd_system_set_velocities(rfd_system *s, const float *V, const float *W) {
   if (!valid(s)) return RFD_E_ARG;
   int st = RFD_OK;
+  if (s->fp64) {
+    if (V) st = upload_vec3_64(s->ctx, s->V64, V, s->N);
+    if (!st && W) st = upload_vec3_64(s->ctx, s->W64, W, s->N);
+    return st;
+  }
   if (V) st = upload_vec3(s->ctx, s->V, V, s->N);
   if (!st && W) st = upload_vec3(s->ctx, s->W, W, s->N);
   return st;
@@ -2059,6 +2526,13 @@ int rfd_system_get_state(rfd_system *s, double *R, float *V, float *W, float *F,
     }
   }
   int st = RFD_OK;
+  if (s->fp64) {
+    if (V) st = download_vec3_64(ctx, V, s->V64, N);
+    if (!st && W) st = download_vec3_64(ctx, W, s->W64, N);
+    if (!st && F) st = download_vec3_64(ctx, F, s->F64, N);
+    if (!st && M) st = download_vec3_64(ctx, M, s->M64, N);
+    return st;
+  }
   if (V) st = download_vec3(ctx, V, s->V, N);
   if (!st && W) st = download_vec3(ctx, W, s->W, N);
   if (!st && F) st = download_vec3(ctx, F, s->F, N);
@@ -2143,6 +2617,9 @@ int rfd_system_fire_load_step(rfd_system *s, double dstrain, const rfd_fire_para
   if (!valid(s)) return RFD_E_ARG;
   rfd_ctx *ctx = s->ctx;
   RFD_CHECK_ARG(ctx, fp && out && s->have_bonds, "rfd_system_fire_load_step: bad arguments");
+  if (s->fp64)
+    return rfd_set_error(ctx, RFD_E_STATE, "rfd_system_fire_load_step: the FP64 path runs "
+                                           "explicit steps only");
   RFD_CHECK_ARG(ctx, fp->dt0 > 0 && fp->dt_max >= fp->dt0 && fp->ftol > 0 && fp->max_iter > 0,
                 "rfd_system_fire_load_step: bad FIRE parameters");
   const auto t0 = std::chrono::steady_clock::now();
@@ -2271,6 +2748,8 @@ int rfd_system_residual(rfd_system *s, double out[2]) {
   if (!valid(s)) return RFD_E_ARG;
   rfd_ctx *ctx = s->ctx;
   RFD_CHECK_ARG(ctx, out && s->have_bonds, "rfd_system_residual: bad arguments");
+  if (s->fp64)
+    return rfd_set_error(ctx, RFD_E_STATE, "rfd_system_residual: FP32 path only");
   int st = fire_alloc(s);
   if (st) return st;
   hipStream_t q = ctx->stream;

@@ -24,7 +24,7 @@ class SystemParams(C.Structure):
                 ("mu", C.c_double), ("mur", C.c_double), ("e", C.c_double),
                 ("dt", C.c_double), ("farcut", C.c_double), ("skin", C.c_double),
                 ("rebuild_every", C.c_uint32), ("contact_capacity", C.c_uint32),
-                ("refmode", C.c_int)]
+                ("refmode", C.c_int), ("fp64", C.c_int)]
 
 
 class Loading(C.Structure):
@@ -396,7 +396,7 @@ def loading_from_compression(cmp: Compression, zcut: float, box=None) -> Loading
 
 
 def params_from_material(mat, refmode: int = 1, skin: float = 0.05, rebuild_every: int = 20,
-                         contact_capacity: int = 16) -> SystemParams:
+                         contact_capacity: int = 16, fp64: bool = False) -> SystemParams:
     p = SystemParams()
     p.E, p.nu, p.rho = mat.E, mat.nu, mat.rho
     p.mu, p.mur, p.e = mat.p_mu, mat.p_mur, mat.p_e
@@ -406,6 +406,7 @@ def params_from_material(mat, refmode: int = 1, skin: float = 0.05, rebuild_ever
     p.rebuild_every = rebuild_every
     p.contact_capacity = contact_capacity
     p.refmode = refmode
+    p.fp64 = 1 if fp64 else 0
     return p
 
 

@@ -1,0 +1,80 @@
+"""The FP64 GPU path (rfd_system_params.fp64): every quantity of the native
+step in double, the GPU precision oracle of the FP32 path.
+
+* its bond forces and moments agree with the CPU FP64 oracle
+  (orc_native_bonds_f64) to 1e-10 of the largest component over several
+  evaluations with large velocities;
+* a uniaxial loading run to fracture: the FP32 path's peak strength is within
+  1 % of the FP64 path's on the same seeded specimen.
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import lib as olib, ptr
+from rfd import native_cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _soa3(a, N):
+    return a.reshape(3, N).T
+
+
+def test_fp64_bonds_match_cpu_oracle(ctx):
+    nc = native_cases.uniaxial_cylinder(ctx, 6.0, 8.0, seed=3, strength=(1e12, 1e12),
+                                        fp64=True)
+    s = nc.sys
+    N = s.N
+    g = np.random.default_rng(9)
+    V = (g.standard_normal(3 * N) * 2e-2).astype(np.float32)
+    W = (g.standard_normal(3 * N) * 4e-2).astype(np.float32)
+    s.set_velocities(V, W)
+    R = s.state()["R"].copy()
+    b = s.bonds
+    Nb = b.n
+    L0 = np.linalg.norm(_soa3(R, N)[b.bj] - _soa3(R, N)[b.bi], axis=1)
+    iL = (1.0 / L0).astype(np.float32).astype(np.float64)
+    state = np.zeros(8 * Nb)
+    act = np.ones(Nb, np.uint8)
+    F = np.zeros(3 * N)
+    M = np.zeros(3 * N)
+    Vd, Wd = V.astype(np.float64), W.astype(np.float64)
+    brd, scd, tcd = (x.astype(np.float64) for x in (b.br, b.sc, b.tc))
+    for it in range(6):
+        s.compute_forces()
+        olib().orc_native_bonds_f64(ptr(R), ptr(Vd), ptr(Wd), N, ptr(b.bi), ptr(b.bj), ptr(iL),
+                                    ptr(brd), ptr(scd), ptr(tcd), ptr(state), ptr(act), Nb,
+                                    nc.mat.dt, nc.mat.E, nc.mat.G, 0, ptr(F), ptr(M))
+        out = s.state()
+        for k, ref in (("F", F), ("M", M)):
+            # the API returns FP32 copies: compare at FP32 resolution of the largest entry
+            scale = np.abs(ref).max()
+            np.testing.assert_allclose(out[k], ref, rtol=0, atol=2e-7 * scale,
+                                       err_msg=f"{k} at evaluation {it}")
+
+
+def _peak(ctx, fp64: bool, steps: int = 6000):
+    """Explicit uniaxial loading (platen speed 10x the reference's, capped
+    ends, platens seated on the end layers) until the loading control
+    detects fracture."""
+    nc = native_cases.uniaxial_cylinder(ctx, 6.0, 10.0, seed=31, fp64=fp64, flat_ends=True,
+                                        v_load=-100.0 * 0.02 / 1e3)
+    s = nc.sys
+    s.seat_platens(preload=0.0)
+    done = 0
+    while done < steps:
+        s.step(200)
+        done += 200
+        L = s.loading()
+        if L.fracture:
+            break
+    L = s.loading()
+    return L, s.history()
+
+
+def test_fp32_peak_strength_within_1pct_of_fp64(ctx):
+    L32, h32 = _peak(ctx, False)
+    L64, h64 = _peak(ctx, True)
+    assert L64.loading and L64.fracture_stress > 0
+    assert L32.fracture_stress == pytest.approx(L64.fracture_stress, rel=0.01), (
+        L32.fracture_stress, L64.fracture_stress, L32.fracture_strain, L64.fracture_strain)
