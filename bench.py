@@ -275,6 +275,38 @@ def bench_warmstart(D: float = 61.0, H: float = 122.0, steps: int = 6) -> dict:
             "total_seconds": time.perf_counter() - t0}
 
 
+def bench_peak_precision(D: float = 10.0, H: float = 20.0, max_steps: int = 20000) -> dict:
+    """Peak strength of one seeded uniaxial cylinder (explicit loading at 10x
+    the reference platen speed, capped ends) on the FP32 path and on the FP64
+    GPU path, and their relative difference."""
+    from rfd import native_cases
+    from rfd.reflayout import Context
+
+    ctx = Context(0)
+    out = {}
+    for name, fp64 in (("fp32", False), ("fp64", True)):
+        nc = native_cases.uniaxial_cylinder(ctx, D, H, seed=31, fp64=fp64, flat_ends=True,
+                                            v_load=-100.0 * 0.02 / 1e3)
+        s = nc.sys
+        s.seat_platens(preload=0.0)
+        done, t0 = 0, time.perf_counter()
+        while done < max_steps:
+            s.step(500)
+            done += 500
+            if s.loading().fracture:
+                break
+        L = s.loading()
+        out[name] = {"peak_stress_MPa": L.fracture_stress * 1e3, "peak_strain": L.fracture_strain,
+                     "steps": done, "fractured": bool(L.fracture),
+                     "seconds": time.perf_counter() - t0, "particles": s.N}
+        print(f"[bench] peak {name}: {out[name]}", file=sys.stderr, flush=True)
+        del s, nc
+    out["relative_difference"] = abs(out["fp32"]["peak_stress_MPa"] / out["fp64"]["peak_stress_MPa"]
+                                     - 1.0)
+    out["workload"] = f"uniaxial cylinder D{D:g}xH{H:g} mm, explicit steps to fracture"
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -313,6 +345,10 @@ def main():
         out["cpu_baseline"] = cpu_baseline_ref(args.cpu_seconds)
         out["cpu_baseline"]["host_cores_available"] = os.cpu_count()
     if not args.no_warmstart and args.tier == "native":
+        try:
+            out["peak_strength"] = bench_peak_precision()
+        except Exception as e:  # reported, never hides the main line
+            out["peak_strength"] = {"error": f"{type(e).__name__}: {e}"}
         try:
             out["warm_start"] = bench_warmstart()
         except Exception as e:  # reported, never hides the main line
