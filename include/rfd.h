@@ -268,6 +268,60 @@ int rfd_system_sizes(rfd_system *sys, uint32_t *N, uint32_t *Nb);
  * clears), then read the summed kernel time in ms and the launch count */
 int rfd_system_timing(rfd_system *sys, int enable);
 int rfd_system_timing_read(rfd_system *sys, double *kernel_ms, uint32_t *launches);
+/* ---- batched specimens ----
+ * S independent specimens held in one system, specimen k owning particles
+ * [first[k], first[k+1]) (first[0] = 0, first[S] = N), each with its own
+ * loading control L[k] (all of mode 1 or all of mode 2) and a history of up to
+ * hist_cap records.  A step then runs one force launch over every specimen's
+ * tiles, one platen/integration launch whose workgroups each work inside one
+ * specimen, and one control workgroup per specimen.  This replaces running
+ * h_CalculationBrazilTest (md_calculationbraziltest.cu:113-198) or
+ * h_CalculationUniaxialCompression (md_calculationuniaxialcompression.cu:113-197)
+ * once per specimen, as the calibration loop does
+ * (mp_determineconjgradient.cu:84-196).  Specimens must not touch each other
+ * (bonds and contacts are not split by specimen).  S = 0 returns to the single
+ * loading control.  FP32 path only. */
+int rfd_system_set_specimens(rfd_system *sys, uint32_t S, const uint32_t *first,
+                             const rfd_loading *L, uint32_t hist_cap);
+/* the S loading controls (S must equal the specimen count) */
+int rfd_system_get_specimen_loading(rfd_system *sys, uint32_t S, rfd_loading *L);
+/* history of specimen k, as rfd_system_get_history */
+int rfd_system_get_specimen_history(rfd_system *sys, uint32_t k, double *hist, uint32_t cap,
+                                    uint32_t *n);
+/* cell grid of the contact search (set from the particle bounds by
+ * rfd_system_set_particles): origin, cell edge (>= 2.5 rmax + skin) and cell
+ * counts; the cell data of cell_data::CalculateParameters (md_data_types.h:42-55) */
+int rfd_system_set_grid(rfd_system *sys, const double origin[3], double cell,
+                        const uint32_t dims[3]);
+int rfd_system_get_grid(rfd_system *sys, double origin[3], double *cell, uint32_t dims[3]);
+/* as rfd_system_step, with the same results bit for bit, but every run of
+ * 2 x rebuild_every steps that starts on a rebuild step is one replay of a HIP
+ * graph captured on first use (and again after any change of particles, bonds,
+ * loading, specimens, grid or launch shapes); other steps run as launches.
+ * The replay runs on a private stream ordered after and before the context
+ * stream's work.  Steps run as launches while force-kernel timing is on. */
+int rfd_system_step_graph(rfd_system *sys, uint32_t nsteps);
+/* ---- launch-shape autotuner ----
+ * Shapes: force-pass tile of 32, 64 (default) or 128 particles per workgroup,
+ * contact-row search block of 64, 128 (default) or 256 threads.  Results do
+ * not depend on the shapes.  rfd_system_autotune times each shape (median of
+ * 5 trial force passes, which write only F, M and the spare state buffer; row
+ * searches into the spare row buffers) and applies the fastest; the choice is
+ * kept per (device architecture, floor(log2 N), batched or not) in a process
+ * table, which use_cache = 1 consults first and rfd_tune_cache_save/load
+ * write to and read from a text file.  F and M hold the forces of the current
+ * configuration afterwards; the trajectory is unchanged. */
+typedef struct rfd_tune_result {
+  int force_tile, nbr_block;    /* chosen shapes                              */
+  int cached;                   /* 1: taken from the table, nothing timed     */
+  int bucket;                   /* floor(log2 N)                              */
+  double force_ms[3];           /* tiles 64, 32, 128                          */
+  double nbr_ms[3];             /* blocks 128, 64, 256                        */
+} rfd_tune_result;
+int rfd_system_autotune(rfd_system *sys, int use_cache, rfd_tune_result *out);
+int rfd_system_set_tiles(rfd_system *sys, int force_tile, int nbr_block);
+int rfd_tune_cache_save(const char *path);
+int rfd_tune_cache_load(const char *path);
 /* ---- quasi-static load steps: axial strain increment + FIRE relaxation ----
  * Standard FIRE (velocity mixing, adaptive dt, half-step back-off on uphill
  * steps) over translations and rotations, run on the device; the reference's

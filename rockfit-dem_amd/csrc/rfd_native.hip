@@ -35,8 +35,12 @@
 // records act through the Hertz-Mindlin law of :199-336 with R* = ri rj /
 // (ri + rj) and m* = mi mj / (mi + mj), which is the reference's law for
 // equal radii.
+#include <algorithm>
 #include <chrono>
 #include <cstddef>
+#include <map>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "rfd_internal.h"
@@ -50,11 +54,11 @@ using namespace rfd;
 
 namespace {
 
-constexpr int kFP = 64;    // particles per force workgroup
-constexpr int kFT = 256;   // threads per force workgroup
+// force-pass tile shapes (particles per workgroup, threads): the default and
+// the alternatives the autotuner times; results do not depend on the choice
+constexpr int kFP = 64;    // particles per force workgroup (default shape)
+constexpr int kFT = 256;   // threads per force workgroup (default shape)
 constexpr int kCH = 512;   // bond ends per LDS chunk
-// the row offsets of a tile (kFP + 1 of them) are loaded one per thread
-static_assert(kFP < kFT, "a force workgroup needs more threads than particles");
 constexpr uint8_t kBond = 1, kBroken = 0, kRemoved = 2;
 constexpr int kHistCap = 1 << 20;
 constexpr uint32_t kMaxBreakRounds = 200;  // relax-and-break rounds per load step
@@ -309,15 +313,18 @@ __device__ __forceinline__ EndData load_end(const SysDev &s, uint32_t i) {
   return e;
 }
 
-__global__ void __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(5, 8)))
+template <int FP, int FT>
+__global__ void __launch_bounds__(FT) __attribute__((amdgpu_waves_per_eu(5, 8)))
 k_nat_forces(SysDev s, Mat mt, uint32_t ntiles, uint32_t per) {
+  // the row offsets of a tile (FP + 1 of them) are loaded one per thread
+  static_assert(FP < FT, "a force workgroup needs more threads than particles");
   __shared__ float sF[3][kCH], sMb[3][kCH], sMc[3][kCH];
   __shared__ uint8_t sC[kCH];
-  __shared__ uint32_t sOff[kFP + 1];
-  __shared__ float sAcc[3][3][kFP];  // F, Mb, Mc running sums per particle
-  __shared__ uint8_t sAnyc[kFP];
-  __shared__ P4 sP[kFP];
-  __shared__ float4 sV[kFP], sW[kFP];
+  __shared__ uint32_t sOff[FP + 1];
+  __shared__ float sAcc[3][3][FP];  // F, Mb, Mc running sums per particle
+  __shared__ uint8_t sAnyc[FP];
+  __shared__ P4 sP[FP];
+  __shared__ float4 sV[FP], sW[FP];
   const uint32_t tile = xcd_tile(per);
   if (tile >= ntiles) return;  // padding tiles of the last XCD run
   if (s.done && *s.done) return;
@@ -327,23 +334,23 @@ k_nat_forces(SysDev s, Mat mt, uint32_t ntiles, uint32_t per) {
   BSt *const st_in = *s.curp ? s.bst1 : s.bst0;
   BSt *const st_out = *s.curp ? s.bst0 : s.bst1;
   const uint32_t tid = threadIdx.x;
-  const uint32_t p0 = tile * kFP;
-  const uint32_t np = min((uint32_t)kFP, s.N - p0);
+  const uint32_t p0 = tile * FP;
+  const uint32_t np = min((uint32_t)FP, s.N - p0);
   if (tid <= np) sOff[tid] = s.hoff[p0 + tid];
   if (tid < np) {
     sP[tid] = s.P[p0 + tid];
     sV[tid] = s.V[p0 + tid];
     sW[tid] = s.W[p0 + tid];
   }
-  for (uint32_t q = tid; q < 9 * kFP; q += kFT) (&sAcc[0][0][0])[q] = 0.f;
-  if (tid < kFP) sAnyc[tid] = 0;
+  for (uint32_t q = tid; q < 9 * FP; q += FT) (&sAcc[0][0][0])[q] = 0.f;
+  if (tid < FP) sAnyc[tid] = 0;
   __syncthreads();
   const uint32_t h0 = sOff[0], n_end = sOff[np] - h0;
   v3 F = kZero, Mb = kZero, Mc = kZero;
   bool anyc = false;
   for (uint32_t c0 = 0; c0 < n_end; c0 += kCH) {
     const uint32_t c1 = min(c0 + (uint32_t)kCH, n_end);
-    for (uint32_t e = c0 + tid; e < c1; e += kFT) {
+    for (uint32_t e = c0 + tid; e < c1; e += FT) {
       const uint32_t h = h0 + e;
       // row particle of half h: last row whose start is <= h
       uint32_t lo = 0, hi = np;
@@ -394,21 +401,23 @@ k_nat_forces(SysDev s, Mat mt, uint32_t ntiles, uint32_t per) {
     __syncthreads();
     // per-particle sums of this chunk: task (component group g, particle p),
     // each adds its particle's entries in row order
-    for (uint32_t task = tid; task < 4 * np; task += kFT) {
+    for (uint32_t task = tid; task < 4 * np; task += FT) {
       const uint32_t p = task % np, g = task / np;
       const uint32_t a0 = max(sOff[p] - h0, c0), a1 = min(sOff[p + 1] - h0, c1);
       if (g < 3) {
         float (*src)[kCH] = g == 0 ? sF : (g == 1 ? sMb : sMc);
-        float x = 0.f, y = 0.f, z = 0.f;
+        // one running sum over the whole row, whatever the chunking: the
+        // result does not depend on the tile shape
+        float x = sAcc[g][0][p], y = sAcc[g][1][p], z = sAcc[g][2][p];
         for (uint32_t e = a0; e < a1; ++e) {
           const uint32_t k = e - c0;
           x = x + src[0][k];
           y = y + src[1][k];
           z = z + src[2][k];
         }
-        sAcc[g][0][p] += x;
-        sAcc[g][1][p] += y;
-        sAcc[g][2][p] += z;
+        sAcc[g][0][p] = x;
+        sAcc[g][1][p] = y;
+        sAcc[g][2][p] = z;
       } else {
         uint8_t c = 0;
         for (uint32_t e = a0; e < a1; ++e) c |= sC[e - c0];
@@ -545,40 +554,39 @@ __device__ __forceinline__ void platen(const rfd_loading &L, P4 &R, VT &V, VT &F
 }
 
 // platens, symplectic Euler with FP64 positions (md_calculateincrementsdem.cu:19-49),
-// cut box (md_parallelepipedrestriction.cu:73-158)
+// cut box (md_parallelepipedrestriction.cu:73-158) for particle i
 template <int MODE>
-__global__ void __launch_bounds__(256) k_nat_pass(PassDev p) {
-  __shared__ double s_part[8];
-  const rfd_loading L = *p.ctl;
-  const uint32_t N = p.N;
-  double sb = 0.0, stp = 0.0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
-    P4 R = p.P[i];
-    v3 V = xyz(p.V[i]), F = xyz(p.F[i]);
-    if (MODE) platen<MODE>(L, R, V, F, sb, stp);
-    const float2 iv = p.inv[i];
-    V = V + F * iv.x;
-    R.x += (double)V.x * p.dt;
-    R.y += (double)V.y * p.dt;
-    R.z += (double)V.z * p.dt;
-    v3 W = xyz(p.W[i]) + xyz(p.M[i]) * iv.y;
-    if (L.cut) {
-      const double dx = R.x - L.box_c[0], dy = R.y - L.box_c[1], dz = R.z - L.box_c[2];
-      if (dx - L.box_h[0] > 0 || dx + L.box_h[0] < 0 || dy - L.box_h[1] > 0 ||
-          dy + L.box_h[1] < 0 || dz - L.box_h[2] > 0 || dz + L.box_h[2] < 0) {
-        R.x = L.hidden[0];
-        R.y = L.hidden[1];
-        R.z = L.hidden[2];
-        V = kZero;
-        F = kZero;
-      }
+__device__ __forceinline__ void advance_one(const PassDev &p, const rfd_loading &L, uint32_t i,
+                                            double &sb, double &stp) {
+  P4 R = p.P[i];
+  v3 V = xyz(p.V[i]), F = xyz(p.F[i]);
+  if (MODE) platen<MODE>(L, R, V, F, sb, stp);
+  const float2 iv = p.inv[i];
+  V = V + F * iv.x;
+  R.x += (double)V.x * p.dt;
+  R.y += (double)V.y * p.dt;
+  R.z += (double)V.z * p.dt;
+  v3 W = xyz(p.W[i]) + xyz(p.M[i]) * iv.y;
+  if (L.cut) {
+    const double dx = R.x - L.box_c[0], dy = R.y - L.box_c[1], dz = R.z - L.box_c[2];
+    if (dx - L.box_h[0] > 0 || dx + L.box_h[0] < 0 || dy - L.box_h[1] > 0 ||
+        dy + L.box_h[1] < 0 || dz - L.box_h[2] > 0 || dz + L.box_h[2] < 0) {
+      R.x = L.hidden[0];
+      R.y = L.hidden[1];
+      R.z = L.hidden[2];
+      V = kZero;
+      F = kZero;
     }
-    p.P[i] = R;
-    p.V[i] = f4(V);
-    p.W[i] = f4(W);
-    p.F[i] = f4(F);
   }
-  if (MODE == 0) return;
+  p.P[i] = R;
+  p.V[i] = f4(V);
+  p.W[i] = f4(W);
+  p.F[i] = f4(F);
+}
+
+// workgroup sum of the two platen reactions, written as this block's partial pair
+__device__ __forceinline__ void write_partials(double *part, double sb, double stp) {
+  __shared__ double s_part[8];
   sb = wave_sum(sb);
   stp = wave_sum(stp);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -593,19 +601,57 @@ __global__ void __launch_bounds__(256) k_nat_pass(PassDev p) {
       a += s_part[2 * w];
       c += s_part[2 * w + 1];
     }
-    p.part[2 * blockIdx.x] = a;
-    p.part[2 * blockIdx.x + 1] = c;
+    part[2 * blockIdx.x] = a;
+    part[2 * blockIdx.x + 1] = c;
   }
 }
 
-// loading control: the platen partial sums are reduced by one workgroup in a
-// fixed order, then one thread applies SumForcesULoading (md_results.cu:34-74)
-// and CompressU / CompressB (md_data_types.h:223-244, 272-293)
-__global__ void __launch_bounds__(256) k_nat_control(rfd_loading *ctl, const double *part,
-                                                     int nblocks, double *hist, double dt) {
+template <int MODE>
+__global__ void __launch_bounds__(256) k_nat_pass(PassDev p) {
+  const rfd_loading L = *p.ctl;
+  const uint32_t N = p.N;
+  double sb = 0.0, stp = 0.0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride())
+    advance_one<MODE>(p, L, i, sb, stp);
+  if (MODE == 0) return;
+  write_partials(p.part, sb, stp);
+}
+
+// one step of the loading control from the reduced platen sums
+// (SumForcesULoading md_results.cu:34-74 and CompressU md_data_types.h:223):
+// history record, stress, loading/fracture detection, platen move
+__device__ __forceinline__ void control_update(rfd_loading &L, double s0, double s1,
+                                               double *hist, uint32_t hist_cap, double dt) {
+  L.steps += 1;
+  if (!L.mode) return;
+  L.sum_bottom = s0;
+  L.sum_top = s1;
+  if (L.hist_len < hist_cap) {
+    hist[3 * (size_t)L.hist_len] = (L.size - L.size0) / L.size0;
+    hist[3 * (size_t)L.hist_len + 1] = s0 * L.inv_area;
+    hist[3 * (size_t)L.hist_len + 2] = s1 * L.inv_area;
+    L.hist_len += 1;
+  }
+  L.stress = 0.5 * (fabs(s0) + fabs(s1)) * L.inv_area;
+  if (!L.loading && L.stress > L.min_stress) L.loading = 1;
+  if (L.fracture_stress < L.stress) {
+    L.fracture_stress = L.stress;
+    L.fracture_strain = (L.size0 - L.size) / L.size0;
+  }
+  if (L.loading && L.stress < L.zero_stress) L.fracture = 1;
+  L.v = L.stress < 1e-9 ? 10.0 * L.v_load : L.v_load;
+  L.size += L.v * dt;
+  L.top += L.v * dt;
+  L.top_r = L.top - L.band;
+  L.strain = (L.size - L.size0) / L.size0;
+}
+
+// reduce nb platen partial pairs in a fixed order within one workgroup
+__device__ __forceinline__ void reduce_pairs(const double *part, int b0, int b1, double &s0,
+                                             double &s1) {
   __shared__ double s_part[8];
   double a = 0, c = 0;
-  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+  for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
     a += part[2 * b];
     c += part[2 * b + 1];
   }
@@ -617,37 +663,54 @@ __global__ void __launch_bounds__(256) k_nat_control(rfd_loading *ctl, const dou
     s_part[2 * wid + 1] = c;
   }
   __syncthreads();
+  s0 = s1 = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+    s0 += s_part[2 * w];
+    s1 += s_part[2 * w + 1];
+  }
+}
+
+// loading control: the platen partial sums are reduced by one workgroup in a
+// fixed order, then one thread applies the control
+__global__ void __launch_bounds__(256) k_nat_control(rfd_loading *ctl, const double *part,
+                                                     int nblocks, double *hist, double dt) {
+  double s0, s1;
+  reduce_pairs(part, 0, nblocks, s0, s1);
   if (threadIdx.x != 0) return;
   rfd_loading L = *ctl;
-  L.steps += 1;
-  if (L.mode) {
-    double s0 = 0, s1 = 0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
-      s0 += s_part[2 * w];
-      s1 += s_part[2 * w + 1];
-    }
-    L.sum_bottom = s0;
-    L.sum_top = s1;
-    if (L.hist_len < (uint32_t)kHistCap) {
-      hist[3 * (size_t)L.hist_len] = (L.size - L.size0) / L.size0;
-      hist[3 * (size_t)L.hist_len + 1] = s0 * L.inv_area;
-      hist[3 * (size_t)L.hist_len + 2] = s1 * L.inv_area;
-      L.hist_len += 1;
-    }
-    L.stress = 0.5 * (fabs(s0) + fabs(s1)) * L.inv_area;
-    if (!L.loading && L.stress > L.min_stress) L.loading = 1;
-    if (L.fracture_stress < L.stress) {
-      L.fracture_stress = L.stress;
-      L.fracture_strain = (L.size0 - L.size) / L.size0;
-    }
-    if (L.loading && L.stress < L.zero_stress) L.fracture = 1;
-    L.v = L.stress < 1e-9 ? 10.0 * L.v_load : L.v_load;
-    L.size += L.v * dt;
-    L.top += L.v * dt;
-    L.top_r = L.top - L.band;
-    L.strain = (L.size - L.size0) / L.size0;
-  }
+  control_update(L, s0, s1, hist, (uint32_t)kHistCap, dt);
   *ctl = L;
+}
+
+// batched specimens: workgroup s controls specimen s from the partials of its
+// own pass workgroups [spec_blk[s], spec_blk[s+1])
+__global__ void __launch_bounds__(256) k_nat_control_batch(rfd_loading *ctls, const double *part,
+                                                           const uint32_t *__restrict__ spec_blk,
+                                                           double *hist, uint32_t hist_cap,
+                                                           double dt) {
+  const uint32_t sp = blockIdx.x;
+  double s0, s1;
+  reduce_pairs(part, (int)spec_blk[sp], (int)spec_blk[sp + 1], s0, s1);
+  if (threadIdx.x != 0) return;
+  rfd_loading L = ctls[sp];
+  control_update(L, s0, s1, hist + 3 * (size_t)hist_cap * sp, hist_cap, dt);
+  ctls[sp] = L;
+}
+
+// batched pass: workgroup b advances particles [blk_first[b], blk_end[b]) of
+// specimen blk_spec[b] against that specimen's platens
+template <int MODE>
+__global__ void __launch_bounds__(256) k_nat_pass_batch(PassDev p,
+                                                        const uint32_t *__restrict__ blk_spec,
+                                                        const uint32_t *__restrict__ blk_first,
+                                                        const uint32_t *__restrict__ blk_end,
+                                                        const rfd_loading *__restrict__ ctls) {
+  const rfd_loading L = ctls[blk_spec[blockIdx.x]];
+  const uint32_t e = blk_end[blockIdx.x];
+  double sb = 0.0, stp = 0.0;
+  for (uint32_t i = blk_first[blockIdx.x] + threadIdx.x; i < e; i += blockDim.x)
+    advance_one<MODE>(p, L, i, sb, stp);
+  write_partials(p.part, sb, stp);
 }
 
 // ----------------------------------------------------- neighbour search
@@ -844,11 +907,11 @@ struct NbrDev {
 // MODE 0: count pairs j>i with |rij| < ri + rj + extra; MODE 1: write them;
 // MODE 2: rebuild the contact row (never-bonded j within ri + rj + extra),
 // carrying each spring O over from the previous row.
-template <int MODE>
-__global__ void __launch_bounds__(128) k_nat_neighbours(NbrDev s, uint32_t per) {
+template <int MODE, int NB = 128>
+__global__ void __launch_bounds__(NB) k_nat_neighbours(NbrDev s, uint32_t per) {
   // XCD-aware: each XCD scans one contiguous run of Morton-ordered particles,
   // so the cells around a run are fetched into one L2 rather than into all
-  const uint32_t i = xcd_tile(per) * 128 + threadIdx.x;
+  const uint32_t i = xcd_tile(per) * NB + threadIdx.x;
   if (i >= s.N) return;
   const uint32_t N = s.N;
   const P4 Ri = s.P[i];
@@ -1596,6 +1659,28 @@ struct rfd_system {
   bool timing = false;  // force-kernel events on the context stream
   std::vector<hipEvent_t> ev;
   uint32_t ev_used = 0;
+  // launch shapes (autotuned or set): force tile particles, neighbour block
+  int force_fp = kFP;
+  int nbr_block = 128;
+  // batched specimens: S independent specimens in contiguous index ranges,
+  // each with its own loading control and history
+  uint32_t S = 0;
+  std::vector<uint32_t> spec_first;   // host copy, S + 1 entries
+  rfd_loading *ctls = nullptr;        // S controls
+  double *hist_b = nullptr;           // S x hist_cap x 3
+  uint32_t hist_cap_b = 0;
+  uint32_t *blk_spec = nullptr, *blk_first = nullptr, *blk_end = nullptr;  // pass work list
+  uint32_t *spec_blk = nullptr;       // S + 1 offsets into the work list
+  uint32_t nblk_b = 0;
+  double *part_b = nullptr;           // per-workgroup platen partials of the batch
+  // step graphs: 2 x rebuild_every steps captured once, replayed
+  uint64_t gen = 1;                   // bumped by every change a captured step depends on
+  hipStream_t gstream = nullptr;
+  hipEvent_t gev[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec = nullptr;
+  uint64_t g_gen = 0;
+  int g_ccur = -1;
+  uint32_t g_steps = 0;
 };
 
 namespace {
@@ -1707,9 +1792,20 @@ NbrDev make_nbr(rfd_system *s, double extra) {
   return d;
 }
 
-// neighbour-search launch: ceil(N/128) tiles spread over kXcd runs
-uint32_t nbr_per(uint32_t N) { return ((N + 127) / 128 + kXcd - 1) / kXcd; }
-uint32_t nbr_grid(uint32_t N) { return nbr_per(N) * kXcd; }
+// neighbour-search launch: ceil(N/nb) tiles spread over kXcd runs
+uint32_t nbr_per(uint32_t N, uint32_t nb = 128) { return ((N + nb - 1) / nb + kXcd - 1) / kXcd; }
+uint32_t nbr_grid(uint32_t N, uint32_t nb = 128) { return nbr_per(N, nb) * kXcd; }
+
+// contact-row rebuild with the chosen block size (rows do not depend on it)
+void launch_rows(rfd_system *s, const NbrDev &d, int nb) {
+  hipStream_t q = s->ctx->stream;
+  if (nb == 64)
+    k_nat_neighbours<2, 64><<<nbr_grid(s->N, 64), 64, 0, q>>>(d, nbr_per(s->N, 64));
+  else if (nb == 256)
+    k_nat_neighbours<2, 256><<<nbr_grid(s->N, 256), 256, 0, q>>>(d, nbr_per(s->N, 256));
+  else
+    k_nat_neighbours<2, 128><<<nbr_grid(s->N), 128, 0, q>>>(d, nbr_per(s->N));
+}
 
 int rebuild_rows(rfd_system *s) {
   int st = bin_particles(s);
@@ -1726,7 +1822,7 @@ int rebuild_rows(rfd_system *s) {
   d.cj_new = s->cj[n];
   d.ccnt_new = s->ccnt[n];
   d.cO_new = s->cO[n];
-  k_nat_neighbours<2><<<nbr_grid(s->N), 128, 0, s->ctx->stream>>>(d, nbr_per(s->N));
+  launch_rows(s, d, s->nbr_block);
   RFD_HIP(s->ctx, hipGetLastError());
   s->ccur = n;
   s->rows_valid = true;
@@ -1790,6 +1886,14 @@ int forces64(rfd_system *s) {
   return RFD_OK;
 }
 
+// one launch over ceil(N / FP) tiles, spread over the XCD runs
+template <int FP, int FT>
+void launch_forces(rfd_system *s, const SysDev &d, const Mat &m) {
+  const uint32_t nb = (s->N + FP - 1) / FP;
+  const uint32_t per = (nb + kXcd - 1) / kXcd;
+  k_nat_forces<FP, FT><<<per * kXcd, FT, 0, s->ctx->stream>>>(d, m, nb, per);
+}
+
 int forces(rfd_system *s, const float *dtp = nullptr, bool trial = false,
            bool nobreak = false) {
   if (s->fp64) return forces64(s);
@@ -1813,11 +1917,15 @@ int forces(rfd_system *s, const float *dtp = nullptr, bool trial = false,
     s->ev_used += 2;
     RFD_HIP(s->ctx, hipEventRecord(e0, s->ctx->stream));
   }
-  const uint32_t nb = (s->N + kFP - 1) / kFP;
-  const uint32_t per = (nb + kXcd - 1) / kXcd;
-  k_nat_forces<<<per * kXcd, kFT, 0, s->ctx->stream>>>(d, m, nb, per);
+  if (s->force_fp == 32)
+    launch_forces<32, 128>(s, d, m);
+  else if (s->force_fp == 128)
+    launch_forces<128, 256>(s, d, m);
+  else
+    launch_forces<kFP, kFT>(s, d, m);
   RFD_HIP(s->ctx, hipGetLastError());
-  k_flip_state<<<1, 64, 0, s->ctx->stream>>>(s->d_cur, d.done);
+  // a trial pass leaves the state index alone: its output buffer is dropped
+  if (!trial) k_flip_state<<<1, 64, 0, s->ctx->stream>>>(s->d_cur, d.done);
   RFD_HIP(s->ctx, hipGetLastError());
   if (e1) RFD_HIP(s->ctx, hipEventRecord(e1, s->ctx->stream));
   return RFD_OK;
@@ -1859,6 +1967,22 @@ int pass_and_control(rfd_system *s) {
   p.ctl = s->ctl;
   p.part = s->part;
   hipStream_t st = s->ctx->stream;
+  if (s->S) {
+    // batched specimens: one launch over every specimen's tiles, then one
+    // control workgroup per specimen
+    p.part = s->part_b;
+    p.ctl = nullptr;
+    if (s->mode == 1)
+      k_nat_pass_batch<1><<<s->nblk_b, 256, 0, st>>>(p, s->blk_spec, s->blk_first, s->blk_end,
+                                                     s->ctls);
+    else
+      k_nat_pass_batch<2><<<s->nblk_b, 256, 0, st>>>(p, s->blk_spec, s->blk_first, s->blk_end,
+                                                     s->ctls);
+    k_nat_control_batch<<<s->S, 256, 0, st>>>(s->ctls, s->part_b, s->spec_blk, s->hist_b,
+                                              s->hist_cap_b, s->p.dt);
+    RFD_HIP(s->ctx, hipGetLastError());
+    return RFD_OK;
+  }
   if (s->mode == 1)
     k_nat_pass<1><<<s->nblocks_pass, 256, 0, st>>>(p);
   else if (s->mode == 2)
@@ -1880,6 +2004,7 @@ int fire_alloc(rfd_system *s) {
   if (!st) st = dalloc(ctx, &s->P0, s->N);
   if (!st) st = dalloc(ctx, &s->Vf, s->N);
   if (!st) st = dalloc(ctx, &s->d_nbreak, 1);
+  s->gen++;  // the force pass now counts breaks
   if (!st) st = dalloc(ctx, &s->Th, s->N);
   for (int q = 0; q < 2 && !st; ++q) {
     st = dalloc(ctx, &s->du[q], 3 * (size_t)s->N);
@@ -1960,15 +2085,12 @@ int begin_pass(rfd_system *s, const FireDev &f, const float *dU, const float *dT
   else
     k_fire_begin<0><<<s->nblocks_pass, 256, 0, q>>>(fpp, dU, dTh, dt0, dtop);
   RFD_HIP(ctx, hipGetLastError());
-  int cur0 = 0;
-  if (trial) RFD_HIP(ctx, hipMemcpy(&cur0, s->d_cur, sizeof(int), hipMemcpyDeviceToHost));
+  // a trial pass writes its state to the spare buffer and leaves the index
   int st = forces(s, s->fire_dt, trial, true);
   if (st) return st;
   k_zero_f4<<<rfd_blocks(s->N, 256), 256, 0, q>>>(s->V, s->W, s->N);
   RFD_HIP(ctx, hipGetLastError());
   if (trial) {
-    // the new state went to the other buffer: drop it
-    RFD_HIP(ctx, hipMemcpy(s->d_cur, &cur0, sizeof(int), hipMemcpyHostToDevice));
     FireDev fm = f;
     fm.measure_only = 1;
     RFD_HIP(ctx, hipMemcpyAsync(s->fire, &fm, sizeof(fm), hipMemcpyHostToDevice, q));
@@ -2005,6 +2127,22 @@ int download_vec3(rfd_ctx *ctx, float *soa, const float4 *src, uint32_t N) {
     soa[i + 2 * (size_t)N] = h[i].z;
   }
   return RFD_OK;
+}
+
+// launch shapes chosen by the autotuner, per (arch, log2 N bucket, batched)
+std::map<std::string, std::pair<int, int>> &tune_table() {
+  static std::map<std::string, std::pair<int, int>> t;
+  return t;
+}
+std::mutex &tune_mu() {
+  static std::mutex m;
+  return m;
+}
+std::string tune_key(const char *arch, int bucket, bool batched) {
+  std::string a(arch);
+  const size_t colon = a.find(':');  // drop feature suffixes (":sramecc+:xnack-")
+  if (colon != std::string::npos) a.resize(colon);
+  return a + "/n" + std::to_string(bucket) + (batched ? "/batch" : "/single");
 }
 
 }  // namespace
@@ -2126,6 +2264,17 @@ int rfd_system_destroy(rfd_system *s) {
     dfree(s->du[q]);
     dfree(s->dth[q]);
   }
+  dfree(s->ctls);
+  dfree(s->hist_b);
+  dfree(s->blk_spec);
+  dfree(s->blk_first);
+  dfree(s->blk_end);
+  dfree(s->spec_blk);
+  dfree(s->part_b);
+  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  for (hipEvent_t e : s->gev)
+    if (e) (void)hipEventDestroy(e);
+  if (s->gstream) (void)hipStreamDestroy(s->gstream);
   for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
   delete s;
   return RFD_OK;
@@ -2219,6 +2368,7 @@ int rfd_system_set_particles(rfd_system *s, const double *R, const float *radius
   for (int q = 0; q < 2; ++q) RFD_HIP(ctx, hipMemset(s->ccnt[q], 0, sizeof(uint32_t) * N));
   s->have_particles = true;
   s->rows_valid = false;
+  s->gen++;
   return RFD_OK;
 }
 
@@ -2360,6 +2510,7 @@ int rfd_system_set_bonds(rfd_system *s, uint32_t Nb, const uint32_t *bi, const u
     RFD_HIP(ctx, hipMemcpy(s->d_cur, &zero, sizeof(int), hipMemcpyHostToDevice));
   }
   s->have_bonds = true;
+  s->gen++;
   s->rows_valid = false;
   RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return RFD_OK;
@@ -2496,7 +2647,9 @@ int rfd_system_set_loading(rfd_system *s, const rfd_loading *L) {
   if (!valid(s)) return RFD_E_ARG;
   rfd_ctx *ctx = s->ctx;
   RFD_CHECK_ARG(ctx, L && L->mode >= 0 && L->mode <= 2, "rfd_system_set_loading: bad loading");
+  RFD_CHECK_ARG(ctx, s->S == 0, "rfd_system_set_loading: the system holds batched specimens");
   RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  s->gen++;
   RFD_HIP(ctx, hipMemcpy(s->ctl, L, sizeof(*L), hipMemcpyHostToDevice));
   s->mode = L->mode;
   return RFD_OK;
@@ -2568,6 +2721,344 @@ int rfd_system_step(rfd_system *s, uint32_t nsteps) {
   return RFD_OK;
 }
 
+// ------------------------------------------------------- batched specimens
+int rfd_system_set_specimens(rfd_system *s, uint32_t S, const uint32_t *first,
+                             const rfd_loading *L, uint32_t hist_cap) {
+  if (!valid(s)) return RFD_E_ARG;
+  rfd_ctx *ctx = s->ctx;
+  RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  s->gen++;
+  if (S == 0) {
+    s->S = 0;
+    s->spec_first.clear();
+    return RFD_OK;
+  }
+  RFD_CHECK_ARG(ctx, first && L && hist_cap > 0, "rfd_system_set_specimens: bad arguments");
+  RFD_CHECK_ARG(ctx, !s->fp64, "rfd_system_set_specimens: batched specimens run the FP32 path");
+  RFD_CHECK_ARG(ctx, first[0] == 0 && first[S] == s->N,
+                "rfd_system_set_specimens: first[0] must be 0 and first[S] = N");
+  const int mode = L[0].mode;
+  RFD_CHECK_ARG(ctx, mode == 1 || mode == 2, "rfd_system_set_specimens: loading mode must be 1 or 2");
+  std::vector<uint32_t> bs, bf, be, sb(S + 1);
+  for (uint32_t k = 0; k < S; ++k) {
+    RFD_CHECK_ARG(ctx, first[k + 1] > first[k], "rfd_system_set_specimens: specimen %u is empty", k);
+    RFD_CHECK_ARG(ctx, L[k].mode == mode, "rfd_system_set_specimens: specimens mix loading modes");
+    sb[k] = (uint32_t)bs.size();
+    for (uint32_t a = first[k]; a < first[k + 1]; a += 256) {
+      bs.push_back(k);
+      bf.push_back(a);
+      be.push_back(std::min(a + 256u, first[k + 1]));
+    }
+  }
+  sb[S] = (uint32_t)bs.size();
+  const uint32_t nblk = (uint32_t)bs.size();
+  int st = dalloc(ctx, &s->ctls, S);
+  if (!st) st = dalloc(ctx, &s->hist_b, 3 * (size_t)hist_cap * S);
+  if (!st) st = dalloc(ctx, &s->blk_spec, nblk);
+  if (!st) st = dalloc(ctx, &s->blk_first, nblk);
+  if (!st) st = dalloc(ctx, &s->blk_end, nblk);
+  if (!st) st = dalloc(ctx, &s->spec_blk, (size_t)S + 1);
+  if (!st) st = dalloc(ctx, &s->part_b, 2 * (size_t)nblk);
+  if (st) return st;
+  std::vector<rfd_loading> Lh(L, L + S);
+  for (auto &x : Lh) {
+    x.steps = 0;
+    x.hist_len = 0;
+  }
+  RFD_HIP(ctx, hipMemcpy(s->ctls, Lh.data(), sizeof(rfd_loading) * S, hipMemcpyHostToDevice));
+  RFD_HIP(ctx, hipMemcpy(s->blk_spec, bs.data(), sizeof(uint32_t) * nblk, hipMemcpyHostToDevice));
+  RFD_HIP(ctx, hipMemcpy(s->blk_first, bf.data(), sizeof(uint32_t) * nblk, hipMemcpyHostToDevice));
+  RFD_HIP(ctx, hipMemcpy(s->blk_end, be.data(), sizeof(uint32_t) * nblk, hipMemcpyHostToDevice));
+  RFD_HIP(ctx, hipMemcpy(s->spec_blk, sb.data(), sizeof(uint32_t) * (S + 1), hipMemcpyHostToDevice));
+  s->S = S;
+  s->spec_first.assign(first, first + S + 1);
+  s->hist_cap_b = hist_cap;
+  s->nblk_b = nblk;
+  s->mode = mode;
+  return RFD_OK;
+}
+
+int rfd_system_get_specimen_loading(rfd_system *s, uint32_t S, rfd_loading *L) {
+  if (!valid(s)) return RFD_E_ARG;
+  rfd_ctx *ctx = s->ctx;
+  RFD_CHECK_ARG(ctx, L && S == s->S && S > 0,
+                "rfd_system_get_specimen_loading: S must equal the specimen count");
+  RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  RFD_HIP(ctx, hipMemcpy(L, s->ctls, sizeof(rfd_loading) * S, hipMemcpyDeviceToHost));
+  return RFD_OK;
+}
+
+int rfd_system_get_specimen_history(rfd_system *s, uint32_t k, double *hist, uint32_t cap,
+                                    uint32_t *n) {
+  if (!valid(s)) return RFD_E_ARG;
+  rfd_ctx *ctx = s->ctx;
+  RFD_CHECK_ARG(ctx, n && k < s->S, "rfd_system_get_specimen_history: bad specimen");
+  rfd_loading L;
+  RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  RFD_HIP(ctx, hipMemcpy(&L, s->ctls + k, sizeof(L), hipMemcpyDeviceToHost));
+  *n = L.hist_len;
+  const uint32_t m = std::min(std::min(L.hist_len, cap), s->hist_cap_b);
+  if (hist && m)
+    RFD_HIP(ctx, hipMemcpy(hist, s->hist_b + 3 * (size_t)s->hist_cap_b * k,
+                           sizeof(double) * 3 * m, hipMemcpyDeviceToHost));
+  return RFD_OK;
+}
+
+// ------------------------------------------------------------- cell grid
+int rfd_system_set_grid(rfd_system *s, const double origin[3], double cell,
+                        const uint32_t dims[3]) {
+  if (!valid(s)) return RFD_E_ARG;
+  rfd_ctx *ctx = s->ctx;
+  RFD_CHECK_ARG(ctx, origin && dims && s->have_particles, "rfd_system_set_grid: bad arguments");
+  const double need = 2.5 * s->rmax + fmax(s->p.skin, 0.0);
+  RFD_CHECK_ARG(ctx, cell >= need, "rfd_system_set_grid: cell %g below 2.5 rmax + skin = %g",
+                cell, need);
+  RFD_CHECK_ARG(ctx, dims[0] && dims[1] && dims[2], "rfd_system_set_grid: empty grid");
+  const uint64_t C = (uint64_t)dims[0] * dims[1] * dims[2];
+  RFD_CHECK_ARG(ctx, C < (1u << 30), "rfd_system_set_grid: grid too large (%llu cells)",
+                (unsigned long long)C);
+  RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  s->g.ox = origin[0];
+  s->g.oy = origin[1];
+  s->g.oz = origin[2];
+  s->g.inv_cell = 1.0 / cell;
+  s->g.nx = dims[0];
+  s->g.ny = dims[1];
+  s->g.nz = dims[2];
+  s->C = (uint32_t)C;
+  int st = dalloc(ctx, &s->cstart, (size_t)s->C + 1);
+  if (!st) st = dalloc(ctx, &s->ccursor, s->C);
+  if (!st) st = dalloc(ctx, &s->ccnt_cells, s->C);
+  if (st) return st;
+  s->rows_valid = false;
+  s->gen++;
+  return RFD_OK;
+}
+
+int rfd_system_get_grid(rfd_system *s, double origin[3], double *cell, uint32_t dims[3]) {
+  if (!valid(s)) return RFD_E_ARG;
+  if (origin) {
+    origin[0] = s->g.ox;
+    origin[1] = s->g.oy;
+    origin[2] = s->g.oz;
+  }
+  if (cell) *cell = 1.0 / s->g.inv_cell;
+  if (dims) {
+    dims[0] = s->g.nx;
+    dims[1] = s->g.ny;
+    dims[2] = s->g.nz;
+  }
+  return RFD_OK;
+}
+
+// ------------------------------------------------------------ step graphs
+int rfd_system_step_graph(rfd_system *s, uint32_t nsteps) {
+  if (!valid(s)) return RFD_E_ARG;
+  rfd_ctx *ctx = s->ctx;
+  RFD_CHECK_ARG(ctx, s->have_bonds, "rfd_system_step_graph: set bonds first");
+  const uint32_t R = s->p.rebuild_every, G = 2 * R;
+  uint32_t k = 0;
+  while (k < nsteps) {
+    // a graph covers two rebuild periods from a rebuild step, so that the
+    // contact-row buffers it reads and writes come back to where they started
+    const bool whole = s->rows_valid && s->step_count % R == 0 && nsteps - k >= G && !s->timing;
+    if (!whole) {
+      if (!s->rows_valid || s->step_count % R == 0) {
+        int st = rebuild_rows(s);
+        if (st) return st;
+      }
+      int st = forces(s);
+      if (!st) st = pass_and_control(s);
+      if (st) return st;
+      s->step_count++;
+      ++k;
+      continue;
+    }
+    if (!s->gexec || s->g_gen != s->gen || s->g_ccur != s->ccur || s->g_steps != G) {
+      if (s->gexec) {
+        RFD_HIP(ctx, hipGraphExecDestroy(s->gexec));
+        s->gexec = nullptr;
+      }
+      if (!s->gstream) {
+        RFD_HIP(ctx, hipStreamCreateWithFlags(&s->gstream, hipStreamNonBlocking));
+        RFD_HIP(ctx, hipEventCreateWithFlags(&s->gev[0], hipEventDisableTiming));
+        RFD_HIP(ctx, hipEventCreateWithFlags(&s->gev[1], hipEventDisableTiming));
+      }
+      // capture: the step code launches onto the context stream, pointed at
+      // the private stream for the duration; nothing is executed
+      const hipStream_t home = ctx->stream;
+      const uint64_t sc0 = s->step_count;
+      const int cc0 = s->ccur;
+      RFD_HIP(ctx, hipStreamBeginCapture(s->gstream, hipStreamCaptureModeThreadLocal));
+      ctx->stream = s->gstream;
+      int st = RFD_OK;
+      for (uint32_t q = 0; q < G && !st; ++q) {
+        if (s->step_count % R == 0) st = rebuild_rows(s);
+        if (!st) st = forces(s);
+        if (!st) st = pass_and_control(s);
+        s->step_count++;
+      }
+      ctx->stream = home;
+      hipGraph_t graph = nullptr;
+      const hipError_t ec = hipStreamEndCapture(s->gstream, &graph);
+      s->step_count = sc0;
+      s->ccur = cc0;
+      if (st) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+      }
+      RFD_HIP(ctx, ec);
+      const hipError_t ei = hipGraphInstantiate(&s->gexec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      RFD_HIP(ctx, ei);
+      s->g_gen = s->gen;
+      s->g_ccur = s->ccur;
+      s->g_steps = G;
+    }
+    // the private stream waits for the context stream's earlier work, and the
+    // context stream for the replay
+    RFD_HIP(ctx, hipEventRecord(s->gev[0], ctx->stream));
+    RFD_HIP(ctx, hipStreamWaitEvent(s->gstream, s->gev[0], 0));
+    RFD_HIP(ctx, hipGraphLaunch(s->gexec, s->gstream));
+    RFD_HIP(ctx, hipEventRecord(s->gev[1], s->gstream));
+    RFD_HIP(ctx, hipStreamWaitEvent(ctx->stream, s->gev[1], 0));
+    s->step_count += G;
+    k += G;
+  }
+  return RFD_OK;
+}
+
+// -------------------------------------------------------------- autotuner
+int rfd_system_set_tiles(rfd_system *s, int force_tile, int nbr_block) {
+  if (!valid(s)) return RFD_E_ARG;
+  rfd_ctx *ctx = s->ctx;
+  RFD_CHECK_ARG(ctx, force_tile == 32 || force_tile == 64 || force_tile == 128,
+                "rfd_system_set_tiles: force tile must be 32, 64 or 128");
+  RFD_CHECK_ARG(ctx, nbr_block == 64 || nbr_block == 128 || nbr_block == 256,
+                "rfd_system_set_tiles: neighbour block must be 64, 128 or 256");
+  s->force_fp = force_tile;
+  s->nbr_block = nbr_block;
+  s->gen++;
+  return RFD_OK;
+}
+
+int rfd_system_autotune(rfd_system *s, int use_cache, rfd_tune_result *out) {
+  if (!valid(s)) return RFD_E_ARG;
+  rfd_ctx *ctx = s->ctx;
+  RFD_CHECK_ARG(ctx, out && s->have_bonds, "rfd_system_autotune: bad arguments");
+  RFD_CHECK_ARG(ctx, !s->fp64, "rfd_system_autotune: the FP64 path has one launch shape");
+  *out = rfd_tune_result{};
+  hipDeviceProp_t prop;
+  RFD_HIP(ctx, hipGetDeviceProperties(&prop, ctx->device));
+  int bucket = 0;
+  while ((2u << bucket) <= s->N && bucket < 31) ++bucket;
+  const std::string key = tune_key(prop.gcnArchName, bucket, s->S > 0);
+  out->bucket = bucket;
+  if (use_cache) {
+    std::lock_guard<std::mutex> lk(tune_mu());
+    auto it = tune_table().find(key);
+    if (it != tune_table().end()) {
+      out->force_tile = it->second.first;
+      out->nbr_block = it->second.second;
+      out->cached = 1;
+      return rfd_system_set_tiles(s, out->force_tile, out->nbr_block);
+    }
+  }
+  hipStream_t q = ctx->stream;
+  if (!s->rows_valid) {
+    int st = rebuild_rows(s);
+    if (st) return st;
+  }
+  const bool timing = s->timing;
+  s->timing = false;
+  hipEvent_t e0, e1;
+  RFD_HIP(ctx, hipEventCreate(&e0));
+  RFD_HIP(ctx, hipEventCreate(&e1));
+  // trial force passes write only F, M and the spare state buffer
+  auto timed = [&](auto &&launch, double *ms) -> int {
+    constexpr int kReps = 5;
+    float t[kReps];
+    int e = launch();  // warm-up
+    for (int r = 0; r < kReps && !e; ++r) {
+      RFD_HIP(ctx, hipEventRecord(e0, q));
+      e = launch();
+      RFD_HIP(ctx, hipEventRecord(e1, q));
+      RFD_HIP(ctx, hipEventSynchronize(e1));
+      RFD_HIP(ctx, hipEventElapsedTime(&t[r], e0, e1));
+    }
+    if (e) return e;
+    std::sort(t, t + kReps);
+    *ms = t[kReps / 2];
+    return RFD_OK;
+  };
+  static const int kTiles[3] = {64, 32, 128};
+  static const int kBlocks[3] = {128, 64, 256};
+  const int fp0 = s->force_fp, nb0 = s->nbr_block;
+  int st = RFD_OK, best_t = 0, best_b = 0;
+  for (int v = 0; v < 3 && !st; ++v) {
+    s->force_fp = kTiles[v];
+    st = timed([&]() { return forces(s, nullptr, true, false); }, &out->force_ms[v]);
+    if (!st && out->force_ms[v] < out->force_ms[best_t]) best_t = v;
+  }
+  s->force_fp = fp0;
+  if (!st) st = bin_particles(s);
+  // the row search writes into the spare row buffers, which the next rebuild
+  // overwrites whole
+  NbrDev d = make_nbr(s, s->p.skin);
+  const int o = s->ccur, n = 1 - s->ccur;
+  d.cj_old = s->cj[o];
+  d.ccnt_old = s->ccnt[o];
+  d.cO_old = s->cO[o];
+  d.cj_new = s->cj[n];
+  d.ccnt_new = s->ccnt[n];
+  d.cO_new = s->cO[n];
+  for (int v = 0; v < 3 && !st; ++v) {
+    st = timed(
+        [&]() {
+          launch_rows(s, d, kBlocks[v]);
+          return (int)hipGetLastError();
+        },
+        &out->nbr_ms[v]);
+    if (!st && out->nbr_ms[v] < out->nbr_ms[best_b]) best_b = v;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  s->timing = timing;
+  if (st) return st > 0 ? rfd_hip_status(ctx, (hipError_t)st, "rfd_system_autotune") : st;
+  out->force_tile = kTiles[best_t];
+  out->nbr_block = kBlocks[best_b];
+  {
+    std::lock_guard<std::mutex> lk(tune_mu());
+    tune_table()[key] = {out->force_tile, out->nbr_block};
+  }
+  return rfd_system_set_tiles(s, out->force_tile, out->nbr_block);
+}
+
+int rfd_tune_cache_save(const char *path) {
+  if (!path) return RFD_E_ARG;
+  FILE *f = fopen(path, "w");
+  if (!f) return RFD_E_ARG;
+  std::lock_guard<std::mutex> lk(tune_mu());
+  for (const auto &kv : tune_table())
+    fprintf(f, "%s %d %d\n", kv.first.c_str(), kv.second.first, kv.second.second);
+  fclose(f);
+  return RFD_OK;
+}
+
+int rfd_tune_cache_load(const char *path) {
+  if (!path) return RFD_E_ARG;
+  FILE *f = fopen(path, "r");
+  if (!f) return RFD_E_ARG;
+  char k[160];
+  int a = 0, b = 0;
+  std::lock_guard<std::mutex> lk(tune_mu());
+  while (fscanf(f, "%159s %d %d", k, &a, &b) == 3)
+    if ((a == 32 || a == 64 || a == 128) && (b == 64 || b == 128 || b == 256))
+      tune_table()[k] = {a, b};
+  fclose(f);
+  return RFD_OK;
+}
+
 int rfd_system_get_history(rfd_system *s, double *hist, uint32_t cap, uint32_t *n) {
   if (!valid(s)) return RFD_E_ARG;
   rfd_ctx *ctx = s->ctx;
@@ -2619,6 +3110,9 @@ int rfd_system_fire_load_step(rfd_system *s, double dstrain, const rfd_fire_para
   RFD_CHECK_ARG(ctx, fp && out && s->have_bonds, "rfd_system_fire_load_step: bad arguments");
   if (s->fp64)
     return rfd_set_error(ctx, RFD_E_STATE, "rfd_system_fire_load_step: the FP64 path runs "
+                                           "explicit steps only");
+  if (s->S)
+    return rfd_set_error(ctx, RFD_E_STATE, "rfd_system_fire_load_step: batched specimens run "
                                            "explicit steps only");
   RFD_CHECK_ARG(ctx, fp->dt0 > 0 && fp->dt_max >= fp->dt0 && fp->ftol > 0 && fp->max_iter > 0,
                 "rfd_system_fire_load_step: bad FIRE parameters");

@@ -64,6 +64,17 @@ class FireResult(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("force_tile", C.c_int), ("nbr_block", C.c_int), ("cached", C.c_int),
+                ("bucket", C.c_int), ("force_ms", C.c_double * 3), ("nbr_ms", C.c_double * 3)]
+
+    def as_dict(self) -> dict:
+        return {"force_tile": self.force_tile, "nbr_block": self.nbr_block,
+                "cached": bool(self.cached), "bucket": self.bucket,
+                "force_ms": {t: self.force_ms[k] for k, t in enumerate((64, 32, 128))},
+                "nbr_ms": {b: self.nbr_ms[k] for k, b in enumerate((128, 64, 256))}}
+
+
 _vp, _u32, _dbl = C.c_void_p, C.c_uint32, C.c_double
 NATIVE_SIGNATURES = {
     "rfd_system_create": [_vp, C.POINTER(SystemParams), _u32, C.POINTER(_vp)],
@@ -94,6 +105,16 @@ NATIVE_SIGNATURES = {
     "rfd_system_bond_graph": [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp),
                               C.POINTER(_vp), _u32p],
     "rfd_system_timing_read": [_vp, C.POINTER(C.c_double), _u32p],
+    "rfd_system_set_specimens": [_vp, _u32, _vp, C.POINTER(Loading), _u32],
+    "rfd_system_get_specimen_loading": [_vp, _u32, C.POINTER(Loading)],
+    "rfd_system_get_specimen_history": [_vp, _u32, _vp, _u32, _u32p],
+    "rfd_system_set_grid": [_vp, C.POINTER(C.c_double), _dbl, _u32p],
+    "rfd_system_get_grid": [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double), _u32p],
+    "rfd_system_step_graph": [_vp, _u32],
+    "rfd_system_autotune": [_vp, C.c_int, C.POINTER(TuneResult)],
+    "rfd_system_set_tiles": [_vp, C.c_int, C.c_int],
+    "rfd_tune_cache_save": [C.c_char_p],
+    "rfd_tune_cache_load": [C.c_char_p],
 }
 
 
@@ -164,6 +185,7 @@ class NativeSystem:
         _, pr = _np(self.radius, np.float32)
         self._call("rfd_system_set_particles", self.h, p, pr)
         self.bonds: Bonds | None = None
+        self.S = 0
 
     def _call(self, name, *args, sys=True):
         st = getattr(self.L, name)(*args)
@@ -328,6 +350,58 @@ class NativeSystem:
     def timing(self, enable: bool = True):
         self._call("rfd_system_timing", self.h, int(enable))
 
+    # -- batched specimens, grid, graphs, launch shapes --------------------
+    def set_specimens(self, first, loadings, hist_cap: int = 1 << 14):
+        """Specimen k owns particles [first[k], first[k+1]) and is driven by
+        loadings[k]; an empty list returns to the single loading control."""
+        S = len(loadings)
+        if S == 0:
+            self._call("rfd_system_set_specimens", self.h, 0, None, None, 0)
+            self.S = 0
+            return
+        f = np.ascontiguousarray(first, np.uint32)
+        arr = (Loading * S)(*loadings)
+        self._call("rfd_system_set_specimens", self.h, S, f.ctypes.data_as(C.c_void_p), arr,
+                   hist_cap)
+        self.S = S
+        self.spec_first = f.copy()
+
+    def specimen_loadings(self) -> list[Loading]:
+        arr = (Loading * self.S)()
+        self._call("rfd_system_get_specimen_loading", self.h, self.S, arr)
+        return list(arr)
+
+    def specimen_history(self, k: int) -> np.ndarray:
+        n = C.c_uint32()
+        self._call("rfd_system_get_specimen_history", self.h, k, None, 0, C.byref(n))
+        h = np.zeros(3 * max(n.value, 1), np.float64)
+        self._call("rfd_system_get_specimen_history", self.h, k, h.ctypes.data_as(C.c_void_p),
+                   n.value, C.byref(n))
+        return h[: 3 * min(n.value, len(h) // 3)].reshape(-1, 3)
+
+    def set_grid(self, origin, cell: float, dims):
+        o = (C.c_double * 3)(*[float(x) for x in origin])
+        d = (C.c_uint32 * 3)(*[int(x) for x in dims])
+        self._call("rfd_system_set_grid", self.h, o, float(cell), d)
+
+    def grid(self):
+        o = (C.c_double * 3)()
+        c = C.c_double()
+        d = (C.c_uint32 * 3)()
+        self._call("rfd_system_get_grid", self.h, o, C.byref(c), d)
+        return tuple(o), c.value, tuple(d)
+
+    def step_graph(self, n: int = 1):
+        self._call("rfd_system_step_graph", self.h, n)
+
+    def set_tiles(self, force_tile: int, nbr_block: int):
+        self._call("rfd_system_set_tiles", self.h, force_tile, nbr_block)
+
+    def autotune(self, use_cache: bool = True) -> TuneResult:
+        r = TuneResult()
+        self._call("rfd_system_autotune", self.h, int(use_cache), C.byref(r))
+        return r
+
     def timing_read(self) -> tuple[float, int]:
         ms, n = C.c_double(), C.c_uint32()
         self._call("rfd_system_timing_read", self.h, C.byref(ms), C.byref(n))
@@ -337,6 +411,16 @@ class NativeSystem:
         out = (C.c_uint32 * 3)()
         self._call("rfd_system_bond_counts", self.h, out)
         return tuple(out)
+
+
+def tune_cache_save(path: str) -> None:
+    if _bind().rfd_tune_cache_save(str(path).encode()) != 0:
+        raise RfdError("rfd_tune_cache_save", -1, f"cannot write {path}")
+
+
+def tune_cache_load(path: str) -> None:
+    if _bind().rfd_tune_cache_load(str(path).encode()) != 0:
+        raise RfdError("rfd_tune_cache_load", -1, f"cannot read {path}")
 
 
 # ------------------------------------------------------------- loading

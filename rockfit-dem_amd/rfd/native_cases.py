@@ -57,3 +57,130 @@ def uniaxial_cylinder(ctx, D: float, H: float, seed: int = specimen.BASE_SEED,
                       min_stress=15e6 / STRESS, max_strain=0.1)
     sysn.set_loading(loading_from_compression(cmp, box.center[2] + box.spacesized2[2], box))
     return NativeCase(sysn, mat, box, cmp, perm)
+
+
+@dataclass
+class BatchCase:
+    sys: NativeSystem
+    mat: Material
+    first: np.ndarray          # specimen k owns particles [first[k], first[k+1])
+    n_target: np.ndarray       # drawn particle counts
+    D: np.ndarray              # disc diameters (mm)
+    T: np.ndarray              # disc thicknesses (mm)
+    mu_t: np.ndarray           # mean bond tensile strength per specimen (MPa)
+    seeds: np.ndarray
+    loadings: list
+    bond_first: np.ndarray     # specimen k owns bond records [bond_first[k], bond_first[k+1])
+
+
+def disc_diameter_for(n: float, d_nn: float = 1.0) -> float:
+    """Diameter of a disc of thickness D/4 holding about n particles of an FCC
+    lattice of spacing d_nn (number density sqrt(2) / d_nn^3)."""
+    return float((16.0 * n * d_nn ** 3 / (np.sqrt(2.0) * np.pi)) ** (1.0 / 3.0))
+
+
+def _disc_count(D: float, T: float) -> int:
+    """Lattice sites of a disc of diameter D and thickness T (the keep rule of
+    specimen.cylinder at the mean radius 0.5 mm)."""
+    pts = specimen.fcc_points((D, D, T), 1.0)
+    return int(np.count_nonzero((pts[:, 0] ** 2 + pts[:, 1] ** 2 <= (0.5 * D - 0.5) ** 2) &
+                                (np.abs(pts[:, 2]) <= 0.5 * T - 0.5)))
+
+
+def _disc_holding(n: float) -> tuple[float, float]:
+    """(D, T): thickness a quarter of the nominal diameter for n particles,
+    diameter bisected until the disc holds n (the count grows by rings in D)."""
+    D0 = disc_diameter_for(n)
+    T = D0 / 4.0
+    lo, hi = 0.7 * D0, 2.0 * D0
+    for _ in range(16):
+        mid = 0.5 * (lo + hi)
+        if _disc_count(mid, T) < n:
+            lo = mid
+        else:
+            hi = mid
+    return hi, T
+
+
+def brazil_batch(ctx, S: int = 256, seed: int = specimen.BASE_SEED, n_lo: float = 1000.0,
+                 n_hi: float = 40000.0, mu_range=(10.0, 30.0), cv: float = 0.2,
+                 spacing: float = 3.0, v_load: float = -0.5 / VELOCITY,
+                 rebuild_every: int = 20, min_stress_mpa: float = 1.0,
+                 hist_cap: int = 1 << 13, build: bool = True):
+    """Batched Brazilian discs: S specimens with particle counts log-uniform in
+    [n_lo, n_hi], each from its own seed (packing jitter and radii
+    r ~ U[0.4, 0.6] mm) and with its own bond strengths N(mu, cv mu) MPa
+    (tensile, mu ~ U(mu_range); shear twice that).  Disc k has diameter D_k
+    holding its drawn count at thickness about D_k / 4 (axis z); the discs are
+    stacked along z with ``spacing`` mm between them, so that no contact forms
+    between specimens; platens act along y on each disc separately.
+    build=False returns the geometry only (no device work)."""
+    g = specimen.rng(seed + 17)
+    n_target = np.exp(g.uniform(np.log(n_lo), np.log(n_hi), S))
+    mu_t = g.uniform(mu_range[0], mu_range[1], S)
+    seeds = seed + 1000 + np.arange(S)
+    # the lattice inset at the boundary loses particles: each diameter is
+    # bisected until the disc holds its drawn count
+    DT = np.array([_disc_holding(n) for n in n_target])
+    D, T = DT[:, 0], DT[:, 1]
+    mat = material_c(0.4)
+    Rs, rads, first, loadings = [], [], [0], []
+    z = 0.0
+    cx = cy = 0.5 * float(D.max()) + 10.0
+    for k in range(S):
+        cz = z + 0.5 * T[k]
+        z += T[k] + spacing
+        sp = specimen.disc(float(D[k]), float(T[k]), center=(cx, cy, cz), d_nn=1.0,
+                           seed=int(seeds[k]))
+        perm = morton_order(sp.R, 2.0)
+        Rs.append(sp.R[perm])
+        rads.append(sp.radius[perm])
+        first.append(first[-1] + len(perm))
+        size = (float(D[k]) - 1.0, float(D[k]) - 1.0, float(T[k]) - 1.0)
+        cmp = Compression("brazil", size, (cx, cy, cz), 0.5, mat.dt, v_load=v_load,
+                          min_stress=min_stress_mpa * 1e6 / STRESS, max_strain=0.1)
+        loadings.append(loading_from_compression(cmp, 1e30))
+    first = np.asarray(first, np.int64)
+    if not build:
+        return BatchCase(None, mat, first, n_target, D, T, mu_t, seeds, loadings, None)
+    R = np.concatenate(Rs)
+    rad = np.concatenate(rads)
+    p = params_from_material(mat, refmode=0, rebuild_every=rebuild_every)
+    p.farcut = 1.45
+    sysn = NativeSystem(ctx, p, R, rad)
+    pairs = sysn.find_pairs(1.0 - 0.8 + 0.01)
+    spec_of = np.searchsorted(first, pairs[:, 0], side="right") - 1
+    if np.any(spec_of != np.searchsorted(first, pairs[:, 1], side="right") - 1):
+        raise ValueError("a bond joins two specimens: increase the spacing")
+    bond_first = np.searchsorted(spec_of, np.arange(S + 1))
+    sc = np.empty(len(pairs), np.float32)
+    tc = np.empty(len(pairs), np.float32)
+    for k in range(S):
+        a, b = bond_first[k], bond_first[k + 1]
+        sc[a:b], tc[a:b] = strengths(specimen.rng(int(seeds[k]) + 1), int(b - a),
+                                     mu_t[k] * 1e6, 2.0 * mu_t[k] * 1e6, cv)
+    br = np.minimum(rad[pairs[:, 0]], rad[pairs[:, 1]]).astype(np.float32)
+    sysn.set_bonds(Bonds(pairs[:, 0].copy(), pairs[:, 1].copy(), br, sc, tc))
+    sysn.set_specimens(first, loadings, hist_cap)
+    return BatchCase(sysn, mat, first, n_target, D, T, mu_t, seeds, loadings, bond_first)
+
+
+def batch_member(ctx, case: BatchCase, k: int) -> NativeSystem:
+    """Specimen k of a batch as a system of its own, at the same place, with
+    the batch's cell grid, bonds and loading control: stepped alone it follows
+    the batch's specimen k bit for bit.  Call it before the batch steps (the
+    bond rest lengths are taken from the current positions)."""
+    bs = case.sys
+    a, b = int(case.first[k]), int(case.first[k + 1])
+    st = bs.state()
+    N = bs.N
+    R = st["R"].reshape(3, N).T[a:b]
+    sysn = NativeSystem(ctx, bs.params, R, bs.radius[a:b])
+    origin, cell, dims = bs.grid()
+    sysn.set_grid(origin, cell, dims)
+    bb = bs.bonds
+    m = (bb.bi >= a) & (bb.bi < b)
+    sysn.set_bonds(Bonds((bb.bi[m] - a).astype(np.uint32), (bb.bj[m] - a).astype(np.uint32),
+                         bb.br[m], bb.sc[m], bb.tc[m], None if bb.L0 is None else bb.L0[m]))
+    sysn.set_loading(case.loadings[k])
+    return sysn
