@@ -43,6 +43,10 @@ def parse():
                    help="skip the learned-warm-start load-step measurement")
     p.add_argument("--warmstart-only", action="store_true",
                    help="only the learned-warm-start load-step measurement")
+    p.add_argument("--no-batch", action="store_true",
+                   help="skip the batched Brazilian-disc measurement")
+    p.add_argument("--batch-only", action="store_true",
+                   help="only the batched Brazilian-disc measurement")
     p.add_argument("--copy-calib", action="store_true",
                    help="only run a plain device copy of 1 GiB (counter calibration and "
                         "stream-copy ceiling)")
@@ -307,6 +311,120 @@ def bench_peak_precision(D: float = 10.0, H: float = 20.0, max_steps: int = 2000
     return out
 
 
+def _quantiles(x) -> dict:
+    import numpy as np
+
+    q = np.percentile(x, [0, 10, 50, 90, 100])
+    return {"min": q[0], "p10": q[1], "median": q[2], "p90": q[3], "max": q[4],
+            "mean": float(np.mean(x)), "std": float(np.std(x))}
+
+
+def bench_batch(S: int = 256, max_steps: int = 16000, chunk: int = 400) -> dict:
+    """Batched Brazilian discs: S specimens (particle counts log-uniform in
+    [1k, 40k], own seeds, own strength means) in one system, launch shapes
+    from the autotuner, stepped with graph replays until every disc has
+    fractured or passed its peak (stress below half of it) or max_steps.
+    Reports the step rate of the batch and the per-specimen peak loads; a
+    2k-particle disc alone measures what graph replays save on small work."""
+    import numpy as np
+
+    from rfd import native_cases
+    from rfd.reflayout import Context
+
+    ctx = Context(0)
+    t0 = time.perf_counter()
+    case = native_cases.brazil_batch(ctx, S=S)
+    s = case.sys
+    setup_s = time.perf_counter() - t0
+    tune = s.autotune(use_cache=True).as_dict()
+    print(f"[bench] batch: {S} discs, N={s.N}, bonds={s.bonds.n}, setup {setup_s:.1f}s, "
+          f"tiles {tune['force_tile']}/{tune['nbr_block']}", file=sys.stderr, flush=True)
+
+    def sync():
+        return s.specimen_loadings()
+
+    sync()
+    # plain launches and graph replays over the first steps of the run
+    t = time.perf_counter()
+    s.step(chunk)
+    sync()
+    ms_plain = (time.perf_counter() - t) / chunk * 1e3
+    t = time.perf_counter()
+    s.step_graph(chunk)
+    Ls = sync()
+    ms_graph = (time.perf_counter() - t) / chunk * 1e3
+    done, t_run = 2 * chunk, time.perf_counter()
+    while done < max_steps:
+        s.step_graph(chunk)
+        done += chunk
+        Ls = sync()
+        st = np.array([L.stress for L in Ls])
+        pk = np.array([L.fracture_stress for L in Ls])
+        fin = np.array([L.fracture for L in Ls], bool) | (st < 0.5 * pk)
+        if done % (10 * chunk) == 0:
+            print(f"[bench] batch step {done}: {fin.sum()} of {S} past peak",
+                  file=sys.stderr, flush=True)
+        if fin.all():
+            break
+    run_s = time.perf_counter() - t_run
+    Ls = s.specimen_loadings()
+    pk = np.array([L.fracture_stress for L in Ls])
+    st = np.array([L.stress for L in Ls])
+    fr = np.array([L.fracture for L in Ls], bool)
+    past = fr | (st < 0.5 * pk)
+    peak_mpa = pk * 1e3
+    load_kn = pk / np.array([L.inv_area for L in Ls])  # force unit: 1 kN
+    n = np.diff(case.first)
+    counts = s.bond_counts()
+    steps_timed = done - 2 * chunk
+    ms_run = run_s / max(steps_timed, 1) * 1e3
+    out = {
+        "workload": f"{S} Brazilian discs, N log-uniform in [1k, 40k] "
+                    f"(total {s.N}), r~U[0.4,0.6] mm, bond strength N(mu,0.2mu), "
+                    "mu~U[10,30] MPa per disc, platens 0.5 m/s, explicit FP32 steps",
+        "specimens": S,
+        "particles": int(s.N),
+        "bonds": int(s.bonds.n),
+        "particles_per_specimen": _quantiles(n),
+        "autotune": tune,
+        "setup_seconds": setup_s,
+        "ms_per_step_plain": ms_plain,
+        "ms_per_step_graph": ms_graph,
+        "ms_per_step_run": ms_run,
+        "bond_updates_per_s": s.bonds.n / (ms_run * 1e-3),
+        "specimen_steps_per_s": S / (ms_run * 1e-3),
+        "steps": done,
+        "run_seconds": run_s,
+        "past_peak": int(past.sum()),
+        "fractured": int(fr.sum()),
+        "bonds_intact_broken_removed": list(counts),
+        "peak_stress_MPa": _quantiles(peak_mpa),
+        "peak_load_kN": _quantiles(load_kn),
+        "corr_peak_vs_mu": float(np.corrcoef(peak_mpa, case.mu_t)[0, 1]),
+        "corr_peak_vs_log_n": float(np.corrcoef(peak_mpa, np.log(n))[0, 1]),
+        "peaks_MPa": [round(float(x), 3) for x in peak_mpa],
+    }
+    s.close()
+    # a single small disc: launch overhead dominates, graph replays pay
+    one = native_cases.brazil_batch(ctx, S=1, n_lo=2000, n_hi=2000, seed=7).sys
+    one.step(40)
+    one.specimen_loadings()
+    small = {}
+    for name, fn in (("plain", one.step), ("graph", one.step_graph)):
+        fn(400)
+        one.specimen_loadings()
+        t = time.perf_counter()
+        fn(2000)
+        one.specimen_loadings()
+        small[name] = (time.perf_counter() - t) / 2000 * 1e6
+    out["small_disc_us_per_step"] = {"particles": int(one.N), "plain": small["plain"],
+                                     "graph": small["graph"]}
+    one.close()
+    print(f"[bench] batch: {out['steps']} steps, {out['past_peak']} past peak, "
+          f"{ms_run:.3f} ms/step; small disc {small}", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -319,6 +437,9 @@ def main():
         return
     if args.warmstart_only:
         print(json.dumps({"warm_start": bench_warmstart()}))
+        return
+    if args.batch_only:
+        print(json.dumps({"batch": bench_batch()}))
         return
     res = bench_ref(args) if args.tier == "ref" else bench_native(args)
     res["roofline"]["traffic"] = load_traffic(res["roofline"]["kernel"],
@@ -353,6 +474,11 @@ def main():
             out["warm_start"] = bench_warmstart()
         except Exception as e:  # reported, never hides the main line
             out["warm_start"] = {"error": f"{type(e).__name__}: {e}"}
+    if not args.no_batch and args.tier == "native":
+        try:
+            out["batch"] = bench_batch()
+        except Exception as e:  # reported, never hides the main line
+            out["batch"] = {"error": f"{type(e).__name__}: {e}"}
     print(json.dumps(out))
 
 

@@ -813,3 +813,91 @@ void orc_native_bonds_f64(const double *R, const double *V, const double *W,
     }
   }
 }
+
+/* Contact law of the native tier per directed pair (i sees j), in double:
+ * md_calculateforcesdem3.cu:199-336 (normal Hertz force with viscous damping
+ * dd_Calculate_fijn_ppHM_2 :235-252, tangential spring with Coulomb cap
+ * dd_Calculate_fijtmijt_ppHM_2 :255-314, rolling resistance
+ * dd_Calculate_mijroll_ppHM_2 :316-336) restated for per-particle radii and
+ * masses: R* = ri rj/(ri+rj), m* = mi mj/(mi+mj), lever arms ri and rj.  The
+ * pair overlaps once its centre distance is below reach[k] (ri + rj; for a
+ * broken bond of the physical mode min(ri + rj, L0)).  Not overlapping: the
+ * spring O[3k] is cleared.  The rolling-moment wipe of the reference mode is
+ * not part of it. */
+void orc_native_contacts_f64(const double *R, const double *V, const double *W,
+                             const double *rad, uint32_t N, const uint32_t *pi,
+                             const uint32_t *pj, const double *reach, double *O,
+                             uint32_t P, double dt, double Estar, double Gstar,
+                             double pA, double mu, double mur, double rho43pi,
+                             double *F, double *M) {
+  memset(F, 0, sizeof(double) * 3 * (size_t)N);
+  memset(M, 0, sizeof(double) * 3 * (size_t)N);
+  for (uint32_t k = 0; k < P; ++k) {
+    const uint32_t i = pi[k], j = pj[k];
+    double r[3], Vi[3], Vj[3], Wi[3], Wj[3];
+    for (int d = 0; d < 3; ++d) {
+      r[d] = R[j + (size_t)d * N] - R[i + (size_t)d * N];
+      Vi[d] = V[i + (size_t)d * N];
+      Vj[d] = V[j + (size_t)d * N];
+      Wi[d] = W[i + (size_t)d * N];
+      Wj[d] = W[j + (size_t)d * N];
+    }
+    double *o = O + 3 * (size_t)k;
+    const double L = sqrt(dot3(r, r));
+    if (!(L < reach[k]) || !(L > 1e-5)) {
+      o[0] = o[1] = o[2] = 0.0;
+      continue;
+    }
+    const double n[3] = {r[0] / L, r[1] / L, r[2] / L};
+    const double ri = rad[i], rj = rad[j];
+    const double Rs = ri * rj / (ri + rj);
+    const double mi = rho43pi * ri * ri * ri, mj = rho43pi * rj * rj * rj;
+    const double ms = mi * mj / (mi + mj);
+    const double q = sqrt(Rs * (reach[k] - L));
+    const double Kn = 2.0 * Estar * q, Kt = 8.0 * Gstar * q;
+    double arm[3] = {Wi[0] * ri + Wj[0] * rj, Wi[1] * ri + Wj[1] * rj,
+                     Wi[2] * ri + Wj[2] * rj}, nxa[3];
+    cross3(n, arm, nxa);
+    double v[3] = {Vj[0] - Vi[0] + nxa[0], Vj[1] - Vi[1] + nxa[1], Vj[2] - Vi[2] + nxa[2]};
+    const double vn = dot3(n, v);
+    const double fn = (L - reach[k]) * Kn * (2.0 / 3.0);
+    const double dmp = -1.8257 * pA * vn * sqrt(Kn * ms);
+    double vt[3], ot[3], ft[3];
+    const double on = dot3(n, o);
+    for (int d = 0; d < 3; ++d) {
+      vt[d] = v[d] - n[d] * vn;
+      ot[d] = o[d] - n[d] * on;
+    }
+    /* the old spring turned into the tangent plane, length kept */
+    const double oo = dot3(ot, ot);
+    if (oo > 1e-18) {
+      const double s = sqrt(dot3(o, o) / oo);
+      for (int d = 0; d < 3; ++d) ot[d] *= s;
+    }
+    for (int d = 0; d < 3; ++d) {
+      ot[d] += vt[d] * dt;
+      ft[d] = ot[d] * Kt;
+    }
+    const double ftm = dot3(ft, ft);
+    if (ftm > mu * mu * fn * fn) {
+      const double s = mu * fabs(fn) / sqrt(ftm);
+      for (int d = 0; d < 3; ++d) {
+        ft[d] *= s;
+        ot[d] = ft[d] / Kt;
+      }
+    } else {
+      const double c = -1.8257 * pA * sqrt(Kt * ms);
+      for (int d = 0; d < 3; ++d) ft[d] += vt[d] * c;
+    }
+    double nxf[3];
+    cross3(n, ft, nxf);
+    const double wi2 = dot3(Wi, Wi);
+    for (int d = 0; d < 3; ++d) {
+      o[d] = ot[d];
+      F[i + (size_t)d * N] += n[d] * (fn + dmp) + ft[d];
+      double m = nxf[d] * ri;
+      if (wi2 > 1e-18) m += Wi[d] * (-mur * fabs(fn) / sqrt(wi2));
+      M[i + (size_t)d * N] += m;
+    }
+  }
+}

@@ -117,6 +117,17 @@ void orc_native_bonds_f64(const double *R, const double *V, const double *W,
                           uint32_t Nb, double dt, double E, double G,
                           int refmode, double *F, double *M);
 
+/* Native-tier contact law per directed pair (i sees j), double precision:
+ * pairs (pi[k], pj[k]) overlap below reach[k]; O[3k] is the pair's shear
+ * spring (updated); F, M (zeroed here) receive i's load.  Per-particle radii
+ * rad[N]; masses rho43pi r^3. */
+void orc_native_contacts_f64(const double *R, const double *V, const double *W,
+                             const double *rad, uint32_t N, const uint32_t *pi,
+                             const uint32_t *pj, const double *reach, double *O,
+                             uint32_t P, double dt, double Estar, double Gstar,
+                             double pA, double mu, double mur, double rho43pi,
+                             double *F, double *M);
+
 #ifdef __cplusplus
 }
 #endif

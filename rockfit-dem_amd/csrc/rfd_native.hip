@@ -139,18 +139,20 @@ struct EndData {
 // Hertz-Mindlin contact for the pair (i, j) (md_calculateforcesdem3.cu:199-336),
 // n from i to j, overlap ri + rj - L; O is i's tangential spring (in/out).
 // Returns the load on i and the contact moments on i and j.
+// reach: the centre distance at which the pair starts to overlap (ri + rj,
+// or less for a broken bond made while overlapping)
 __device__ __forceinline__ void contact_law(const Mat &mt, v3 n, float L, float ri, float rj,
-                                            v3 Vi, v3 Vj, v3 Wi, v3 Wj, v3 &O, v3 &Fi, v3 &Mi,
-                                            v3 &Mj) {
+                                            float reach, v3 Vi, v3 Vj, v3 Wi, v3 Wj, v3 &O,
+                                            v3 &Fi, v3 &Mi, v3 &Mj) {
   const float Rs = ri * rj / (ri + rj);
   const float mi = mt.rho43pi * ri * ri * ri, mj = mt.rho43pi * rj * rj * rj;
   const float ms = mi * mj / (mi + mj);
-  const float q = sqrtf(Rs * (ri + rj - L));
+  const float q = sqrtf(Rs * (reach - L));
   const float Kn = 2.0f * mt.Estar * q;
   const float Kt = 8.0f * mt.Gstar * q;
   const v3 v = Vj - Vi + cross(n, Wi * ri + Wj * rj);
   const float vn = dot(n, v);
-  const float fn = (L - (ri + rj)) * Kn * kTwoThirdsF;
+  const float fn = (L - reach) * Kn * kTwoThirdsF;
   const float dmp = -1.8257f * mt.pA * vn * sqrtf(Kn * ms);
   Fi = n * (fn + dmp);
   const v3 vt = v - n * vn;
@@ -271,9 +273,13 @@ __device__ __forceinline__ void eval_bond(const Mat &mt, const EndData &A, const
     }
   }
   if (contact_now) {
-    if (L < ri + rj) {
+    // a broken bond meets the contact law at its rest length when it was made
+    // overlapping, so the overlap it was made with does not spring apart
+    // (reference mode: at ri + rj, as md_calculateforcesdem3.cu does)
+    const float reach = mt.refmode ? ri + rj : fminf(ri + rj, 1.0f / c.il);
+    if (L < reach) {
       v3 Fc;
-      contact_law(mt, n, L, ri, rj, A.V, B.V, A.W, B.W, O, Fc, o.Mci, o.Mcj);
+      contact_law(mt, n, L, ri, rj, reach, A.V, B.V, A.W, B.W, O, Fc, o.Mci, o.Mcj);
       o.Fi = o.Fi + Fc;
       o.hasc = true;
     } else {
@@ -452,7 +458,8 @@ k_nat_forces(SysDev s, Mat mt, uint32_t ntiles, uint32_t per) {
       const v3 n = {(float)(rx * inv), (float)(ry * inv), (float)(rz * inv)};
       v3 O = xyz(s.cO[sl]);
       v3 Fi, Mi, Mj;
-      contact_law(mt, n, (float)L64, ri, rj, me.V, other.V, me.W, other.W, O, Fi, Mi, Mj);
+      contact_law(mt, n, (float)L64, ri, rj, ri + rj, me.V, other.V, me.W, other.W, O, Fi, Mi,
+                  Mj);
       F = F + Fi;
       Mc = Mc + Mi;
       anyc = true;
@@ -1366,16 +1373,16 @@ struct Rot64 {
 };
 
 __device__ __forceinline__ void contact_law64(const Mat64 &mt, d3 n, double L, double ri,
-                                              double rj, d3 Vi, d3 Vj, d3 Wi, d3 Wj, d3 &O,
-                                              d3 &Fi, d3 &Mi, d3 &Mj) {
+                                              double rj, double reach, d3 Vi, d3 Vj, d3 Wi,
+                                              d3 Wj, d3 &O, d3 &Fi, d3 &Mi, d3 &Mj) {
   const double Rs = ri * rj / (ri + rj);
   const double mi = mt.rho43pi * ri * ri * ri, mj = mt.rho43pi * rj * rj * rj;
   const double ms = mi * mj / (mi + mj);
-  const double q = __builtin_sqrt(Rs * (ri + rj - L));
+  const double q = __builtin_sqrt(Rs * (reach - L));
   const double Kn = 2.0 * mt.Estar * q, Kt = 8.0 * mt.Gstar * q;
   const d3 v = Vj - Vi + cross(n, Wi * ri + Wj * rj);
   const double vn = dot(n, v);
-  const double fn = (L - (ri + rj)) * Kn * (2.0 / 3.0);
+  const double fn = (L - reach) * Kn * (2.0 / 3.0);
   const double dmp = -1.8257 * mt.pA * vn * __builtin_sqrt(Kn * ms);
   Fi = n * (fn + dmp);
   const d3 vt = v - n * vn;
@@ -1493,9 +1500,10 @@ __global__ void __launch_bounds__(128) k_f64_forces(Sys64 s, Mat64 mt) {
         }
       }
       if (contact_now) {
-        if (L < A.r + B.r) {
+        const double reach = mt.refmode ? A.r + B.r : fmin(A.r + B.r, 1.0 / (double)c.il);
+        if (L < reach) {
           d3 Fc, Mci, Mcj;
-          contact_law64(mt, n, L, A.r, B.r, Vi, Vj, Wi, Wj, O, Fc, Mci, Mcj);
+          contact_law64(mt, n, L, A.r, B.r, reach, Vi, Vj, Wi, Wj, O, Fc, Mci, Mcj);
           F = F + Fc;
           Mc = Mc + Mci;
           anyc = true;
@@ -1522,8 +1530,8 @@ __global__ void __launch_bounds__(128) k_f64_forces(Sys64 s, Mat64 mt) {
       if (L < A.r + B.r && L > 1e-5) {
         d3 O = xyz(s.cO[sl]);
         d3 Fi, Mi, Mj;
-        contact_law64(mt, r * (1.0 / L), L, A.r, B.r, Vi, xyz(s.V[j]), Wi, xyz(s.W[j]), O, Fi,
-                      Mi, Mj);
+        contact_law64(mt, r * (1.0 / L), L, A.r, B.r, A.r + B.r, Vi, xyz(s.V[j]), Wi,
+                      xyz(s.W[j]), O, Fi, Mi, Mj);
         F = F + Fi;
         Mc = Mc + Mi;
         anyc = true;

@@ -205,6 +205,15 @@ class NativeSystem:
             pass
 
     # -- setup ---------------------------------------------------------
+    def set_positions(self, R: np.ndarray):
+        """New positions (N, 3) for the same particles; velocities, loads and
+        contact rows are cleared, bonds and their state are kept."""
+        R = np.asarray(R, np.float64)
+        assert R.shape == (self.N, 3)
+        _, p = _np(R.T.reshape(-1), np.float64)
+        _, pr = _np(self.radius, np.float32)
+        self._call("rfd_system_set_particles", self.h, p, pr)
+
     def find_pairs(self, gap: float) -> np.ndarray:
         cnt = C.c_uint32()
         self._call("rfd_system_find_pairs", self.h, gap, None, 0, C.byref(cnt))

@@ -139,7 +139,15 @@ def brazil_batch(ctx, S: int = 256, seed: int = specimen.BASE_SEED, n_lo: float 
         size = (float(D[k]) - 1.0, float(D[k]) - 1.0, float(T[k]) - 1.0)
         cmp = Compression("brazil", size, (cx, cy, cz), 0.5, mat.dt, v_load=v_load,
                           min_stress=min_stress_mpa * 1e6 / STRESS, max_strain=0.1)
-        loadings.append(loading_from_compression(cmp, 1e30))
+        L = loading_from_compression(cmp, 1e30)
+        # platens start on the outermost particle centres (they hold a particle
+        # once its centre reaches them), so that the load builds from the first
+        # step instead of arriving with an impact after the fast approach
+        y = sp.R[:, 1]
+        L.top, L.bottom = float(y.max()), float(y.min())
+        L.top_r, L.bottom_r = L.top - 0.5, L.bottom + 0.5
+        L.size = L.size0 = L.top - L.bottom
+        loadings.append(L)
     first = np.asarray(first, np.int64)
     if not build:
         return BatchCase(None, mat, first, n_target, D, T, mu_t, seeds, loadings, None)

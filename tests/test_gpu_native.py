@@ -11,7 +11,7 @@ Tolerances are stated per test (relative to the largest component).
 import numpy as np
 import pytest
 
-from oracle.oracle import OracleLayout, lib as olib, ptr
+from oracle.oracle import OracleLayout, lib as olib, native_contacts, ptr
 from rfd import cases, native, native_cases
 from rfd.reflayout import UniaxialDriver
 
@@ -117,6 +117,62 @@ def test_physical_mode_matches_f64_oracle(ctx):
             np.testing.assert_allclose(out[k], ref, rtol=0, atol=1e-4 * scale,
                                        err_msg=f"{k} at evaluation {it}")
     assert act.all()
+
+
+@pytest.mark.parametrize("refmode", [0, 1])
+def test_contacts_of_broken_bonds_and_new_pairs_match_oracle(ctx, refmode):
+    """Every bond broken, the specimen squeezed along z and jittered so that
+    bonded pairs come closer than their rest length and never-bonded pairs
+    come into contact, random V and W: one force
+    evaluation against the FP64 native contact law of the oracle, per
+    directed pair (broken bonds and never-bonded pairs within ri + rj).  A
+    broken bond overlaps from min(ri + rj, L0) in the physical mode and from
+    ri + rj in the reference mode.  Tolerance 1e-4 of max |F| and max |M|."""
+    from scipy.spatial import cKDTree
+
+    nc = native_cases.uniaxial_cylinder(ctx, 6.0, 8.0, seed=21, refmode=refmode)
+    s = nc.sys
+    N = s.N
+    R0 = _soa3(s.state()["R"], N)
+    b = s.bonds
+    L0 = np.linalg.norm(R0[b.bj] - R0[b.bi], axis=1)
+    iL = (1.0 / L0).astype(np.float32).astype(np.float64)
+    g = np.random.default_rng(23)
+    # squeeze along z: second lattice neighbours along z (sqrt(2) apart) come
+    # within ri + rj, bonded pairs along z come closer than L0
+    c = R0.mean(0)
+    R = c + (R0 - c) * np.array([1.0, 1.0, 0.82]) + g.standard_normal((N, 3)) * 0.02
+    s.set_positions(R)
+    s.set_bond_state(np.zeros((b.n, 12), np.float32), np.zeros(b.n, np.uint8))
+    V = (g.standard_normal((N, 3)) * 2e-2).astype(np.float32)
+    W = (g.standard_normal((N, 3)) * 4e-2).astype(np.float32)
+    s.set_velocities(V.T.reshape(-1), W.T.reshape(-1))
+    s.compute_forces()
+    out = s.state()
+    rad = s.radius.astype(np.float64)
+    sumr = rad[b.bi] + rad[b.bj]
+    reach_b = sumr if refmode else np.minimum(sumr, 1.0 / iL)
+    bonded = set(zip(b.bi.tolist(), b.bj.tolist()))
+    cand = cKDTree(R).query_pairs(1.3, output_type="ndarray")
+    d = np.linalg.norm(R[cand[:, 1]] - R[cand[:, 0]], axis=1)
+    keep = np.array([(int(i), int(j)) not in bonded for i, j in cand]) & \
+        (d < rad[cand[:, 0]] + rad[cand[:, 1]])
+    newp = cand[keep]
+    pairs = np.concatenate([np.stack([b.bi, b.bj], 1), np.stack([b.bj, b.bi], 1),
+                            newp, newp[:, ::-1]]).astype(np.uint32)
+    reach = np.concatenate([reach_b, reach_b, rad[newp[:, 0]] + rad[newp[:, 1]],
+                            rad[newp[:, 0]] + rad[newp[:, 1]]])
+    m = nc.mat
+    O = np.zeros((len(pairs), 3))
+    Fo, Mo = native_contacts(R, V, W, rad, pairs, reach, O, m.dt, m.hm_E, m.hm_G, m.p_A,
+                             m.p_mu, m.p_mur, m.rho * 4.0 / 3.0 * np.pi)
+    n_touch = int(np.count_nonzero(np.linalg.norm(R[pairs[:, 1]] - R[pairs[:, 0]], axis=1)
+                                   < reach))
+    assert n_touch > 50 and len(newp) > 0
+    for k, ref in (("F", Fo), ("M", Mo)):
+        got = _soa3(out[k], N)
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-4 * np.abs(ref).max(),
+                                   err_msg=k)
 
 
 def test_newton3_whole_specimen(ctx):
