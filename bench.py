@@ -202,7 +202,7 @@ def bench_native(args) -> dict:
                    "contact_rebuild_every": 20},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": None,
-                     "kernel": "k_nat_forces", "kernel_ms": kern_ms,
+                     "kernel": s.force_kernel(), "kernel_ms": kern_ms,
                      "alg_bytes_per_launch": alg_bytes},
     }
 

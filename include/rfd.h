@@ -303,23 +303,30 @@ int rfd_system_get_grid(rfd_system *sys, double origin[3], double *cell, uint32_
 int rfd_system_step_graph(rfd_system *sys, uint32_t nsteps);
 /* ---- launch-shape autotuner ----
  * Shapes: force-pass tile of 32, 64 (default) or 128 particles per workgroup,
- * contact-row search block of 64, 128 (default) or 256 threads.  Results do
- * not depend on the shapes.  rfd_system_autotune times each shape (median of
- * 5 trial force passes, which write only F, M and the spare state buffer; row
- * searches into the spare row buffers) and applies the fastest; the choice is
- * kept per (device architecture, floor(log2 N), batched or not) in a process
- * table, which use_cache = 1 consults first and rfd_tune_cache_save/load
- * write to and read from a text file.  F and M hold the forces of the current
- * configuration afterwards; the trajectory is unchanged. */
+ * with or without single evaluation (physical mode: a bond whose ends share a
+ * tile is evaluated once and loads both ends; used where the tile's half-bonds
+ * fit the workgroup's LDS), and contact-row search block of 64, 128 (default)
+ * or 256 threads.  Results do not depend on the shapes.  rfd_system_autotune
+ * times each shape (median of 5 trial force passes, which write only F, M and
+ * the spare state buffer; row searches into the spare row buffers) and
+ * applies the fastest; the choice is kept per (device architecture,
+ * floor(log2 N), batched or not) in a process table, which use_cache = 1
+ * consults first and rfd_tune_cache_save/load write to and read from a text
+ * file.  F and M hold the forces of the current configuration afterwards; the
+ * trajectory is unchanged. */
 typedef struct rfd_tune_result {
   int force_tile, nbr_block;    /* chosen shapes                              */
+  int single_eval;              /* 1: in-tile bonds evaluated once            */
   int cached;                   /* 1: taken from the table, nothing timed     */
   int bucket;                   /* floor(log2 N)                              */
-  double force_ms[3];           /* tiles 64, 32, 128                          */
+  double force_ms[4];           /* tiles 64, 32, 128 single evaluation; 64 every half */
   double nbr_ms[3];             /* blocks 128, 64, 256                        */
 } rfd_tune_result;
 int rfd_system_autotune(rfd_system *sys, int use_cache, rfd_tune_result *out);
-int rfd_system_set_tiles(rfd_system *sys, int force_tile, int nbr_block);
+int rfd_system_set_tiles(rfd_system *sys, int force_tile, int nbr_block, int single_eval);
+/* the shapes in use; single_eval reports whether the single-evaluation force
+ * pass runs (physical mode, FP32 path, tile halves within its LDS) */
+int rfd_system_get_tiles(rfd_system *sys, int *force_tile, int *nbr_block, int *single_eval);
 int rfd_tune_cache_save(const char *path);
 int rfd_tune_cache_load(const char *path);
 /* ---- quasi-static load steps: axial strain increment + FIRE relaxation ----

@@ -90,6 +90,7 @@ struct v3 {
 };
 __device__ __forceinline__ v3 operator+(v3 a, v3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 __device__ __forceinline__ v3 operator-(v3 a, v3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ v3 operator-(v3 a) { return {-a.x, -a.y, -a.z}; }
 __device__ __forceinline__ v3 operator*(v3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
 __device__ __forceinline__ float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ __forceinline__ v3 cross(v3 a, v3 b) {
@@ -186,10 +187,12 @@ struct BondOut {
   BSt st;       // new state, i's frame of the owner half
 };
 
-// evaluate bond (A, B) from its old state, seen from A; sg = +1 when A is the
-// owner end (the state is stored in the owner's frame), -1 otherwise
+// evaluate bond (A, B) from its old state, A the lower-index end (the state
+// is kept in its frame).  Both halves of a bond call this with the same
+// orientation and take their own end's loads (B's force is -Fi), so the two
+// ends, and any tiling of the pass, see the same bits.
 __device__ __forceinline__ void eval_bond(const Mat &mt, const EndData &A, const EndData &B,
-                                          const BR &c, float sg, const BSt &old, BondOut &o) {
+                                          const BR &c, const BSt &old, BondOut &o) {
   const double rx = B.P.x - A.P.x, ry = B.P.y - A.P.y, rz = B.P.z - A.P.z;
   const double r2 = rx * rx + ry * ry + rz * rz;
   o.Fi = o.Mbi = o.Mbj = o.Mci = o.Mcj = kZero;
@@ -209,14 +212,14 @@ __device__ __forceinline__ void eval_bond(const Mat &mt, const EndData &A, const
   const float L = r2f * invL;
   const v3 n = r * invL;
   const float ri = (float)A.P.r, rj = (float)B.P.r;
-  v3 O = xyz(old.a) * sg;
+  v3 O = xyz(old.a);
   bool contact_now = old.fl == kBroken;
   if (old.fl == kBond) {
     // bond direction one move back: both ends at R - V dt
     const v3 rp = r - (B.V - A.V) * mt.dt;
     const float pp = dot(rp, rp);
     const v3 np = pp > 0.f ? rp * __builtin_amdgcn_rsqf(pp) : n;
-    const v3 Mt0 = v3{old.mx, old.my, old.mz} * sg;
+    const v3 Mt0 = v3{old.mx, old.my, old.mz};
     const float il = c.il;
     const float A2 = kPiF * c.br * c.br, J = kPiF * c.br * c.br * c.br * c.br;
     // stretch without cancellation: eps = (L^2/L0^2 - 1) / (1 + L/L0), the
@@ -264,11 +267,10 @@ __device__ __forceinline__ void eval_bond(const Mat &mt, const EndData &A, const
       o.st.mx = o.st.my = o.st.mz = 0.f;
       contact_now = true;  // the freed pair meets the contact law this pass
     } else {
-      const v3 Os = O * sg, Ms = Mt * sg;
-      o.st.a = make_float4(Os.x, Os.y, Os.z, mn);
-      o.st.mx = Ms.x;
-      o.st.my = Ms.y;
-      o.st.mz = Ms.z;
+      o.st.a = make_float4(O.x, O.y, O.z, mn);
+      o.st.mx = Mt.x;
+      o.st.my = Mt.y;
+      o.st.mz = Mt.z;
       return;
     }
   }
@@ -285,10 +287,9 @@ __device__ __forceinline__ void eval_bond(const Mat &mt, const EndData &A, const
     } else {
       O = kZero;
     }
-    const v3 Os = O * sg;
-    o.st.a.x = Os.x;
-    o.st.a.y = Os.y;
-    o.st.a.z = Os.z;
+    o.st.a.x = O.x;
+    o.st.a.y = O.y;
+    o.st.a.z = O.z;
   }
 }
 
@@ -308,6 +309,7 @@ struct SysDev {
   const float *dtp;  // when set, the time step of the last move (FIRE) replaces mt.dt
   const int *done;   // when set and nonzero (relaxation finished), the pass does nothing
   int trial;         // trial pass: writes only F, M and st_out (nothing the next pass reads)
+  const uint32_t *hm;  // mirror half of each half
   uint32_t *nbreak;  // when set, counts the bonds this pass breaks
 };
 
@@ -368,16 +370,17 @@ k_nat_forces(SysDev s, Mat mt, uint32_t ntiles, uint32_t per) {
       const uint32_t i = p0 + lo;
       const BSt old = st_in[hr.b];
       BondOut o;
+      const bool own = i < hr.j;
       if (old.fl == kRemoved) {
-        o.Fi = o.Mbi = o.Mci = kZero;
+        o.Fi = o.Mbi = o.Mbj = o.Mci = o.Mcj = kZero;
         o.hasc = false;
       } else {
-        const bool own = i < hr.j;
-        EndData A;
-        A.P = sP[lo];
-        A.V = xyz(sV[lo]);
-        A.W = xyz(sW[lo]);
-        eval_bond(mt, A, load_end(s, hr.j), s.brc[hr.b], own ? 1.0f : -1.0f, old, o);
+        EndData me;
+        me.P = sP[lo];
+        me.V = xyz(sV[lo]);
+        me.W = xyz(sW[lo]);
+        const EndData other = load_end(s, hr.j);
+        eval_bond(mt, own ? me : other, own ? other : me, s.brc[hr.b], old, o);
         if (own) {
           if (o.st.fl == kRemoved) {
             // both buffers: no later pass may read a stale live flag
@@ -393,15 +396,24 @@ k_nat_forces(SysDev s, Mat mt, uint32_t ntiles, uint32_t per) {
         }
       }
       const uint32_t k = e - c0;
-      sF[0][k] = o.Fi.x;
-      sF[1][k] = o.Fi.y;
-      sF[2][k] = o.Fi.z;
-      sMb[0][k] = o.Mbi.x;
-      sMb[1][k] = o.Mbi.y;
-      sMb[2][k] = o.Mbi.z;
-      sMc[0][k] = o.Mci.x;
-      sMc[1][k] = o.Mci.y;
-      sMc[2][k] = o.Mci.z;
+      // this end's loads; the physical mode keeps bond and contact moments in
+      // one sum (the reference mode needs the contact part for its wipe rule)
+      const v3 Fm = own ? o.Fi : -o.Fi;
+      v3 Mb = own ? o.Mbi : o.Mbj;
+      v3 Mcx = own ? o.Mci : o.Mcj;
+      if (!mt.refmode) {
+        Mb = Mb + Mcx;
+        Mcx = kZero;
+      }
+      sF[0][k] = Fm.x;
+      sF[1][k] = Fm.y;
+      sF[2][k] = Fm.z;
+      sMb[0][k] = Mb.x;
+      sMb[1][k] = Mb.y;
+      sMb[2][k] = Mb.z;
+      sMc[0][k] = Mcx.x;
+      sMc[1][k] = Mcx.y;
+      sMc[2][k] = Mcx.z;
       sC[k] = o.hasc ? 1 : 0;
     }
     __syncthreads();
@@ -473,6 +485,281 @@ k_nat_forces(SysDev s, Mat mt, uint32_t ntiles, uint32_t per) {
   if (mt.refmode && anyc && !((double)dot(me.W, me.W) > 1e-18)) Mc = kZero;
   s.F[i] = f4(F);
   s.M[i] = f4(Mb + Mc);
+}
+
+// Half-bond capacity of one tile in the single-evaluation pass: the tile's
+// per-half loads stay in LDS for the whole pass (tiles with more halves run
+// the chunked pass above)
+template <int FP> struct OnceCap;
+template <> struct OnceCap<32> { static constexpr uint32_t v = 512; };
+template <> struct OnceCap<64> { static constexpr uint32_t v = 896; };
+template <> struct OnceCap<128> { static constexpr uint32_t v = 1664; };
+
+// Work record of the single-evaluation pass, one per evaluated half-bond of a
+// tile, built once per tile shape (k_work_*): a = slot e (11 bits) | row
+// particle lo << 11 (7 bits) | partner's local index << 18 (7 bits, in-tile
+// records); b = bond; c = the mirror half's slot (in-tile) or the partner id.
+struct WorkRec {
+  uint32_t a, b, c;
+};
+
+struct WorkDev {
+  const WorkRec *rec;
+  const uint32_t *ofs;   // ntiles + 1 record offsets
+  const uint32_t *n_in;  // in-tile records per tile (they come first)
+};
+
+// Force pass of the physical mode with every bond whose ends share a tile
+// evaluated once: its lower-index half computes the law with the partner from
+// LDS and writes both halves' loads; bonds leaving the tile are evaluated by
+// each of their halves in the tile.  The two kinds run as separate loops over
+// precomputed records, so no lane waits on a load another lane needs.
+// Per-particle sums in row order, then the contact rows: the same bits as
+// k_nat_forces.
+template <int FP, int FT>
+__global__ void __launch_bounds__(FT) __attribute__((amdgpu_waves_per_eu(5, 8)))
+k_nat_forces1(SysDev s, Mat mt, WorkDev w, uint32_t ntiles, uint32_t per) {
+  static_assert(FP < FT && FP <= 128, "tile shape");
+  constexpr uint32_t CAP = OnceCap<FP>::v;
+  __shared__ float sF[3][CAP], sM[3][CAP];
+  __shared__ uint32_t sOff[FP + 1];
+  __shared__ float sAcc[2][3][FP];
+  __shared__ P4 sP[FP];
+  __shared__ float4 sV[FP], sW[FP];
+  const uint32_t tile = xcd_tile(per);
+  if (tile >= ntiles) return;
+  if (s.done && *s.done) return;
+  if (s.dtp) mt.dt = *s.dtp;
+  BSt *const st_in = *s.curp ? s.bst1 : s.bst0;
+  BSt *const st_out = *s.curp ? s.bst0 : s.bst1;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t p0 = tile * FP;
+  const uint32_t np = min((uint32_t)FP, s.N - p0);
+  if (tid <= np) sOff[tid] = s.hoff[p0 + tid];
+  if (tid < np) {
+    sP[tid] = s.P[p0 + tid];
+    sV[tid] = s.V[p0 + tid];
+    sW[tid] = s.W[p0 + tid];
+  }
+  __syncthreads();
+  const uint32_t h0 = sOff[0];
+  const uint32_t w0 = w.ofs[tile], w1 = w.ofs[tile + 1], wx = w0 + w.n_in[tile];
+  // owner side of a bond: state out, flag copy, break count
+  auto commit = [&](uint32_t bnd, const BSt &old, const BondOut &o) {
+    if (o.st.fl == kRemoved) {
+      st_out[bnd].fl = kRemoved;
+      if (!s.trial) st_in[bnd].fl = kRemoved;
+    } else {
+      st_out[bnd] = o.st;
+    }
+    if (!s.trial && o.st.fl != old.fl) {
+      s.fl8[bnd] = (uint8_t)o.st.fl;
+      if (s.nbreak && o.st.fl == kBroken) atomicAdd(s.nbreak, 1u);
+    }
+  };
+  auto put = [&](uint32_t e, v3 F, v3 M) {
+    sF[0][e] = F.x;
+    sF[1][e] = F.y;
+    sF[2][e] = F.z;
+    sM[0][e] = M.x;
+    sM[1][e] = M.y;
+    sM[2][e] = M.z;
+  };
+  // 1. bonds inside the tile, once each (the lower end owns them)
+  for (uint32_t k = w0 + tid; k < wx; k += FT) {
+    const WorkRec r = w.rec[k];
+    const uint32_t e = r.a & 0x7FFu, lo = (r.a >> 11) & 0x7Fu, jl = (r.a >> 18) & 0x7Fu;
+    const BSt old = st_in[r.b];
+    BondOut o;
+    if (old.fl == kRemoved) {
+      o.Fi = o.Mbi = o.Mbj = o.Mci = o.Mcj = kZero;
+    } else {
+      EndData A, B;
+      A.P = sP[lo];
+      A.V = xyz(sV[lo]);
+      A.W = xyz(sW[lo]);
+      B.P = sP[jl];
+      B.V = xyz(sV[jl]);
+      B.W = xyz(sW[jl]);
+      eval_bond(mt, A, B, s.brc[r.b], old, o);
+      commit(r.b, old, o);
+    }
+    put(e, o.Fi, o.Mbi + o.Mci);
+    put(r.c, -o.Fi, o.Mbj + o.Mcj);
+  }
+  // 2. halves whose partner is outside the tile, each evaluated by its end
+  for (uint32_t k = wx + tid; k < w1; k += FT) {
+    const WorkRec r = w.rec[k];
+    const uint32_t e = r.a & 0x7FFu, lo = (r.a >> 11) & 0x7Fu, j = r.c;
+    const bool own = p0 + lo < j;
+    const BSt old = st_in[r.b];
+    BondOut o;
+    if (old.fl == kRemoved) {
+      o.Fi = o.Mbi = o.Mbj = o.Mci = o.Mcj = kZero;
+    } else {
+      EndData me;
+      me.P = sP[lo];
+      me.V = xyz(sV[lo]);
+      me.W = xyz(sW[lo]);
+      const EndData other = load_end(s, j);
+      eval_bond(mt, own ? me : other, own ? other : me, s.brc[r.b], old, o);
+      if (own) commit(r.b, old, o);
+    }
+    if (own)
+      put(e, o.Fi, o.Mbi + o.Mci);
+    else
+      put(e, -o.Fi, o.Mbj + o.Mcj);
+  }
+  __syncthreads();
+  // 3. per-particle sums in row order: task (group g, particle p)
+  for (uint32_t task = tid; task < 2 * np; task += FT) {
+    const uint32_t p = task % np, g = task / np;
+    const uint32_t a0 = sOff[p] - h0, a1 = sOff[p + 1] - h0;
+    float (*src)[CAP] = g == 0 ? sF : sM;
+    float x = 0.f, y = 0.f, z = 0.f;
+    for (uint32_t e = a0; e < a1; ++e) {
+      x = x + src[0][e];
+      y = y + src[1][e];
+      z = z + src[2][e];
+    }
+    sAcc[g][0][p] = x;
+    sAcc[g][1][p] = y;
+    sAcc[g][2][p] = z;
+  }
+  __syncthreads();
+  if (tid >= np) return;
+  // 4. never-bonded contact rows of this particle (per-end spring, in place)
+  v3 F = v3{sAcc[0][0][tid], sAcc[0][1][tid], sAcc[0][2][tid]};
+  const v3 Mb = v3{sAcc[1][0][tid], sAcc[1][1][tid], sAcc[1][2][tid]};
+  v3 Mc = kZero;
+  const uint32_t i = p0 + tid, N = s.N;
+  EndData me;
+  me.P = sP[tid];
+  me.V = xyz(sV[tid]);
+  me.W = xyz(sW[tid]);
+  const uint32_t nc = s.ccnt[i];
+  for (uint32_t k = 0; k < nc; ++k) {
+    const size_t sl = (size_t)k * N + i;
+    const uint32_t j = s.cj[sl];
+    const EndData other = load_end(s, j);
+    const double rx = other.P.x - me.P.x, ry = other.P.y - me.P.y, rz = other.P.z - me.P.z;
+    const double L64 = __builtin_sqrt(rx * rx + ry * ry + rz * rz);
+    const float ri = (float)me.P.r, rj = (float)other.P.r;
+    if ((float)L64 < ri + rj && L64 > 1e-5) {
+      const double inv = 1.0 / L64;
+      const v3 n = {(float)(rx * inv), (float)(ry * inv), (float)(rz * inv)};
+      v3 O = xyz(s.cO[sl]);
+      v3 Fi, Mi, Mj;
+      contact_law(mt, n, (float)L64, ri, rj, ri + rj, me.V, other.V, me.W, other.W, O, Fi, Mi,
+                  Mj);
+      F = F + Fi;
+      Mc = Mc + Mi;
+      if (!s.trial) s.cO[sl] = f4(O);
+    } else if (!s.trial) {
+      s.cO[sl] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  }
+  s.F[i] = f4(F);
+  s.M[i] = f4(Mb + Mc);
+}
+
+// work records of the single-evaluation pass: per tile, count the in-tile
+// bonds (once, at their lower end) and the halves leaving the tile
+template <int FP>
+__global__ void __launch_bounds__(256) k_work_count(const uint32_t *__restrict__ hoff,
+                                                    const HR *__restrict__ hr, uint32_t N,
+                                                    uint32_t ntiles, uint32_t *cnt,
+                                                    uint32_t *n_in) {
+  const uint32_t t = blockIdx.x;
+  if (t >= ntiles) return;
+  __shared__ uint32_t s_in, s_x;
+  if (threadIdx.x == 0) s_in = s_x = 0;
+  __syncthreads();
+  const uint32_t p0 = t * FP, np = min((uint32_t)FP, N - p0);
+  const uint32_t h0 = hoff[p0], h1 = hoff[p0 + np];
+  uint32_t a = 0, x = 0;
+  for (uint32_t h = h0 + threadIdx.x; h < h1; h += blockDim.x) {
+    // row particle of h
+    uint32_t lo = 0, hi = np;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (hoff[p0 + mid] <= h) lo = mid; else hi = mid;
+    }
+    const uint32_t jl = hr[h].j - p0;
+    if (jl >= np) ++x;
+    else if (lo < jl) ++a;
+  }
+  atomicAdd(&s_in, a);
+  atomicAdd(&s_x, x);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    cnt[t] = s_in + s_x;
+    n_in[t] = s_in;
+  }
+}
+
+// fills the records in row order (ordered wave-ballot compaction per class)
+template <int FP>
+__global__ void __launch_bounds__(256) k_work_fill(const uint32_t *__restrict__ hoff,
+                                                   const HR *__restrict__ hr,
+                                                   const uint32_t *__restrict__ hm, uint32_t N,
+                                                   uint32_t ntiles, const uint32_t *ofs,
+                                                   const uint32_t *n_in, WorkRec *rec) {
+  const uint32_t t = blockIdx.x;
+  if (t >= ntiles) return;
+  __shared__ uint32_t s_cnt[2][4];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const uint32_t p0 = t * FP, np = min((uint32_t)FP, N - p0);
+  const uint32_t h0 = hoff[p0], h1 = hoff[p0 + np];
+  uint32_t base_in = ofs[t], base_x = ofs[t] + n_in[t];
+  for (uint32_t b0 = h0; b0 < h1; b0 += 256) {
+    const uint32_t h = b0 + tid;
+    bool isin = false, isx = false;
+    uint32_t lo = 0, jl = 0;
+    HR r{0, 0};
+    if (h < h1) {
+      uint32_t hi = np;
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (hoff[p0 + mid] <= h) lo = mid; else hi = mid;
+      }
+      r = hr[h];
+      jl = r.j - p0;
+      isx = jl >= np;
+      isin = !isx && lo < jl;
+    }
+    const uint64_t bi = __ballot(isin), bx = __ballot(isx);
+    const uint64_t below = (1ull << lane) - 1ull;
+    if (lane == 0) {
+      s_cnt[0][wid] = (uint32_t)__popcll(bi);
+      s_cnt[1][wid] = (uint32_t)__popcll(bx);
+    }
+    __syncthreads();
+    uint32_t oi = 0, ox = 0, ti = 0, tx = 0;
+    for (uint32_t q = 0; q < 4; ++q) {
+      oi += q < wid ? s_cnt[0][q] : 0u;
+      ox += q < wid ? s_cnt[1][q] : 0u;
+      ti += s_cnt[0][q];
+      tx += s_cnt[1][q];
+    }
+    const uint32_t e = h - h0;
+    if (isin) {
+      WorkRec w;
+      w.a = e | (lo << 11) | (jl << 18);
+      w.b = r.b;
+      w.c = hm[h] - h0;
+      rec[base_in + oi + (uint32_t)__popcll(bi & below)] = w;
+    } else if (isx) {
+      WorkRec w;
+      w.a = e | (lo << 11);
+      w.b = r.b;
+      w.c = r.j;
+      rec[base_x + ox + (uint32_t)__popcll(bx & below)] = w;
+    }
+    base_in += ti;
+    base_x += tx;
+    __syncthreads();
+  }
 }
 
 // flips the state-buffer index after a force pass that ran
@@ -1670,6 +1957,12 @@ struct rfd_system {
   // launch shapes (autotuned or set): force tile particles, neighbour block
   int force_fp = kFP;
   int nbr_block = 128;
+  uint32_t tile_halves[3] = {0, 0, 0};  // most half-bonds in a tile of 32, 64, 128
+  bool once_off = false;  // force the chunked pass (comparison runs)
+  // work records of the single-evaluation pass, for tiles of work_fp particles
+  WorkRec *wrec = nullptr;
+  uint32_t *wofs = nullptr, *wnin = nullptr, *wcnt = nullptr, *wcur = nullptr;
+  int work_fp = 0;
   // batched specimens: S independent specimens in contiguous index ranges,
   // each with its own loading control and history
   uint32_t S = 0;
@@ -1739,6 +2032,7 @@ SysDev make_dev(rfd_system *s) {
   d.M = s->M;
   d.hoff = s->hoff;
   d.hr = s->hr;
+  d.hm = s->hm;
   d.brc = s->brc;
   d.bst0 = s->bst[0];
   d.bst1 = s->bst[1];
@@ -1895,11 +2189,60 @@ int forces64(rfd_system *s) {
 }
 
 // one launch over ceil(N / FP) tiles, spread over the XCD runs
+// (the single-evaluation pass in the physical mode when every tile's halves
+// fit its LDS, else the chunked pass)
 template <int FP, int FT>
 void launch_forces(rfd_system *s, const SysDev &d, const Mat &m) {
   const uint32_t nb = (s->N + FP - 1) / FP;
   const uint32_t per = (nb + kXcd - 1) / kXcd;
-  k_nat_forces<FP, FT><<<per * kXcd, FT, 0, s->ctx->stream>>>(d, m, nb, per);
+  if (!m.refmode && !s->once_off && s->work_fp == FP) {
+    WorkDev w{s->wrec, s->wofs, s->wnin};
+    k_nat_forces1<FP, FT><<<per * kXcd, FT, 0, s->ctx->stream>>>(d, m, w, nb, per);
+  } else {
+    k_nat_forces<FP, FT><<<per * kXcd, FT, 0, s->ctx->stream>>>(d, m, nb, per);
+  }
+}
+
+// work records for tiles of FP particles (none when a tile's halves exceed the
+// single-evaluation pass's LDS); allocates and synchronises, so it runs at
+// set-up, never inside a captured step
+template <int FP>
+int build_work_fp(rfd_system *s) {
+  rfd_ctx *ctx = s->ctx;
+  hipStream_t q = ctx->stream;
+  const uint32_t nt = (s->N + FP - 1) / FP;
+  int st = dalloc(ctx, &s->wcnt, nt);
+  if (!st) st = dalloc(ctx, &s->wnin, nt);
+  if (!st) st = dalloc(ctx, &s->wofs, (size_t)nt + 1);
+  if (!st) st = dalloc(ctx, &s->wcur, nt);
+  if (st) return st;
+  k_work_count<FP><<<nt, 256, 0, q>>>(s->hoff, s->hr, s->N, nt, s->wcnt, s->wnin);
+  RFD_HIP(ctx, hipGetLastError());
+  st = scan_u32(s, s->wcnt, nt, s->wofs, s->wcur);
+  if (st) return st;
+  uint32_t total = 0;
+  RFD_HIP(ctx, hipMemcpyAsync(&total, s->wofs + nt, sizeof(uint32_t), hipMemcpyDeviceToHost, q));
+  RFD_HIP(ctx, hipStreamSynchronize(q));
+  st = dalloc(ctx, &s->wrec, total);
+  if (st) return st;
+  k_work_fill<FP><<<nt, 256, 0, q>>>(s->hoff, s->hr, s->hm, s->N, nt, s->wofs, s->wnin,
+                                      s->wrec);
+  RFD_HIP(ctx, hipGetLastError());
+  s->work_fp = FP;
+  return RFD_OK;
+}
+
+int ensure_work(rfd_system *s) {
+  const int fp = s->force_fp;
+  if (!s->have_bonds || s->p.refmode || s->fp64 || s->work_fp == fp) return RFD_OK;
+  const int q = fp == 32 ? 0 : (fp == 64 ? 1 : 2);
+  s->work_fp = 0;
+  s->gen++;
+  const uint32_t cap = fp == 32 ? OnceCap<32>::v : (fp == 64 ? OnceCap<64>::v : OnceCap<128>::v);
+  if (s->tile_halves[q] > cap) return RFD_OK;  // the chunked pass serves this shape
+  if (fp == 32) return build_work_fp<32>(s);
+  if (fp == 128) return build_work_fp<128>(s);
+  return build_work_fp<64>(s);
 }
 
 int forces(rfd_system *s, const float *dtp = nullptr, bool trial = false,
@@ -2138,8 +2481,11 @@ int download_vec3(rfd_ctx *ctx, float *soa, const float4 *src, uint32_t N) {
 }
 
 // launch shapes chosen by the autotuner, per (arch, log2 N bucket, batched)
-std::map<std::string, std::pair<int, int>> &tune_table() {
-  static std::map<std::string, std::pair<int, int>> t;
+struct TuneChoice {
+  int first, second, third;  // force tile, row-search block, single evaluation
+};
+std::map<std::string, TuneChoice> &tune_table() {
+  static std::map<std::string, TuneChoice> t;
   return t;
 }
 std::mutex &tune_mu() {
@@ -2272,6 +2618,11 @@ int rfd_system_destroy(rfd_system *s) {
     dfree(s->du[q]);
     dfree(s->dth[q]);
   }
+  dfree(s->wrec);
+  dfree(s->wofs);
+  dfree(s->wnin);
+  dfree(s->wcnt);
+  dfree(s->wcur);
   dfree(s->ctls);
   dfree(s->hist_b);
   dfree(s->blk_spec);
@@ -2480,6 +2831,13 @@ int rfd_system_set_bonds(rfd_system *s, uint32_t Nb, const uint32_t *bi, const u
   s->hbj.assign(bj, bj + Nb);
   s->Nb = Nb;
   s->H = H;
+  for (int q = 0; q < 3; ++q) {
+    const uint32_t fp = 32u << q;
+    uint32_t most = 0;
+    for (uint32_t p = 0; p < N; p += fp)
+      most = std::max(most, hoff[std::min(p + fp, N)] - hoff[p]);
+    s->tile_halves[q] = most;
+  }
   int st = dalloc(ctx, &s->hr, H);
   if (!st) st = dalloc(ctx, &s->hm, H);
   if (!st) st = dalloc(ctx, &s->brc, Nb);
@@ -2519,6 +2877,11 @@ int rfd_system_set_bonds(rfd_system *s, uint32_t Nb, const uint32_t *bi, const u
   }
   s->have_bonds = true;
   s->gen++;
+  s->work_fp = 0;
+  {
+    int e = ensure_work(s);
+    if (e) return e;
+  }
   s->rows_valid = false;
   RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return RFD_OK;
@@ -2937,7 +3300,7 @@ int rfd_system_step_graph(rfd_system *s, uint32_t nsteps) {
 }
 
 // -------------------------------------------------------------- autotuner
-int rfd_system_set_tiles(rfd_system *s, int force_tile, int nbr_block) {
+int rfd_system_set_tiles(rfd_system *s, int force_tile, int nbr_block, int single_eval) {
   if (!valid(s)) return RFD_E_ARG;
   rfd_ctx *ctx = s->ctx;
   RFD_CHECK_ARG(ctx, force_tile == 32 || force_tile == 64 || force_tile == 128,
@@ -2946,7 +3309,17 @@ int rfd_system_set_tiles(rfd_system *s, int force_tile, int nbr_block) {
                 "rfd_system_set_tiles: neighbour block must be 64, 128 or 256");
   s->force_fp = force_tile;
   s->nbr_block = nbr_block;
+  s->once_off = !single_eval;
   s->gen++;
+  return ensure_work(s);
+}
+
+int rfd_system_get_tiles(rfd_system *s, int *force_tile, int *nbr_block, int *single_eval) {
+  if (!valid(s)) return RFD_E_ARG;
+  if (force_tile) *force_tile = s->force_fp;
+  if (nbr_block) *nbr_block = s->nbr_block;
+  if (single_eval)
+    *single_eval = !s->p.refmode && !s->fp64 && !s->once_off && s->work_fp == s->force_fp;
   return RFD_OK;
 }
 
@@ -2968,8 +3341,9 @@ int rfd_system_autotune(rfd_system *s, int use_cache, rfd_tune_result *out) {
     if (it != tune_table().end()) {
       out->force_tile = it->second.first;
       out->nbr_block = it->second.second;
+      out->single_eval = it->second.third;
       out->cached = 1;
-      return rfd_system_set_tiles(s, out->force_tile, out->nbr_block);
+      return rfd_system_set_tiles(s, out->force_tile, out->nbr_block, out->single_eval);
     }
   }
   hipStream_t q = ctx->stream;
@@ -2999,16 +3373,24 @@ int rfd_system_autotune(rfd_system *s, int use_cache, rfd_tune_result *out) {
     *ms = t[kReps / 2];
     return RFD_OK;
   };
-  static const int kTiles[3] = {64, 32, 128};
+  // force variants: tiles of 64, 32, 128 with single evaluation of in-tile
+  // bonds (where the tile's halves fit), and 64 with every half evaluated
+  static const int kTiles[4] = {64, 32, 128, 64};
+  static const bool kOnceOff[4] = {false, false, false, true};
   static const int kBlocks[3] = {128, 64, 256};
-  const int fp0 = s->force_fp, nb0 = s->nbr_block;
+  const int fp0 = s->force_fp;
+  const bool off0 = s->once_off;
   int st = RFD_OK, best_t = 0, best_b = 0;
-  for (int v = 0; v < 3 && !st; ++v) {
+  for (int v = 0; v < 4 && !st; ++v) {
     s->force_fp = kTiles[v];
-    st = timed([&]() { return forces(s, nullptr, true, false); }, &out->force_ms[v]);
+    s->once_off = kOnceOff[v];
+    st = ensure_work(s);
+    if (!st) st = timed([&]() { return forces(s, nullptr, true, false); }, &out->force_ms[v]);
     if (!st && out->force_ms[v] < out->force_ms[best_t]) best_t = v;
   }
   s->force_fp = fp0;
+  s->once_off = off0;
+  if (!st) st = ensure_work(s);
   if (!st) st = bin_particles(s);
   // the row search writes into the spare row buffers, which the next rebuild
   // overwrites whole
@@ -3034,12 +3416,13 @@ int rfd_system_autotune(rfd_system *s, int use_cache, rfd_tune_result *out) {
   s->timing = timing;
   if (st) return st > 0 ? rfd_hip_status(ctx, (hipError_t)st, "rfd_system_autotune") : st;
   out->force_tile = kTiles[best_t];
+  out->single_eval = kOnceOff[best_t] ? 0 : 1;
   out->nbr_block = kBlocks[best_b];
   {
     std::lock_guard<std::mutex> lk(tune_mu());
-    tune_table()[key] = {out->force_tile, out->nbr_block};
+    tune_table()[key] = {out->force_tile, out->nbr_block, out->single_eval};
   }
-  return rfd_system_set_tiles(s, out->force_tile, out->nbr_block);
+  return rfd_system_set_tiles(s, out->force_tile, out->nbr_block, out->single_eval);
 }
 
 int rfd_tune_cache_save(const char *path) {
@@ -3048,7 +3431,8 @@ int rfd_tune_cache_save(const char *path) {
   if (!f) return RFD_E_ARG;
   std::lock_guard<std::mutex> lk(tune_mu());
   for (const auto &kv : tune_table())
-    fprintf(f, "%s %d %d\n", kv.first.c_str(), kv.second.first, kv.second.second);
+    fprintf(f, "%s %d %d %d\n", kv.first.c_str(), kv.second.first, kv.second.second,
+            kv.second.third);
   fclose(f);
   return RFD_OK;
 }
@@ -3058,11 +3442,12 @@ int rfd_tune_cache_load(const char *path) {
   FILE *f = fopen(path, "r");
   if (!f) return RFD_E_ARG;
   char k[160];
-  int a = 0, b = 0;
+  int a = 0, b = 0, c = 0;
   std::lock_guard<std::mutex> lk(tune_mu());
-  while (fscanf(f, "%159s %d %d", k, &a, &b) == 3)
-    if ((a == 32 || a == 64 || a == 128) && (b == 64 || b == 128 || b == 256))
-      tune_table()[k] = {a, b};
+  while (fscanf(f, "%159s %d %d %d", k, &a, &b, &c) == 4)
+    if ((a == 32 || a == 64 || a == 128) && (b == 64 || b == 128 || b == 256) &&
+        (c == 0 || c == 1))
+      tune_table()[k] = {a, b, c};
   fclose(f);
   return RFD_OK;
 }

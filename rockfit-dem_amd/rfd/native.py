@@ -65,13 +65,17 @@ class FireResult(C.Structure):
 
 
 class TuneResult(C.Structure):
-    _fields_ = [("force_tile", C.c_int), ("nbr_block", C.c_int), ("cached", C.c_int),
-                ("bucket", C.c_int), ("force_ms", C.c_double * 3), ("nbr_ms", C.c_double * 3)]
+    _fields_ = [("force_tile", C.c_int), ("nbr_block", C.c_int), ("single_eval", C.c_int),
+                ("cached", C.c_int), ("bucket", C.c_int), ("force_ms", C.c_double * 4),
+                ("nbr_ms", C.c_double * 3)]
+
+    FORCE_VARIANTS = ("64", "32", "128", "64 every half")
 
     def as_dict(self) -> dict:
         return {"force_tile": self.force_tile, "nbr_block": self.nbr_block,
-                "cached": bool(self.cached), "bucket": self.bucket,
-                "force_ms": {t: self.force_ms[k] for k, t in enumerate((64, 32, 128))},
+                "single_eval": bool(self.single_eval), "cached": bool(self.cached),
+                "bucket": self.bucket,
+                "force_ms": {t: self.force_ms[k] for k, t in enumerate(self.FORCE_VARIANTS)},
                 "nbr_ms": {b: self.nbr_ms[k] for k, b in enumerate((128, 64, 256))}}
 
 
@@ -112,7 +116,8 @@ NATIVE_SIGNATURES = {
     "rfd_system_get_grid": [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double), _u32p],
     "rfd_system_step_graph": [_vp, _u32],
     "rfd_system_autotune": [_vp, C.c_int, C.POINTER(TuneResult)],
-    "rfd_system_set_tiles": [_vp, C.c_int, C.c_int],
+    "rfd_system_set_tiles": [_vp, C.c_int, C.c_int, C.c_int],
+    "rfd_system_get_tiles": [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "rfd_tune_cache_save": [C.c_char_p],
     "rfd_tune_cache_load": [C.c_char_p],
 }
@@ -403,8 +408,20 @@ class NativeSystem:
     def step_graph(self, n: int = 1):
         self._call("rfd_system_step_graph", self.h, n)
 
-    def set_tiles(self, force_tile: int, nbr_block: int):
-        self._call("rfd_system_set_tiles", self.h, force_tile, nbr_block)
+    def set_tiles(self, force_tile: int, nbr_block: int, single_eval: bool = True):
+        self._call("rfd_system_set_tiles", self.h, force_tile, nbr_block, int(single_eval))
+
+    def tiles(self) -> tuple[int, int, bool]:
+        """(force tile, row-search block, single-evaluation pass running)."""
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        self._call("rfd_system_get_tiles", self.h, C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, bool(c.value)
+
+    def force_kernel(self) -> str:
+        """Name of the force-pass kernel the next step launches."""
+        if self.params.fp64:
+            return "k_f64_forces"
+        return "k_nat_forces1" if self.tiles()[2] else "k_nat_forces"
 
     def autotune(self, use_cache: bool = True) -> TuneResult:
         r = TuneResult()

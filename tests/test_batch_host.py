@@ -35,15 +35,15 @@ def test_batch_geometry_follows_the_drawn_counts():
 
 def test_tune_table_round_trip(tmp_path):
     good = tmp_path / "tune.txt"
-    good.write_text("gfx950/n21/single 64 128\n"
-                    "gfx950/n17/batch 32 256\n"
-                    "gfx950/n9/single 48 128\n")  # 48 is not a tile shape
+    good.write_text("gfx950/n21/single 64 128 1\n"
+                    "gfx950/n17/batch 32 256 0\n"
+                    "gfx950/n9/single 48 128 1\n")  # 48 is not a tile shape
     native.tune_cache_load(good)
     out = tmp_path / "out.txt"
     native.tune_cache_save(out)
     lines = set(out.read_text().split("\n")) - {""}
-    assert "gfx950/n21/single 64 128" in lines
-    assert "gfx950/n17/batch 32 256" in lines
+    assert "gfx950/n21/single 64 128 1" in lines
+    assert "gfx950/n17/batch 32 256 0" in lines
     assert not any(l.startswith("gfx950/n9/") for l in lines)
     native.tune_cache_load(out)
     out2 = tmp_path / "out2.txt"

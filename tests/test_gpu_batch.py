@@ -132,11 +132,11 @@ def test_tile_shapes_and_autotune_leave_results_unchanged(ctx):
 
     ref = build()
     ref.step(75)
-    for fp, nb in ((32, 64), (128, 256), (64, 128)):
+    for fp, nb, once in ((32, 64, True), (128, 256, True), (64, 128, False), (64, 128, True)):
         s = build()
-        s.set_tiles(fp, nb)
+        s.set_tiles(fp, nb, once)
         s.step(75)
-        _same_state(ref, s, label=f"tile {fp} block {nb}")
+        _same_state(ref, s, label=f"tile {fp} block {nb} single {once}")
         _same_loading(ref.loading(), s.loading(), f"tile {fp} block {nb}")
         s.close()
     # autotune mid-run: trial passes must not move the trajectory
@@ -149,13 +149,42 @@ def test_tile_shapes_and_autotune_leave_results_unchanged(ctx):
     assert not r.cached
     assert all(v > 0 for v in d["force_ms"].values())
     assert all(v > 0 for v in d["nbr_ms"].values())
-    assert d["force_ms"][r.force_tile] == min(d["force_ms"].values())
+    key = str(r.force_tile) if r.single_eval else "64 every half"
+    assert d["force_ms"][key] == min(d["force_ms"].values())
     t.step(45)
     _same_state(ref, t, label="after autotune")
     _same_loading(ref.loading(), t.loading(), "after autotune")
     # the same size bucket on the same device comes from the table
     u = build()
     r2 = u.autotune(use_cache=True)
-    assert r2.cached and (r2.force_tile, r2.nbr_block) == (r.force_tile, r.nbr_block)
+    assert r2.cached and (r2.force_tile, r2.nbr_block, r2.single_eval) == \
+        (r.force_tile, r.nbr_block, r.single_eval)
     for s in (ref, t, u):
         s.close()
+
+
+def test_single_evaluation_matches_every_half_with_breaking_bonds(ctx):
+    """Weak bonds under large random velocities break during the run, so the
+    broken-bond contact law and the break bookkeeping run in both passes:
+    the single-evaluation pass, the every-half pass and every tile shape give
+    the same trajectory bit for bit."""
+    def build():
+        nc = native_cases.uniaxial_cylinder(ctx, 8.0, 12.0, seed=6, strength=(2e6, 4e6))
+        g = np.random.default_rng(2)
+        nc.sys.set_velocities((g.standard_normal(3 * nc.sys.N) * 5e-2).astype(np.float32),
+                              (g.standard_normal(3 * nc.sys.N) * 5e-2).astype(np.float32))
+        return nc.sys
+
+    ref = build()
+    ref.set_tiles(64, 128, False)
+    ref.step(60)
+    counts = ref.bond_counts()
+    assert counts[1] > 0, counts  # bonds broke
+    for fp, once in ((64, True), (32, True), (128, True), (32, False)):
+        s = build()
+        s.set_tiles(fp, 128, once)
+        s.step(60)
+        _same_state(ref, s, label=f"tile {fp} single {once}")
+        assert s.bond_counts() == counts
+        s.close()
+    ref.close()
