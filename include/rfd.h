@@ -305,22 +305,23 @@ int rfd_system_step_graph(rfd_system *sys, uint32_t nsteps);
  * Shapes: force-pass tile of 32, 64 (default) or 128 particles per workgroup,
  * with or without single evaluation (physical mode: a bond whose ends share a
  * tile is evaluated once and loads both ends; used where the tile's half-bonds
- * fit the workgroup's LDS), and contact-row search block of 64, 128 (default)
- * or 256 threads.  Results do not depend on the shapes.  rfd_system_autotune
- * times each shape (median of 5 trial force passes, which write only F, M and
- * the spare state buffer; row searches into the spare row buffers) and
- * applies the fastest; the choice is kept per (device architecture,
- * floor(log2 N), batched or not) in a process table, which use_cache = 1
- * consults first and rfd_tune_cache_save/load write to and read from a text
- * file.  F and M hold the forces of the current configuration afterwards; the
- * trajectory is unchanged. */
+ * fit the workgroup's LDS); contact-row search by 2 x 2 x 2 cell blocks with
+ * the surrounding cells staged in LDS (0, default), or one thread per particle
+ * in blocks of 64, 128 or 256.  Results do not depend on the shapes.
+ * rfd_system_autotune times each shape (median of 5 trial force passes, which
+ * write only F, M and the spare state buffer; row searches into the spare row
+ * buffers) and applies the fastest; the choice is kept per (device
+ * architecture, floor(log2 N), batched or not) in a process table, which
+ * use_cache = 1 consults first and rfd_tune_cache_save/load write to and read
+ * from a text file.  F and M hold the forces of the current configuration
+ * afterwards; the trajectory is unchanged. */
 typedef struct rfd_tune_result {
   int force_tile, nbr_block;    /* chosen shapes                              */
   int single_eval;              /* 1: in-tile bonds evaluated once            */
   int cached;                   /* 1: taken from the table, nothing timed     */
   int bucket;                   /* floor(log2 N)                              */
   double force_ms[4];           /* tiles 64, 32, 128 single evaluation; 64 every half */
-  double nbr_ms[3];             /* blocks 128, 64, 256                        */
+  double nbr_ms[4];             /* thread blocks 128, 64, 256; cell blocks (0) */
 } rfd_tune_result;
 int rfd_system_autotune(rfd_system *sys, int use_cache, rfd_tune_result *out);
 int rfd_system_set_tiles(rfd_system *sys, int force_tile, int nbr_block, int single_eval);

@@ -1142,24 +1142,22 @@ __global__ void k_nat_scatter(const uint32_t *__restrict__ CI, uint32_t N,
     tmp[atomicAdd(&cursor[CI[i]], 1u)] = i;
 }
 
-// stable order inside each cell: rank members by id (one wave per cell)
-// also writes the members' positions relative to the grid origin (FP32) and
-// radii in cell order, for the contact-row search
+// stable order inside each cell: each particle counts the members of its cell
+// with a smaller id (one thread per particle) and writes itself at that rank,
+// with its position relative to the grid origin (FP32) and radius in cell
+// order, for the contact-row search
 __global__ void k_nat_rank(const uint32_t *__restrict__ tmp, const uint32_t *__restrict__ start,
-                           uint32_t C, uint32_t *__restrict__ sorted, const P4 *__restrict__ P,
-                           GridDev g, float4 *__restrict__ cpos) {
-  const uint32_t lane = threadIdx.x & 63, waves = blockDim.x >> 6;
-  for (uint32_t c = blockIdx.x * waves + (threadIdx.x >> 6); c < C; c += gridDim.x * waves) {
-    const uint32_t s0 = start[c], n = start[c + 1] - s0;
-    for (uint32_t e = lane; e < n; e += 64) {
-      const uint32_t id = tmp[s0 + e];
-      uint32_t rank = 0;
-      for (uint32_t t = 0; t < n; ++t) rank += (tmp[s0 + t] < id) ? 1u : 0u;
-      sorted[s0 + rank] = id;
-      const P4 R = P[id];
-      cpos[s0 + rank] = make_float4((float)(R.x - g.ox), (float)(R.y - g.oy),
-                                    (float)(R.z - g.oz), (float)R.r);
-    }
+                           const uint32_t *__restrict__ CI, uint32_t N,
+                           uint32_t *__restrict__ sorted, const P4 *__restrict__ P, GridDev g,
+                           float4 *__restrict__ cpos) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
+    const uint32_t c = CI[i], s0 = start[c], n = start[c + 1] - s0;
+    uint32_t rank = 0;
+    for (uint32_t t = 0; t < n; ++t) rank += (tmp[s0 + t] < i) ? 1u : 0u;
+    sorted[s0 + rank] = i;
+    const P4 R = P[i];
+    cpos[s0 + rank] = make_float4((float)(R.x - g.ox), (float)(R.y - g.oy), (float)(R.z - g.oz),
+                                  (float)R.r);
   }
 }
 
@@ -1283,6 +1281,222 @@ __global__ void __launch_bounds__(NB) k_nat_neighbours(NbrDev s, uint32_t per) {
   }
   if (MODE == 0) s.count[i] = nfound;
   if (MODE == 2) s.ccnt_new[i] = nfound < s.Kc ? nfound : s.Kc;
+}
+
+// contact-row entry for candidate j of particle i that passed the distance
+// and bond tests: spring carried over from the previous row by partner id
+__device__ __forceinline__ void row_entry(const NbrDev &s, uint32_t i, uint32_t j,
+                                          uint32_t &nfound) {
+  const uint32_t N = s.N;
+  if (nfound < s.Kc) {
+    const size_t sl = (size_t)nfound * N + i;
+    s.cj_new[sl] = j;
+    float4 O = make_float4(0.f, 0.f, 0.f, 0.f);
+    double4 O64 = make_double4(0.0, 0.0, 0.0, 0.0);
+    const uint32_t no = s.ccnt_old ? s.ccnt_old[i] : 0;
+    for (uint32_t k = 0; k < no; ++k) {
+      const size_t so = (size_t)k * N + i;
+      if (s.cj_old[so] == j) {
+        O = s.cO_old[so];
+        if (s.cO64_new) O64 = s.cO64_old[so];
+        break;
+      }
+    }
+    s.cO_new[sl] = O;
+    if (s.cO64_new) s.cO64_new[sl] = O64;
+  } else {
+    atomicOr(s.flags, RFD_FLAG_ROW_FULL);
+  }
+  ++nfound;
+}
+
+// Contact-row rebuild by cell blocks: a workgroup takes the particles of a
+// 2 x 2 x 2 block of cells and copies the candidates of the surrounding
+// 4 x 4 x 4 cells into LDS, one contiguous range per x-row of cells, in box
+// order (z, y, x, then members by id).  Every lane then scans the whole box in
+// that order: the candidates of its 27 neighbour cells come in the order the
+// per-particle search visits them, and every other candidate lies more than a
+// cell edge (> ri + rj + skin) away and fails the same FP32 test, so the rows
+// are the same as k_nat_neighbours<2>'s.  The scan is uniform across lanes
+// (broadcast LDS reads).  A block whose candidates exceed the LDS copy reads
+// them from global memory in the same order.
+constexpr int kRowCB = 2;                       // home block edge (cells)
+constexpr int kRowHB = kRowCB + 2;              // halo block edge
+constexpr uint32_t kRowHC = kRowHB * kRowHB * kRowHB;
+constexpr uint32_t kRowCand = 512;              // candidates held in LDS
+constexpr uint32_t kRowStage = 16;              // bonded partners held per particle
+constexpr uint32_t kRowFound = 32;              // in-reach candidates listed per particle
+
+__global__ void __launch_bounds__(64) k_nat_rows_cells(NbrDev s, uint32_t nblocks, uint32_t per,
+                                                       uint32_t nbx, uint32_t nby) {
+  __shared__ float4 sC[kRowCand];
+  __shared__ uint32_t sJ[kRowCand];
+  __shared__ uint32_t sS[kRowHC + 1], sG[kRowHC];
+  __shared__ uint16_t sF[kRowFound][64];  // box index of each in-reach candidate
+  __shared__ uint32_t sHome[kRowCB * kRowCB + 1], sHomeG[kRowCB * kRowCB];
+  const uint32_t b = xcd_tile(per);
+  if (b >= nblocks) return;
+  const GridDev &g = s.g;
+  const uint32_t tid = threadIdx.x;
+  const int x0 = (int)(b % nbx) * kRowCB, y0 = (int)((b / nbx) % nby) * kRowCB,
+            z0 = (int)(b / (nbx * nby)) * kRowCB;
+  const int nx = (int)g.nx, ny = (int)g.ny, nz = (int)g.nz;
+  auto gcell = [&](int x, int y, int z) {
+    return (uint32_t)x + (uint32_t)y * g.nx + (uint32_t)z * g.nx * g.ny;
+  };
+  // home particles: one contiguous range per home x-row (y, z), cells x0..x1
+  const int x1 = min(x0 + kRowCB - 1, nx - 1);
+  {
+    uint32_t hc = 0;
+    if (tid < (uint32_t)(kRowCB * kRowCB)) {
+      const int y = y0 + (int)tid % kRowCB, z = z0 + (int)tid / kRowCB;
+      uint32_t a = 0;
+      if (y < ny && z < nz) {
+        a = s.start[gcell(x0, y, z)];
+        hc = s.start[gcell(x1, y, z) + 1] - a;
+      }
+      sHomeG[tid] = a;
+    }
+    uint32_t incl = hc;
+#pragma unroll
+    for (int o = 1; o < kRowCB * kRowCB; o <<= 1) {
+      const uint32_t v = __shfl_up(incl, o, 64);
+      if ((int)tid >= o) incl += v;
+    }
+    if (tid < (uint32_t)(kRowCB * kRowCB)) sHome[tid] = incl - hc;
+    if (tid == kRowCB * kRowCB - 1) sHome[kRowCB * kRowCB] = incl;
+  }
+  __syncthreads();
+  const uint32_t nhome = sHome[kRowCB * kRowCB];
+  if (nhome == 0) return;
+  // halo cells [x0-1, x0+kRowCB] x ... clipped to the grid, box order (x fastest)
+  const int hx0 = max(x0 - 1, 0), hy0 = max(y0 - 1, 0), hz0 = max(z0 - 1, 0);
+  const int hx1 = min(x0 + kRowCB, nx - 1), hy1 = min(y0 + kRowCB, ny - 1),
+            hz1 = min(z0 + kRowCB, nz - 1);
+  const int hnx = hx1 - hx0 + 1, hny = hy1 - hy0 + 1, hnz = hz1 - hz0 + 1;
+  const uint32_t nh = (uint32_t)(hnx * hny * hnz);
+  uint32_t cnt = 0;
+  if (tid < nh) {
+    const int qx = (int)tid % hnx, qy = ((int)tid / hnx) % hny, qz = (int)tid / (hnx * hny);
+    const uint32_t gc = gcell(hx0 + qx, hy0 + qy, hz0 + qz);
+    const uint32_t a = s.start[gc];
+    sG[tid] = a;
+    cnt = s.start[gc + 1] - a;
+  }
+  // exclusive scan of the counts over the (at most 64) halo cells: one wave
+  uint32_t incl = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t v = __shfl_up(incl, o, 64);
+    if ((int)tid >= o) incl += v;
+  }
+  if (tid < nh) sS[tid] = incl - cnt;
+  if (tid == nh - 1) sS[nh] = incl;
+  __syncthreads();
+  const uint32_t total = sS[nh];
+  const bool staged = total <= kRowCand;
+  if (staged) {
+    for (uint32_t e = tid; e < total; e += 64) {
+      uint32_t lo = 0, hi = nh;
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (sS[mid] <= e) lo = mid; else hi = mid;
+      }
+      const uint32_t src = sG[lo] + (e - sS[lo]);
+      sC[e] = s.cpos[src];
+      sJ[e] = s.sorted[src];
+    }
+  }
+  __syncthreads();
+  const float exf = (float)s.extra;
+  for (uint32_t k0 = 0; k0 < nhome; k0 += 64) {
+    const uint32_t k = k0 + tid;
+    const bool live = k < nhome;
+    uint32_t i = 0;
+    float4 me = make_float4(0.f, 0.f, 0.f, 0.f);
+    bool fits = true;
+    uint32_t jb[kRowStage];  // bonded partners (kSentinel: none or removed)
+#pragma unroll
+    for (uint32_t t = 0; t < kRowStage; ++t) jb[t] = kSentinel;
+    if (live) {
+      uint32_t r = 0;
+      while (r + 1 < kRowCB * kRowCB && sHome[r + 1] <= k) ++r;
+      const uint32_t eg = sHomeG[r] + (k - sHome[r]);
+      i = s.sorted[eg];
+      me = s.cpos[eg];
+      const uint32_t r0 = s.hoff[i], nb = s.hoff[i + 1] - r0;
+      fits = nb <= kRowStage;
+      if (fits) {
+        HR hh[kRowStage];
+#pragma unroll
+        for (uint32_t t = 0; t < kRowStage; ++t)
+          if (t < nb) hh[t] = s.hr[r0 + t];
+#pragma unroll
+        for (uint32_t t = 0; t < kRowStage; ++t)
+          if (t < nb && s.fl8[hh[t].b] != kRemoved) jb[t] = hh[t].j;
+      }
+    }
+    // in-reach test of one candidate (FP32, as the per-particle search)
+    auto reach = [&](const float4 q4) {
+      const float ddx = q4.x - me.x, ddy = q4.y - me.y, ddz = q4.z - me.z;
+      const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+      const float lim = me.w + q4.w + exf;
+      return d2 >= 1e-10f && d2 < lim * lim;
+    };
+    auto is_bonded = [&](uint32_t j) {
+      if (!fits) return bonded(s.hoff, s.hr, s.fl8, i, j);
+      bool bd = false;
+#pragma unroll
+      for (uint32_t t = 0; t < kRowStage; ++t) bd |= jb[t] == j;
+      return bd;
+    };
+    uint32_t nfound = 0;
+    if (staged) {
+      // 1. in-reach candidates in scan order (box indices), per lane
+      uint32_t nf = 0;
+      uint32_t e = 0;
+      for (; e + 4 <= total; e += 4) {
+        const float4 q0 = sC[e], q1 = sC[e + 1], q2 = sC[e + 2], q3 = sC[e + 3];
+        const bool h0 = reach(q0), h1 = reach(q1), h2 = reach(q2), h3 = reach(q3);
+        if (live && (h0 || h1 || h2 || h3)) {
+          if (h0) { if (nf < kRowFound) sF[nf][tid] = (uint16_t)e; ++nf; }
+          if (h1) { if (nf < kRowFound) sF[nf][tid] = (uint16_t)(e + 1); ++nf; }
+          if (h2) { if (nf < kRowFound) sF[nf][tid] = (uint16_t)(e + 2); ++nf; }
+          if (h3) { if (nf < kRowFound) sF[nf][tid] = (uint16_t)(e + 3); ++nf; }
+        }
+      }
+      for (; e < total; ++e)
+        if (live && reach(sC[e])) {
+          if (nf < kRowFound) sF[nf][tid] = (uint16_t)e;
+          ++nf;
+        }
+      // 2. drop self and bonded partners, write the row in scan order
+      if (live && nf <= kRowFound) {
+        for (uint32_t f = 0; f < nf; ++f) {
+          const uint32_t j = sJ[sF[f][tid]];
+          if (j != i && !is_bonded(j)) row_entry(s, i, j, nfound);
+        }
+      } else if (live) {
+        // more in-reach candidates than the list holds: scan again
+        for (uint32_t q = 0; q < total; ++q)
+          if (reach(sC[q])) {
+            const uint32_t j = sJ[q];
+            if (j != i && !is_bonded(j)) row_entry(s, i, j, nfound);
+          }
+      }
+    } else if (live) {
+      // candidates read from global memory in box order
+      for (uint32_t q = 0; q < nh; ++q) {
+        const uint32_t a = sG[q], n = sS[q + 1] - sS[q];
+        for (uint32_t e = a; e < a + n; ++e)
+          if (reach(s.cpos[e])) {
+            const uint32_t j = s.sorted[e];
+            if (j != i && !is_bonded(j)) row_entry(s, i, j, nfound);
+          }
+      }
+    }
+    if (live) s.ccnt_new[i] = nfound < s.Kc ? nfound : s.Kc;
+  }
 }
 
 // rest length = current distance, set by the owner half
@@ -1956,7 +2170,7 @@ struct rfd_system {
   uint32_t ev_used = 0;
   // launch shapes (autotuned or set): force tile particles, neighbour block
   int force_fp = kFP;
-  int nbr_block = 128;
+  int nbr_block = 0;  // contact-row search by cell blocks
   uint32_t tile_halves[3] = {0, 0, 0};  // most half-bonds in a tile of 32, 64, 128
   bool once_off = false;  // force the chunked pass (comparison runs)
   // work records of the single-evaluation pass, for tiles of work_fp particles
@@ -2071,8 +2285,8 @@ int bin_particles(rfd_system *s) {
     if (e) return e;
   }
   k_nat_scatter<<<rfd_blocks(s->N, 256), 256, 0, st>>>(s->CI, s->N, s->ccursor, s->ctmp);
-  k_nat_rank<<<rfd_blocks(s->C, 4), 256, 0, st>>>(s->ctmp, s->cstart, s->C, s->csorted, s->P,
-                                                  s->g, s->cpos);
+  k_nat_rank<<<rfd_blocks(s->N, 256), 256, 0, st>>>(s->ctmp, s->cstart, s->CI, s->N, s->csorted,
+                                                   s->P, s->g, s->cpos);
   RFD_HIP(s->ctx, hipGetLastError());
   return RFD_OK;
 }
@@ -2098,10 +2312,16 @@ NbrDev make_nbr(rfd_system *s, double extra) {
 uint32_t nbr_per(uint32_t N, uint32_t nb = 128) { return ((N + nb - 1) / nb + kXcd - 1) / kXcd; }
 uint32_t nbr_grid(uint32_t N, uint32_t nb = 128) { return nbr_per(N, nb) * kXcd; }
 
-// contact-row rebuild with the chosen block size (rows do not depend on it)
+// contact-row rebuild with the chosen shape (rows do not depend on it):
+// 0 = by cell blocks with LDS staging, else threads per particle block
 void launch_rows(rfd_system *s, const NbrDev &d, int nb) {
   hipStream_t q = s->ctx->stream;
-  if (nb == 64)
+  if (nb == 0) {
+    const uint32_t nbx = (s->g.nx + kRowCB - 1) / kRowCB, nby = (s->g.ny + kRowCB - 1) / kRowCB,
+                   nbz = (s->g.nz + kRowCB - 1) / kRowCB;
+    const uint32_t nblocks = nbx * nby * nbz, per = (nblocks + kXcd - 1) / kXcd;
+    k_nat_rows_cells<<<per * kXcd, 64, 0, q>>>(d, nblocks, per, nbx, nby);
+  } else if (nb == 64)
     k_nat_neighbours<2, 64><<<nbr_grid(s->N, 64), 64, 0, q>>>(d, nbr_per(s->N, 64));
   else if (nb == 256)
     k_nat_neighbours<2, 256><<<nbr_grid(s->N, 256), 256, 0, q>>>(d, nbr_per(s->N, 256));
@@ -3305,8 +3525,8 @@ int rfd_system_set_tiles(rfd_system *s, int force_tile, int nbr_block, int singl
   rfd_ctx *ctx = s->ctx;
   RFD_CHECK_ARG(ctx, force_tile == 32 || force_tile == 64 || force_tile == 128,
                 "rfd_system_set_tiles: force tile must be 32, 64 or 128");
-  RFD_CHECK_ARG(ctx, nbr_block == 64 || nbr_block == 128 || nbr_block == 256,
-                "rfd_system_set_tiles: neighbour block must be 64, 128 or 256");
+  RFD_CHECK_ARG(ctx, nbr_block == 0 || nbr_block == 64 || nbr_block == 128 || nbr_block == 256,
+                "rfd_system_set_tiles: row search must be 0 (cell blocks), 64, 128 or 256");
   s->force_fp = force_tile;
   s->nbr_block = nbr_block;
   s->once_off = !single_eval;
@@ -3377,7 +3597,7 @@ int rfd_system_autotune(rfd_system *s, int use_cache, rfd_tune_result *out) {
   // bonds (where the tile's halves fit), and 64 with every half evaluated
   static const int kTiles[4] = {64, 32, 128, 64};
   static const bool kOnceOff[4] = {false, false, false, true};
-  static const int kBlocks[3] = {128, 64, 256};
+  static const int kBlocks[4] = {128, 64, 256, 0};
   const int fp0 = s->force_fp;
   const bool off0 = s->once_off;
   int st = RFD_OK, best_t = 0, best_b = 0;
@@ -3402,7 +3622,7 @@ int rfd_system_autotune(rfd_system *s, int use_cache, rfd_tune_result *out) {
   d.cj_new = s->cj[n];
   d.ccnt_new = s->ccnt[n];
   d.cO_new = s->cO[n];
-  for (int v = 0; v < 3 && !st; ++v) {
+  for (int v = 0; v < 4 && !st; ++v) {
     st = timed(
         [&]() {
           launch_rows(s, d, kBlocks[v]);
@@ -3445,7 +3665,7 @@ int rfd_tune_cache_load(const char *path) {
   int a = 0, b = 0, c = 0;
   std::lock_guard<std::mutex> lk(tune_mu());
   while (fscanf(f, "%159s %d %d %d", k, &a, &b, &c) == 4)
-    if ((a == 32 || a == 64 || a == 128) && (b == 64 || b == 128 || b == 256) &&
+    if ((a == 32 || a == 64 || a == 128) && (b == 0 || b == 64 || b == 128 || b == 256) &&
         (c == 0 || c == 1))
       tune_table()[k] = {a, b, c};
   fclose(f);

@@ -67,7 +67,7 @@ class FireResult(C.Structure):
 class TuneResult(C.Structure):
     _fields_ = [("force_tile", C.c_int), ("nbr_block", C.c_int), ("single_eval", C.c_int),
                 ("cached", C.c_int), ("bucket", C.c_int), ("force_ms", C.c_double * 4),
-                ("nbr_ms", C.c_double * 3)]
+                ("nbr_ms", C.c_double * 4)]
 
     FORCE_VARIANTS = ("64", "32", "128", "64 every half")
 
@@ -76,7 +76,7 @@ class TuneResult(C.Structure):
                 "single_eval": bool(self.single_eval), "cached": bool(self.cached),
                 "bucket": self.bucket,
                 "force_ms": {t: self.force_ms[k] for k, t in enumerate(self.FORCE_VARIANTS)},
-                "nbr_ms": {b: self.nbr_ms[k] for k, b in enumerate((128, 64, 256))}}
+                "nbr_ms": {b: self.nbr_ms[k] for k, b in enumerate((128, 64, 256, 0))}}
 
 
 _vp, _u32, _dbl = C.c_void_p, C.c_uint32, C.c_double

@@ -32,7 +32,8 @@ def uniaxial_cylinder(ctx, D: float, H: float, seed: int = specimen.BASE_SEED,
                       bond_gap: float | None = None, refmode: int = 0,
                       strength=(20e6, 40e6), morton: bool = True, rebuild_every: int = 20,
                       v_load: float = -10.0 * 0.02 / VELOCITY,
-                      flat_ends: bool = False, fp64: bool = False) -> NativeCase:
+                      flat_ends: bool = False, fp64: bool = False,
+                      skin: float = 0.05) -> NativeCase:
     """Polydisperse cylinder (r ~ U[0.4, 0.6] mm) on a jittered FCC lattice of
     spacing 1 mm.  bond_gap=None joins every nearest lattice neighbour (Nb/N
     close to 6); otherwise pairs with surface gap < bond_gap are joined."""
@@ -44,7 +45,8 @@ def uniaxial_cylinder(ctx, D: float, H: float, seed: int = specimen.BASE_SEED,
     R, rad = sp.R, sp.radius
     perm = morton_order(R, 2.0) if morton else np.arange(len(R))
     R, rad = R[perm], rad[perm]
-    p = params_from_material(mat, refmode=refmode, rebuild_every=rebuild_every, fp64=fp64)
+    p = params_from_material(mat, refmode=refmode, rebuild_every=rebuild_every, fp64=fp64,
+                             skin=skin)
     p.farcut = 1.45
     sysn = NativeSystem(ctx, p, R, rad)
     gap = bond_gap if bond_gap is not None else 1.0 - 0.8 + 0.01

@@ -132,7 +132,8 @@ def test_tile_shapes_and_autotune_leave_results_unchanged(ctx):
 
     ref = build()
     ref.step(75)
-    for fp, nb, once in ((32, 64, True), (128, 256, True), (64, 128, False), (64, 128, True)):
+    for fp, nb, once in ((32, 64, True), (128, 256, True), (64, 128, False), (64, 128, True),
+                         (64, 0, True)):
         s = build()
         s.set_tiles(fp, nb, once)
         s.step(75)
@@ -145,7 +146,7 @@ def test_tile_shapes_and_autotune_leave_results_unchanged(ctx):
     r = t.autotune(use_cache=False)
     d = r.as_dict()
     print("autotune", d, flush=True)
-    assert r.force_tile in (32, 64, 128) and r.nbr_block in (64, 128, 256)
+    assert r.force_tile in (32, 64, 128) and r.nbr_block in (0, 64, 128, 256)
     assert not r.cached
     assert all(v > 0 for v in d["force_ms"].values())
     assert all(v > 0 for v in d["nbr_ms"].values())
@@ -180,11 +181,41 @@ def test_single_evaluation_matches_every_half_with_breaking_bonds(ctx):
     ref.step(60)
     counts = ref.bond_counts()
     assert counts[1] > 0, counts  # bonds broke
-    for fp, once in ((64, True), (32, True), (128, True), (32, False)):
+    for fp, once, nb in ((64, True, 128), (32, True, 128), (128, True, 0), (32, False, 0)):
         s = build()
-        s.set_tiles(fp, 128, once)
+        s.set_tiles(fp, nb, once)
         s.step(60)
         _same_state(ref, s, label=f"tile {fp} single {once}")
         assert s.bond_counts() == counts
         s.close()
     ref.close()
+
+
+@pytest.mark.parametrize("skin", [0.05, 0.35])
+def test_row_search_by_cell_blocks_matches_per_particle(ctx, skin):
+    """The contact rows of the cell-block search (LDS-staged candidates; with a
+    0.35 mm skin the cells hold about 9 particles and many blocks exceed the
+    staged copy and read candidates from global memory) equal those of the
+    per-particle search, on a specimen with weak bonds and large random
+    velocities, so contacts form and bonds break.  No row may overflow."""
+    def build():
+        nc = native_cases.uniaxial_cylinder(ctx, 8.0, 12.0, seed=12, strength=(2e6, 4e6),
+                                            skin=skin)
+        N = nc.sys.N
+        g = np.random.default_rng(4)
+        nc.sys.set_velocities((g.standard_normal(3 * N) * 5e-2).astype(np.float32),
+                              (g.standard_normal(3 * N) * 5e-2).astype(np.float32))
+        return nc.sys
+
+    a, b = build(), build()
+    a.set_tiles(64, 128, True)
+    b.set_tiles(64, 0, True)
+    for k in range(4):
+        a.step(30)
+        b.step(30)
+        _same_state(a, b, label=f"skin {skin} chunk {k}")
+    assert a.bond_counts() == b.bond_counts()
+    assert a.bond_counts()[1] > 0
+    assert ctx.flags() == 0
+    a.close()
+    b.close()
