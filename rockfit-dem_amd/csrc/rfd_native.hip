@@ -487,13 +487,14 @@ k_nat_forces(SysDev s, Mat mt, uint32_t ntiles, uint32_t per) {
   s.M[i] = f4(Mb + Mc);
 }
 
-// Half-bond capacity of one tile in the single-evaluation pass: the tile's
-// per-half loads stay in LDS for the whole pass (tiles with more halves run
-// the chunked pass above)
+// Largest half-bond count of one tile in the single-evaluation pass: the
+// tile's per-half loads stay in LDS for the whole pass, in dynamic LDS sized
+// at launch to the system's largest tile (systems with larger tiles run the
+// chunked pass above)
 template <int FP> struct OnceCap;
-template <> struct OnceCap<32> { static constexpr uint32_t v = 512; };
-template <> struct OnceCap<64> { static constexpr uint32_t v = 896; };
-template <> struct OnceCap<128> { static constexpr uint32_t v = 1664; };
+template <> struct OnceCap<32> { static constexpr uint32_t v = 1024; };
+template <> struct OnceCap<64> { static constexpr uint32_t v = 2040; };
+template <> struct OnceCap<128> { static constexpr uint32_t v = 2040; };
 
 // Work record of the single-evaluation pass, one per evaluated half-bond of a
 // tile, built once per tile shape (k_work_*): a = slot e (11 bits) | row
@@ -518,14 +519,18 @@ struct WorkDev {
 // k_nat_forces.
 template <int FP, int FT>
 __global__ void __launch_bounds__(FT) __attribute__((amdgpu_waves_per_eu(5, 8)))
-k_nat_forces1(SysDev s, Mat mt, WorkDev w, uint32_t ntiles, uint32_t per) {
+k_nat_forces1(SysDev s, Mat mt, WorkDev w, uint32_t ntiles, uint32_t per, uint32_t cap) {
   static_assert(FP < FT && FP <= 128, "tile shape");
-  constexpr uint32_t CAP = OnceCap<FP>::v;
-  __shared__ float sF[3][CAP], sM[3][CAP];
+  static_assert(2 * FP <= FT, "one sum task per thread");
+  // per-half loads: F (3 x cap floats) then M (3 x cap), cap >= 2 FP
+  extern __shared__ float sDyn[];
+  float *const sF0 = sDyn, *const sF1 = sDyn + cap, *const sF2 = sDyn + 2 * cap;
+  float *const sM0 = sDyn + 3 * cap, *const sM1 = sDyn + 4 * cap, *const sM2 = sDyn + 5 * cap;
   __shared__ uint32_t sOff[FP + 1];
-  __shared__ float sAcc[2][3][FP];
   __shared__ P4 sP[FP];
   __shared__ float4 sV[FP], sW[FP];
+  // per-particle sums, kept in the slot memory once the slots are read
+  float (*const sAcc)[3][FP] = reinterpret_cast<float (*)[3][FP]>(sDyn);
   const uint32_t tile = xcd_tile(per);
   if (tile >= ntiles) return;
   if (s.done && *s.done) return;
@@ -558,12 +563,12 @@ k_nat_forces1(SysDev s, Mat mt, WorkDev w, uint32_t ntiles, uint32_t per) {
     }
   };
   auto put = [&](uint32_t e, v3 F, v3 M) {
-    sF[0][e] = F.x;
-    sF[1][e] = F.y;
-    sF[2][e] = F.z;
-    sM[0][e] = M.x;
-    sM[1][e] = M.y;
-    sM[2][e] = M.z;
+    sF0[e] = F.x;
+    sF1[e] = F.y;
+    sF2[e] = F.z;
+    sM0[e] = M.x;
+    sM1[e] = M.y;
+    sM2[e] = M.z;
   };
   // 1. bonds inside the tile, once each (the lower end owns them)
   for (uint32_t k = w0 + tid; k < wx; k += FT) {
@@ -611,20 +616,24 @@ k_nat_forces1(SysDev s, Mat mt, WorkDev w, uint32_t ntiles, uint32_t per) {
       put(e, -o.Fi, o.Mbj + o.Mcj);
   }
   __syncthreads();
-  // 3. per-particle sums in row order: task (group g, particle p)
-  for (uint32_t task = tid; task < 2 * np; task += FT) {
-    const uint32_t p = task % np, g = task / np;
-    const uint32_t a0 = sOff[p] - h0, a1 = sOff[p + 1] - h0;
-    float (*src)[CAP] = g == 0 ? sF : sM;
-    float x = 0.f, y = 0.f, z = 0.f;
+  // 3. per-particle sums in row order: task (group g, particle p), one per
+  //    thread; the results go to the slot memory after every slot is read
+  float x = 0.f, y = 0.f, z = 0.f;
+  const uint32_t tp = tid % np, tg = tid / np;
+  if (tid < 2 * np) {
+    const uint32_t a0 = sOff[tp] - h0, a1 = sOff[tp + 1] - h0;
+    const float *const src = sDyn + 3 * cap * tg;
     for (uint32_t e = a0; e < a1; ++e) {
-      x = x + src[0][e];
-      y = y + src[1][e];
-      z = z + src[2][e];
+      x = x + src[e];
+      y = y + src[cap + e];
+      z = z + src[2 * cap + e];
     }
-    sAcc[g][0][p] = x;
-    sAcc[g][1][p] = y;
-    sAcc[g][2][p] = z;
+  }
+  __syncthreads();
+  if (tid < 2 * np) {
+    sAcc[tg][0][tp] = x;
+    sAcc[tg][1][tp] = y;
+    sAcc[tg][2][tp] = z;
   }
   __syncthreads();
   if (tid >= np) return;
@@ -2417,7 +2426,11 @@ void launch_forces(rfd_system *s, const SysDev &d, const Mat &m) {
   const uint32_t per = (nb + kXcd - 1) / kXcd;
   if (!m.refmode && !s->once_off && s->work_fp == FP) {
     WorkDev w{s->wrec, s->wofs, s->wnin};
-    k_nat_forces1<FP, FT><<<per * kXcd, FT, 0, s->ctx->stream>>>(d, m, w, nb, per);
+    const int q = FP == 32 ? 0 : (FP == 64 ? 1 : 2);
+    // slots for the system's largest tile, at least the per-particle sums
+    const uint32_t cap = std::max((s->tile_halves[q] + 31u) & ~31u, 2u * FP);
+    k_nat_forces1<FP, FT><<<per * kXcd, FT, sizeof(float) * 6 * cap, s->ctx->stream>>>(
+        d, m, w, nb, per, cap);
   } else {
     k_nat_forces<FP, FT><<<per * kXcd, FT, 0, s->ctx->stream>>>(d, m, nb, per);
   }
