@@ -47,6 +47,8 @@ def parse():
                    help="skip the batched Brazilian-disc measurement")
     p.add_argument("--batch-only", action="store_true",
                    help="only the batched Brazilian-disc measurement")
+    p.add_argument("--packed-only", action="store_true",
+                   help="only the random-packing cylinder measurement")
     p.add_argument("--copy-calib", action="store_true",
                    help="only run a plain device copy of 1 GiB (counter calibration and "
                         "stream-copy ceiling)")
@@ -311,6 +313,48 @@ def bench_peak_precision(D: float = 10.0, H: float = 20.0, max_steps: int = 2000
     return out
 
 
+def bench_packed(D: float = 100.0, H: float = 200.0, steps: int = 50, warmup: int = 5) -> dict:
+    """The uniaxial cylinder on a seeded random packing at solid fraction 0.6
+    (rfd_system_pack) instead of the lattice: bonds join pairs with gap below
+    0.05 r_mean.  Same step and roofline accounting as the main line."""
+    from rfd import native_cases
+    from rfd.reflayout import Context
+
+    ctx = Context(0)
+    t0 = time.perf_counter()
+    pk = native_cases.packed_cylinder(ctx, D, H, 0.6)
+    pack_s = time.perf_counter() - t0
+    nc = native_cases.packed_uniaxial_cylinder(ctx, D, H, packing=pk)
+    s = nc.sys
+    N, Nb = s.N, s.bonds.n
+    print(f"[bench] packed: N={N} Nb={Nb} phi={pk.phi:.4f} packed in {pack_s:.1f}s "
+          f"({pk.result['iterations']} iterations)", file=sys.stderr, flush=True)
+    s.step(warmup)
+    ctx.sync()
+    s.timing(True)
+    t1 = time.perf_counter()
+    s.step(steps)
+    ctx.sync()
+    t2 = time.perf_counter()
+    kern_ms, launches = s.timing_read()
+    kern_ms /= max(launches, 1)
+    active, broken, removed = s.bond_counts()
+    alg = 128 * Nb + 72 * N
+    out = {
+        "workload": f"uniaxial cylinder D{D:g}xH{H:g} mm, seeded random packing phi=0.6, "
+                    "r~U[0.4,0.6] mm, bonds at gap < 0.05 r_mean",
+        "particles": N, "bonds": Nb, "bonds_per_particle": Nb / N, "phi": pk.phi,
+        "pack": {"seconds": pack_s, **pk.result},
+        "bond_updates_per_s": active * steps / (t2 - t1),
+        "ms_per_step": (t2 - t1) * 1e3 / steps,
+        "force_kernel": s.force_kernel(), "force_ms": kern_ms,
+        "force_roofline_frac": alg / (kern_ms * 1e-3) / PEAK_HBM,
+    }
+    assert ctx.flags() == 0, "device flags raised in the packed run"
+    s.close()
+    return out
+
+
 def _quantiles(x) -> dict:
     import numpy as np
 
@@ -441,6 +485,9 @@ def main():
     if args.batch_only:
         print(json.dumps({"batch": bench_batch()}))
         return
+    if args.packed_only:
+        print(json.dumps({"packed": bench_packed()}))
+        return
     res = bench_ref(args) if args.tier == "ref" else bench_native(args)
     res["roofline"]["traffic"] = load_traffic(res["roofline"]["kernel"],
                                               res["config"]["particles"],
@@ -479,6 +526,10 @@ def main():
             out["batch"] = bench_batch()
         except Exception as e:  # reported, never hides the main line
             out["batch"] = {"error": f"{type(e).__name__}: {e}"}
+        try:
+            out["packed"] = bench_packed()
+        except Exception as e:  # reported, never hides the main line
+            out["packed"] = {"error": f"{type(e).__name__}: {e}"}
     print(json.dumps(out))
 
 

@@ -268,6 +268,38 @@ int rfd_system_sizes(rfd_system *sys, uint32_t *N, uint32_t *Nb);
  * clears), then read the summed kernel time in ms and the launch count */
 int rfd_system_timing(rfd_system *sys, int enable);
 int rfd_system_timing_read(rfd_system *sys, double *kernel_ms, uint32_t *launches);
+/* ---- seeded packer ----
+ * Packs the system's particles (positions as given, e.g. seeded uniform
+ * random points inside the container; radii as given are the final radii)
+ * into a cylinder of radius `radius` about the axis through (cx, cy) along
+ * z, between z0 and z1: the radii start at s0 times their final value and
+ * grow by `growth` per stage; each iteration rebuilds the contact rows and
+ * moves every particle by beta/2 of each of its overlaps plus `momentum`
+ * times its last move (heavy-ball relaxation), then keeps its sphere inside
+ * the container (a clamped particle forgets its last move).  A stage ends when the largest overlap,
+ * relative to the smaller radius of the pair, is below tol, or after
+ * stage_iters iterations (final_iters at full size).  Replaces the
+ * soft-sphere packing of h_CreateCylinderSample
+ * (md_createcylindersample.cu:136-343, d_CalculateForces md_calculateforces.cu:20,
+ * d_CylinderRestrictionZ md_cylinderrestriction.cu:74).  The system must hold
+ * no bonds (set an empty bond list); velocities and loads are zero after it,
+ * and the contact rows are rebuilt by the next step.  Row overflow during the
+ * packing is reported in the result (and raised in the context flags). */
+typedef struct rfd_pack_params {
+  double cx, cy, radius, z0, z1;  /* container                                 */
+  double s0, growth;              /* initial radius scale, growth per stage    */
+  double beta;                    /* relaxation factor (0, 1]                  */
+  double momentum;                /* share of the last move kept, [0, 1)       */
+  double tol;                     /* overlap / smaller radius at convergence   */
+  uint32_t stage_iters, final_iters;
+} rfd_pack_params;
+typedef struct rfd_pack_result {
+  uint32_t iterations, stages;
+  int converged, row_overflow;
+  double max_overlap;             /* relative to the smaller radius            */
+  double seconds;
+} rfd_pack_result;
+int rfd_system_pack(rfd_system *sys, const rfd_pack_params *p, rfd_pack_result *out);
 /* ---- batched specimens ----
  * S independent specimens held in one system, specimen k owning particles
  * [first[k], first[k+1]) (first[0] = 0, first[S] = N), each with its own

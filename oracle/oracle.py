@@ -248,3 +248,41 @@ def native_contacts(R, V, W, rad, pairs, reach, O, dt, Estar, Gstar, pA, mu, mur
                                   ptr(reach), ptr(O), len(pi), dt, Estar, Gstar, pA, mu, mur,
                                   rho43pi, ptr(F), ptr(M))
     return F.T.copy(), M.T.copy()
+
+
+def pack_iteration(R, rad, beta, cx, cy, rc, z0, z1, reach_pairs=None):
+    """One iteration of the packer (k_pack_disp + k_pack_apply), restated in
+    float64: every particle moves by -beta/2 * overlap along the pair
+    direction for each overlapping partner (Jacobi, from the positions at the
+    start of the iteration; the displacement is rounded to float32 as the
+    device keeps it), then its sphere is clamped inside the cylinder
+    (md_cylinderrestriction.cu:74-118 applied to sphere surfaces).  Pairs are
+    found by brute force (all pairs, or the given candidate pairs).  Returns
+    the new positions and the largest overlap relative to the smaller radius."""
+    R = np.asarray(R, np.float64)
+    rad = np.asarray(rad, np.float64)
+    N = len(R)
+    if reach_pairs is None:
+        ii, jj = np.triu_indices(N, 1)
+    else:
+        ii, jj = reach_pairs[:, 0], reach_pairs[:, 1]
+    r = R[jj] - R[ii]
+    L = np.linalg.norm(r, axis=1)
+    ov = rad[ii] + rad[jj] - L
+    m = (ov > 0) & (L > 1e-12)
+    ii, jj, r, L, ov = ii[m], jj[m], r[m], L[m], ov[m]
+    c = (-0.5 * beta * ov / L)[:, None] * r
+    D = np.zeros_like(R)
+    np.add.at(D, ii, c)
+    np.add.at(D, jj, -c)
+    mo = float((ov / np.minimum(rad[ii], rad[jj])).max()) if len(ov) else 0.0
+    out = R + D.astype(np.float32).astype(np.float64)
+    ex, ey = out[:, 0] - cx, out[:, 1] - cy
+    lim = rc - rad
+    rho = np.sqrt(ex * ex + ey * ey)
+    f = np.where(rho > lim, lim / np.where(rho > 0, rho, 1.0), 1.0)
+    out[:, 0] = cx + ex * f
+    out[:, 1] = cy + ey * f
+    out[:, 2] = np.minimum(out[:, 2], z1 - rad)
+    out[:, 2] = np.maximum(out[:, 2], z0 + rad)
+    return out, mo

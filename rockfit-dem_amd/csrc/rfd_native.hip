@@ -1791,6 +1791,88 @@ __global__ void k_fire_step_disp(const P4 *__restrict__ P, const P4 *__restrict_
   }
 }
 
+// ------------------------------------------------------------- packer
+// Seeded packing by radius growth with overlap removal (the role of
+// md_createcylindersample.cu:136-343, which relaxes a monodisperse soft-sphere
+// gas for 200k viscous steps and then by FIRE): radii grow in stages and each
+// iteration moves every particle by half of each of its overlaps (Jacobi, a
+// relaxation factor beta), then keeps it inside the container.
+
+// radii of the current stage: r = s r_final
+__global__ void k_pack_scale(P4 *__restrict__ P, const float *__restrict__ rfin, double sc,
+                             uint32_t N) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride())
+    P[i].r = sc * (double)rfin[i];
+}
+
+// displacement of each particle from its overlaps (contact rows), and the
+// largest overlap relative to the smaller radius (as ordered float bits)
+__global__ void k_pack_disp(const P4 *__restrict__ P, const uint32_t *__restrict__ ccnt,
+                            const uint32_t *__restrict__ cj, uint32_t N, double beta,
+                            float4 *__restrict__ D, uint32_t *__restrict__ maxov) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
+    const P4 A = P[i];
+    double dx = 0.0, dy = 0.0, dz = 0.0;
+    float mo = 0.f;
+    const uint32_t nc = ccnt[i];
+    for (uint32_t k = 0; k < nc; ++k) {
+      const P4 B = P[cj[(size_t)k * N + i]];
+      const double rx = B.x - A.x, ry = B.y - A.y, rz = B.z - A.z;
+      const double L = __builtin_sqrt(rx * rx + ry * ry + rz * rz);
+      const double ov = A.r + B.r - L;
+      if (ov > 0.0 && L > 1e-12) {
+        const double c = -0.5 * beta * ov / L;
+        dx += c * rx;
+        dy += c * ry;
+        dz += c * rz;
+        mo = fmaxf(mo, (float)(ov / fmin(A.r, B.r)));
+      }
+    }
+    D[i] = make_float4((float)dx, (float)dy, (float)dz, 0.f);
+    if (mo > 0.f) atomicMax(maxov, __float_as_uint(mo));
+  }
+}
+
+// apply the moves, then keep each sphere inside the container: a cylinder of
+// radius rc about the axis through (cx, cy) along z, between z0 and z1 (the
+// clamp of d_CylinderRestrictionZ, md_cylinderrestriction.cu:74-118, applied
+// to sphere surfaces).  The move is the displacement plus mom times the last
+// move (heavy-ball relaxation); a clamp drops the last move.
+__global__ void k_pack_apply(P4 *__restrict__ P, const float4 *__restrict__ D,
+                             float4 *__restrict__ U, uint32_t N, double mom, double cx,
+                             double cy, double rc, double z0, double z1) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
+    P4 A = P[i];
+    const float4 d = D[i], u0 = U[i];
+    const double ux = (double)d.x + mom * (double)u0.x, uy = (double)d.y + mom * (double)u0.y,
+                 uz = (double)d.z + mom * (double)u0.z;
+    const double x0 = A.x, y0 = A.y, z00 = A.z;
+    A.x += ux;
+    A.y += uy;
+    A.z += uz;
+    bool hit = false;
+    const double ex = A.x - cx, ey = A.y - cy, lim = rc - A.r;
+    const double rho2 = ex * ex + ey * ey;
+    if (rho2 > lim * lim) {
+      const double f = lim / __builtin_sqrt(rho2);
+      A.x = cx + ex * f;
+      A.y = cy + ey * f;
+      hit = true;
+    }
+    if (A.z > z1 - A.r) {
+      A.z = z1 - A.r;
+      hit = true;
+    }
+    if (A.z < z0 + A.r) {
+      A.z = z0 + A.r;
+      hit = true;
+    }
+    P[i] = A;
+    U[i] = hit ? make_float4(0.f, 0.f, 0.f, 0.f)
+               : make_float4((float)(A.x - x0), (float)(A.y - y0), (float)(A.z - z00), 0.f);
+  }
+}
+
 __global__ void k_zero_f4(float4 *__restrict__ a, float4 *__restrict__ b, uint32_t N) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
     a[i] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -3696,6 +3778,93 @@ int rfd_system_get_history(rfd_system *s, double *hist, uint32_t cap, uint32_t *
   const uint32_t m = L.hist_len < cap ? L.hist_len : cap;
   if (hist && m)
     RFD_HIP(ctx, hipMemcpy(hist, s->hist, sizeof(double) * 3 * m, hipMemcpyDeviceToHost));
+  return RFD_OK;
+}
+
+int rfd_system_pack(rfd_system *s, const rfd_pack_params *pp, rfd_pack_result *out) {
+  if (!valid(s)) return RFD_E_ARG;
+  rfd_ctx *ctx = s->ctx;
+  RFD_CHECK_ARG(ctx, pp && out, "rfd_system_pack: null argument");
+  RFD_CHECK_ARG(ctx, s->have_bonds && s->Nb == 0 && !s->fp64,
+                "rfd_system_pack: the packer runs on particles without bonds (set an empty "
+                "bond list first)");
+  RFD_CHECK_ARG(ctx, pp->s0 > 0 && pp->s0 <= 1 && pp->growth > 1 && pp->beta > 0 &&
+                         pp->beta <= 1 && pp->momentum >= 0 && pp->momentum < 1 &&
+                         pp->tol > 0 && pp->stage_iters > 0 &&
+                         pp->final_iters > 0 && pp->radius > 0 && pp->z1 > pp->z0,
+                "rfd_system_pack: bad parameters");
+  *out = rfd_pack_result{};
+  const auto t0 = std::chrono::steady_clock::now();
+  hipStream_t q = ctx->stream;
+  const uint32_t N = s->N;
+  // final radii, kept while the stages scale them
+  float *rfin = nullptr;
+  uint32_t *d_max = nullptr;
+  int st = dalloc(ctx, &rfin, N);
+  if (!st) st = dalloc(ctx, &d_max, 1);
+  if (st) return st;
+  {
+    std::vector<P4> hp(N);
+    RFD_HIP(ctx, hipMemcpy(hp.data(), s->P, sizeof(P4) * N, hipMemcpyDeviceToHost));
+    std::vector<float> hr(N);
+    for (uint32_t i = 0; i < N; ++i) hr[i] = (float)hp[i].r;
+    RFD_HIP(ctx, hipMemcpy(rfin, hr.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+  }
+  const uint32_t nb = rfd_blocks(N, 256);
+  k_zero_f4<<<nb, 256, 0, q>>>(s->V, nullptr, N);  // no last move yet
+  double sc = pp->s0;
+  uint32_t iters = 0, stages = 0;
+  float mo = 0.f;
+  auto relax = [&](uint32_t max_it) -> int {
+    for (uint32_t k = 0; k < max_it; ++k) {
+      int e = rebuild_rows(s);
+      if (e) return e;
+      k_fill_u32n<<<1, 64, 0, q>>>(d_max, 1, 0u);
+      k_pack_disp<<<nb, 256, 0, q>>>(s->P, s->ccnt[s->ccur], s->cj[s->ccur], N, pp->beta, s->F,
+                                     d_max);
+      k_pack_apply<<<nb, 256, 0, q>>>(s->P, s->F, s->V, N, pp->momentum, pp->cx, pp->cy,
+                                      pp->radius, pp->z0, pp->z1);
+      RFD_HIP(ctx, hipGetLastError());
+      ++iters;
+      if (k % 8 == 7 || k + 1 == max_it) {
+        uint32_t bits = 0;
+        RFD_HIP(ctx, hipMemcpyAsync(&bits, d_max, sizeof(bits), hipMemcpyDeviceToHost, q));
+        RFD_HIP(ctx, hipStreamSynchronize(q));
+        memcpy(&mo, &bits, sizeof(mo));
+        if (mo < pp->tol) break;
+      }
+    }
+    return RFD_OK;
+  };
+  for (;;) {
+    k_pack_scale<<<nb, 256, 0, q>>>(s->P, rfin, sc, N);
+    RFD_HIP(ctx, hipGetLastError());
+    const bool last = sc >= 1.0;
+    st = relax(last ? pp->final_iters : pp->stage_iters);
+    if (st) break;
+    ++stages;
+    if (last) break;
+    sc = std::min(1.0, sc * pp->growth);
+  }
+  (void)hipStreamSynchronize(q);
+  dfree(rfin);
+  dfree(d_max);
+  if (st) return st;
+  // the packing leaves no motion and no loads behind
+  k_zero_f4<<<nb, 256, 0, q>>>(s->V, s->W, N);
+  k_zero_f4<<<nb, 256, 0, q>>>(s->F, s->M, N);
+  RFD_HIP(ctx, hipGetLastError());
+  RFD_HIP(ctx, hipStreamSynchronize(q));
+  uint32_t fl = 0;
+  RFD_HIP(ctx, hipMemcpy(&fl, ctx->d_flags, sizeof(fl), hipMemcpyDeviceToHost));
+  out->iterations = iters;
+  out->stages = stages;
+  out->max_overlap = mo;
+  out->converged = mo < pp->tol ? 1 : 0;
+  out->row_overflow = (fl & RFD_FLAG_ROW_FULL) ? 1 : 0;
+  out->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  s->rows_valid = false;
+  s->gen++;
   return RFD_OK;
 }
 

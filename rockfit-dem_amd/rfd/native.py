@@ -79,6 +79,22 @@ class TuneResult(C.Structure):
                 "nbr_ms": {b: self.nbr_ms[k] for k, b in enumerate((128, 64, 256, 0))}}
 
 
+class PackParams(C.Structure):
+    _fields_ = [("cx", C.c_double), ("cy", C.c_double), ("radius", C.c_double),
+                ("z0", C.c_double), ("z1", C.c_double), ("s0", C.c_double),
+                ("growth", C.c_double), ("beta", C.c_double), ("momentum", C.c_double),
+                ("tol", C.c_double),
+                ("stage_iters", C.c_uint32), ("final_iters", C.c_uint32)]
+
+
+class PackResult(C.Structure):
+    _fields_ = [("iterations", C.c_uint32), ("stages", C.c_uint32), ("converged", C.c_int),
+                ("row_overflow", C.c_int), ("max_overlap", C.c_double), ("seconds", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _vp, _u32, _dbl = C.c_void_p, C.c_uint32, C.c_double
 NATIVE_SIGNATURES = {
     "rfd_system_create": [_vp, C.POINTER(SystemParams), _u32, C.POINTER(_vp)],
@@ -119,6 +135,7 @@ NATIVE_SIGNATURES = {
     "rfd_system_set_tiles": [_vp, C.c_int, C.c_int, C.c_int],
     "rfd_system_get_tiles": [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "rfd_tune_cache_save": [C.c_char_p],
+    "rfd_system_pack": [_vp, C.POINTER(PackParams), C.POINTER(PackResult)],
     "rfd_tune_cache_load": [C.c_char_p],
 }
 
@@ -407,6 +424,11 @@ class NativeSystem:
 
     def step_graph(self, n: int = 1):
         self._call("rfd_system_step_graph", self.h, n)
+
+    def pack(self, p: PackParams) -> PackResult:
+        r = PackResult()
+        self._call("rfd_system_pack", self.h, C.byref(p), C.byref(r))
+        return r
 
     def set_tiles(self, force_tile: int, nbr_block: int, single_eval: bool = True):
         self._call("rfd_system_set_tiles", self.h, force_tile, nbr_block, int(single_eval))

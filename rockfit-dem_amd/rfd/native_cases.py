@@ -7,7 +7,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import specimen
-from .native import (Bonds, NativeSystem, loading_from_compression, morton_order,
+from .native import (Bonds, NativeSystem, PackParams, loading_from_compression, morton_order,
                      params_from_material, strengths)
 from .params import Compression, Material, SampleBox, STRESS, VELOCITY
 
@@ -194,3 +194,99 @@ def batch_member(ctx, case: BatchCase, k: int) -> NativeSystem:
                          bb.br[m], bb.sc[m], bb.tc[m], None if bb.L0 is None else bb.L0[m]))
     sysn.set_loading(case.loadings[k])
     return sysn
+
+
+@dataclass
+class Packing:
+    R: np.ndarray        # (N, 3) positions, Morton order
+    radius: np.ndarray   # (N,)
+    phi: float           # solid fraction of the container
+    result: dict         # the packer's report
+
+
+def mean_sphere_volume(r_lo: float, r_hi: float) -> float:
+    """E[4/3 pi r^3] for r ~ U[r_lo, r_hi]."""
+    if r_hi == r_lo:
+        return 4.0 / 3.0 * np.pi * r_lo ** 3
+    return 4.0 / 3.0 * np.pi * (r_hi ** 4 - r_lo ** 4) / (4.0 * (r_hi - r_lo))
+
+
+def packed_cylinder(ctx, D: float, H: float, phi: float = 0.6, seed: int = specimen.BASE_SEED,
+                    r_lo: float = 0.4, r_hi: float = 0.6, center=(0.0, 0.0, 0.0),
+                    s0: float = 0.7, growth: float = 1.005, beta: float = 0.5,
+                    momentum: float = 0.9, tol: float = 0.05, stage_iters: int = 64,
+                    final_iters: int = 3000) -> Packing:
+    """Seeded random packing of a cylinder of diameter D and height H (axis z)
+    at solid fraction phi: N = phi V / E[sphere volume] spheres with radii
+    U[r_lo, r_hi], uniform random centres (Philox, `seed`), packed on the
+    device by radius growth with overlap removal (rfd_system_pack).  At
+    phi = 0.6 in small containers the wall layers bring the interior close to
+    jamming, so a packing ends with small overlaps: at most `tol` of the
+    smaller radius (the mean is about 1 %), the scale of the bonding gap
+    0.05 r_mean."""
+    g = specimen.rng(seed + 7)
+    V = np.pi * (0.5 * D) ** 2 * H
+    N = int(round(phi * V / mean_sphere_volume(r_lo, r_hi)))
+    # radii as the device keeps them (FP32 values)
+    rad = g.uniform(r_lo, r_hi, N).astype(np.float32).astype(np.float64)
+    # uniform points in the cylinder, clear of the wall at the initial scale
+    rr = 0.5 * D - s0 * rad
+    rho = rr * np.sqrt(g.uniform(0.0, 1.0, N))
+    th = g.uniform(0.0, 2 * np.pi, N)
+    z = g.uniform(-0.5 * H + s0 * rad, 0.5 * H - s0 * rad)
+    c = np.asarray(center, np.float64)
+    R = np.stack([c[0] + rho * np.cos(th), c[1] + rho * np.sin(th), c[2] + z], 1)
+    perm = morton_order(R, 2.0)
+    R, rad = R[perm], rad[perm]
+    mat = material_c(r_lo)
+    p = params_from_material(mat, refmode=0, contact_capacity=32)
+    sysn = NativeSystem(ctx, p, R, rad)
+    sysn.set_bonds(Bonds(np.zeros(0, np.uint32), np.zeros(0, np.uint32),
+                         np.zeros(0, np.float32), np.zeros(0, np.float32),
+                         np.zeros(0, np.float32)))
+    pp = PackParams(cx=c[0], cy=c[1], radius=0.5 * D, z0=c[2] - 0.5 * H, z1=c[2] + 0.5 * H,
+                    s0=s0, growth=growth, beta=beta, momentum=momentum, tol=tol,
+                    stage_iters=stage_iters,
+                    final_iters=final_iters)
+    res = sysn.pack(pp).as_dict()
+    Rp = sysn.state()["R"].reshape(3, N).T.copy()
+    radius = np.ascontiguousarray(rad, np.float64)
+    sysn.close()
+    perm = morton_order(Rp, 2.0)
+    phi_out = float((4.0 / 3.0 * np.pi * radius ** 3).sum() / V)
+    return Packing(Rp[perm], radius[perm], phi_out, res)
+
+
+def packed_uniaxial_cylinder(ctx, D: float, H: float, phi: float = 0.6,
+                             seed: int = specimen.BASE_SEED, gap_frac: float = 0.05,
+                             strength=(20e6, 40e6), rebuild_every: int = 20,
+                             v_load: float = -10.0 * 0.02 / VELOCITY, fp64: bool = False,
+                             packing: Packing | None = None) -> NativeCase:
+    """Uniaxial specimen on a seeded random packing (packed_cylinder): bonds
+    join pairs with surface gap < gap_frac * r_mean, bond radius min(ri, rj),
+    strengths N(mu, 0.2 mu).  Platens start on the outermost particle centres
+    along z."""
+    pk = packing or packed_cylinder(ctx, D, H, phi, seed)
+    assert np.allclose(0.5 * (pk.R.max(0) + pk.R.min(0))[:2], 0.0, atol=0.5 * D)
+    R, rad = pk.R, pk.radius
+    mat = material_c(0.4)
+    a = 2.0 * 2.9 * 0.6
+    box = SampleBox(D, D, H, space=(D + 16.0, D + 16.0, H + 12.0), cell_a=a, p_r=0.5)
+    R = R + np.asarray(box.center)  # the packing is centred on the origin
+    p = params_from_material(mat, refmode=0, rebuild_every=rebuild_every, fp64=fp64)
+    p.farcut = 1.45
+    sysn = NativeSystem(ctx, p, R, rad.astype(np.float32))
+    pairs = sysn.find_pairs(gap_frac * float(rad.mean()))
+    g = specimen.rng(seed + 1)
+    sc, tc = strengths(g, len(pairs), strength[0], strength[1])
+    br = np.minimum(rad[pairs[:, 0]], rad[pairs[:, 1]]).astype(np.float32)
+    sysn.set_bonds(Bonds(pairs[:, 0].copy(), pairs[:, 1].copy(), br, sc, tc))
+    cmp = Compression("uniaxial", box.L, box.center, 0.5, mat.dt, v_load=v_load,
+                      min_stress=15e6 / STRESS, max_strain=0.1)
+    L = loading_from_compression(cmp, box.center[2] + box.spacesized2[2], box)
+    z = R[:, 2]
+    L.top, L.bottom = float(z.max()), float(z.min())
+    L.top_r, L.bottom_r = L.top - 0.5, L.bottom + 0.5
+    L.size = L.size0 = L.top - L.bottom
+    sysn.set_loading(L)
+    return NativeCase(sysn, mat, box, cmp, np.arange(len(R)))
