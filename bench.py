@@ -47,6 +47,8 @@ def parse():
                    help="skip the batched Brazilian-disc measurement")
     p.add_argument("--batch-only", action="store_true",
                    help="only the batched Brazilian-disc measurement")
+    p.add_argument("--calibrate", action="store_true",
+                   help="only the calibration outer loop (CG steps on batched tests)")
     p.add_argument("--packed-only", action="store_true",
                    help="only the random-packing cylinder measurement")
     p.add_argument("--copy-calib", action="store_true",
@@ -355,6 +357,55 @@ def bench_packed(D: float = 100.0, H: float = 200.0, steps: int = 50, warmup: in
     return out
 
 
+def bench_calibrate(steps: int = 2) -> dict:
+    """The calibration outer loop on the reference's test geometry (uniaxial
+    D25 x H50, Brazilian D50 x T20, packed at solid fraction 0.61): CG steps
+    from the granite start, each evaluating 6 parameter sets x 2 tests in
+    batches; then the same 12 runs of one step one at a time."""
+    import numpy as np
+
+    from rfd import calibrate as cal
+    from rfd.params import STRESS
+    from rfd.reflayout import Context
+
+    ctx = Context(0)
+    t0 = time.perf_counter()
+    ev = cal.Evaluator(ctx)
+    setup = time.perf_counter() - t0
+    st = cal.CGState(params=cal.GRANITE_START.copy(), goals=cal.GRANITE_GOALS.copy())
+    times = []
+    for k in range(steps):
+        t = time.perf_counter()
+        R = ev.evaluate(st.parameter_sets())
+        times.append(time.perf_counter() - t)
+        st.update(R)
+        print(f"[bench] CG step {k}: {times[-1]:.1f}s, functional "
+              f"{st.history[-1]['functional']:.4g}", file=sys.stderr, flush=True)
+    sets = st.parameter_sets()
+    dt = min(cal._material(p).dt for p in sets)
+    t = time.perf_counter()
+    for p in sets:
+        ev.run(p[None, :], "uniaxial", dt)
+        ev.run(p[None, :], "brazil", dt)
+    seq = time.perf_counter() - t
+    return {
+        "workload": "granite CG calibration, uniaxial D25xH50 and Brazilian D50xT20 mm packed "
+                    "at phi 0.61 (monodisperse r 0.5 mm), 6 parameter sets x 2 tests per step",
+        "particles": {"uniaxial": int(len(ev.uni.radius)), "brazil": int(len(ev.bra.radius))},
+        "setup_seconds": setup,
+        "seconds_per_cg_step_batched": float(np.mean(times)),
+        "seconds_12_runs_one_at_a_time": seq,
+        "steps": [{"functional": h["functional"],
+                   "uniaxial_peak_strain": float(h["results"][0]),
+                   "uniaxial_peak_MPa": float(h["results"][1] * STRESS / 1e6),
+                   "brazil_peak_MPa": float(h["results"][2] * STRESS / 1e6),
+                   "params": [float(x) for x in h["params"]]} for h in st.history],
+        "goals": {"uniaxial_peak_strain": float(cal.GRANITE_GOALS[0]),
+                  "uniaxial_peak_MPa": float(cal.GRANITE_GOALS[1] * STRESS / 1e6),
+                  "brazil_peak_MPa": float(cal.GRANITE_GOALS[2] * STRESS / 1e6)},
+    }
+
+
 def _quantiles(x) -> dict:
     import numpy as np
 
@@ -487,6 +538,9 @@ def main():
         return
     if args.packed_only:
         print(json.dumps({"packed": bench_packed()}))
+        return
+    if args.calibrate:
+        print(json.dumps({"calibrate": bench_calibrate()}))
         return
     res = bench_ref(args) if args.tier == "ref" else bench_native(args)
     res["roofline"]["traffic"] = load_traffic(res["roofline"]["kernel"],
