@@ -5,6 +5,8 @@
 * Uniaxial.txt / Brazil.txt: the stress-strain record "EpsZ StressZ_bottom
   StressZ_top" with stresses in Pa (SaveSumForcesLoading, md_results.cu:119-131);
 * LAMMPS data with bonds (SaveLammpsDATASimple, md_savelammpsdata.cpp:8-74);
+* particle position files: N whitespace-separated "x y z" triples in metres,
+  placed about the sample centre (ReadParticlesCSV, md_readparticlesfile.cu:9-72);
 * DATA.dat: the binary checkpoint of the slot layout (SaveAllData /
   LoadAllData and the write_/read_ helpers, md_savealldata.cpp:12-277).  The
   reference is built with MSVC, where uint_fast32_t is 4 bytes.
@@ -45,6 +47,21 @@ def read_sample(path) -> np.ndarray:
     n = int(tok[0])
     v = np.array(tok[1:1 + 3 * n], np.float64).reshape(n, 3)
     return (v * (1.0 / LENGTH)).astype(np.float32).astype(np.float64)
+
+
+def read_particles_csv(path, n: int, center=(0.0, 0.0, 0.0)):
+    """Positions of the first n triples of a particle file, shifted to the
+    sample centre, and the bounding box the reference derives from them
+    (A_real, B_real, size_real, center_real; md_readparticlesfile.cu:42-72).
+    Returns (R (n, 3), dict of the four float3)."""
+    tok = pathlib.Path(path).read_text().replace(",", " ").split()
+    if len(tok) < 3 * n:
+        raise ValueError(f"{path}: {len(tok) // 3} triples, {n} asked for")
+    v = np.array(tok[:3 * n], np.float64).reshape(n, 3)
+    R = (v / LENGTH + np.asarray(center, np.float64)).astype(np.float32)
+    lo, hi = R.min(0).astype(np.float64), R.max(0).astype(np.float64)
+    box = {"A_real": lo, "B_real": hi, "size_real": hi - lo, "center_real": 0.5 * (hi + lo)}
+    return R.astype(np.float64), box
 
 
 # ------------------------------------------------- Uniaxial.txt / Brazil.txt
@@ -99,6 +116,29 @@ def write_lammps(path, R, bonds, box_lo, box_hi, dt, V=None, W=None, mass=None,
         f.write("\nBonds\n\n")
         for k, (a, b) in enumerate(bonds):
             f.write(f"{k + 1} 2 {a + 1} {b + 1}\n")
+
+
+def write_lammps_system(path, sys, box_lo=None, box_hi=None) -> None:
+    """LAMMPS data of a native-tier system (rfd.native.NativeSystem): its
+    current positions, kinetic energies from its density and radii, and the
+    bonds still intact (flag 1).  The box defaults to the particles' bounds."""
+    st = sys.state()
+    N = sys.N
+    R = st["R"].reshape(3, N).T
+    V = st["V"].reshape(3, N).T.astype(np.float64)
+    W = st["W"].reshape(3, N).T.astype(np.float64)
+    r = sys.radius.astype(np.float64)
+    mass = sys.params.rho * (4.0 / 3.0) * np.pi * r ** 3
+    inertia = 0.4 * mass * r * r
+    if sys.bonds is not None and sys.bonds.n:
+        _, flag = sys.bond_state()
+        keep = flag == 1
+        bonds = np.stack([sys.bonds.bi[keep], sys.bonds.bj[keep]], 1)
+    else:
+        bonds = np.zeros((0, 2), np.int64)
+    lo = R.min(0) if box_lo is None else box_lo
+    hi = R.max(0) if box_hi is None else box_hi
+    write_lammps(path, R, bonds, lo, hi, sys.params.dt, V, W, mass, inertia)
 
 
 @dataclass

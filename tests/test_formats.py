@@ -137,3 +137,17 @@ def test_restore_rejects_other_sizes(tmp_path):
     ck = formats.checkpoint_from_layout(OracleLayout(c.R, c.K, c.grid()))
     with pytest.raises(ValueError):
         formats.restore_layout(OracleLayout(c.R[: 3 * 10], c.K, c.grid()), ck)
+
+
+def test_particles_csv(tmp_path):
+    """Metres about the sample centre; the real bounds of the read set."""
+    p = tmp_path / "particles.csv"
+    p.write_text("0.001 -0.002 0.0005\n-0.003,0.004,0.0\n0.0 0.0 0.01\n9 9 9\n")
+    R, box = formats.read_particles_csv(p, 3, center=(10.0, 20.0, 30.0))
+    np.testing.assert_allclose(R, [[11, 18, 30.5], [7, 24, 30], [10, 20, 40]], rtol=1e-6)
+    np.testing.assert_allclose(box["A_real"], [7, 18, 30], rtol=1e-6)
+    np.testing.assert_allclose(box["B_real"], [11, 24, 40], rtol=1e-6)
+    np.testing.assert_allclose(box["size_real"], [4, 6, 10], rtol=1e-6)
+    np.testing.assert_allclose(box["center_real"], [9, 21, 35], rtol=1e-6)
+    with pytest.raises(ValueError):
+        formats.read_particles_csv(p, 5)
