@@ -192,7 +192,10 @@ def bench_native(args) -> dict:
     kern_ms /= max(launches, 1)
     active, broken, removed = s.bond_counts()
     assert ctx.flags() == 0, "device flags raised during the benchmark"
-    alg_bytes = 128 * Nb + (24 + 48) * N
+    kernel = s.force_kernel()
+    # 128 B per bond, 72 B per particle; the fused particle update adds 1/m,
+    # 1/I (8 B) read and the next R, V, W (48 B) written per particle
+    alg_bytes = 128 * Nb + (72 + (56 if kernel.endswith("<fused>") else 0)) * N
     achieved = alg_bytes / (kern_ms * 1e-3)
     return {
         "value": active * args.steps / (t1 - t0),
@@ -206,7 +209,7 @@ def bench_native(args) -> dict:
                    "contact_rebuild_every": 20},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": None,
-                     "kernel": s.force_kernel(), "kernel_ms": kern_ms,
+                     "kernel": kernel, "kernel_ms": kern_ms,
                      "alg_bytes_per_launch": alg_bytes},
     }
 

@@ -360,6 +360,17 @@ int rfd_system_set_tiles(rfd_system *sys, int force_tile, int nbr_block, int sin
 /* the shapes in use; single_eval reports whether the single-evaluation force
  * pass runs (physical mode, FP32 path, tile halves within its LDS) */
 int rfd_system_get_tiles(rfd_system *sys, int *force_tile, int *nbr_block, int *single_eval);
+
+/* ---- fused explicit steps ----
+ * Replaces the separate d_CalculateIncrementsDEM / platen launches after the
+ * force kernels (md_calculationuniaxialcompression.cu:173-184): with one
+ * specimen in the FP32 physical mode and the single-evaluation force pass,
+ * rfd_system_step runs the platens, integration and cut box of each particle
+ * inside the force pass, pairs of steps alternating between two particle
+ * buffers.  On by default; the results are the same bits with it off.
+ * rfd_system_get_fusion reports whether steps currently run fused. */
+int rfd_system_set_fusion(rfd_system *sys, int on);
+int rfd_system_get_fusion(rfd_system *sys, int *fused);
 int rfd_tune_cache_save(const char *path);
 int rfd_tune_cache_load(const char *path);
 /* ---- quasi-static load steps: axial strain increment + FIRE relaxation ----

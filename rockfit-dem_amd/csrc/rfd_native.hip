@@ -510,6 +510,142 @@ struct WorkDev {
   const uint32_t *n_in;  // in-tile records per tile (they come first)
 };
 
+// 1/sqrt for the platen friction: correctly rounded in FP32, plain in FP64
+__device__ __forceinline__ float platen_rsqrt(float x) { return rsqrt_rn(x); }
+__device__ __forceinline__ double platen_rsqrt(double x) { return 1.0 / __builtin_sqrt(x); }
+
+// Platen reactions of the FP32 explicit steps are summed in fixed point:
+// each particle's reaction is rounded to a multiple of 2^-40 force units
+// (1e3 N x 2^-40, about 1e-9 N) and the sums are 64-bit integers, so that
+// they do not depend on the order of summation (tile shape, fused or separate
+// pass, workgroup schedule) and can be gathered with integer atomics.  Sums
+// up to 2^22 force units (4e9 N) are represented exactly.
+constexpr double kFxScale = 0x1p40, kFxInv = 0x1p-40;
+__device__ __forceinline__ long long to_fx(double f) {
+  const double x = fmin(fmax(f * kFxScale, -0x1p62), 0x1p62);  // NaN goes to -2^62
+  return (long long)__builtin_rint(x);
+}
+__device__ __forceinline__ void acc_add(double &a, double f) { a += f; }
+__device__ __forceinline__ void acc_add(long long &a, double f) { a += to_fx(f); }
+
+// wave-wide integer sum, then one atomic per wave that has something to add
+__device__ __forceinline__ void fx_wave_add(unsigned long long *acc, long long qb,
+                                            long long qt) {
+  for (int off = 32; off > 0; off >>= 1) {
+    qb += __shfl_xor(qb, off, 64);
+    qt += __shfl_xor(qt, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (qb) atomicAdd(acc, (unsigned long long)qb);
+    if (qt) atomicAdd(acc + 1, (unsigned long long)qt);
+  }
+}
+
+// platen clamp + reaction sums (md_uniaxialcompression3_simple.cu:19-76,
+// md_braziltest_simple2.cu:19-92).  MODE 1: load axis z, 2: load axis y.
+// VT is the velocity/force vector type (FP32 v3 or FP64 d3); AT the
+// accumulator (double, or fixed-point long long).
+template <int MODE, class VT, class AT>
+__device__ __forceinline__ void platen(const rfd_loading &L, P4 &R, VT &V, VT &F, AT &sb,
+                                       AT &stp) {
+  using T = decltype(V.x);
+  double ra, rb, h, zc;
+  if (MODE == 1) {
+    ra = R.x - L.cx;
+    rb = R.y - L.cy;
+    h = R.z;
+    zc = R.z;
+  } else {
+    ra = R.x - L.cx;
+    rb = R.z - L.cz;
+    h = R.y;
+    zc = R.z - L.cz;
+  }
+  if (!(ra * ra + rb * rb < L.RR && zc < L.zcut)) return;
+  T &Fa = MODE == 1 ? F.z : F.y;
+  T &Va = MODE == 1 ? V.z : V.y;
+  double &Ra = MODE == 1 ? R.z : R.y;
+  const T fr = (T)0.45f;
+  if (h - L.top_r > -1e-9) {
+    const T f = Fa;
+    if (f > 0) {
+      acc_add(stp, f);
+      if (MODE == 2) {
+        T vv = V.x * V.x + V.z * V.z;
+        if ((double)vv > 1e-9) {
+          vv = platen_rsqrt(vv);
+          F.x = F.x - fr * f * V.x * vv;
+          F.z = F.z - fr * f * V.z * vv;
+        }
+      }
+    }
+    if (h - L.top > -1e-9) {
+      Ra = L.top;
+      Va = (T)L.v;
+      Fa = (T)0;
+    }
+  }
+  if (h - L.bottom_r < 1e-9) {
+    const T f = Fa;
+    if (f < 0) {
+      acc_add(sb, f);
+      if (MODE == 2) {
+        T vv = V.x * V.x + V.z * V.z;
+        if ((double)vv > 1e-9) {
+          vv = platen_rsqrt(vv);
+          F.x = F.x + fr * f * V.x * vv;
+          F.z = F.z + fr * f * V.z * vv;
+        }
+      }
+    }
+    if (h - L.bottom < 1e-9) {
+      Ra = L.bottom;
+      Va = (T)0;
+      Fa = (T)0;
+    }
+  }
+}
+
+// platens, symplectic Euler with FP64 positions (md_calculateincrementsdem.cu:19-49),
+// cut box (md_parallelepipedrestriction.cu:73-158) for one particle held in
+// registers: F and M are its summed loads, iv its (1/m, 1/I)
+template <int MODE>
+__device__ __forceinline__ void advance_regs(const rfd_loading &L, P4 &R, v3 &V, v3 &W, v3 &F,
+                                             const v3 &M, float2 iv, float dt, long long &qb,
+                                             long long &qt) {
+  if (MODE) platen<MODE>(L, R, V, F, qb, qt);
+  V = V + F * iv.x;
+  R.x += (double)V.x * dt;
+  R.y += (double)V.y * dt;
+  R.z += (double)V.z * dt;
+  W = W + M * iv.y;
+  if (L.cut) {
+    const double dx = R.x - L.box_c[0], dy = R.y - L.box_c[1], dz = R.z - L.box_c[2];
+    if (dx - L.box_h[0] > 0 || dx + L.box_h[0] < 0 || dy - L.box_h[1] > 0 ||
+        dy + L.box_h[1] < 0 || dz - L.box_h[2] > 0 || dz + L.box_h[2] < 0) {
+      R.x = L.hidden[0];
+      R.y = L.hidden[1];
+      R.z = L.hidden[2];
+      V = kZero;
+      F = kZero;
+    }
+  }
+}
+
+// The explicit step's particle update fused into the force pass: each tile
+// advances its own particles once their loads are summed, writing positions,
+// velocities and spins to the other buffers (other tiles still read the
+// current ones), with the platen reactions summed in fixed point.
+struct AdvDev {
+  P4 *P;
+  float4 *V, *W;
+  const float2 *inv;
+  const rfd_loading *ctl;
+  unsigned long long *acc;
+  float dt;
+  int mode;  // loading mode: 0 none, 1 uniaxial (z), 2 Brazilian (y)
+};
+
 // Force pass of the physical mode with every bond whose ends share a tile
 // evaluated once: its lower-index half computes the law with the partner from
 // LDS and writes both halves' loads; bonds leaving the tile are evaluated by
@@ -517,9 +653,10 @@ struct WorkDev {
 // precomputed records, so no lane waits on a load another lane needs.
 // Per-particle sums in row order, then the contact rows: the same bits as
 // k_nat_forces.
-template <int FP, int FT>
+template <int FP, int FT, bool ADV>
 __global__ void __launch_bounds__(FT) __attribute__((amdgpu_waves_per_eu(5, 8)))
-k_nat_forces1(SysDev s, Mat mt, WorkDev w, uint32_t ntiles, uint32_t per, uint32_t cap) {
+k_nat_forces1(SysDev s, Mat mt, WorkDev w, uint32_t ntiles, uint32_t per, uint32_t cap,
+              AdvDev a) {
   static_assert(FP < FT && FP <= 128, "tile shape");
   static_assert(2 * FP <= FT, "one sum task per thread");
   // per-half loads: F (3 x cap floats) then M (3 x cap), cap >= 2 FP
@@ -636,40 +773,68 @@ k_nat_forces1(SysDev s, Mat mt, WorkDev w, uint32_t ntiles, uint32_t per, uint32
     sAcc[tg][2][tp] = z;
   }
   __syncthreads();
-  if (tid >= np) return;
-  // 4. never-bonded contact rows of this particle (per-end spring, in place)
-  v3 F = v3{sAcc[0][0][tid], sAcc[0][1][tid], sAcc[0][2][tid]};
-  const v3 Mb = v3{sAcc[1][0][tid], sAcc[1][1][tid], sAcc[1][2][tid]};
-  v3 Mc = kZero;
+  // the fused update keeps whole waves for its wave-wide sums
+  if (tid >= (ADV ? (FP + 63u) & ~63u : np)) return;
+  const bool act = tid < np;
   const uint32_t i = p0 + tid, N = s.N;
+  v3 F = kZero, M = kZero;
   EndData me;
-  me.P = sP[tid];
-  me.V = xyz(sV[tid]);
-  me.W = xyz(sW[tid]);
-  const uint32_t nc = s.ccnt[i];
-  for (uint32_t k = 0; k < nc; ++k) {
-    const size_t sl = (size_t)k * N + i;
-    const uint32_t j = s.cj[sl];
-    const EndData other = load_end(s, j);
-    const double rx = other.P.x - me.P.x, ry = other.P.y - me.P.y, rz = other.P.z - me.P.z;
-    const double L64 = __builtin_sqrt(rx * rx + ry * ry + rz * rz);
-    const float ri = (float)me.P.r, rj = (float)other.P.r;
-    if ((float)L64 < ri + rj && L64 > 1e-5) {
-      const double inv = 1.0 / L64;
-      const v3 n = {(float)(rx * inv), (float)(ry * inv), (float)(rz * inv)};
-      v3 O = xyz(s.cO[sl]);
-      v3 Fi, Mi, Mj;
-      contact_law(mt, n, (float)L64, ri, rj, ri + rj, me.V, other.V, me.W, other.W, O, Fi, Mi,
-                  Mj);
-      F = F + Fi;
-      Mc = Mc + Mi;
-      if (!s.trial) s.cO[sl] = f4(O);
-    } else if (!s.trial) {
-      s.cO[sl] = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (act) {
+    // 4. never-bonded contact rows of this particle (per-end spring, in place)
+    F = v3{sAcc[0][0][tid], sAcc[0][1][tid], sAcc[0][2][tid]};
+    const v3 Mb = v3{sAcc[1][0][tid], sAcc[1][1][tid], sAcc[1][2][tid]};
+    v3 Mc = kZero;
+    me.P = sP[tid];
+    me.V = xyz(sV[tid]);
+    me.W = xyz(sW[tid]);
+    const uint32_t nc = s.ccnt[i];
+    for (uint32_t k = 0; k < nc; ++k) {
+      const size_t sl = (size_t)k * N + i;
+      const uint32_t j = s.cj[sl];
+      const EndData other = load_end(s, j);
+      const double rx = other.P.x - me.P.x, ry = other.P.y - me.P.y, rz = other.P.z - me.P.z;
+      const double L64 = __builtin_sqrt(rx * rx + ry * ry + rz * rz);
+      const float ri = (float)me.P.r, rj = (float)other.P.r;
+      if ((float)L64 < ri + rj && L64 > 1e-5) {
+        const double inv = 1.0 / L64;
+        const v3 n = {(float)(rx * inv), (float)(ry * inv), (float)(rz * inv)};
+        v3 O = xyz(s.cO[sl]);
+        v3 Fi, Mi, Mj;
+        contact_law(mt, n, (float)L64, ri, rj, ri + rj, me.V, other.V, me.W, other.W, O, Fi, Mi,
+                    Mj);
+        F = F + Fi;
+        Mc = Mc + Mi;
+        if (!s.trial) s.cO[sl] = f4(O);
+      } else if (!s.trial) {
+        s.cO[sl] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
     }
+    M = Mb + Mc;
   }
-  s.F[i] = f4(F);
-  s.M[i] = f4(Mb + Mc);
+  if (!ADV) {
+    s.F[i] = f4(F);
+    s.M[i] = f4(M);
+    return;
+  }
+  // 5. platens, integration and cut box of this particle (the separate
+  //    particle pass's arithmetic, on the sums in registers)
+  long long qb = 0, qt = 0;
+  if (act) {
+    const rfd_loading L = *a.ctl;
+    const float2 iv = a.inv[i];
+    if (a.mode == 1)
+      advance_regs<1>(L, me.P, me.V, me.W, F, M, iv, a.dt, qb, qt);
+    else if (a.mode == 2)
+      advance_regs<2>(L, me.P, me.V, me.W, F, M, iv, a.dt, qb, qt);
+    else
+      advance_regs<0>(L, me.P, me.V, me.W, F, M, iv, a.dt, qb, qt);
+    a.P[i] = me.P;
+    a.V[i] = f4(me.V);
+    a.W[i] = f4(me.W);
+    s.F[i] = f4(F);
+    s.M[i] = f4(M);
+  }
+  if (a.mode) fx_wave_add(a.acc, qb, qt);
 }
 
 // work records of the single-evaluation pass: per tile, count the in-tile
@@ -785,109 +950,23 @@ struct PassDev {
   const float2 *inv;
   float dt;
   rfd_loading *ctl;
-  double *part;
+  unsigned long long *acc;  // fixed-point platen sums (bottom, top), per specimen
 };
 
-// 1/sqrt for the platen friction: correctly rounded in FP32, plain in FP64
-__device__ __forceinline__ float platen_rsqrt(float x) { return rsqrt_rn(x); }
-__device__ __forceinline__ double platen_rsqrt(double x) { return 1.0 / __builtin_sqrt(x); }
-
-// platen clamp + reaction sums (md_uniaxialcompression3_simple.cu:19-76,
-// md_braziltest_simple2.cu:19-92).  MODE 1: load axis z, 2: load axis y.
-// VT is the velocity/force vector type (FP32 v3 or FP64 d3).
-template <int MODE, class VT>
-__device__ __forceinline__ void platen(const rfd_loading &L, P4 &R, VT &V, VT &F, double &sb,
-                                       double &stp) {
-  using T = decltype(V.x);
-  double ra, rb, h, zc;
-  if (MODE == 1) {
-    ra = R.x - L.cx;
-    rb = R.y - L.cy;
-    h = R.z;
-    zc = R.z;
-  } else {
-    ra = R.x - L.cx;
-    rb = R.z - L.cz;
-    h = R.y;
-    zc = R.z - L.cz;
-  }
-  if (!(ra * ra + rb * rb < L.RR && zc < L.zcut)) return;
-  T &Fa = MODE == 1 ? F.z : F.y;
-  T &Va = MODE == 1 ? V.z : V.y;
-  double &Ra = MODE == 1 ? R.z : R.y;
-  const T fr = (T)0.45f;
-  if (h - L.top_r > -1e-9) {
-    const T f = Fa;
-    if (f > 0) {
-      stp += f;
-      if (MODE == 2) {
-        T vv = V.x * V.x + V.z * V.z;
-        if ((double)vv > 1e-9) {
-          vv = platen_rsqrt(vv);
-          F.x = F.x - fr * f * V.x * vv;
-          F.z = F.z - fr * f * V.z * vv;
-        }
-      }
-    }
-    if (h - L.top > -1e-9) {
-      Ra = L.top;
-      Va = (T)L.v;
-      Fa = (T)0;
-    }
-  }
-  if (h - L.bottom_r < 1e-9) {
-    const T f = Fa;
-    if (f < 0) {
-      sb += f;
-      if (MODE == 2) {
-        T vv = V.x * V.x + V.z * V.z;
-        if ((double)vv > 1e-9) {
-          vv = platen_rsqrt(vv);
-          F.x = F.x + fr * f * V.x * vv;
-          F.z = F.z + fr * f * V.z * vv;
-        }
-      }
-    }
-    if (h - L.bottom < 1e-9) {
-      Ra = L.bottom;
-      Va = (T)0;
-      Fa = (T)0;
-    }
-  }
-}
-
-// platens, symplectic Euler with FP64 positions (md_calculateincrementsdem.cu:19-49),
-// cut box (md_parallelepipedrestriction.cu:73-158) for particle i
 template <int MODE>
 __device__ __forceinline__ void advance_one(const PassDev &p, const rfd_loading &L, uint32_t i,
-                                            double &sb, double &stp) {
+                                            long long &qb, long long &qt) {
   P4 R = p.P[i];
-  v3 V = xyz(p.V[i]), F = xyz(p.F[i]);
-  if (MODE) platen<MODE>(L, R, V, F, sb, stp);
-  const float2 iv = p.inv[i];
-  V = V + F * iv.x;
-  R.x += (double)V.x * p.dt;
-  R.y += (double)V.y * p.dt;
-  R.z += (double)V.z * p.dt;
-  v3 W = xyz(p.W[i]) + xyz(p.M[i]) * iv.y;
-  if (L.cut) {
-    const double dx = R.x - L.box_c[0], dy = R.y - L.box_c[1], dz = R.z - L.box_c[2];
-    if (dx - L.box_h[0] > 0 || dx + L.box_h[0] < 0 || dy - L.box_h[1] > 0 ||
-        dy + L.box_h[1] < 0 || dz - L.box_h[2] > 0 || dz + L.box_h[2] < 0) {
-      R.x = L.hidden[0];
-      R.y = L.hidden[1];
-      R.z = L.hidden[2];
-      V = kZero;
-      F = kZero;
-    }
-  }
+  v3 V = xyz(p.V[i]), F = xyz(p.F[i]), W = xyz(p.W[i]);
+  advance_regs<MODE>(L, R, V, W, F, xyz(p.M[i]), p.inv[i], p.dt, qb, qt);
   p.P[i] = R;
   p.V[i] = f4(V);
   p.W[i] = f4(W);
   p.F[i] = f4(F);
 }
 
-// workgroup sum of the two platen reactions, written as this block's partial pair
+// workgroup sum of the two platen reactions, written as this block's partial
+// pair (FP64 path)
 __device__ __forceinline__ void write_partials(double *part, double sb, double stp) {
   __shared__ double s_part[8];
   sb = wave_sum(sb);
@@ -913,11 +992,10 @@ template <int MODE>
 __global__ void __launch_bounds__(256) k_nat_pass(PassDev p) {
   const rfd_loading L = *p.ctl;
   const uint32_t N = p.N;
-  double sb = 0.0, stp = 0.0;
+  long long qb = 0, qt = 0;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride())
-    advance_one<MODE>(p, L, i, sb, stp);
-  if (MODE == 0) return;
-  write_partials(p.part, sb, stp);
+    advance_one<MODE>(p, L, i, qb, qt);
+  if (MODE) fx_wave_add(p.acc, qb, qt);
 }
 
 // one step of the loading control from the reduced platen sums
@@ -973,8 +1051,8 @@ __device__ __forceinline__ void reduce_pairs(const double *part, int b0, int b1,
   }
 }
 
-// loading control: the platen partial sums are reduced by one workgroup in a
-// fixed order, then one thread applies the control
+// loading control of the FP64 path: the platen partial sums are reduced by
+// one workgroup in a fixed order, then one thread applies the control
 __global__ void __launch_bounds__(256) k_nat_control(rfd_loading *ctl, const double *part,
                                                      int nblocks, double *hist, double dt) {
   double s0, s1;
@@ -985,16 +1063,18 @@ __global__ void __launch_bounds__(256) k_nat_control(rfd_loading *ctl, const dou
   *ctl = L;
 }
 
-// batched specimens: workgroup s controls specimen s from the partials of its
-// own pass workgroups [spec_blk[s], spec_blk[s+1])
-__global__ void __launch_bounds__(256) k_nat_control_batch(rfd_loading *ctls, const double *part,
-                                                           const uint32_t *__restrict__ spec_blk,
-                                                           double *hist, uint32_t hist_cap,
-                                                           double dt) {
-  const uint32_t sp = blockIdx.x;
-  double s0, s1;
-  reduce_pairs(part, (int)spec_blk[sp], (int)spec_blk[sp + 1], s0, s1);
-  if (threadIdx.x != 0) return;
+// loading control of the FP32 steps, thread k for specimen k: the fixed-point
+// sums of the pass are read and cleared for the next step
+__global__ void __launch_bounds__(64) k_nat_control_fx(rfd_loading *ctls,
+                                                       unsigned long long *acc, uint32_t S,
+                                                       double *hist, uint32_t hist_cap,
+                                                       double dt) {
+  const uint32_t sp = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sp >= S) return;
+  const double s0 = (double)(long long)acc[2 * sp] * kFxInv;
+  const double s1 = (double)(long long)acc[2 * sp + 1] * kFxInv;
+  acc[2 * sp] = 0ull;
+  acc[2 * sp + 1] = 0ull;
   rfd_loading L = ctls[sp];
   control_update(L, s0, s1, hist + 3 * (size_t)hist_cap * sp, hist_cap, dt);
   ctls[sp] = L;
@@ -1008,12 +1088,13 @@ __global__ void __launch_bounds__(256) k_nat_pass_batch(PassDev p,
                                                         const uint32_t *__restrict__ blk_first,
                                                         const uint32_t *__restrict__ blk_end,
                                                         const rfd_loading *__restrict__ ctls) {
-  const rfd_loading L = ctls[blk_spec[blockIdx.x]];
+  const uint32_t sp = blk_spec[blockIdx.x];
+  const rfd_loading L = ctls[sp];
   const uint32_t e = blk_end[blockIdx.x];
-  double sb = 0.0, stp = 0.0;
+  long long qb = 0, qt = 0;
   for (uint32_t i = blk_first[blockIdx.x] + threadIdx.x; i < e; i += blockDim.x)
-    advance_one<MODE>(p, L, i, sb, stp);
-  write_partials(p.part, sb, stp);
+    advance_one<MODE>(p, L, i, qb, qt);
+  fx_wave_add(p.acc + 2 * sp, qb, qt);
 }
 
 // ----------------------------------------------------- neighbour search
@@ -2150,7 +2231,6 @@ struct Pass64 {
 
 template <int MODE>
 __global__ void __launch_bounds__(256) k_f64_pass(Pass64 p) {
-  __shared__ double s_part[8];
   const rfd_loading L = *p.ctl;
   double sb = 0.0, stp = 0.0;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.N; i += grid_stride()) {
@@ -2180,23 +2260,7 @@ __global__ void __launch_bounds__(256) k_f64_pass(Pass64 p) {
     p.F[i] = d4(F);
   }
   if (MODE == 0) return;
-  sb = wave_sum(sb);
-  stp = wave_sum(stp);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) {
-    s_part[2 * wid] = sb;
-    s_part[2 * wid + 1] = stp;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double a = 0, c = 0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
-      a += s_part[2 * w];
-      c += s_part[2 * w + 1];
-    }
-    p.part[2 * blockIdx.x] = a;
-    p.part[2 * blockIdx.x + 1] = c;
-  }
+  write_partials(p.part, sb, stp);
 }
 
 __global__ void k_f32_to_f64(const float *__restrict__ soa, uint32_t N,
@@ -2238,8 +2302,15 @@ struct rfd_system {
   double4 *cO64[2] = {nullptr, nullptr};
   uint8_t *fl8 = nullptr;    // per-bond flag, kept beside the state (contact search)
   rfd_loading *ctl = nullptr;
-  double *part = nullptr;
+  double *part = nullptr;               // FP64 path: per-workgroup platen partials
+  unsigned long long *acc = nullptr;    // FP32 path: fixed-point platen sums
   double *hist = nullptr;
+  // explicit steps with the particle update fused into the force pass write
+  // positions, velocities and spins to these and swap them with P, V, W; the
+  // steps run in pairs, so that P, V, W hold the state between calls
+  P4 *P2 = nullptr;
+  float4 *V2 = nullptr, *W2 = nullptr;
+  bool fuse = true;
   uint32_t nblocks_pass = 0;
   uint64_t step_count = 0;
   int mode = 0;  // host copy of the loading mode (selects the pass kernel)
@@ -2276,9 +2347,8 @@ struct rfd_system {
   double *hist_b = nullptr;           // S x hist_cap x 3
   uint32_t hist_cap_b = 0;
   uint32_t *blk_spec = nullptr, *blk_first = nullptr, *blk_end = nullptr;  // pass work list
-  uint32_t *spec_blk = nullptr;       // S + 1 offsets into the work list
   uint32_t nblk_b = 0;
-  double *part_b = nullptr;           // per-workgroup platen partials of the batch
+  unsigned long long *acc_b = nullptr;  // S fixed-point platen sum pairs
   // step graphs: 2 x rebuild_every steps captured once, replayed
   uint64_t gen = 1;                   // bumped by every change a captured step depends on
   hipStream_t gstream = nullptr;
@@ -2503,7 +2573,7 @@ int forces64(rfd_system *s) {
 // (the single-evaluation pass in the physical mode when every tile's halves
 // fit its LDS, else the chunked pass)
 template <int FP, int FT>
-void launch_forces(rfd_system *s, const SysDev &d, const Mat &m) {
+void launch_forces(rfd_system *s, const SysDev &d, const Mat &m, const AdvDev *adv) {
   const uint32_t nb = (s->N + FP - 1) / FP;
   const uint32_t per = (nb + kXcd - 1) / kXcd;
   if (!m.refmode && !s->once_off && s->work_fp == FP) {
@@ -2511,8 +2581,13 @@ void launch_forces(rfd_system *s, const SysDev &d, const Mat &m) {
     const int q = FP == 32 ? 0 : (FP == 64 ? 1 : 2);
     // slots for the system's largest tile, at least the per-particle sums
     const uint32_t cap = std::max((s->tile_halves[q] + 31u) & ~31u, 2u * FP);
-    k_nat_forces1<FP, FT><<<per * kXcd, FT, sizeof(float) * 6 * cap, s->ctx->stream>>>(
-        d, m, w, nb, per, cap);
+    const size_t lds = sizeof(float) * 6 * cap;
+    if (adv)
+      k_nat_forces1<FP, FT, true><<<per * kXcd, FT, lds, s->ctx->stream>>>(d, m, w, nb, per,
+                                                                            cap, *adv);
+    else
+      k_nat_forces1<FP, FT, false><<<per * kXcd, FT, lds, s->ctx->stream>>>(d, m, w, nb, per,
+                                                                             cap, AdvDev{});
   } else {
     k_nat_forces<FP, FT><<<per * kXcd, FT, 0, s->ctx->stream>>>(d, m, nb, per);
   }
@@ -2561,7 +2636,7 @@ int ensure_work(rfd_system *s) {
 }
 
 int forces(rfd_system *s, const float *dtp = nullptr, bool trial = false,
-           bool nobreak = false) {
+           bool nobreak = false, const AdvDev *adv = nullptr) {
   if (s->fp64) return forces64(s);
   SysDev d = make_dev(s);
   d.dtp = dtp;
@@ -2584,16 +2659,24 @@ int forces(rfd_system *s, const float *dtp = nullptr, bool trial = false,
     RFD_HIP(s->ctx, hipEventRecord(e0, s->ctx->stream));
   }
   if (s->force_fp == 32)
-    launch_forces<32, 128>(s, d, m);
+    launch_forces<32, 128>(s, d, m, adv);
   else if (s->force_fp == 128)
-    launch_forces<128, 256>(s, d, m);
+    launch_forces<128, 256>(s, d, m, adv);
   else
-    launch_forces<kFP, kFT>(s, d, m);
+    launch_forces<kFP, kFT>(s, d, m, adv);
   RFD_HIP(s->ctx, hipGetLastError());
   // a trial pass leaves the state index alone: its output buffer is dropped
   if (!trial) k_flip_state<<<1, 64, 0, s->ctx->stream>>>(s->d_cur, d.done);
   RFD_HIP(s->ctx, hipGetLastError());
   if (e1) RFD_HIP(s->ctx, hipEventRecord(e1, s->ctx->stream));
+  return RFD_OK;
+}
+
+// loading control of a single specimen's FP32 step
+int control(rfd_system *s) {
+  k_nat_control_fx<<<1, 64, 0, s->ctx->stream>>>(s->ctl, s->acc, 1u, s->hist,
+                                                 (uint32_t)kHistCap, s->p.dt);
+  RFD_HIP(s->ctx, hipGetLastError());
   return RFD_OK;
 }
 
@@ -2631,12 +2714,12 @@ int pass_and_control(rfd_system *s) {
   p.inv = s->inv;
   p.dt = (float)s->p.dt;
   p.ctl = s->ctl;
-  p.part = s->part;
+  p.acc = s->acc;
   hipStream_t st = s->ctx->stream;
   if (s->S) {
     // batched specimens: one launch over every specimen's tiles, then one
-    // control workgroup per specimen
-    p.part = s->part_b;
+    // control thread per specimen
+    p.acc = s->acc_b;
     p.ctl = nullptr;
     if (s->mode == 1)
       k_nat_pass_batch<1><<<s->nblk_b, 256, 0, st>>>(p, s->blk_spec, s->blk_first, s->blk_end,
@@ -2644,8 +2727,8 @@ int pass_and_control(rfd_system *s) {
     else
       k_nat_pass_batch<2><<<s->nblk_b, 256, 0, st>>>(p, s->blk_spec, s->blk_first, s->blk_end,
                                                      s->ctls);
-    k_nat_control_batch<<<s->S, 256, 0, st>>>(s->ctls, s->part_b, s->spec_blk, s->hist_b,
-                                              s->hist_cap_b, s->p.dt);
+    k_nat_control_fx<<<(s->S + 63) / 64, 64, 0, st>>>(s->ctls, s->acc_b, s->S, s->hist_b,
+                                                      s->hist_cap_b, s->p.dt);
     RFD_HIP(s->ctx, hipGetLastError());
     return RFD_OK;
   }
@@ -2655,12 +2738,67 @@ int pass_and_control(rfd_system *s) {
     k_nat_pass<2><<<s->nblocks_pass, 256, 0, st>>>(p);
   else
     k_nat_pass<0><<<s->nblocks_pass, 256, 0, st>>>(p);
-  k_nat_control<<<1, 256, 0, st>>>(s->ctl, s->part, (int)s->nblocks_pass, s->hist, s->p.dt);
-  RFD_HIP(s->ctx, hipGetLastError());
-  return RFD_OK;
+  return control(s);
 }
 
 bool valid(rfd_system *s) { return s && s->ctx; }
+
+// explicit steps fuse the particle update into the force pass for one
+// specimen in the FP32 physical mode, when the single-evaluation pass serves
+// the current tile shape
+bool can_fuse(const rfd_system *s) {
+  return s->fuse && !s->fp64 && !s->S && !s->p.refmode && !s->once_off && s->P2 &&
+         s->work_fp == s->force_fp;
+}
+
+int step_plain(rfd_system *s) {
+  if (!s->rows_valid || s->step_count % s->p.rebuild_every == 0) {
+    int st = rebuild_rows(s);
+    if (st) return st;
+  }
+  int st = forces(s);
+  if (!st) st = pass_and_control(s);
+  if (st) return st;
+  s->step_count++;
+  return RFD_OK;
+}
+
+// one fused step: reads P, V, W, writes P2, V2, W2 and swaps the two sets
+// (also after an error, so that a pair of calls always restores them)
+int step_fused(rfd_system *s) {
+  int st = RFD_OK;
+  if (!s->rows_valid || s->step_count % s->p.rebuild_every == 0) st = rebuild_rows(s);
+  if (!st) {
+    const AdvDev a{s->P2, s->V2, s->W2, s->inv, s->ctl, s->acc, (float)s->p.dt, s->mode};
+    st = forces(s, nullptr, false, false, &a);
+  }
+  if (!st) st = control(s);
+  std::swap(s->P, s->P2);
+  std::swap(s->V, s->V2);
+  std::swap(s->W, s->W2);
+  if (!st) s->step_count++;
+  return st;
+}
+
+// n explicit steps; fused ones run in pairs, so that P, V, W hold the state
+// when this returns (the particle update is the same arithmetic either way,
+// and the two orders of steps give the same bits)
+int run_steps(rfd_system *s, uint32_t n) {
+  uint32_t k = 0;
+  if (can_fuse(s)) {
+    for (; k + 2 <= n; k += 2) {
+      const int st = step_fused(s);
+      const int st2 = step_fused(s);
+      if (st) return st;
+      if (st2) return st2;
+    }
+  }
+  for (; k < n; ++k) {
+    const int st = step_plain(s);
+    if (st) return st;
+  }
+  return RFD_OK;
+}
 
 int fire_alloc(rfd_system *s) {
   if (s->fire) return RFD_OK;
@@ -2866,10 +3004,17 @@ int rfd_system_create(rfd_ctx *ctx, const rfd_system_params *p, uint32_t N, rfd_
   }
   s->nblocks_pass = rfd_blocks(N, 256, 2048);
   if (!st) st = dalloc(ctx, &s->part, 2 * (size_t)s->nblocks_pass);
+  if (!st) st = dalloc(ctx, &s->acc, 2);
   if (!st) st = dalloc(ctx, &s->hist, 3 * (size_t)kHistCap);
+  if (!st && !s->fp64) {
+    st = dalloc(ctx, &s->P2, N);
+    if (!st) st = dalloc(ctx, &s->V2, N);
+    if (!st) st = dalloc(ctx, &s->W2, N);
+  }
   if (!st) {
     rfd_loading L{};
     hipError_t e = hipMemcpy(s->ctl, &L, sizeof(L), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(s->acc, 0, 2 * sizeof(unsigned long long));
     for (int q = 0; q < 2 && e == hipSuccess; ++q)
       e = hipMemset(s->ccnt[q], 0, sizeof(uint32_t) * N);
     if (e != hipSuccess) st = rfd_hip_status(ctx, e, "rfd_system_create");
@@ -2922,6 +3067,10 @@ int rfd_system_destroy(rfd_system *s) {
     dfree(s->cO64[q]);
   }
   dfree(s->part);
+  dfree(s->acc);
+  dfree(s->P2);
+  dfree(s->V2);
+  dfree(s->W2);
   dfree(s->hist);
   dfree(s->scan_tot);
   dfree(s->scan_off);
@@ -2943,8 +3092,7 @@ int rfd_system_destroy(rfd_system *s) {
   dfree(s->blk_spec);
   dfree(s->blk_first);
   dfree(s->blk_end);
-  dfree(s->spec_blk);
-  dfree(s->part_b);
+  dfree(s->acc_b);
   if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
   for (hipEvent_t e : s->gev)
     if (e) (void)hipEventDestroy(e);
@@ -3394,16 +3542,19 @@ int rfd_system_step(rfd_system *s, uint32_t nsteps) {
   if (!valid(s)) return RFD_E_ARG;
   rfd_ctx *ctx = s->ctx;
   RFD_CHECK_ARG(ctx, s->have_bonds, "rfd_system_step: set bonds first");
-  for (uint32_t k = 0; k < nsteps; ++k) {
-    if (!s->rows_valid || s->step_count % s->p.rebuild_every == 0) {
-      int st = rebuild_rows(s);
-      if (st) return st;
-    }
-    int st = forces(s);
-    if (!st) st = pass_and_control(s);
-    if (st) return st;
-    s->step_count++;
-  }
+  return run_steps(s, nsteps);
+}
+
+int rfd_system_set_fusion(rfd_system *s, int on) {
+  if (!valid(s)) return RFD_E_ARG;
+  s->fuse = on != 0;
+  s->gen++;
+  return RFD_OK;
+}
+
+int rfd_system_get_fusion(rfd_system *s, int *fused) {
+  if (!valid(s) || !fused) return RFD_E_ARG;
+  *fused = can_fuse(s) ? 1 : 0;
   return RFD_OK;
 }
 
@@ -3425,27 +3576,25 @@ int rfd_system_set_specimens(rfd_system *s, uint32_t S, const uint32_t *first,
                 "rfd_system_set_specimens: first[0] must be 0 and first[S] = N");
   const int mode = L[0].mode;
   RFD_CHECK_ARG(ctx, mode == 1 || mode == 2, "rfd_system_set_specimens: loading mode must be 1 or 2");
-  std::vector<uint32_t> bs, bf, be, sb(S + 1);
+  std::vector<uint32_t> bs, bf, be;
   for (uint32_t k = 0; k < S; ++k) {
     RFD_CHECK_ARG(ctx, first[k + 1] > first[k], "rfd_system_set_specimens: specimen %u is empty", k);
     RFD_CHECK_ARG(ctx, L[k].mode == mode, "rfd_system_set_specimens: specimens mix loading modes");
-    sb[k] = (uint32_t)bs.size();
     for (uint32_t a = first[k]; a < first[k + 1]; a += 256) {
       bs.push_back(k);
       bf.push_back(a);
       be.push_back(std::min(a + 256u, first[k + 1]));
     }
   }
-  sb[S] = (uint32_t)bs.size();
   const uint32_t nblk = (uint32_t)bs.size();
   int st = dalloc(ctx, &s->ctls, S);
   if (!st) st = dalloc(ctx, &s->hist_b, 3 * (size_t)hist_cap * S);
   if (!st) st = dalloc(ctx, &s->blk_spec, nblk);
   if (!st) st = dalloc(ctx, &s->blk_first, nblk);
   if (!st) st = dalloc(ctx, &s->blk_end, nblk);
-  if (!st) st = dalloc(ctx, &s->spec_blk, (size_t)S + 1);
-  if (!st) st = dalloc(ctx, &s->part_b, 2 * (size_t)nblk);
+  if (!st) st = dalloc(ctx, &s->acc_b, 2 * (size_t)S);
   if (st) return st;
+  RFD_HIP(ctx, hipMemset(s->acc_b, 0, sizeof(unsigned long long) * 2 * S));
   std::vector<rfd_loading> Lh(L, L + S);
   for (auto &x : Lh) {
     x.steps = 0;
@@ -3455,7 +3604,6 @@ int rfd_system_set_specimens(rfd_system *s, uint32_t S, const uint32_t *first,
   RFD_HIP(ctx, hipMemcpy(s->blk_spec, bs.data(), sizeof(uint32_t) * nblk, hipMemcpyHostToDevice));
   RFD_HIP(ctx, hipMemcpy(s->blk_first, bf.data(), sizeof(uint32_t) * nblk, hipMemcpyHostToDevice));
   RFD_HIP(ctx, hipMemcpy(s->blk_end, be.data(), sizeof(uint32_t) * nblk, hipMemcpyHostToDevice));
-  RFD_HIP(ctx, hipMemcpy(s->spec_blk, sb.data(), sizeof(uint32_t) * (S + 1), hipMemcpyHostToDevice));
   s->S = S;
   s->spec_first.assign(first, first + S + 1);
   s->hist_cap_b = hist_cap;
@@ -3549,14 +3697,8 @@ int rfd_system_step_graph(rfd_system *s, uint32_t nsteps) {
     // contact-row buffers it reads and writes come back to where they started
     const bool whole = s->rows_valid && s->step_count % R == 0 && nsteps - k >= G && !s->timing;
     if (!whole) {
-      if (!s->rows_valid || s->step_count % R == 0) {
-        int st = rebuild_rows(s);
-        if (st) return st;
-      }
-      int st = forces(s);
-      if (!st) st = pass_and_control(s);
+      const int st = step_plain(s);
       if (st) return st;
-      s->step_count++;
       ++k;
       continue;
     }
@@ -3577,13 +3719,7 @@ int rfd_system_step_graph(rfd_system *s, uint32_t nsteps) {
       const int cc0 = s->ccur;
       RFD_HIP(ctx, hipStreamBeginCapture(s->gstream, hipStreamCaptureModeThreadLocal));
       ctx->stream = s->gstream;
-      int st = RFD_OK;
-      for (uint32_t q = 0; q < G && !st; ++q) {
-        if (s->step_count % R == 0) st = rebuild_rows(s);
-        if (!st) st = forces(s);
-        if (!st) st = pass_and_control(s);
-        s->step_count++;
-      }
+      const int st = run_steps(s, G);
       ctx->stream = home;
       hipGraph_t graph = nullptr;
       const hipError_t ec = hipStreamEndCapture(s->gstream, &graph);

@@ -134,6 +134,8 @@ NATIVE_SIGNATURES = {
     "rfd_system_autotune": [_vp, C.c_int, C.POINTER(TuneResult)],
     "rfd_system_set_tiles": [_vp, C.c_int, C.c_int, C.c_int],
     "rfd_system_get_tiles": [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "rfd_system_set_fusion": [_vp, C.c_int],
+    "rfd_system_get_fusion": [_vp, C.POINTER(C.c_int)],
     "rfd_tune_cache_save": [C.c_char_p],
     "rfd_system_pack": [_vp, C.POINTER(PackParams), C.POINTER(PackResult)],
     "rfd_tune_cache_load": [C.c_char_p],
@@ -439,10 +441,23 @@ class NativeSystem:
         self._call("rfd_system_get_tiles", self.h, C.byref(a), C.byref(b), C.byref(c))
         return a.value, b.value, bool(c.value)
 
+    def set_fusion(self, on: bool = True):
+        """Fuse the particle update into the force pass of explicit steps
+        (on by default; the same bits either way)."""
+        self._call("rfd_system_set_fusion", self.h, int(on))
+
+    def fused(self) -> bool:
+        """Whether explicit steps currently run the fused pass."""
+        f = C.c_int()
+        self._call("rfd_system_get_fusion", self.h, C.byref(f))
+        return bool(f.value)
+
     def force_kernel(self) -> str:
-        """Name of the force-pass kernel the next step launches."""
+        """Name of the force-pass kernel the next explicit step launches."""
         if self.params.fp64:
             return "k_f64_forces"
+        if self.fused():
+            return "k_nat_forces1<fused>"  # the particle update runs in the same launch
         return "k_nat_forces1" if self.tiles()[2] else "k_nat_forces"
 
     def autotune(self, use_cache: bool = True) -> TuneResult:
