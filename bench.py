@@ -78,6 +78,13 @@ def stream_copy(reps: int = 5) -> float:
     return 2 * COPY_BYTES * reps / (a.elapsed_time(b) * 1e-3)
 
 
+def force_alg_bytes(kernel: str, bonds: int, particles: int) -> int:
+    """Algorithmic bytes of one native force launch: 128 B per bond and 72 B
+    per particle; the fused particle update adds 1/m, 1/I (8 B) read and the
+    next R, V, W (48 B) written per particle."""
+    return 128 * bonds + (72 + (56 if kernel.endswith("<fused>") else 0)) * particles
+
+
 def load_traffic(kernel: str, particles: int, bonds: int):
     """HBM bytes per launch of `kernel` measured by the FETCH_SIZE / WRITE_SIZE
     counter runs recorded under profiles/ for this workload, or None."""
@@ -193,9 +200,7 @@ def bench_native(args) -> dict:
     active, broken, removed = s.bond_counts()
     assert ctx.flags() == 0, "device flags raised during the benchmark"
     kernel = s.force_kernel()
-    # 128 B per bond, 72 B per particle; the fused particle update adds 1/m,
-    # 1/I (8 B) read and the next R, V, W (48 B) written per particle
-    alg_bytes = 128 * Nb + (72 + (56 if kernel.endswith("<fused>") else 0)) * N
+    alg_bytes = force_alg_bytes(kernel, Nb, N)
     achieved = alg_bytes / (kern_ms * 1e-3)
     return {
         "value": active * args.steps / (t1 - t0),
@@ -344,7 +349,8 @@ def bench_packed(D: float = 100.0, H: float = 200.0, steps: int = 50, warmup: in
     kern_ms, launches = s.timing_read()
     kern_ms /= max(launches, 1)
     active, broken, removed = s.bond_counts()
-    alg = 128 * Nb + 72 * N
+    kernel = s.force_kernel()
+    alg = force_alg_bytes(kernel, Nb, N)
     out = {
         "workload": f"uniaxial cylinder D{D:g}xH{H:g} mm, seeded random packing phi=0.6, "
                     "r~U[0.4,0.6] mm, bonds at gap < 0.05 r_mean",
@@ -352,7 +358,7 @@ def bench_packed(D: float = 100.0, H: float = 200.0, steps: int = 50, warmup: in
         "pack": {"seconds": pack_s, **pk.result},
         "bond_updates_per_s": active * steps / (t2 - t1),
         "ms_per_step": (t2 - t1) * 1e3 / steps,
-        "force_kernel": s.force_kernel(), "force_ms": kern_ms,
+        "force_kernel": kernel, "force_ms": kern_ms,
         "force_roofline_frac": alg / (kern_ms * 1e-3) / PEAK_HBM,
     }
     assert ctx.flags() == 0, "device flags raised in the packed run"

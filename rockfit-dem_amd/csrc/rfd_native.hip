@@ -640,10 +640,14 @@ struct AdvDev {
   P4 *P;
   float4 *V, *W;
   const float2 *inv;
-  const rfd_loading *ctl;
-  unsigned long long *acc;
+  const rfd_loading *ctl;    // per specimen
+  unsigned long long *acc;   // per specimen: (bottom, top)
   float dt;
   int mode;  // loading mode: 0 none, 1 uniaxial (z), 2 Brazilian (y)
+  // batched specimens (null for one): the specimen of particle 32 k, and the
+  // specimens' first particles (S + 1); every specimen holds at least 128
+  // particles, so a tile meets at most two
+  const uint32_t *spec32, *spec_first;
 };
 
 // Force pass of the physical mode with every bond whose ends share a tile
@@ -819,8 +823,13 @@ k_nat_forces1(SysDev s, Mat mt, WorkDev w, uint32_t ntiles, uint32_t per, uint32
   // 5. platens, integration and cut box of this particle (the separate
   //    particle pass's arithmetic, on the sums in registers)
   long long qb = 0, qt = 0;
+  uint32_t sp0 = 0, sp = 0;
+  if (a.spec32) {
+    sp0 = a.spec32[p0 >> 5];
+    sp = sp0 + (i >= a.spec_first[sp0 + 1] ? 1u : 0u);
+  }
   if (act) {
-    const rfd_loading L = *a.ctl;
+    const rfd_loading L = a.ctl[sp];
     const float2 iv = a.inv[i];
     if (a.mode == 1)
       advance_regs<1>(L, me.P, me.V, me.W, F, M, iv, a.dt, qb, qt);
@@ -834,7 +843,17 @@ k_nat_forces1(SysDev s, Mat mt, WorkDev w, uint32_t ntiles, uint32_t per, uint32
     s.F[i] = f4(F);
     s.M[i] = f4(M);
   }
-  if (a.mode) fx_wave_add(a.acc, qb, qt);
+  if (a.mode) {
+    if (!a.spec32) {
+      fx_wave_add(a.acc, qb, qt);
+    } else {
+      // the tile's first specimen and the next one: the lanes' reactions
+      // split between the two (the second adds nothing when no lane has it)
+      const bool hi = sp != sp0;
+      fx_wave_add(a.acc + 2 * sp0, hi ? 0 : qb, hi ? 0 : qt);
+      fx_wave_add(a.acc + 2 * (sp0 + 1), hi ? qb : 0, hi ? qt : 0);
+    }
+  }
 }
 
 // work records of the single-evaluation pass: per tile, count the in-tile
@@ -1064,12 +1083,14 @@ __global__ void __launch_bounds__(256) k_nat_control(rfd_loading *ctl, const dou
 }
 
 // loading control of the FP32 steps, thread k for specimen k: the fixed-point
-// sums of the pass are read and cleared for the next step
+// sums of the pass are read and cleared for the next step.  curp, when set,
+// is the bond-state index that the step's force pass leaves to be flipped.
 __global__ void __launch_bounds__(64) k_nat_control_fx(rfd_loading *ctls,
                                                        unsigned long long *acc, uint32_t S,
                                                        double *hist, uint32_t hist_cap,
-                                                       double dt) {
+                                                       double dt, int *curp) {
   const uint32_t sp = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sp == 0 && curp) curp[0] = 1 - curp[0];
   if (sp >= S) return;
   const double s0 = (double)(long long)acc[2 * sp] * kFxInv;
   const double s1 = (double)(long long)acc[2 * sp + 1] * kFxInv;
@@ -2349,6 +2370,9 @@ struct rfd_system {
   uint32_t *blk_spec = nullptr, *blk_first = nullptr, *blk_end = nullptr;  // pass work list
   uint32_t nblk_b = 0;
   unsigned long long *acc_b = nullptr;  // S fixed-point platen sum pairs
+  uint32_t *spec32 = nullptr;           // specimen of particle 32 k (fused batch steps)
+  uint32_t *spec_first_d = nullptr;     // S + 1 first particles, on the device
+  bool batch_fusable = false;           // every specimen holds at least 128 particles
   // step graphs: 2 x rebuild_every steps captured once, replayed
   uint64_t gen = 1;                   // bumped by every change a captured step depends on
   hipStream_t gstream = nullptr;
@@ -2635,8 +2659,10 @@ int ensure_work(rfd_system *s) {
   return build_work_fp<64>(s);
 }
 
+// flip = false leaves the flip of the bond-state index to the step's
+// loading control (FP32 explicit steps)
 int forces(rfd_system *s, const float *dtp = nullptr, bool trial = false,
-           bool nobreak = false, const AdvDev *adv = nullptr) {
+           bool nobreak = false, const AdvDev *adv = nullptr, bool flip = true) {
   if (s->fp64) return forces64(s);
   SysDev d = make_dev(s);
   d.dtp = dtp;
@@ -2666,16 +2692,21 @@ int forces(rfd_system *s, const float *dtp = nullptr, bool trial = false,
     launch_forces<kFP, kFT>(s, d, m, adv);
   RFD_HIP(s->ctx, hipGetLastError());
   // a trial pass leaves the state index alone: its output buffer is dropped
-  if (!trial) k_flip_state<<<1, 64, 0, s->ctx->stream>>>(s->d_cur, d.done);
+  if (!trial && flip) k_flip_state<<<1, 64, 0, s->ctx->stream>>>(s->d_cur, d.done);
   RFD_HIP(s->ctx, hipGetLastError());
   if (e1) RFD_HIP(s->ctx, hipEventRecord(e1, s->ctx->stream));
   return RFD_OK;
 }
 
-// loading control of a single specimen's FP32 step
+// loading control of an FP32 step, for the one specimen or every batched
+// specimen (and the flip of the bond-state index the step's force pass left)
 int control(rfd_system *s) {
-  k_nat_control_fx<<<1, 64, 0, s->ctx->stream>>>(s->ctl, s->acc, 1u, s->hist,
-                                                 (uint32_t)kHistCap, s->p.dt);
+  if (s->S)
+    k_nat_control_fx<<<(s->S + 63) / 64, 64, 0, s->ctx->stream>>>(
+        s->ctls, s->acc_b, s->S, s->hist_b, s->hist_cap_b, s->p.dt, s->d_cur);
+  else
+    k_nat_control_fx<<<1, 64, 0, s->ctx->stream>>>(s->ctl, s->acc, 1u, s->hist,
+                                                   (uint32_t)kHistCap, s->p.dt, s->d_cur);
   RFD_HIP(s->ctx, hipGetLastError());
   return RFD_OK;
 }
@@ -2727,10 +2758,7 @@ int pass_and_control(rfd_system *s) {
     else
       k_nat_pass_batch<2><<<s->nblk_b, 256, 0, st>>>(p, s->blk_spec, s->blk_first, s->blk_end,
                                                      s->ctls);
-    k_nat_control_fx<<<(s->S + 63) / 64, 64, 0, st>>>(s->ctls, s->acc_b, s->S, s->hist_b,
-                                                      s->hist_cap_b, s->p.dt);
-    RFD_HIP(s->ctx, hipGetLastError());
-    return RFD_OK;
+    return control(s);
   }
   if (s->mode == 1)
     k_nat_pass<1><<<s->nblocks_pass, 256, 0, st>>>(p);
@@ -2747,8 +2775,8 @@ bool valid(rfd_system *s) { return s && s->ctx; }
 // specimen in the FP32 physical mode, when the single-evaluation pass serves
 // the current tile shape
 bool can_fuse(const rfd_system *s) {
-  return s->fuse && !s->fp64 && !s->S && !s->p.refmode && !s->once_off && s->P2 &&
-         s->work_fp == s->force_fp;
+  return s->fuse && !s->fp64 && (!s->S || s->batch_fusable) && !s->p.refmode &&
+         !s->once_off && s->P2 && s->work_fp == s->force_fp;
 }
 
 int step_plain(rfd_system *s) {
@@ -2756,7 +2784,8 @@ int step_plain(rfd_system *s) {
     int st = rebuild_rows(s);
     if (st) return st;
   }
-  int st = forces(s);
+  // the FP32 control flips the bond-state index (the FP64 pass flips it itself)
+  int st = forces(s, nullptr, false, false, nullptr, false);
   if (!st) st = pass_and_control(s);
   if (st) return st;
   s->step_count++;
@@ -2769,8 +2798,15 @@ int step_fused(rfd_system *s) {
   int st = RFD_OK;
   if (!s->rows_valid || s->step_count % s->p.rebuild_every == 0) st = rebuild_rows(s);
   if (!st) {
-    const AdvDev a{s->P2, s->V2, s->W2, s->inv, s->ctl, s->acc, (float)s->p.dt, s->mode};
-    st = forces(s, nullptr, false, false, &a);
+    AdvDev a{s->P2, s->V2, s->W2, s->inv, s->ctl, s->acc, (float)s->p.dt, s->mode,
+             nullptr, nullptr};
+    if (s->S) {
+      a.ctl = s->ctls;
+      a.acc = s->acc_b;
+      a.spec32 = s->spec32;
+      a.spec_first = s->spec_first_d;
+    }
+    st = forces(s, nullptr, false, false, &a, false);
   }
   if (!st) st = control(s);
   std::swap(s->P, s->P2);
@@ -3093,6 +3129,8 @@ int rfd_system_destroy(rfd_system *s) {
   dfree(s->blk_first);
   dfree(s->blk_end);
   dfree(s->acc_b);
+  dfree(s->spec32);
+  dfree(s->spec_first_d);
   if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
   for (hipEvent_t e : s->gev)
     if (e) (void)hipEventDestroy(e);
@@ -3593,8 +3631,25 @@ int rfd_system_set_specimens(rfd_system *s, uint32_t S, const uint32_t *first,
   if (!st) st = dalloc(ctx, &s->blk_first, nblk);
   if (!st) st = dalloc(ctx, &s->blk_end, nblk);
   if (!st) st = dalloc(ctx, &s->acc_b, 2 * (size_t)S);
+  const size_t n32 = ((size_t)s->N + 31) / 32;
+  if (!st) st = dalloc(ctx, &s->spec32, n32);
+  if (!st) st = dalloc(ctx, &s->spec_first_d, (size_t)S + 1);
   if (st) return st;
   RFD_HIP(ctx, hipMemset(s->acc_b, 0, sizeof(unsigned long long) * 2 * S));
+  {
+    std::vector<uint32_t> sp32(n32);
+    uint32_t k = 0, smallest = UINT32_MAX;
+    for (size_t t = 0; t < n32; ++t) {
+      while (first[k + 1] <= 32 * t) ++k;
+      sp32[t] = k;
+    }
+    for (uint32_t q = 0; q < S; ++q) smallest = std::min(smallest, first[q + 1] - first[q]);
+    RFD_HIP(ctx, hipMemcpy(s->spec32, sp32.data(), sizeof(uint32_t) * n32,
+                           hipMemcpyHostToDevice));
+    RFD_HIP(ctx, hipMemcpy(s->spec_first_d, first, sizeof(uint32_t) * (S + 1),
+                           hipMemcpyHostToDevice));
+    s->batch_fusable = smallest >= 128;
+  }
   std::vector<rfd_loading> Lh(L, L + S);
   for (auto &x : Lh) {
     x.steps = 0;

@@ -5,7 +5,8 @@ loads in registers, and the platen reactions are fixed-point sums, so both
 give the same bits: positions, velocities, spins, loads, bond state and
 flags, the loading control and its history.  Checked over steps with
 contact-row rebuilds, breaking bonds, odd step counts (the last step of an
-odd count runs the separate pass) and graph replays, for both loading modes.
+odd count runs the separate pass) and graph replays, for both loading modes
+and for batched specimens.
 """
 import numpy as np
 import pytest
@@ -72,6 +73,36 @@ def test_fused_graph_replays_match_separate_pass(ctx):
             a.step(n)
         b.step(n)
         _compare(a, b, f"{how} +{n}")
+    assert ctx.flags() == 0
+    a.close()
+    b.close()
+
+
+def test_fused_batch_matches_separate_pass(ctx):
+    """Batched discs: tiles that span two specimens split their platen sums
+    between them; the fused batch equals the separate-pass batch bit for bit,
+    per-specimen loading and history included."""
+
+    def build():
+        return native_cases.brazil_batch(ctx, S=4, seed=53, n_lo=300, n_hi=1200,
+                                         v_load=-5e-3, hist_cap=4096).sys
+
+    a, b = build(), build()
+    b.set_fusion(False)
+    assert a.fused() and not b.fused()
+    for n, how in ((3, "plain"), (37, "plain"), (80, "graph"), (41, "plain")):
+        print(f"batch {how}: {n} steps", flush=True)
+        if how == "graph":
+            a.step_graph(n)
+        else:
+            a.step(n)
+        b.step(n)
+        _same_state(a, b, label=f"batch {how} +{n}")
+        for k, (La, Lb) in enumerate(zip(a.specimen_loadings(), b.specimen_loadings())):
+            _same_loading(La, Lb, f"batch specimen {k}")
+            assert np.array_equal(a.specimen_history(k), b.specimen_history(k))
+        assert np.array_equal(a.bond_state()[1], b.bond_state()[1])
+    assert any(L.loading for L in a.specimen_loadings())
     assert ctx.flags() == 0
     a.close()
     b.close()
