@@ -246,6 +246,33 @@ int rfd_system_set_bonds(rfd_system *sys, uint32_t Nb, const uint32_t *bi,
 int rfd_system_set_bond_state(rfd_system *sys, const float *state, const uint8_t *flag);
 int rfd_system_get_bond_state(rfd_system *sys, float *state, uint8_t *flag);
 int rfd_system_set_velocities(rfd_system *sys, const float *V, const float *W);
+
+/* ---- the reference slot layout in and out ----
+ * Device buffers of the reference's layout (md_data_types.h:12-56,124-144:
+ * R, V, W SoA float[3N]; IL, ILtype, 1d_iL per slot, rows of K slots;
+ * rij, Oijt, Mijn, Mijt float3 per slot), synchronous.
+ * Import replaces the system's particles, bonds and contacts (N must be the
+ * system's): radius, bond radius and strengths uniform (potential_data); a
+ * pair i < j is a bond when slot i -> j has ILtype 1, with rest length 1/iL
+ * (md_calculateforcesdem3.cu:96) and that slot's rij, Oijt, Mijn, Mijt as its
+ * state (as d_ConstructBoundInteractions / d_CalculateForcesDEM_22 keep it);
+ * type-0 slots with a spring or within 2 radius + skin become contact rows
+ * with that spring, then a row search completes them.
+ * Export writes the state back as rows of K slots: per particle its intact
+ * bonds in both directions (partners ascending; the higher end's slot holds
+ * the mirror image -rij, -Oijt, -Mijn, -Mijt, as the reference's own second
+ * slot does), then broken bonds and contact rows as type-0 slots with their
+ * springs; rij is the direction at R - V dt; free slots hold 0xFFFFFFFF.
+ * A row that needs more than K slots is an argument error. */
+int rfd_system_import_reference_layout(rfd_system *sys, const float *R, const float *V,
+                                       const float *W, const uint32_t *IL,
+                                       const uint8_t *ILtype, const float *iL, const float *rij,
+                                       const float *Oijt, const float *Mijn, const float *Mijt,
+                                       uint32_t N, uint32_t K, float radius, float b_r, float sc,
+                                       float tc);
+int rfd_system_export_reference_layout(rfd_system *sys, float *R, float *V, float *W,
+                                       uint32_t *IL, uint8_t *ILtype, float *iL, float *rij,
+                                       float *Oijt, float *Mijn, float *Mijt, uint32_t K);
 int rfd_system_set_loading(rfd_system *sys, const rfd_loading *L);
 int rfd_system_get_loading(rfd_system *sys, rfd_loading *L);
 /* host copies; any pointer may be NULL */

@@ -3502,6 +3502,251 @@ int download_vec3_64(rfd_ctx *ctx, float *soa, const double4 *src, uint32_t N) {
   return RFD_OK;
 }
 
+// ------------------------------------------- reference slot layout in/out
+int rfd_system_import_reference_layout(rfd_system *s, const float *R, const float *V,
+                                       const float *W, const uint32_t *IL,
+                                       const uint8_t *ILtype, const float *iL, const float *rij,
+                                       const float *Oijt, const float *Mijn, const float *Mijt,
+                                       uint32_t N, uint32_t K, float radius, float b_r, float sc,
+                                       float tc) {
+  if (!valid(s)) return RFD_E_ARG;
+  rfd_ctx *ctx = s->ctx;
+  RFD_CHECK_ARG(ctx, R && V && W && IL && ILtype && iL && rij && Oijt && Mijn && Mijt,
+                "rfd_system_import_reference_layout: null buffer");
+  RFD_CHECK_ARG(ctx, N == s->N && K > 0,
+                "rfd_system_import_reference_layout: N = %u, the system holds %u", N, s->N);
+  RFD_CHECK_ARG(ctx, radius > 0 && b_r > 0, "rfd_system_import_reference_layout: radii <= 0");
+  const size_t S = (size_t)N * K, N3 = 3 * (size_t)N;
+  RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<float> hR(N3), hV(N3), hW(N3), hiL(S), hn(3 * S), hO(3 * S), hMn(3 * S),
+      hMt(3 * S);
+  std::vector<uint32_t> hIL(S);
+  std::vector<uint8_t> hty(S);
+  RFD_HIP(ctx, hipMemcpy(hR.data(), R, sizeof(float) * N3, hipMemcpyDeviceToHost));
+  RFD_HIP(ctx, hipMemcpy(hV.data(), V, sizeof(float) * N3, hipMemcpyDeviceToHost));
+  RFD_HIP(ctx, hipMemcpy(hW.data(), W, sizeof(float) * N3, hipMemcpyDeviceToHost));
+  RFD_HIP(ctx, hipMemcpy(hIL.data(), IL, sizeof(uint32_t) * S, hipMemcpyDeviceToHost));
+  RFD_HIP(ctx, hipMemcpy(hty.data(), ILtype, S, hipMemcpyDeviceToHost));
+  RFD_HIP(ctx, hipMemcpy(hiL.data(), iL, sizeof(float) * S, hipMemcpyDeviceToHost));
+  RFD_HIP(ctx, hipMemcpy(hn.data(), rij, sizeof(float) * 3 * S, hipMemcpyDeviceToHost));
+  RFD_HIP(ctx, hipMemcpy(hO.data(), Oijt, sizeof(float) * 3 * S, hipMemcpyDeviceToHost));
+  RFD_HIP(ctx, hipMemcpy(hMn.data(), Mijn, sizeof(float) * 3 * S, hipMemcpyDeviceToHost));
+  RFD_HIP(ctx, hipMemcpy(hMt.data(), Mijt, sizeof(float) * 3 * S, hipMemcpyDeviceToHost));
+  // particles (FP32 positions are exact in FP64), uniform radius
+  std::vector<double> R64(hR.begin(), hR.end());
+  std::vector<float> rad(N, radius);
+  int st = rfd_system_set_particles(s, R64.data(), rad.data());
+  if (st) return st;
+  // bonds: slot i -> j of type 1 with i < j, in (i, j) order; state of that slot
+  std::vector<std::pair<uint64_t, size_t>> own;
+  for (uint32_t i = 0; i < N; ++i)
+    for (uint32_t k = 0; k < K; ++k) {
+      const size_t sl = (size_t)i * K + k;
+      const uint32_t j = hIL[sl];
+      if (j == 0xFFFFFFFFu || hty[sl] != 1 || j <= i) continue;
+      RFD_CHECK_ARG(ctx, j < N, "rfd_system_import_reference_layout: IL[%zu] = %u >= N", sl, j);
+      own.emplace_back(((uint64_t)i << 32) | j, sl);
+    }
+  std::sort(own.begin(), own.end());
+  const uint32_t Nb = (uint32_t)own.size();
+  std::vector<uint32_t> bi(Nb), bj(Nb);
+  std::vector<float> br(Nb, b_r), bsc(Nb, sc), btc(Nb, tc), L0(Nb), state(12 * (size_t)Nb);
+  std::vector<uint8_t> flag(Nb, (uint8_t)kBond);
+  for (uint32_t b = 0; b < Nb; ++b) {
+    RFD_CHECK_ARG(ctx, b == 0 || own[b].first != own[b - 1].first,
+                  "rfd_system_import_reference_layout: pair (%u,%u) bonded twice",
+                  (uint32_t)(own[b].first >> 32), (uint32_t)own[b].first);
+    const size_t sl = own[b].second;
+    bi[b] = (uint32_t)(own[b].first >> 32);
+    bj[b] = (uint32_t)own[b].first;
+    // the bond law uses 1/iL as the rest length (md_calculateforcesdem3.cu:96)
+    L0[b] = 1.0f / hiL[sl];
+    float *q = state.data() + 12 * (size_t)b;
+    for (int d = 0; d < 3; ++d) {
+      q[d] = hn[3 * sl + d];
+      q[3 + d] = hO[3 * sl + d];
+      q[6 + d] = hMn[3 * sl + d];
+      q[9 + d] = hMt[3 * sl + d];
+    }
+  }
+  st = rfd_system_set_bonds(s, Nb, bi.data(), bj.data(), br.data(), bsc.data(), btc.data(),
+                            L0.data());
+  if (!st && Nb) st = rfd_system_set_bond_state(s, state.data(), flag.data());
+  if (!st) st = rfd_system_set_velocities(s, hV.data(), hW.data());
+  if (st) return st;
+  // contact slots with a spring, or within ri + rj + skin, become the contact
+  // rows (each end keeps its own spring, as each slot does), then one row
+  // search completes the rows and carries the springs over
+  const uint32_t Kc = s->p.contact_capacity;
+  std::vector<uint32_t> cj((size_t)Kc * N, 0), ccnt(N, 0);
+  std::vector<float4> cO((size_t)Kc * N, make_float4(0.f, 0.f, 0.f, 0.f));
+  const double reach = 2.0 * radius + std::max(s->p.skin, 0.0);
+  for (uint32_t i = 0; i < N; ++i)
+    for (uint32_t k = 0; k < K; ++k) {
+      const size_t sl = (size_t)i * K + k;
+      const uint32_t j = hIL[sl];
+      if (j == 0xFFFFFFFFu || hty[sl] != 0 || j >= N || j == i) continue;
+      const bool spring = hO[3 * sl] != 0.f || hO[3 * sl + 1] != 0.f || hO[3 * sl + 2] != 0.f;
+      double rr = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        const double x = (double)hR[j + (size_t)d * N] - hR[i + (size_t)d * N];
+        rr += x * x;
+      }
+      if (!spring && rr >= reach * reach) continue;
+      RFD_CHECK_ARG(ctx, ccnt[i] < Kc,
+                    "rfd_system_import_reference_layout: particle %u has more than %u contacts",
+                    i, Kc);
+      const size_t c = (size_t)ccnt[i]++ * N + i;
+      cj[c] = j;
+      cO[c] = make_float4(hO[3 * sl], hO[3 * sl + 1], hO[3 * sl + 2], 0.f);
+    }
+  const int o = s->ccur;
+  RFD_HIP(ctx, hipMemcpy(s->cj[o], cj.data(), sizeof(uint32_t) * cj.size(),
+                         hipMemcpyHostToDevice));
+  RFD_HIP(ctx, hipMemcpy(s->ccnt[o], ccnt.data(), sizeof(uint32_t) * N, hipMemcpyHostToDevice));
+  RFD_HIP(ctx, hipMemcpy(s->cO[o], cO.data(), sizeof(float4) * cO.size(),
+                         hipMemcpyHostToDevice));
+  if (s->fp64) {
+    std::vector<double4> c64(cO.size());
+    for (size_t c = 0; c < cO.size(); ++c) c64[c] = make_double4(cO[c].x, cO[c].y, cO[c].z, 0.0);
+    RFD_HIP(ctx, hipMemcpy(s->cO64[o], c64.data(), sizeof(double4) * c64.size(),
+                           hipMemcpyHostToDevice));
+  }
+  s->rows_valid = true;
+  st = rebuild_rows(s);
+  if (st) return st;
+  s->gen++;
+  RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return RFD_OK;
+}
+
+int rfd_system_export_reference_layout(rfd_system *s, float *R, float *V, float *W,
+                                       uint32_t *IL, uint8_t *ILtype, float *iL, float *rij,
+                                       float *Oijt, float *Mijn, float *Mijt, uint32_t K) {
+  if (!valid(s)) return RFD_E_ARG;
+  rfd_ctx *ctx = s->ctx;
+  RFD_CHECK_ARG(ctx, R && V && W && IL && ILtype && iL && rij && Oijt && Mijn && Mijt && K > 0,
+                "rfd_system_export_reference_layout: bad arguments");
+  RFD_CHECK_ARG(ctx, s->have_particles, "rfd_system_export_reference_layout: no particles");
+  RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const uint32_t N = s->N, Nb = s->have_bonds ? s->Nb : 0;
+  const size_t S = (size_t)N * K, N3 = 3 * (size_t)N;
+  std::vector<P4> hp(N);
+  RFD_HIP(ctx, hipMemcpy(hp.data(), s->P, sizeof(P4) * N, hipMemcpyDeviceToHost));
+  std::vector<float> hR(N3), hV(N3), hW(N3);
+  if (s->fp64) {
+    int st = download_vec3_64(ctx, hV.data(), s->V64, N);
+    if (!st) st = download_vec3_64(ctx, hW.data(), s->W64, N);
+    if (st) return st;
+  } else {
+    int st = download_vec3(ctx, hV.data(), s->V, N);
+    if (!st) st = download_vec3(ctx, hW.data(), s->W, N);
+    if (st) return st;
+  }
+  for (uint32_t i = 0; i < N; ++i) {
+    hR[i] = (float)hp[i].x;
+    hR[i + (size_t)N] = (float)hp[i].y;
+    hR[i + 2 * (size_t)N] = (float)hp[i].z;
+  }
+  std::vector<float> state(12 * (size_t)std::max(Nb, 1u));
+  std::vector<uint8_t> flag(std::max(Nb, 1u));
+  if (Nb) {
+    int st = rfd_system_get_bond_state(s, state.data(), flag.data());
+    if (st) return st;
+  }
+  std::vector<uint32_t> hIL(S, 0xFFFFFFFFu), cnt(N, 0);
+  std::vector<uint8_t> hty(S, 0);
+  std::vector<float> hiL(S, 0.f), hn(3 * S, 0.f), hO(3 * S, 0.f), hMn(3 * S, 0.f),
+      hMt(3 * S, 0.f);
+  uint32_t full = UINT32_MAX;
+  // one slot of row i; sgn = -1 writes the mirror image (the other end's view)
+  auto put = [&](uint32_t i, uint32_t j, uint8_t ty, float il, const float *n3, const float *o3,
+                 const float *mn3, const float *mt3, float sgn) {
+    if (cnt[i] >= K) {
+      full = std::min(full, i);
+      return;
+    }
+    const size_t sl = (size_t)i * K + cnt[i]++;
+    hIL[sl] = j;
+    hty[sl] = ty;
+    hiL[sl] = il;
+    for (int d = 0; d < 3; ++d) {
+      hn[3 * sl + d] = n3 ? sgn * n3[d] : 0.f;
+      hO[3 * sl + d] = o3 ? sgn * o3[d] : 0.f;
+      hMn[3 * sl + d] = mn3 ? sgn * mn3[d] : 0.f;
+      hMt[3 * sl + d] = mt3 ? sgn * mt3[d] : 0.f;
+    }
+  };
+  std::vector<BR> hb(Nb);
+  if (Nb) RFD_HIP(ctx, hipMemcpy(hb.data(), s->brc, sizeof(BR) * Nb, hipMemcpyDeviceToHost));
+  const double dt = s->p.dt;
+  // intact bonds first, both directions, partners ascending; then broken bonds
+  for (int pass = 0; pass < 2; ++pass)
+    for (uint32_t b = 0; b < Nb; ++b) {
+      if (flag[b] != (pass == 0 ? kBond : kBroken)) continue;
+      const uint32_t i = s->hbi[b], j = s->hbj[b];
+      const float *q = state.data() + 12 * (size_t)b;
+      // previous direction at R - V dt, as the force pass forms it
+      const P4 &a = hp[i], &c = hp[j];
+      double r[3] = {c.x - a.x, c.y - a.y, c.z - a.z};
+      for (int d = 0; d < 3; ++d)
+        r[d] -= ((double)hV[j + (size_t)d * N] - hV[i + (size_t)d * N]) * dt;
+      const double L = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+      const float n3[3] = {(float)(r[0] / L), (float)(r[1] / L), (float)(r[2] / L)};
+      // twist: the stored component along the current direction q[0..2]
+      const float mn = q[6] * q[0] + q[7] * q[1] + q[8] * q[2];
+      const float mn3[3] = {mn * n3[0], mn * n3[1], mn * n3[2]};
+      const float il = hb[b].il;
+      if (pass == 0) {
+        put(i, j, 1, il, n3, q + 3, mn3, q + 9, 1.f);
+        put(j, i, 1, il, n3, q + 3, mn3, q + 9, -1.f);
+      } else {
+        put(i, j, 0, il, nullptr, q + 3, nullptr, nullptr, 1.f);
+        put(j, i, 0, il, nullptr, q + 3, nullptr, nullptr, -1.f);
+      }
+    }
+  // never-bonded contact rows with their per-end springs
+  if (s->rows_valid) {
+    const uint32_t Kc = s->p.contact_capacity;
+    const int o = s->ccur;
+    std::vector<uint32_t> cj((size_t)Kc * N), ccnt(N);
+    std::vector<float4> cO((size_t)Kc * N);
+    RFD_HIP(ctx, hipMemcpy(ccnt.data(), s->ccnt[o], sizeof(uint32_t) * N, hipMemcpyDeviceToHost));
+    RFD_HIP(ctx, hipMemcpy(cj.data(), s->cj[o], sizeof(uint32_t) * cj.size(),
+                           hipMemcpyDeviceToHost));
+    if (s->fp64) {
+      std::vector<double4> c64(cO.size());
+      RFD_HIP(ctx, hipMemcpy(c64.data(), s->cO64[o], sizeof(double4) * c64.size(),
+                             hipMemcpyDeviceToHost));
+      for (size_t c = 0; c < c64.size(); ++c)
+        cO[c] = make_float4((float)c64[c].x, (float)c64[c].y, (float)c64[c].z, 0.f);
+    } else {
+      RFD_HIP(ctx, hipMemcpy(cO.data(), s->cO[o], sizeof(float4) * cO.size(),
+                             hipMemcpyDeviceToHost));
+    }
+    for (uint32_t i = 0; i < N; ++i)
+      for (uint32_t k = 0; k < std::min(ccnt[i], Kc); ++k) {
+        const size_t c = (size_t)k * N + i;
+        const float o3[3] = {cO[c].x, cO[c].y, cO[c].z};
+        put(i, cj[c], 0, 0.f, nullptr, o3, nullptr, nullptr, 1.f);
+      }
+  }
+  RFD_CHECK_ARG(ctx, full == UINT32_MAX,
+                "rfd_system_export_reference_layout: row of particle %u needs more than %u slots",
+                full, K);
+  RFD_HIP(ctx, hipMemcpy(R, hR.data(), sizeof(float) * N3, hipMemcpyHostToDevice));
+  RFD_HIP(ctx, hipMemcpy(V, hV.data(), sizeof(float) * N3, hipMemcpyHostToDevice));
+  RFD_HIP(ctx, hipMemcpy(W, hW.data(), sizeof(float) * N3, hipMemcpyHostToDevice));
+  RFD_HIP(ctx, hipMemcpy(IL, hIL.data(), sizeof(uint32_t) * S, hipMemcpyHostToDevice));
+  RFD_HIP(ctx, hipMemcpy(ILtype, hty.data(), S, hipMemcpyHostToDevice));
+  RFD_HIP(ctx, hipMemcpy(iL, hiL.data(), sizeof(float) * S, hipMemcpyHostToDevice));
+  RFD_HIP(ctx, hipMemcpy(rij, hn.data(), sizeof(float) * 3 * S, hipMemcpyHostToDevice));
+  RFD_HIP(ctx, hipMemcpy(Oijt, hO.data(), sizeof(float) * 3 * S, hipMemcpyHostToDevice));
+  RFD_HIP(ctx, hipMemcpy(Mijn, hMn.data(), sizeof(float) * 3 * S, hipMemcpyHostToDevice));
+  RFD_HIP(ctx, hipMemcpy(Mijt, hMt.data(), sizeof(float) * 3 * S, hipMemcpyHostToDevice));
+  return RFD_OK;
+}
+
 int rfd_system_set_velocities(rfd_system *s, const float *V, const float *W) {
   if (!valid(s)) return RFD_E_ARG;
   int st = RFD_OK;

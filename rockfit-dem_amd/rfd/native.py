@@ -105,6 +105,9 @@ NATIVE_SIGNATURES = {
     "rfd_system_set_bond_state": [_vp, _vp, _vp],
     "rfd_system_get_bond_state": [_vp, _vp, _vp],
     "rfd_system_set_velocities": [_vp, _vp, _vp],
+    "rfd_system_import_reference_layout": [_vp] + [_vp] * 10 + [C.c_uint32] * 2
+                                          + [C.c_float] * 4,
+    "rfd_system_export_reference_layout": [_vp] + [_vp] * 10 + [C.c_uint32],
     "rfd_system_set_loading": [_vp, C.POINTER(Loading)],
     "rfd_system_get_loading": [_vp, C.POINTER(Loading)],
     "rfd_system_get_state": [_vp, _vp, _vp, _vp, _vp, _vp],
@@ -270,6 +273,31 @@ class NativeSystem:
         self._call("rfd_system_get_bond_state", self.h, st.ctypes.data_as(C.c_void_p),
                    fl.ctypes.data_as(C.c_void_p))
         return st.reshape(nb, 12), fl
+
+    # -- the reference slot layout ---------------------------------------
+    _SLOT_ARGS = ("rij", "Oijt", "Mijn", "Mijt")
+
+    def import_reference_layout(self, lay, mat):
+        """Particles, bonds with their state, and contact springs from a
+        reference-layout system (rfd.reflayout.RefLayout, device tensors);
+        radius, bond radius and strengths from the material.  The bond list
+        and order are those of bonds_from_slots."""
+        d = [t.data_ptr() for t in (lay.R, lay.V, lay.W, lay.IL, lay.ILtype, lay.iL)]
+        d += [lay.slot[k].data_ptr() for k in self._SLOT_ARGS]
+        self._call("rfd_system_import_reference_layout", self.h, *d, lay.N, lay.K,
+                   mat.p_r, mat.b_r, mat.sc, mat.tc)
+        # the host copy of the bond list (the library selects and orders the
+        # same way)
+        self.bonds, _, _ = bonds_from_slots(lay.host(), lay.N, lay.K, mat)
+        self.radius = np.full(self.N, mat.p_r, np.float32)
+
+    def export_reference_layout(self, lay):
+        """Write the state into a reference-layout system's device tensors
+        (the same N; rows of lay.K slots)."""
+        assert lay.N == self.N
+        d = [t.data_ptr() for t in (lay.R, lay.V, lay.W, lay.IL, lay.ILtype, lay.iL)]
+        d += [lay.slot[k].data_ptr() for k in self._SLOT_ARGS]
+        self._call("rfd_system_export_reference_layout", self.h, *d, lay.K)
 
     def set_velocities(self, V=None, W=None):
         a = _np(V, np.float32) if V is not None else (None, None)
