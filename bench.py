@@ -49,6 +49,8 @@ def parse():
                    help="only the batched Brazilian-disc measurement")
     p.add_argument("--calibrate", action="store_true",
                    help="only the calibration outer loop (CG steps on batched tests)")
+    p.add_argument("--fire-only", action="store_true",
+                   help="only the plain-FIRE load steps on the large cylinder")
     p.add_argument("--packed-only", action="store_true",
                    help="only the random-packing cylinder measurement")
     p.add_argument("--copy-calib", action="store_true",
@@ -289,6 +291,50 @@ def bench_warmstart(D: float = 61.0, H: float = 122.0, steps: int = 6) -> dict:
                          "val_loss": min(hist["val"]) if hist["val"] else None,
                          "val_loss_zero_correction": hist["val_zero"]},
             "total_seconds": time.perf_counter() - t0}
+
+
+def bench_fire_large(D: float = 100.0, H: float = 200.0, steps: int = 2) -> dict:
+    """Plain GPU FIRE load steps on the large cylinder: strain increment
+    -1e-5, FIRE to |F|max < 1e-6 F_ref, contact rows rebuilt every 20 FIRE
+    iterations.  The first load step (seating) is not timed.  Bond-updates/s
+    counts active bonds x FIRE iterations (one force pass each)."""
+    from rfd import native, native_cases
+    from rfd.reflayout import Context
+
+    ctx = Context(0)
+    nc = native_cases.uniaxial_cylinder(ctx, D, H, flat_ends=True)
+    s = nc.sys
+    s.seat_platens(preload=0.0)
+    fp = native.fire_params(nc.mat.E, 0.5, nc.mat.dt, rel_tol=1e-6, max_iter=100000,
+                            check_every=200)
+    res = []
+    for k in range(1 + steps):
+        if k == 1:
+            s.timing(True)
+        r = s.fire_load_step(-1e-5, fp)
+        print(f"[bench] fire D{D:g}xH{H:g} step {k}: {r.iterations} iterations, "
+              f"{r.seconds:.2f}s, converged {r.converged}", file=sys.stderr, flush=True)
+        res.append(r)
+    kern_ms, launches = s.timing_read()
+    kern_ms /= max(launches, 1)
+    timed = res[1:]
+    sec = sum(r.seconds for r in timed)
+    it = sum(r.iterations for r in timed)
+    active = s.bond_counts()[0]
+    kernel = s.force_kernel().replace("<fused>", "")  # FIRE passes run the force pass alone
+    alg = force_alg_bytes(kernel, s.bonds.n, s.N)
+    assert ctx.flags() == 0, "device flags raised in the FIRE run"
+    out = {"workload": f"uniaxial cylinder D{D:g}xH{H:g} mm (as the main line), plain FIRE "
+                       "load steps of strain -1e-5 to |F|max < 1e-6 E pi r_mean^2",
+           "particles": s.N, "bonds": s.bonds.n, "load_steps_per_s": len(timed) / sec,
+           "s_per_step": sec / len(timed), "fire_iterations_per_step": it / len(timed),
+           "all_converged": all(bool(r.converged) for r in timed),
+           "max_final_residual": max(r.fmax for r in timed), "ftol": fp.ftol,
+           "bond_updates_per_s": active * it / sec, "ms_per_iteration": sec * 1e3 / it,
+           "force_kernel": kernel, "force_ms": kern_ms,
+           "force_roofline_frac": alg / (kern_ms * 1e-3) / PEAK_HBM}
+    s.close()
+    return out
 
 
 def bench_peak_precision(D: float = 10.0, H: float = 20.0, max_steps: int = 20000) -> dict:
@@ -548,6 +594,9 @@ def main():
     if args.packed_only:
         print(json.dumps({"packed": bench_packed()}))
         return
+    if args.fire_only:
+        print(json.dumps({"fire_large": bench_fire_large(args.D, args.H)}))
+        return
     if args.calibrate:
         print(json.dumps({"calibrate": bench_calibrate()}))
         return
@@ -584,6 +633,10 @@ def main():
             out["warm_start"] = bench_warmstart()
         except Exception as e:  # reported, never hides the main line
             out["warm_start"] = {"error": f"{type(e).__name__}: {e}"}
+        try:
+            out["fire_large"] = bench_fire_large(args.D, args.H)
+        except Exception as e:  # reported, never hides the main line
+            out["fire_large"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_batch and args.tier == "native":
         try:
             out["batch"] = bench_batch()
