@@ -293,6 +293,69 @@ def bench_warmstart(D: float = 61.0, H: float = 122.0, steps: int = 6) -> dict:
             "total_seconds": time.perf_counter() - t0}
 
 
+def bench_warmstart_damage(pre: int = 18, steps: int = 26) -> dict:
+    """The warm start where bonds break: load steps of strain -5e-5 from the
+    onset of breaking (after `pre` plain steps) towards the peak.  The
+    network is trained on two D21xH42 cylinders and validated on a third over
+    the same range, then three copies of a D28xH56 cylinder run the range
+    with plain FIRE, with the skip path alone (an untrained network, whose
+    zero decoder repeats the last increment), and with the trained network.
+    The first two steps of the range are plain in all three and not timed."""
+    from rfd import native, native_cases, warmstart as ws
+    from rfd.reflayout import Context
+
+    ctx = Context(0)
+    ds = -5e-5
+    t0 = time.perf_counter()
+
+    def spec(d, h, seed):
+        nc = native_cases.uniaxial_cylinder(ctx, d, h, seed=seed, flat_ends=True)
+        nc.sys.seat_platens(preload=0.0)
+        return nc
+
+    first = spec(21.0, 42.0, 101)
+    fp = native.fire_params(first.mat.E, 0.5, first.mat.dt, rel_tol=1e-6, max_iter=60000,
+                            check_every=200)
+
+    def loaded(c):
+        for _ in range(pre):
+            c.sys.fire_load_step(ds, fp)
+        return c
+
+    samples = []
+    for c in (first, spec(21.0, 42.0, 102)):
+        samples += ws.collect_samples(loaded(c).sys, ds, fp, steps)[0]
+    val = ws.collect_samples(loaded(spec(21.0, 42.0, 103)).sys, ds, fp, steps)[0]
+    net = ws.WarmStartNet(ctx, seed=7)
+    hist = ws.train(net, samples, epochs=40, lr=1e-3, val=val)
+    skip = ws.WarmStartNet(ctx, seed=7)
+    out = {}
+    for name, nt in (("plain", None), ("skip", skip), ("learned", net)):
+        c = loaded(spec(28.0, 56.0, 104))
+        warm = (lambda s_, nt=nt: ws.predict(nt, s_, ds)) if nt is not None else None
+        res = ws.collect_samples(c.sys, ds, fp, steps, warm=warm)[1][2:]
+        sec = sum(r.seconds for r in res)
+        out[name] = {"s_per_step": sec / len(res),
+                     "fire_iterations_per_step": sum(r.iterations for r in res) / len(res),
+                     "bonds_broken": sum(r.breaks for r in res),
+                     "all_converged": all(bool(r.converged) for r in res),
+                     "warm_starts_taken": sum(r.warm_used for r in res)}
+        print(f"[bench] damage-regime warm start {name}: {out[name]}", file=sys.stderr,
+              flush=True)
+    return {"workload": f"uniaxial cylinder D28xH56 mm (seed 104), load steps {pre + 2}-"
+                        f"{pre + steps - 1} of strain {ds:g} (onset of breaking to near the "
+                        "peak), FIRE to |F|max < 1e-6 E pi r_mean^2",
+            "particles": c.sys.N, **out,
+            "speedup_learned_vs_plain": out["plain"]["s_per_step"] / out["learned"]["s_per_step"],
+            "speedup_learned_vs_skip": out["skip"]["s_per_step"] / out["learned"]["s_per_step"],
+            "training": {"specimens": "D21xH42 seeds 101, 102; validation seed 103",
+                         "samples": len(samples), "epochs": 40,
+                         "correction_head": hist["head"],
+                         "val_loss": min(hist["val"]), "val_loss_zero_correction":
+                             hist["val_zero"]},
+            "total_seconds": time.perf_counter() - t0}
+
+
 def bench_fire_large(D: float = 100.0, H: float = 200.0, steps: int = 2) -> dict:
     """Plain GPU FIRE load steps on the large cylinder: strain increment
     -1e-5, FIRE to |F|max < 1e-6 F_ref, contact rows rebuilt every 20 FIRE
@@ -586,7 +649,8 @@ def main():
                           "stream_copy_GBps": stream_copy() / 1e9}))
         return
     if args.warmstart_only:
-        print(json.dumps({"warm_start": bench_warmstart()}))
+        print(json.dumps({"warm_start": bench_warmstart(),
+                          "warm_start_damage": bench_warmstart_damage()}))
         return
     if args.batch_only:
         print(json.dumps({"batch": bench_batch()}))
@@ -633,6 +697,10 @@ def main():
             out["warm_start"] = bench_warmstart()
         except Exception as e:  # reported, never hides the main line
             out["warm_start"] = {"error": f"{type(e).__name__}: {e}"}
+        try:
+            out["warm_start_damage"] = bench_warmstart_damage()
+        except Exception as e:  # reported, never hides the main line
+            out["warm_start_damage"] = {"error": f"{type(e).__name__}: {e}"}
         try:
             out["fire_large"] = bench_fire_large(args.D, args.H)
         except Exception as e:  # reported, never hides the main line
