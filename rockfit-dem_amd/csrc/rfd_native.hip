@@ -3516,6 +3516,8 @@ int rfd_system_import_reference_layout(rfd_system *s, const float *R, const floa
   RFD_CHECK_ARG(ctx, N == s->N && K > 0,
                 "rfd_system_import_reference_layout: N = %u, the system holds %u", N, s->N);
   RFD_CHECK_ARG(ctx, radius > 0 && b_r > 0, "rfd_system_import_reference_layout: radii <= 0");
+  RFD_CHECK_ARG(ctx, s->S == 0,
+                "rfd_system_import_reference_layout: the system holds batched specimens");
   const size_t S = (size_t)N * K, N3 = 3 * (size_t)N;
   RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
   std::vector<float> hR(N3), hV(N3), hW(N3), hiL(S), hn(3 * S), hO(3 * S), hMn(3 * S),
