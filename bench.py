@@ -50,7 +50,8 @@ def parse():
     p.add_argument("--calibrate", action="store_true",
                    help="only the calibration outer loop (CG steps on batched tests)")
     p.add_argument("--fire-only", action="store_true",
-                   help="only the plain-FIRE load steps on the large cylinder")
+                   help="only the plain-FIRE load steps on the large cylinder and the "
+                        "FP32 / FP64 FIRE comparison")
     p.add_argument("--packed-only", action="store_true",
                    help="only the random-packing cylinder measurement")
     p.add_argument("--copy-calib", action="store_true",
@@ -400,6 +401,41 @@ def bench_fire_large(D: float = 100.0, H: float = 200.0, steps: int = 2) -> dict
     return out
 
 
+def bench_fire_fp64(D: float = 25.0, H: float = 50.0, steps: int = 4) -> dict:
+    """Quasi-static load steps (strain -1e-5, FIRE to |F|max < 1e-6 F_ref) on
+    the same seeded cylinder on the FP32 path and on the FP64 GPU path: the
+    relaxed platen reactions of each step and their largest relative
+    difference."""
+    from rfd import native, native_cases
+    from rfd.reflayout import Context
+
+    ctx = Context(0)
+    out = {}
+    for name, fp64 in (("fp32", False), ("fp64", True)):
+        nc = native_cases.uniaxial_cylinder(ctx, D, H, flat_ends=True, fp64=fp64)
+        s = nc.sys
+        s.seat_platens(preload=0.0)
+        fp = native.fire_params(nc.mat.E, 0.5, nc.mat.dt, rel_tol=1e-6, max_iter=100000,
+                                check_every=200)
+        rr = [s.fire_load_step(-1e-5, fp) for _ in range(1 + steps)][1:]
+        out[name] = {"reaction_top": [r.sum_top for r in rr],
+                     "reaction_bottom": [r.sum_bottom for r in rr],
+                     "fire_iterations": [r.iterations for r in rr],
+                     "s_per_step": sum(r.seconds for r in rr) / len(rr),
+                     "all_converged": all(bool(r.converged) for r in rr),
+                     "particles": s.N, "bonds": s.bonds.n}
+        print(f"[bench] FIRE {name} D{D:g}xH{H:g}: {out[name]['fire_iterations']} iterations, "
+              f"{out[name]['s_per_step']:.3f}s per step", file=sys.stderr, flush=True)
+        s.close()
+    a = [out["fp32"]["reaction_top"], out["fp32"]["reaction_bottom"]]
+    b = [out["fp64"]["reaction_top"], out["fp64"]["reaction_bottom"]]
+    scale = max(abs(x) for row in b for x in row)
+    diff = max(abs(x - y) for ra, rb in zip(a, b) for x, y in zip(ra, rb)) / scale
+    return {"workload": f"uniaxial cylinder D{D:g}xH{H:g} mm, load steps 2-{steps + 1} of "
+                        "strain -1e-5, FIRE to |F|max < 1e-6 E pi r_mean^2",
+            **out, "max_relative_reaction_difference": diff}
+
+
 def bench_peak_precision(D: float = 10.0, H: float = 20.0, max_steps: int = 20000) -> dict:
     """Peak strength of one seeded uniaxial cylinder (explicit loading at 10x
     the reference platen speed, capped ends) on the FP32 path and on the FP64
@@ -659,7 +695,8 @@ def main():
         print(json.dumps({"packed": bench_packed()}))
         return
     if args.fire_only:
-        print(json.dumps({"fire_large": bench_fire_large(args.D, args.H)}))
+        print(json.dumps({"fire_fp64": bench_fire_fp64(),
+                          "fire_large": bench_fire_large(args.D, args.H)}))
         return
     if args.calibrate:
         print(json.dumps({"calibrate": bench_calibrate()}))
@@ -693,6 +730,10 @@ def main():
             out["peak_strength"] = bench_peak_precision()
         except Exception as e:  # reported, never hides the main line
             out["peak_strength"] = {"error": f"{type(e).__name__}: {e}"}
+        try:
+            out["fire_fp64"] = bench_fire_fp64()
+        except Exception as e:  # reported, never hides the main line
+            out["fire_fp64"] = {"error": f"{type(e).__name__}: {e}"}
         try:
             out["warm_start"] = bench_warmstart()
         except Exception as e:  # reported, never hides the main line

@@ -1646,7 +1646,8 @@ struct FireDev {
   double dt_max, dt_min, alpha0, f_inc, f_dec, f_alpha, ftol;
   uint32_t n_pos, iter, max_iter, n_delay;
   int done, converged, backoff, first, measure_only;
-  float dt_f;  // time step of the last move, read by the force pass
+  float dt_f;   // time step of the last move, read by the force pass
+  double dt_m;  // the same for the FP64 force pass
 };
 
 constexpr int kFireVals = 9;  // P, |F|^2, |V|^2, |M|^2, |W|^2, max|F|^2, max(|M|/r)^2, sb, st
@@ -1779,6 +1780,7 @@ __global__ void __launch_bounds__(256) k_fire_control(FireDev *fire, const doubl
     f.alpha = f.alpha0;
   }
   f.dt_f = (float)f.dt;
+  f.dt_m = f.dt;
   *fire = f;
 }
 
@@ -2045,6 +2047,7 @@ struct alignas(16) BSt64 {
 struct Mat64 {
   double E, G, Estar, Gstar, pA, mu, mur, rho43pi, dt, farcut;
   int refmode;
+  int nobreak;  // quasi-static relaxation: bonds are judged after convergence
 };
 
 struct Rot64 {
@@ -2115,9 +2118,15 @@ struct Sys64 {
   uint8_t *fl8;
   const uint32_t *cj, *ccnt;
   double4 *cO;
+  const double *dtp;  // when set, the time step of the last move (FIRE) replaces mt.dt
+  const int *done;    // when set and nonzero (relaxation finished), the pass does nothing
+  int trial;          // trial pass: writes only F, M and st_out
+  uint32_t *nbreak;   // when set, counts the bonds this pass breaks
 };
 
 __global__ void __launch_bounds__(128) k_f64_forces(Sys64 s, Mat64 mt) {
+  if (s.done && *s.done) return;
+  if (s.dtp) mt.dt = *s.dtp;
   BSt64 *const st_in = *s.curp ? s.b1 : s.b0;
   BSt64 *const st_out = *s.curp ? s.b0 : s.b1;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < s.N; i += grid_stride()) {
@@ -2141,8 +2150,10 @@ __global__ void __launch_bounds__(128) k_f64_forces(Sys64 s, Mat64 mt) {
       if (r2 > lim * lim || !(r2 > 1e-10)) {
         if (own) {
           st_out[q.b].fl = kRemoved;
-          st_in[q.b].fl = kRemoved;
-          s.fl8[q.b] = kRemoved;
+          if (!s.trial) {
+            st_in[q.b].fl = kRemoved;
+            s.fl8[q.b] = kRemoved;
+          }
         }
         continue;
       }
@@ -2177,7 +2188,7 @@ __global__ void __launch_bounds__(128) k_f64_forces(Sys64 s, Mat64 mt) {
         Mb = Mb + n * (mn * J) + Mt * mtf + cross(r, Ft) * 0.5;
         const double Sc = sig + __builtin_sqrt(dot(Mt, Mt)) * 4.0 * br;
         const double Mcr = mt.G * il * __builtin_sqrt(dot(O, O)) + fabs(mn) * 2.0 * br;
-        if (Sc > (double)c.sc || Mcr > (double)c.tc) {
+        if (!mt.nobreak && (Sc > (double)c.sc || Mcr > (double)c.tc)) {
           nw.fl = kBroken;
           O = kZero64;
           nw.mn = nw.mx = nw.my = nw.mz = 0.0;
@@ -2211,7 +2222,10 @@ __global__ void __launch_bounds__(128) k_f64_forces(Sys64 s, Mat64 mt) {
       }
       if (own) {
         st_out[q.b] = nw;
-        if (nw.fl != old.fl) s.fl8[q.b] = (uint8_t)nw.fl;
+        if (!s.trial && nw.fl != old.fl) {
+          s.fl8[q.b] = (uint8_t)nw.fl;
+          if (s.nbreak && nw.fl == kBroken) atomicAdd(s.nbreak, 1u);
+        }
       }
     }
     const uint32_t nc = s.ccnt[i];
@@ -2229,8 +2243,8 @@ __global__ void __launch_bounds__(128) k_f64_forces(Sys64 s, Mat64 mt) {
         F = F + Fi;
         Mc = Mc + Mi;
         anyc = true;
-        s.cO[sl] = d4(O);
-      } else {
+        if (!s.trial) s.cO[sl] = d4(O);
+      } else if (!s.trial) {
         s.cO[sl] = make_double4(0.0, 0.0, 0.0, 0.0);
       }
     }
@@ -2290,6 +2304,172 @@ __global__ void k_f32_to_f64(const float *__restrict__ soa, uint32_t N,
     out[i] = make_double4(soa[i], soa[i + (size_t)N], soa[i + 2 * (size_t)N], 0.0);
 }
 
+// ------------------------------------------------------ FP64 FIRE load steps
+// The FP32 FIRE kernels (k_fire_begin / measure / update / step_disp) with
+// velocities, spins, loads and the accumulated rotation in double, and 1/m,
+// 1/I from the radius as the FP64 pass takes them.
+struct FireParticles64 {
+  uint32_t N;
+  P4 *P;
+  double4 *V, *Vf, *W, *F, *Th;
+  const P4 *P0;
+  const double4 *M;
+  double rho43pi;
+  rfd_loading *ctl;
+  FireDev *fire;
+  double *part;
+};
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_fire64_measure(FireParticles64 p) {
+  __shared__ double s_part[4 * kFireVals];
+  if (p.fire->done) return;
+  rfd_loading L = *p.ctl;
+  L.v = 0.0;
+  double v[kFireVals] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.N; i += grid_stride()) {
+    P4 R = p.P[i];
+    d3 V = xyz(p.Vf[i]), F = xyz(p.F[i]);
+    if (MODE) {
+      const d3 V0 = V, F0 = F;
+      const double z0 = R.z, y0 = R.y;
+      platen<MODE>(L, R, V, F, v[7], v[8]);
+      if (R.z != z0 || R.y != y0) p.P[i] = R;
+      if (V.x != V0.x || V.y != V0.y || V.z != V0.z) p.Vf[i] = d4(V);
+      if (F.x != F0.x || F.y != F0.y || F.z != F0.z) p.F[i] = d4(F);
+    }
+    const d3 W = xyz(p.W[i]), M = xyz(p.M[i]);
+    v[0] += dot(F, V) + dot(M, W);
+    const double f2 = dot(F, F), m2 = dot(M, M);
+    v[1] += f2;
+    v[2] += dot(V, V);
+    v[3] += m2;
+    v[4] += dot(W, W);
+    v[5] = fmax(v[5], f2);
+    v[6] = fmax(v[6], m2 / (R.r * R.r));
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < kFireVals; ++q) v[q] = (q == 5 || q == 6) ? wave_max(v[q]) : wave_sum(v[q]);
+  if (lane == 0)
+    for (int q = 0; q < kFireVals; ++q) s_part[wid * kFireVals + q] = v[q];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < kFireVals; ++q) {
+      double a = 0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
+        a = (q == 5 || q == 6) ? fmax(a, s_part[w * kFireVals + q]) : a + s_part[w * kFireVals + q];
+      p.part[blockIdx.x * kFireVals + q] = a;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_fire64_update(FireParticles64 p) {
+  const FireDev f = *p.fire;
+  if (f.done) return;
+  const double a1 = 1.0 - f.alpha, inv_dt = 1.0 / f.dt;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.N; i += grid_stride()) {
+    const P4 R0 = p.P[i];
+    P4 R = R0;
+    d3 V = xyz(p.Vf[i]), W = xyz(p.W[i]);
+    const d3 F = xyz(p.F[i]), M = xyz(p.M[i]);
+    if (f.backoff) {
+      const double h = 0.5 * f.dt_last;
+      R.x -= V.x * h;
+      R.y -= V.y * h;
+      R.z -= V.z * h;
+      V = kZero64;
+      W = kZero64;
+    } else {
+      V = V * a1 + F * f.cV;
+      W = W * a1 + M * f.cW;
+    }
+    const double m = p.rho43pi * R.r * R.r * R.r, I = 0.4 * m * R.r * R.r;
+    V = V + F * (f.dt / m);
+    W = W + M * (f.dt / I);
+    R.x += V.x * f.dt;
+    R.y += V.y * f.dt;
+    R.z += V.z * f.dt;
+    p.P[i] = R;
+    p.Vf[i] = d4(V);
+    p.V[i] = make_double4((R.x - R0.x) * inv_dt, (R.y - R0.y) * inv_dt, (R.z - R0.z) * inv_dt,
+                          0.0);
+    p.W[i] = d4(W);
+    p.Th[i] = d4(xyz(p.Th[i]) + W * f.dt);
+  }
+}
+
+template <int MODE>
+__global__ void k_fire64_begin(FireParticles64 p, const float *__restrict__ dU,
+                               const float *__restrict__ dTh, double dt0, double dtop) {
+  rfd_loading L = *p.ctl;
+  L.v = 0.0;
+  const double inv_dt = 1.0 / dt0;
+  double sb = 0, stp = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.N; i += grid_stride()) {
+    const P4 R0 = p.P[i];
+    P4 R = R0;
+    if (dU) {
+      R.x += (double)dU[3 * (size_t)i];
+      R.y += (double)dU[3 * (size_t)i + 1];
+      R.z += (double)dU[3 * (size_t)i + 2];
+    }
+    d3 Vd = kZero64, F = kZero64;
+    if (MODE) {
+      double ra, rb, h0, zc;
+      if (MODE == 1) {
+        ra = R0.x - L.cx;
+        rb = R0.y - L.cy;
+        h0 = R0.z;
+        zc = R0.z;
+      } else {
+        ra = R0.x - L.cx;
+        rb = R0.z - L.cz;
+        h0 = R0.y;
+        zc = R0.z - L.cz;
+      }
+      if (ra * ra + rb * rb < L.RR && zc < L.zcut) {
+        double &Ra = MODE == 1 ? R.z : R.y;
+        if (h0 - (L.top - dtop) > -1e-9)
+          Ra = L.top;
+        else if (h0 - L.bottom < 1e-9)
+          Ra = L.bottom;
+      }
+      platen<MODE>(L, R, Vd, F, sb, stp);
+    }
+    const d3 th = dTh ? d3{dTh[3 * (size_t)i], dTh[3 * (size_t)i + 1], dTh[3 * (size_t)i + 2]}
+                      : kZero64;
+    p.P[i] = R;
+    p.V[i] = make_double4((R.x - R0.x) * inv_dt, (R.y - R0.y) * inv_dt, (R.z - R0.z) * inv_dt,
+                          0.0);
+    p.W[i] = d4(th * inv_dt);
+    p.Th[i] = d4(th);
+    p.Vf[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+  }
+}
+
+__global__ void k_fire64_step_disp(const P4 *__restrict__ P, const P4 *__restrict__ P0,
+                                   const double4 *__restrict__ Th, uint32_t N,
+                                   float *__restrict__ du, float *__restrict__ dth) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
+    const P4 a = P[i], b = P0[i];
+    du[3 * i] = (float)(a.x - b.x);
+    du[3 * i + 1] = (float)(a.y - b.y);
+    du[3 * i + 2] = (float)(a.z - b.z);
+    const double4 t = Th[i];
+    dth[3 * i] = (float)t.x;
+    dth[3 * i + 1] = (float)t.y;
+    dth[3 * i + 2] = (float)t.z;
+  }
+}
+
+__global__ void k_zero_d4(double4 *__restrict__ a, double4 *__restrict__ b, uint32_t N) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
+    a[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (b) b[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+  }
+}
+
 // ================================================================= host side
 struct rfd_system {
   rfd_ctx *ctx = nullptr;
@@ -2346,6 +2526,8 @@ struct rfd_system {
   uint32_t *d_nbreak = nullptr;  // bonds broken by the last break-check pass
   bool forces_current = false;  // the load step's first force pass is done
   float4 *Th = nullptr;      // rotation accumulated during the load step
+  double4 *Vf64 = nullptr, *Th64 = nullptr;  // the same two for the FP64 path
+  double *fire_dt64 = nullptr;  // &fire->dt_m, handed to the FP64 force pass
   float *du[2] = {nullptr, nullptr}, *dth[2] = {nullptr, nullptr};
   int dcur = 0;              // index of the newest step displacement
   bool timing = false;  // force-kernel events on the context stream
@@ -2558,8 +2740,9 @@ Mat64 make_mat64(const rfd_system_params &p) {
   return m;
 }
 
-// FP64 force pass (explicit steps only)
-int forces64(rfd_system *s) {
+// FP64 force pass; dtp, trial, nobreak and done as in forces() (FIRE passes)
+int forces64(rfd_system *s, const double *dtp = nullptr, bool trial = false,
+             bool nobreak = false, const int *done = nullptr) {
   Sys64 d{};
   d.N = s->N;
   d.Kc = s->p.contact_capacity;
@@ -2578,6 +2761,16 @@ int forces64(rfd_system *s) {
   d.cj = s->cj[s->ccur];
   d.ccnt = s->ccnt[s->ccur];
   d.cO = s->cO64[s->ccur];
+  d.dtp = dtp;
+  d.done = done;
+  d.trial = trial ? 1 : 0;
+  d.nbreak = s->d_nbreak;
+  Mat64 m = make_mat64(s->p);
+  if (dtp) {
+    m.pA = 0.0;
+    m.mur = 0.0;
+  }
+  m.nobreak = nobreak ? 1 : 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (s->timing && s->ev_used + 2 <= s->ev.size()) {
     e0 = s->ev[s->ev_used];
@@ -2585,10 +2778,10 @@ int forces64(rfd_system *s) {
     s->ev_used += 2;
     RFD_HIP(s->ctx, hipEventRecord(e0, s->ctx->stream));
   }
-  k_f64_forces<<<rfd_blocks(s->N, 128), 128, 0, s->ctx->stream>>>(d, make_mat64(s->p));
+  k_f64_forces<<<rfd_blocks(s->N, 128), 128, 0, s->ctx->stream>>>(d, m);
   RFD_HIP(s->ctx, hipGetLastError());
   if (e1) RFD_HIP(s->ctx, hipEventRecord(e1, s->ctx->stream));
-  k_flip_state<<<1, 64, 0, s->ctx->stream>>>(s->d_cur, nullptr);
+  if (!trial) k_flip_state<<<1, 64, 0, s->ctx->stream>>>(s->d_cur, done);
   RFD_HIP(s->ctx, hipGetLastError());
   return RFD_OK;
 }
@@ -2663,7 +2856,10 @@ int ensure_work(rfd_system *s) {
 // loading control (FP32 explicit steps)
 int forces(rfd_system *s, const float *dtp = nullptr, bool trial = false,
            bool nobreak = false, const AdvDev *adv = nullptr, bool flip = true) {
-  if (s->fp64) return forces64(s);
+  if (s->fp64)
+    return forces64(s, dtp ? s->fire_dt64 : nullptr, trial, nobreak,
+                    dtp ? (const int *)((const char *)s->fire + offsetof(FireDev, done))
+                        : nullptr);
   SysDev d = make_dev(s);
   d.dtp = dtp;
   d.trial = trial ? 1 : 0;
@@ -2846,6 +3042,10 @@ int fire_alloc(rfd_system *s) {
   if (!st) st = dalloc(ctx, &s->d_nbreak, 1);
   s->gen++;  // the force pass now counts breaks
   if (!st) st = dalloc(ctx, &s->Th, s->N);
+  if (!st && s->fp64) {
+    st = dalloc(ctx, &s->Vf64, s->N);
+    if (!st) st = dalloc(ctx, &s->Th64, s->N);
+  }
   for (int q = 0; q < 2 && !st; ++q) {
     st = dalloc(ctx, &s->du[q], 3 * (size_t)s->N);
     if (!st) st = dalloc(ctx, &s->dth[q], 3 * (size_t)s->N);
@@ -2856,8 +3056,10 @@ int fire_alloc(rfd_system *s) {
     RFD_HIP(ctx, hipMemset(s->dth[q], 0, sizeof(float) * 3 * s->N));
   }
   s->fire_dt = (float *)((char *)s->fire + offsetof(FireDev, dt_f));
+  s->fire_dt64 = (double *)((char *)s->fire + offsetof(FireDev, dt_m));
   return RFD_OK;
 }
+
 
 FireParticles fire_particles(rfd_system *s) {
   FireParticles p{};
@@ -2878,6 +3080,57 @@ FireParticles fire_particles(rfd_system *s) {
   return p;
 }
 
+FireParticles64 fire_particles64(rfd_system *s) {
+  FireParticles64 p{};
+  p.N = s->N;
+  p.P = s->P;
+  p.V = s->V64;
+  p.Vf = s->Vf64;
+  p.W = s->W64;
+  p.F = s->F64;
+  p.Th = s->Th64;
+  p.P0 = s->P0;
+  p.M = s->M64;
+  p.rho43pi = s->p.rho * 4.0 / 3.0 * M_PI;
+  p.ctl = s->ctl;
+  p.fire = s->fire;
+  p.part = s->fire_part;
+  return p;
+}
+
+// the FIRE measure of the current forces (platens at rest, power, norms,
+// reaction sums) into the per-workgroup partials
+void fire_measure(rfd_system *s, hipStream_t q) {
+  if (s->fp64) {
+    const FireParticles64 fp = fire_particles64(s);
+    if (s->mode == 1)
+      k_fire64_measure<1><<<s->nblocks_pass, 256, 0, q>>>(fp);
+    else if (s->mode == 2)
+      k_fire64_measure<2><<<s->nblocks_pass, 256, 0, q>>>(fp);
+    else
+      k_fire64_measure<0><<<s->nblocks_pass, 256, 0, q>>>(fp);
+    return;
+  }
+  const FireParticles fp = fire_particles(s);
+  if (s->mode == 1)
+    k_fire_measure<1><<<s->nblocks_pass, 256, 0, q>>>(fp);
+  else if (s->mode == 2)
+    k_fire_measure<2><<<s->nblocks_pass, 256, 0, q>>>(fp);
+  else
+    k_fire_measure<0><<<s->nblocks_pass, 256, 0, q>>>(fp);
+}
+
+// velocities, spins and rotations of the load step back to rest (FIRE velocity too)
+void fire_rest(rfd_system *s, hipStream_t q, bool fire_velocity) {
+  if (s->fp64) {
+    k_zero_d4<<<rfd_blocks(s->N, 256), 256, 0, q>>>(s->V64, s->W64, s->N);
+    if (fire_velocity) k_zero_d4<<<rfd_blocks(s->N, 256), 256, 0, q>>>(s->Vf64, nullptr, s->N);
+  } else {
+    k_zero_f4<<<rfd_blocks(s->N, 256), 256, 0, q>>>(s->V, s->W, s->N);
+    if (fire_velocity) k_zero_f4<<<rfd_blocks(s->N, 256), 256, 0, q>>>(s->Vf, nullptr, s->N);
+  }
+}
+
 // one FIRE iteration: force pass at the current configuration, constraint +
 // reductions, control, update
 int fire_iteration(rfd_system *s) {
@@ -2891,16 +3144,13 @@ int fire_iteration(rfd_system *s) {
     int st = forces(s, s->fire_dt, false, true);
     if (st) return st;
   }
-  FireParticles fp = fire_particles(s);
   hipStream_t q = s->ctx->stream;
-  if (s->mode == 1)
-    k_fire_measure<1><<<s->nblocks_pass, 256, 0, q>>>(fp);
-  else if (s->mode == 2)
-    k_fire_measure<2><<<s->nblocks_pass, 256, 0, q>>>(fp);
-  else
-    k_fire_measure<0><<<s->nblocks_pass, 256, 0, q>>>(fp);
+  fire_measure(s, q);
   k_fire_control<<<1, 256, 0, q>>>(s->fire, s->fire_part, (int)s->nblocks_pass, s->ctl);
-  k_fire_update<<<s->nblocks_pass, 256, 0, q>>>(fp);
+  if (s->fp64)
+    k_fire64_update<<<s->nblocks_pass, 256, 0, q>>>(fire_particles64(s));
+  else
+    k_fire_update<<<s->nblocks_pass, 256, 0, q>>>(fire_particles(s));
   RFD_HIP(s->ctx, hipGetLastError());
   s->step_count++;
   return RFD_OK;
@@ -2916,30 +3166,35 @@ int begin_pass(rfd_system *s, const FireDev &f, const float *dU, const float *dT
   hipStream_t q = ctx->stream;
   RFD_HIP(ctx, hipMemcpyAsync(s->fire, &f, sizeof(f), hipMemcpyHostToDevice, q));
   RFD_HIP(ctx, hipMemcpyAsync(s->P, s->P0, sizeof(P4) * s->N, hipMemcpyDeviceToDevice, q));
-  FireParticles fpp = fire_particles(s);
-  const float dt0 = (float)f.dt;
-  if (s->mode == 1)
-    k_fire_begin<1><<<s->nblocks_pass, 256, 0, q>>>(fpp, dU, dTh, dt0, dtop);
-  else if (s->mode == 2)
-    k_fire_begin<2><<<s->nblocks_pass, 256, 0, q>>>(fpp, dU, dTh, dt0, dtop);
-  else
-    k_fire_begin<0><<<s->nblocks_pass, 256, 0, q>>>(fpp, dU, dTh, dt0, dtop);
+  if (s->fp64) {
+    const FireParticles64 fpp = fire_particles64(s);
+    if (s->mode == 1)
+      k_fire64_begin<1><<<s->nblocks_pass, 256, 0, q>>>(fpp, dU, dTh, f.dt, dtop);
+    else if (s->mode == 2)
+      k_fire64_begin<2><<<s->nblocks_pass, 256, 0, q>>>(fpp, dU, dTh, f.dt, dtop);
+    else
+      k_fire64_begin<0><<<s->nblocks_pass, 256, 0, q>>>(fpp, dU, dTh, f.dt, dtop);
+  } else {
+    const FireParticles fpp = fire_particles(s);
+    const float dt0 = (float)f.dt;
+    if (s->mode == 1)
+      k_fire_begin<1><<<s->nblocks_pass, 256, 0, q>>>(fpp, dU, dTh, dt0, dtop);
+    else if (s->mode == 2)
+      k_fire_begin<2><<<s->nblocks_pass, 256, 0, q>>>(fpp, dU, dTh, dt0, dtop);
+    else
+      k_fire_begin<0><<<s->nblocks_pass, 256, 0, q>>>(fpp, dU, dTh, dt0, dtop);
+  }
   RFD_HIP(ctx, hipGetLastError());
   // a trial pass writes its state to the spare buffer and leaves the index
   int st = forces(s, s->fire_dt, trial, true);
   if (st) return st;
-  k_zero_f4<<<rfd_blocks(s->N, 256), 256, 0, q>>>(s->V, s->W, s->N);
+  fire_rest(s, q, false);
   RFD_HIP(ctx, hipGetLastError());
   if (trial) {
     FireDev fm = f;
     fm.measure_only = 1;
     RFD_HIP(ctx, hipMemcpyAsync(s->fire, &fm, sizeof(fm), hipMemcpyHostToDevice, q));
-    if (s->mode == 1)
-      k_fire_measure<1><<<s->nblocks_pass, 256, 0, q>>>(fpp);
-    else if (s->mode == 2)
-      k_fire_measure<2><<<s->nblocks_pass, 256, 0, q>>>(fpp);
-    else
-      k_fire_measure<0><<<s->nblocks_pass, 256, 0, q>>>(fpp);
+    fire_measure(s, q);
     k_fire_control<<<1, 256, 0, q>>>(s->fire, s->fire_part, (int)s->nblocks_pass, s->ctl);
     RFD_HIP(ctx, hipGetLastError());
     FireDev r;
@@ -3104,6 +3359,8 @@ int rfd_system_destroy(rfd_system *s) {
   }
   dfree(s->part);
   dfree(s->acc);
+  dfree(s->Vf64);
+  dfree(s->Th64);
   dfree(s->P2);
   dfree(s->V2);
   dfree(s->W2);
@@ -4341,9 +4598,6 @@ int rfd_system_fire_load_step(rfd_system *s, double dstrain, const rfd_fire_para
   if (!valid(s)) return RFD_E_ARG;
   rfd_ctx *ctx = s->ctx;
   RFD_CHECK_ARG(ctx, fp && out && s->have_bonds, "rfd_system_fire_load_step: bad arguments");
-  if (s->fp64)
-    return rfd_set_error(ctx, RFD_E_STATE, "rfd_system_fire_load_step: the FP64 path runs "
-                                           "explicit steps only");
   if (s->S)
     return rfd_set_error(ctx, RFD_E_STATE, "rfd_system_fire_load_step: batched specimens run "
                                            "explicit steps only");
@@ -4382,6 +4636,7 @@ int rfd_system_fire_load_step(rfd_system *s, double dstrain, const rfd_fire_para
   f.max_iter = fp->max_iter;
   f.first = 1;
   f.dt_f = (float)fp->dt0;
+  f.dt_m = fp->dt0;
   if (!s->rows_valid || s->step_count % s->p.rebuild_every == 0) {
     st = rebuild_rows(s);
     if (st) return st;
@@ -4424,6 +4679,7 @@ int rfd_system_fire_load_step(rfd_system *s, double dstrain, const rfd_fire_para
   while (f.converged) {
     FireDev fc = f_start;
     fc.dt_f = 0.0f;
+    fc.dt_m = 0.0;
     RFD_HIP(ctx, hipMemcpyAsync(s->fire, &fc, sizeof(fc), hipMemcpyHostToDevice, q));
     RFD_HIP(ctx, hipMemsetAsync(s->d_nbreak, 0, sizeof(uint32_t), q));
     st = forces(s, s->fire_dt, false, false);
@@ -4435,8 +4691,7 @@ int rfd_system_fire_load_step(rfd_system *s, double dstrain, const rfd_fire_para
     breaks += nb;
     ++rounds;
     RFD_HIP(ctx, hipMemcpyAsync(s->fire, &f_start, sizeof(f_start), hipMemcpyHostToDevice, q));
-    k_zero_f4<<<rfd_blocks(s->N, 256), 256, 0, q>>>(s->V, s->W, s->N);
-    k_zero_f4<<<rfd_blocks(s->N, 256), 256, 0, q>>>(s->Vf, nullptr, s->N);
+    fire_rest(s, q, true);
     RFD_HIP(ctx, hipGetLastError());
     s->forces_current = true;  // the check pass left the forces of this state
     st = relax();
@@ -4447,9 +4702,13 @@ int rfd_system_fire_load_step(rfd_system *s, double dstrain, const rfd_fire_para
   // the step's displacement and rotation, newest first; the relaxed state is
   // at rest
   s->dcur = 1 - s->dcur;
-  k_fire_step_disp<<<rfd_blocks(s->N, 256), 256, 0, q>>>(s->P, s->P0, s->Th, s->N,
-                                                          s->du[s->dcur], s->dth[s->dcur]);
-  k_zero_f4<<<rfd_blocks(s->N, 256), 256, 0, q>>>(s->V, s->W, s->N);
+  if (s->fp64)
+    k_fire64_step_disp<<<rfd_blocks(s->N, 256), 256, 0, q>>>(s->P, s->P0, s->Th64, s->N,
+                                                              s->du[s->dcur], s->dth[s->dcur]);
+  else
+    k_fire_step_disp<<<rfd_blocks(s->N, 256), 256, 0, q>>>(s->P, s->P0, s->Th, s->N,
+                                                            s->du[s->dcur], s->dth[s->dcur]);
+  fire_rest(s, q, false);
   RFD_HIP(ctx, hipGetLastError());
   st = rfd_system_get_loading(s, &L);
   if (st) return st;
@@ -4483,6 +4742,7 @@ int rfd_system_residual(rfd_system *s, double out[2]) {
   FireDev f{};
   f.measure_only = 1;
   f.dt_f = 0.0f;  // no spring increments: the state is only re-expressed
+  f.dt_m = 0.0;
   RFD_HIP(ctx, hipMemcpyAsync(s->fire, &f, sizeof(f), hipMemcpyHostToDevice, q));
   if (!s->rows_valid) {
     st = rebuild_rows(s);

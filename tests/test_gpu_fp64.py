@@ -5,7 +5,9 @@ step in double, the GPU precision oracle of the FP32 path.
   (orc_native_bonds_f64) to 1e-10 of the largest component over several
   evaluations with large velocities;
 * a uniaxial loading run to fracture: the FP32 path's peak strength is within
-  1 % of the FP64 path's on the same seeded specimen.
+  1 % of the FP64 path's on the same seeded specimen;
+* quasi-static FIRE load steps: the relaxed platen reactions of the FP32
+  path agree with the FP64 path's to 1e-3 of the largest.
 """
 import numpy as np
 import pytest
@@ -78,3 +80,34 @@ def test_fp32_peak_strength_within_1pct_of_fp64(ctx):
     assert L64.loading and L64.fracture_stress > 0
     assert L32.fracture_stress == pytest.approx(L64.fracture_stress, rel=0.01), (
         L32.fracture_stress, L64.fracture_stress, L32.fracture_strain, L64.fracture_strain)
+
+
+def test_fire_load_steps_fp32_vs_fp64(ctx):
+    """Quasi-static load steps (axial strain -1e-4, FIRE to |F|max < 1e-8
+    F_ref) on the same seeded specimen on the FP32 path and on the FP64 path:
+    every step converges on both, and the relaxed platen reactions agree to
+    1e-4 of the largest.  Both relaxations stop anywhere below the tolerance,
+    so the reactions may differ by up to N tol F_ref (about 1e-5 of them
+    here); the FP32 rounding itself is smaller (2e-7 measured)."""
+    from rfd import native
+
+    res = {}
+    for fp64 in (False, True):
+        nc = native_cases.uniaxial_cylinder(ctx, 8.0, 12.0, seed=5, strength=(1e12, 1e12),
+                                            fp64=fp64)
+        s = nc.sys
+        s.seat_platens(preload=0.05)
+        fp = native.fire_params(nc.mat.E, 0.5, nc.mat.dt, rel_tol=1e-8, max_iter=50000,
+                                check_every=200)
+        rr = [s.fire_load_step(-1e-4, fp) for _ in range(4)]
+        print(f"fp64={fp64}: iterations {[r.iterations for r in rr]}", flush=True)
+        assert all(r.converged and r.fmax < fp.ftol for r in rr), [r.as_dict() for r in rr]
+        res[fp64] = np.array([[r.sum_bottom, r.sum_top] for r in rr])
+        s.close()
+    scale = np.abs(res[True]).max()
+    err = np.abs(res[False] - res[True]).max() / scale
+    print(f"reactions: FP64 {res[True][:, 1]}, FP32 - FP64 max {err:.2e} of the largest",
+          flush=True)
+    assert scale > 0
+    np.testing.assert_allclose(res[False], res[True], rtol=0, atol=1e-4 * scale)
+    assert ctx.flags() == 0
