@@ -197,8 +197,9 @@ typedef struct rfd_system_params {
                             (md_calculateforcesdem3.cu:148) and rolling-moment
                             wipe (:330-335); 0: physical law                */
   int fp64;              /* 1: every quantity of the step in double (the GPU
-                            precision oracle of the FP32 path; explicit steps
-                            only, no FIRE)                                  */
+                            precision oracle of the FP32 path), explicit and
+                            FIRE load steps; no batched specimens, no warm-
+                            start residual query                            */
 } rfd_system_params;
 
 /* loading control, device resident; mirrors compression_data
