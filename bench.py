@@ -678,7 +678,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU runs are not part of this build", file=sys.stderr)
+              "the path runs on one GPU; how it would shard is described, not run",
+              file=sys.stderr)
         sys.exit(2)
     if args.copy_calib:
         print(json.dumps({"copy_bytes_read": COPY_BYTES, "copy_bytes_written": COPY_BYTES,

@@ -1,7 +1,7 @@
 """numpy/ctypes wrapper of the CPU oracle (oracle/rfd_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py as the checker.  The product never imports it.
+cpu_baseline leg of bench.py, to compare against.  The product never imports it.
 
 ``OracleLayout`` holds the reference slot-layout arrays in host memory and
 has the same stage methods as rfd.reflayout.RefLayout, so that a parity test

@@ -1,7 +1,8 @@
 /*
  * rfd_oracle.c -- CPU restatement of the RockFit-DEM BPM step.
  *
- * TEST INFRASTRUCTURE ONLY: the checker for the HIP path (see rfd_oracle.h).
+ * TEST INFRASTRUCTURE ONLY: what the HIP path is compared against (see
+ * rfd_oracle.h).
  * Built with -O2 -ffp-contract=off -fno-fast-math so that every FP32
  * expression below rounds after each operation, left to right, exactly as
  * written.  Each function cites the reference file:line it restates.

@@ -1,8 +1,8 @@
 /*
  * rfd_oracle.h -- CPU oracle for the RockFit-DEM bonded-particle-model step.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP path is
- * compared against.  Only tests/, __graft_entry__.smoke() and the cpu_baseline
+ * TEST INFRASTRUCTURE ONLY.  This library is what the HIP path is compared
+ * against.  Only tests/, __graft_entry__.smoke() and the cpu_baseline
  * leg of bench.py may load it; the product library (librfd.so) never links,
  * loads or calls it.
  *
