@@ -49,6 +49,8 @@ def parse():
                    help="only the batched Brazilian-disc measurement")
     p.add_argument("--calibrate", action="store_true",
                    help="only the calibration outer loop (CG steps on batched tests)")
+    p.add_argument("--disc-only", action="store_true",
+                   help="only the 2k-sphere Brazilian disc (FP64 and FP32 GPU, CPU oracle)")
     p.add_argument("--fire-only", action="store_true",
                    help="only the plain-FIRE load steps on the large cylinder and the "
                         "FP32 / FP64 FIRE comparison")
@@ -436,6 +438,61 @@ def bench_fire_fp64(D: float = 25.0, H: float = 50.0, steps: int = 4) -> dict:
             **out, "max_relative_reaction_difference": diff}
 
 
+def bench_c1(steps: int = 5000, cpu_seconds: float = 5.0, cpu: bool = True) -> dict:
+    """The CPU-runnable Brazilian disc (D20 x T5 mm, random packing at
+    phi = 0.6, bonds at gap < 0.05 r_mean, strengths N(20,4)/N(40,8) MPa,
+    platens closing at 0.5 m/s): `steps` explicit steps on the FP64 GPU path
+    and on the FP32 path; beside them the oracle's step loop (single thread,
+    reference slot layout) on the reference's monodisperse disc of the same
+    size, for a bounded time."""
+    import numpy as np
+
+    from rfd import native_cases
+    from rfd.reflayout import Context
+
+    ctx = Context(0)
+    out = {}
+    for name, fp64 in (("gpu_fp64", True), ("gpu_fp32", False)):
+        nc = native_cases.packed_brazil_disc(ctx, fp64=fp64)
+        s = nc.sys
+        s.step(10)
+        ctx.sync()
+        t0 = time.perf_counter()
+        s.step(steps)
+        ctx.sync()
+        sec = time.perf_counter() - t0
+        h = s.history()
+        active = s.bond_counts()[0]
+        out[name] = {"particles": s.N, "bonds": s.bonds.n, "steps": steps,
+                     "ms_per_step": sec * 1e3 / steps,
+                     "bond_updates_per_s": active * steps / sec,
+                     "bonds_active_end": active,
+                     "peak_stress_MPa": float(np.abs(h[:, 1:]).max(1).max()) * 1e3}
+        assert ctx.flags() == 0, "device flags raised in the disc run"
+        s.close()
+    if cpu:
+        from oracle.oracle import OracleUniaxialDriver
+        from rfd import cases
+
+        c = cases.reference_brazil(20.0, 5.0)
+        d = OracleUniaxialDriver(c.R, c.mat, c.box, c.cmp, c.grid(), c.K)
+        d.setup()
+        d.step()
+        nb = int((d.s.ILtype == 1).sum()) // 2
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < cpu_seconds and n < steps:
+            d.step()
+            n += 1
+        sec = time.perf_counter() - t0
+        out["cpu_port"] = {"particles": c.N, "bonds": nb, "steps": n,
+                           "ms_per_step": sec * 1e3 / n, "bond_updates_per_s": nb * n / sec,
+                           "cores": 1, "kind": "port",
+                           "sample": "oracle step loop on the reference's monodisperse "
+                                     "D20xT5 mm disc, single thread"}
+    return {"workload": "Brazilian disc D20xT5 mm, random packing phi=0.6, r~U[0.4,0.6] mm, "
+                        f"{steps} explicit steps", **out}
+
+
 def bench_peak_precision(D: float = 10.0, H: float = 20.0, max_steps: int = 20000) -> dict:
     """Peak strength of one seeded uniaxial cylinder (explicit loading at 10x
     the reference platen speed, capped ends) on the FP32 path and on the FP64
@@ -695,6 +752,9 @@ def main():
     if args.packed_only:
         print(json.dumps({"packed": bench_packed()}))
         return
+    if args.disc_only:
+        print(json.dumps({"disc_c1": bench_c1(cpu=not args.no_cpu)}))
+        return
     if args.fire_only:
         print(json.dumps({"fire_fp64": bench_fire_fp64(),
                           "fire_large": bench_fire_large(args.D, args.H)}))
@@ -710,6 +770,7 @@ def main():
     out = {
         "metric": "bond-updates/s",
         "value": res["value"],
+        "particle_updates_per_s": res["config"]["particles"] * 1e3 / res["ms_per_step"],
         "unit": "bond-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -731,6 +792,10 @@ def main():
             out["peak_strength"] = bench_peak_precision()
         except Exception as e:  # reported, never hides the main line
             out["peak_strength"] = {"error": f"{type(e).__name__}: {e}"}
+        try:
+            out["disc_c1"] = bench_c1(cpu=not args.no_cpu)
+        except Exception as e:  # reported, never hides the main line
+            out["disc_c1"] = {"error": f"{type(e).__name__}: {e}"}
         try:
             out["fire_fp64"] = bench_fire_fp64()
         except Exception as e:  # reported, never hides the main line

@@ -290,3 +290,33 @@ def packed_uniaxial_cylinder(ctx, D: float, H: float, phi: float = 0.6,
     L.size = L.size0 = L.top - L.bottom
     sysn.set_loading(L)
     return NativeCase(sysn, mat, box, cmp, np.arange(len(R)))
+
+
+def packed_brazil_disc(ctx, D: float = 20.0, T: float = 5.0, phi: float = 0.6,
+                       seed: int = specimen.BASE_SEED, gap_frac: float = 0.05,
+                       strength=(20e6, 40e6), v_load: float = -0.5 / VELOCITY,
+                       fp64: bool = False, rebuild_every: int = 20) -> NativeCase:
+    """Brazilian disc on a seeded random packing (packed_cylinder, axis z):
+    bonds join pairs with surface gap < gap_frac * r_mean, bond radius
+    min(ri, rj), strengths N(mu, 0.2 mu) truncated > 0; diametral platens along
+    y close at v_load from the outermost particle centres."""
+    pk = packed_cylinder(ctx, D, T, phi, seed)
+    mat = material_c(0.4)
+    c = np.array([0.5 * D + 10.0, 0.5 * D + 10.0, 0.5 * T + 10.0])
+    R, rad = pk.R + c, pk.radius
+    p = params_from_material(mat, refmode=0, rebuild_every=rebuild_every, fp64=fp64)
+    p.farcut = 1.45
+    sysn = NativeSystem(ctx, p, R, rad.astype(np.float32))
+    pairs = sysn.find_pairs(gap_frac * float(rad.mean()))
+    sc, tc = strengths(specimen.rng(seed + 1), len(pairs), strength[0], strength[1])
+    br = np.minimum(rad[pairs[:, 0]], rad[pairs[:, 1]]).astype(np.float32)
+    sysn.set_bonds(Bonds(pairs[:, 0].copy(), pairs[:, 1].copy(), br, sc, tc))
+    cmp = Compression("brazil", (D - 1.0, D - 1.0, T - 1.0), tuple(c), 0.5, mat.dt,
+                      v_load=v_load, min_stress=1e6 / STRESS, max_strain=0.1)
+    L = loading_from_compression(cmp, 1e30)
+    y = R[:, 1]
+    L.top, L.bottom = float(y.max()), float(y.min())
+    L.top_r, L.bottom_r = L.top - 0.5, L.bottom + 0.5
+    L.size = L.size0 = L.top - L.bottom
+    sysn.set_loading(L)
+    return NativeCase(sysn, mat, None, cmp, np.arange(len(R)))
