@@ -102,14 +102,16 @@ def load_traffic(kernel: str, particles: int, bonds: int):
     return None
 
 
-def cpu_baseline_ref(seconds: float) -> dict:
-    """Oracle driver (single thread) on a 25 x 50 mm cylinder, the size of the
-    reference's own 28,593-particle sample (kernel.cu:166)."""
-    from oracle.oracle import OracleUniaxialDriver
+def cpu_baseline_ref(seconds: float, omp: bool = False) -> dict:
+    """Oracle driver on a 25 x 50 mm cylinder, the size of the reference's own
+    28,593-particle sample (kernel.cu:166): single thread as the reference's
+    CPU path runs, or (omp) its per-particle loops on every OpenMP thread."""
+    from oracle.oracle import OracleUniaxialDriver, lib
     from rfd import cases
 
     c = cases.reference_uniaxial(25.0, 50.0)
-    d = OracleUniaxialDriver(c.R, c.mat, c.box, c.cmp, c.grid(), c.K)
+    d = OracleUniaxialDriver(c.R, c.mat, c.box, c.cmp, c.grid(), c.K, omp=omp)
+    threads = lib(omp).orc_threads()
     d.setup()
     d.step()
     nb = int((d.s.ILtype == 1).sum()) // 2
@@ -118,9 +120,10 @@ def cpu_baseline_ref(seconds: float) -> dict:
         d.step()
         n += 1
     dt = time.perf_counter() - t0
-    return {"value": nb * n / dt, "unit": "bond-updates/s", "cores": 1, "kind": "port",
+    return {"value": nb * n / dt, "unit": "bond-updates/s", "cores": threads, "kind": "port",
             "sample": f"oracle step loop, {n} explicit steps of a {c.N}-particle "
-                      f"D25xH50 mm cylinder ({nb} bonds), single thread"}
+                      f"D25xH50 mm cylinder ({nb} bonds), "
+                      + (f"{threads} OpenMP threads" if omp else "single thread")}
 
 
 def bench_ref(args) -> dict:
@@ -787,6 +790,7 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_ref(args.cpu_seconds)
         out["cpu_baseline"]["host_cores_available"] = os.cpu_count()
+        out["cpu_baseline_threads"] = cpu_baseline_ref(args.cpu_seconds, omp=True)
     if not args.no_warmstart and args.tier == "native":
         try:
             out["peak_strength"] = bench_peak_precision()

@@ -17,6 +17,7 @@ import numpy as np
 
 HERE = pathlib.Path(__file__).resolve().parent
 LIB_PATH = HERE / "_build" / "librfd_oracle.so"
+LIB_OMP_PATH = HERE / "_build" / "librfd_oracle_omp.so"  # per-particle loops on OpenMP
 
 _u32, _f32, _int = C.c_uint32, C.c_float, C.c_int
 _P = C.c_void_p
@@ -42,13 +43,14 @@ _SIG = {
     "orc_fire_power": ([_P, _P, _P, _P, _u32, _D], None),
     "orc_norm3": ([_f32, _f32, _f32], _f32),
     "orc_rsqrt": ([_f32], _f32),
+    "orc_threads": ([], _int),
     "orc_native_bonds_f64": ([_P] * 3 + [_u32] + [_P] * 8 + [_u32, C.c_double, C.c_double,
                                                              C.c_double, _int, _P, _P], None),
     "orc_native_contacts_f64": ([_P] * 4 + [_u32] + [_P] * 4 + [_u32] + [C.c_double] * 7
                                 + [_P, _P], None),
 }
 
-_lib = None
+_libs: dict = {}
 
 
 def build() -> pathlib.Path:
@@ -56,18 +58,20 @@ def build() -> pathlib.Path:
     return LIB_PATH
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not LIB_PATH.exists():
+def lib(omp: bool = False):
+    """The oracle library; omp=True: the build whose per-particle loops run on
+    OpenMP threads (the same results)."""
+    path = LIB_OMP_PATH if omp else LIB_PATH
+    if path not in _libs:
+        if not path.exists():
             build()
-        h = C.CDLL(str(LIB_PATH))
+        h = C.CDLL(str(path))
         for name, (args, res) in _SIG.items():
             f = getattr(h, name)
             f.argtypes = args
             f.restype = res
-        _lib = h
-    return _lib
+        _libs[path] = h
+    return _libs[path]
 
 
 def ptr(a: np.ndarray | None):
@@ -83,7 +87,8 @@ ERR = {0: "ok", -10: "cell range", -11: "row full"}
 class OracleLayout:
     """Host twin of rfd.reflayout.RefLayout evaluated by the C oracle."""
 
-    def __init__(self, R: np.ndarray, K: int, grid):
+    def __init__(self, R: np.ndarray, K: int, grid, omp: bool = False):
+        self._lib = lib(omp)
         R = np.asarray(R, dtype=np.float32).reshape(-1).copy()
         self.N = N = R.size // 3
         self.K = K
@@ -109,29 +114,29 @@ class OracleLayout:
 
     def cell_distribution(self):
         g = self.grid
-        self.last_status = lib().orc_cell_distribution(ptr(self.R), self.N, g.inv_a, g.Nx, g.Ny,
+        self.last_status = self._lib.orc_cell_distribution(ptr(self.R), self.N, g.inv_a, g.Nx, g.Ny,
                                                        g.Nz, ptr(self.CI), ptr(self.CIs),
                                                        ptr(self.pnC))
 
     def il_construct(self, aacut):
         g = self.grid
-        self.last_status = lib().orc_il_construct(ptr(self.R), self.N, ptr(self.CIs),
+        self.last_status = self._lib.orc_il_construct(ptr(self.R), self.N, ptr(self.CIs),
                                                   ptr(self.pnC), ptr(self.IL), self.K, g.a,
                                                   g.inv_a, aacut, g.Nx, g.Ny, g.Nz)
 
     def il_reconstruct_bpm(self, aacut, quirk_p0=True):
         g = self.grid
-        self.last_status = lib().orc_il_reconstruct_bpm(
+        self.last_status = self._lib.orc_il_reconstruct_bpm(
             ptr(self.R), self.N, ptr(self.CIs), ptr(self.pnC), ptr(self.IL), ptr(self.ILtype),
             self.K, g.a, g.inv_a, aacut, g.Nx, g.Ny, g.Nz, int(quirk_p0))
 
     def bonds_construct(self, bdd):
-        lib().orc_bonds_construct(ptr(self.R), ptr(self.IL), ptr(self.ILtype), ptr(self.iL),
+        self._lib.orc_bonds_construct(ptr(self.R), ptr(self.IL), ptr(self.ILtype), ptr(self.iL),
                                   *self._s("rij", "Oijt", "Fijn", "Fijt", "Mijn", "Mijt",
                                            "Mijadd"), self.N, self.K, bdd)
 
     def delete_far_links(self, aa_farcut):
-        lib().orc_delete_far_links(ptr(self.R), self.N, ptr(self.IL), ptr(self.ILtype),
+        self._lib.orc_delete_far_links(ptr(self.R), self.N, ptr(self.IL), ptr(self.ILtype),
                                    *self._s("rij", "Oijt", "Fijn", "Fijt", "Mijn", "Mijt",
                                             "Mijadd"), self.K, aa_farcut)
 
@@ -140,51 +145,51 @@ class OracleLayout:
         self.M[:] = 0
 
     def forces_bpm(self, m):
-        lib().orc_forces_bpm(ptr(self.R), ptr(self.V), ptr(self.W), ptr(self.IL),
+        self._lib.orc_forces_bpm(ptr(self.R), ptr(self.V), ptr(self.W), ptr(self.IL),
                              ptr(self.ILtype), ptr(self.iL), *self._s("rij", "Oijt", "Mijn",
                                                                        "Mijt"),
                              ptr(self.F), ptr(self.M), self.N, self.K, m.dt, m.E, m.G, m.b_r,
                              m.sc, m.tc)
 
     def forces_contact(self, m):
-        lib().orc_forces_contact(ptr(self.R), ptr(self.V), ptr(self.W), ptr(self.IL),
+        self._lib.orc_forces_contact(ptr(self.R), ptr(self.V), ptr(self.W), ptr(self.IL),
                                  ptr(self.ILtype), ptr(self.slot["Oijt"]), ptr(self.F),
                                  ptr(self.M), self.N, self.K, m.dt, m.hm_E, m.hm_G, m.p_A,
                                  m.p_mu, m.p_mur, m.p_m, m.p_r)
 
     def sumup_forces(self):
-        lib().orc_sumup_forces(*self._s("Fijn", "Fijt", "Mijn", "Mijt", "Mijadd"), ptr(self.F),
+        self._lib.orc_sumup_forces(*self._s("Fijn", "Fijt", "Mijn", "Mijt", "Mijadd"), ptr(self.F),
                                ptr(self.M), self.N, self.K)
 
     def check_break(self, sc, tc, b_r):
-        lib().orc_check_break(*self._s("rij", "Oijt", "Fijn", "Fijt", "Mijn", "Mijt"),
+        self._lib.orc_check_break(*self._s("rij", "Oijt", "Fijn", "Fijt", "Mijn", "Mijt"),
                               ptr(self.ILtype), self.N * self.K, sc, tc, b_r)
 
     def increments(self, m):
-        lib().orc_increments(ptr(self.F), ptr(self.V), ptr(self.R), ptr(self.M), ptr(self.W),
+        self._lib.orc_increments(ptr(self.F), ptr(self.V), ptr(self.R), ptr(self.M), ptr(self.W),
                              self.N, m.dt_d_m, m.dt, m.dt_d_I)
 
     def platen_uniaxial(self, cmp, zcut):
         out = (C.c_double * 2)()
-        lib().orc_platen_uniaxial(ptr(self.R), ptr(self.V), ptr(self.F), self.N, cmp.center_d[0],
+        self._lib.orc_platen_uniaxial(ptr(self.R), ptr(self.V), ptr(self.F), self.N, cmp.center_d[0],
                                   cmp.center_d[1], cmp.RR, cmp.top, cmp.bottom, cmp.top_r,
                                   cmp.bottom_r, cmp.V, zcut, out)
         return out[0], out[1]
 
     def platen_brazil(self, cmp, zcut):
         out = (C.c_double * 2)()
-        lib().orc_platen_brazil(ptr(self.R), ptr(self.V), ptr(self.F), self.N, cmp.center_d[0],
+        self._lib.orc_platen_brazil(ptr(self.R), ptr(self.V), ptr(self.F), self.N, cmp.center_d[0],
                                 cmp.center_d[2], cmp.RR, cmp.top, cmp.bottom, cmp.top_r,
                                 cmp.bottom_r, cmp.V, zcut, out)
         return out[0], out[1]
 
     def cut_restriction(self, center, sized2, hidden_point):
         a = [np.asarray(x, np.float32) for x in (center, sized2, hidden_point)]
-        lib().orc_cut_restriction(ptr(self.R), ptr(self.V), ptr(self.F), self.N, *map(ptr, a))
+        self._lib.orc_cut_restriction(ptr(self.R), ptr(self.V), ptr(self.F), self.N, *map(ptr, a))
 
     def fire_power(self, rotations=False):
         out = (C.c_double * 4)()
-        lib().orc_fire_power(ptr(self.F), ptr(self.V), ptr(self.M) if rotations else None,
+        self._lib.orc_fire_power(ptr(self.F), ptr(self.V), ptr(self.M) if rotations else None,
                              ptr(self.W) if rotations else None, self.N, out)
         return tuple(out)
 
@@ -200,9 +205,9 @@ class OracleUniaxialDriver:
     (md_calculationuniaxialcompression.cu:131-189), with the same host logic
     objects (rfd.params.Compression) as the product driver."""
 
-    def __init__(self, R, mat, box, cmp, grid, K=32):
+    def __init__(self, R, mat, box, cmp, grid, K=32, omp: bool = False):
         self.mat, self.box, self.cmp = mat, box, cmp
-        self.s = OracleLayout(R, K, grid)
+        self.s = OracleLayout(R, K, grid, omp)
         self.aacut = float(np.float32(mat.a_farcut) ** 2)
         self.zcut = float(box.center[2] + box.spacesized2[2])
 

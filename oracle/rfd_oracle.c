@@ -21,6 +21,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define PI_F 3.14159265358979f /* MCf_pi, md_math_constants.h:87 */
 #define TWO_THIRDS_F 0.6666666667f /* MCf_2d3, md_math_constants.h:63 */
@@ -129,11 +132,17 @@ int orc_il_construct(const float *R, uint32_t N, const uint32_t *CIs,
                      uint32_t Nz) {
   uint32_t C = Nx * Ny * Nz;
   int status = ORC_OK;
+  int cell_status = ORC_OK;
   for (size_t s = 0; s < (size_t)N * K; ++s) IL[s] = ORC_SENTINEL;
+#pragma omp parallel for schedule(static)
   for (uint32_t i = 0; i < N; ++i) {
     uint32_t cells[8];
     int st = neighbour_cells(R, N, i, a, inv_a, Nx, Ny, Nz, cells);
-    if (st) return st;
+    if (st) {
+#pragma omp atomic write
+      cell_status = st;
+      continue;
+    }
     uint32_t nk = 0;
     for (int q = 0; q < 8; ++q) {
       uint32_t c = cells[q];
@@ -143,16 +152,18 @@ int orc_il_construct(const float *R, uint32_t N, const uint32_t *CIs,
         uint32_t j = CIs[N + s0 + t];
         float rr = dist2(R, N, i, j);
         if (rr < aacut && j != i) {
-          if (nk < K)
+          if (nk < K) {
             IL[(size_t)i * K + nk] = j;
-          else
+          } else {
+#pragma omp atomic write
             status = ORC_ERR_ROW_FULL;
+          }
           ++nk;
         }
       }
     }
   }
-  return status;
+  return cell_status ? cell_status : status;
 }
 
 /* md_interaction_list_reconstructbpm.cu:13-46, 80-139.  New contacts go to
@@ -165,11 +176,16 @@ int orc_il_reconstruct_bpm(const float *R, uint32_t N, const uint32_t *CIs,
                            uint32_t K, float a, float inv_a, float aacut,
                            uint32_t Nx, uint32_t Ny, uint32_t Nz, int quirk_p0) {
   uint32_t C = Nx * Ny * Nz;
-  int status = ORC_OK;
+  int status = ORC_OK, cell_status = ORC_OK;
+#pragma omp parallel for schedule(static)
   for (uint32_t i = 0; i < N; ++i) {
     uint32_t cells[8];
     int st = neighbour_cells(R, N, i, a, inv_a, Nx, Ny, Nz, cells);
-    if (st) return st;
+    if (st) {
+#pragma omp atomic write
+      cell_status = st;
+      continue;
+    }
     uint32_t *row = IL + (size_t)i * K;
     for (int q = 0; q < 8; ++q) {
       uint32_t c = cells[q];
@@ -195,6 +211,7 @@ int orc_il_reconstruct_bpm(const float *R, uint32_t N, const uint32_t *CIs,
           }
           if (!found) {
             if (freeslot < 0) {
+#pragma omp atomic write
               status = ORC_ERR_ROW_FULL;
             } else {
               row[freeslot] = j;
@@ -205,7 +222,7 @@ int orc_il_reconstruct_bpm(const float *R, uint32_t N, const uint32_t *CIs,
       }
     }
   }
-  return status;
+  return cell_status ? cell_status : status;
 }
 
 static void zero3(float *p, size_t s) { p[3 * s] = p[3 * s + 1] = p[3 * s + 2] = 0.0f; }
@@ -216,6 +233,7 @@ void orc_bonds_construct(const float *R, const uint32_t *IL, uint8_t *ILtype,
                          float *iL, float *rij, float *Oijt, float *Fijn,
                          float *Fijt, float *Mijn, float *Mijt, float *Mijadd,
                          uint32_t N, uint32_t K, float bdd) {
+#pragma omp parallel for schedule(static)
   for (uint32_t i = 0; i < N; ++i) {
     for (uint32_t k = 0; k < K; ++k) {
       size_t s = (size_t)i * K + k;
@@ -250,6 +268,7 @@ void orc_delete_far_links(const float *R, uint32_t N, uint32_t *IL,
                           uint8_t *ILtype, float *rij, float *Oijt, float *Fijn,
                           float *Fijt, float *Mijn, float *Mijt, float *Mijadd,
                           uint32_t K, float aa_farcut) {
+#pragma omp parallel for schedule(static)
   for (uint32_t i = 0; i < N; ++i) {
     for (uint32_t k = 0; k < K; ++k) {
       size_t s = (size_t)i * K + k;
@@ -280,6 +299,7 @@ void orc_forces_bpm(const float *R, const float *V, const float *W,
                     float *M, uint32_t N, uint32_t K, float dt, float E, float G,
                     float b_r, float sc, float tc) {
   const size_t N1 = N, N2 = 2 * (size_t)N;
+#pragma omp parallel for schedule(static)
   for (uint32_t i = 0; i < N; ++i) {
     float fsx = 0, fsy = 0, fsz = 0, msx = 0, msy = 0, msz = 0;
     float Sc = 0, Mc = 0;
@@ -399,6 +419,7 @@ void orc_forces_contact(const float *R, const float *V, const float *W,
                         float Estar, float Gstar, float pA, float mu, float mur,
                         float m, float r) {
   const size_t N1 = N, N2 = 2 * (size_t)N;
+#pragma omp parallel for schedule(static)
   for (uint32_t i = 0; i < N; ++i) {
     float fsx = 0, fsy = 0, fsz = 0, msx = 0, msy = 0, msz = 0;
     for (uint32_t k = 0; k < K; ++k) {
@@ -544,6 +565,7 @@ void orc_check_break(const float *rij, float *Oijt, const float *Fijn,
 /* md_calculateincrementsdem.cu:19-49 */
 void orc_increments(const float *F, float *V, float *R, const float *M, float *W,
                     uint32_t N, float dt_m, float dt, float dt_I) {
+#pragma omp parallel for schedule(static)
   for (uint32_t i = 0; i < N; ++i) {
     for (int d = 0; d < 3; ++d) V[i + (size_t)d * N] += F[i + (size_t)d * N] * dt_m;
     for (int d = 0; d < 3; ++d) R[i + (size_t)d * N] += V[i + (size_t)d * N] * dt;
@@ -638,6 +660,7 @@ void orc_platen_brazil(float *R, float *V, float *F, uint32_t N, float cx,
 void orc_cut_restriction(float *R, float *V, float *F, uint32_t N,
                          const float center[3], const float sized2[3],
                          const float hp[3]) {
+#pragma omp parallel for schedule(static)
   for (uint32_t i = 0; i < N; ++i) {
     int out = 0;
     for (int d = 0; d < 3; ++d) {
@@ -901,4 +924,15 @@ void orc_native_contacts_f64(const double *R, const double *V, const double *W,
       M[i + (size_t)d * N] += m;
     }
   }
+}
+
+/* threads the per-particle loops run on: 1 in the plain build, the OpenMP
+ * team size in the -fopenmp build (the loops are independent per particle,
+ * so both builds give the same bits) */
+int orc_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
 }

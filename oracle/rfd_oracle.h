@@ -131,4 +131,8 @@ void orc_native_contacts_f64(const double *R, const double *V, const double *W,
 #ifdef __cplusplus
 }
 #endif
+/* threads of the per-particle loops (1, or the OpenMP team of the -fopenmp
+ * build, librfd_oracle_omp.so) */
+int orc_threads(void);
+
 #endif
