@@ -544,10 +544,13 @@ __device__ __forceinline__ void fx_wave_add(unsigned long long *acc, long long q
 // platen clamp + reaction sums (md_uniaxialcompression3_simple.cu:19-76,
 // md_braziltest_simple2.cu:19-92).  MODE 1: load axis z, 2: load axis y.
 // VT is the velocity/force vector type (FP32 v3 or FP64 d3); AT the
-// accumulator (double, or fixed-point long long).
+// accumulator (double, or fixed-point long long).  friction = false drops the
+// Brazilian platens' sliding friction against the in-plane velocity, which
+// quasi-static (FIRE) passes leave out like the contact law's
+// velocity-dependent terms.
 template <int MODE, class VT, class AT>
 __device__ __forceinline__ void platen(const rfd_loading &L, P4 &R, VT &V, VT &F, AT &sb,
-                                       AT &stp) {
+                                       AT &stp, bool friction = true) {
   using T = decltype(V.x);
   double ra, rb, h, zc;
   if (MODE == 1) {
@@ -570,7 +573,7 @@ __device__ __forceinline__ void platen(const rfd_loading &L, P4 &R, VT &V, VT &F
     const T f = Fa;
     if (f > 0) {
       acc_add(stp, f);
-      if (MODE == 2) {
+      if (MODE == 2 && friction) {
         T vv = V.x * V.x + V.z * V.z;
         if ((double)vv > 1e-9) {
           vv = platen_rsqrt(vv);
@@ -589,7 +592,7 @@ __device__ __forceinline__ void platen(const rfd_loading &L, P4 &R, VT &V, VT &F
     const T f = Fa;
     if (f < 0) {
       acc_add(sb, f);
-      if (MODE == 2) {
+      if (MODE == 2 && friction) {
         T vv = V.x * V.x + V.z * V.z;
         if ((double)vv > 1e-9) {
           vv = platen_rsqrt(vv);
@@ -1682,7 +1685,7 @@ __global__ void __launch_bounds__(256) k_fire_measure(FireParticles p) {
     if (MODE) {
       const v3 V0 = V, F0 = F;
       const double z0 = R.z, y0 = R.y;
-      platen<MODE>(L, R, V, F, v[7], v[8]);
+      platen<MODE>(L, R, V, F, v[7], v[8], false);
       if (R.z != z0 || R.y != y0) p.P[i] = R;
       if (V.x != V0.x || V.y != V0.y || V.z != V0.z) p.Vf[i] = f4(V);
       if (F.x != F0.x || F.y != F0.y || F.z != F0.z) p.F[i] = f4(F);
@@ -2333,7 +2336,7 @@ __global__ void __launch_bounds__(256) k_fire64_measure(FireParticles64 p) {
     if (MODE) {
       const d3 V0 = V, F0 = F;
       const double z0 = R.z, y0 = R.y;
-      platen<MODE>(L, R, V, F, v[7], v[8]);
+      platen<MODE>(L, R, V, F, v[7], v[8], false);
       if (R.z != z0 || R.y != y0) p.P[i] = R;
       if (V.x != V0.x || V.y != V0.y || V.z != V0.z) p.Vf[i] = d4(V);
       if (F.x != F0.x || F.y != F0.y || F.z != F0.z) p.F[i] = d4(F);

@@ -175,17 +175,20 @@ def brazil_batch(ctx, S: int = 256, seed: int = specimen.BASE_SEED, n_lo: float 
     return BatchCase(sysn, mat, first, n_target, D, T, mu_t, seeds, loadings, bond_first)
 
 
-def batch_member(ctx, case: BatchCase, k: int) -> NativeSystem:
+def batch_member(ctx, case: BatchCase, k: int, fp64: bool = False) -> NativeSystem:
     """Specimen k of a batch as a system of its own, at the same place, with
     the batch's cell grid, bonds and loading control: stepped alone it follows
     the batch's specimen k bit for bit.  Call it before the batch steps (the
-    bond rest lengths are taken from the current positions)."""
+    bond rest lengths are taken from the current positions).  fp64: on the
+    FP64 path instead."""
     bs = case.sys
     a, b = int(case.first[k]), int(case.first[k + 1])
     st = bs.state()
     N = bs.N
     R = st["R"].reshape(3, N).T[a:b]
-    sysn = NativeSystem(ctx, bs.params, R, bs.radius[a:b])
+    params = type(bs.params).from_buffer_copy(bs.params)
+    params.fp64 = int(fp64)
+    sysn = NativeSystem(ctx, params, R, bs.radius[a:b])
     origin, cell, dims = bs.grid()
     sysn.set_grid(origin, cell, dims)
     bb = bs.bonds

@@ -111,3 +111,33 @@ def test_fire_load_steps_fp32_vs_fp64(ctx):
     assert scale > 0
     np.testing.assert_allclose(res[False], res[True], rtol=0, atol=1e-4 * scale)
     assert ctx.flags() == 0
+
+
+def test_fire_brazil_fp32_vs_fp64(ctx):
+    """The same comparison on a Brazilian disc (platens along y, the FIRE
+    start, measure and update of loading mode 2, frictionless platens), with
+    bonds too strong to break, relaxed to |F|max < 1e-6 F_ref: every step
+    converges on both paths and the reactions agree to 1e-3 of the largest
+    (the spread of where two relaxations stop, N tol F_ref, against the
+    reaction of a point-loaded disc)."""
+    from rfd import native
+
+    case = native_cases.brazil_batch(ctx, S=1, seed=13, n_lo=1500, n_hi=1600,
+                                     mu_range=(1e3, 1e3), hist_cap=4096)
+    res = {}
+    for fp64 in (False, True):
+        s = native_cases.batch_member(ctx, case, 0, fp64=fp64)
+        fp = native.fire_params(case.mat.E, 0.5, case.mat.dt, rel_tol=1e-6, max_iter=50000,
+                                check_every=200)
+        rr = [s.fire_load_step(-1e-4, fp) for _ in range(4)]
+        print(f"brazil fp64={fp64}: N={s.N} iterations {[r.iterations for r in rr]}, "
+              f"fmax {[r.fmax for r in rr]}", flush=True)
+        assert all(r.converged and r.fmax < fp.ftol for r in rr), [r.as_dict() for r in rr]
+        res[fp64] = np.array([[r.sum_bottom, r.sum_top] for r in rr])
+        s.close()
+    scale = np.abs(res[True]).max()
+    print(f"reactions: FP64 {res[True][:, 1]}, FP32 - FP64 max "
+          f"{np.abs(res[False] - res[True]).max() / scale:.2e} of the largest", flush=True)
+    assert scale > 0
+    np.testing.assert_allclose(res[False], res[True], rtol=0, atol=1e-3 * scale)
+    assert ctx.flags() == 0
