@@ -1716,9 +1716,12 @@ __global__ void __launch_bounds__(256) k_fire_measure(FireParticles p) {
   }
 }
 
-// reduce the partials in a fixed order, then one thread runs the FIRE logic
+// reduce the partials in a fixed order, then one thread runs the FIRE logic;
+// curp, when set, is the bond-state index the iteration's force pass left to
+// be flipped
 __global__ void __launch_bounds__(256) k_fire_control(FireDev *fire, const double *part,
-                                                      int nblocks, rfd_loading *ctl) {
+                                                      int nblocks, rfd_loading *ctl,
+                                                      int *curp) {
   __shared__ double s_part[4 * kFireVals];
   if (fire->done) return;
   double v[kFireVals] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -1733,6 +1736,7 @@ __global__ void __launch_bounds__(256) k_fire_control(FireDev *fire, const doubl
     for (int q = 0; q < kFireVals; ++q) s_part[wid * kFireVals + q] = v[q];
   __syncthreads();
   if (threadIdx.x != 0) return;
+  if (curp) *curp = 1 - *curp;
   for (int q = 0; q < kFireVals; ++q) {
     double a = 0;
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
@@ -2745,7 +2749,7 @@ Mat64 make_mat64(const rfd_system_params &p) {
 
 // FP64 force pass; dtp, trial, nobreak and done as in forces() (FIRE passes)
 int forces64(rfd_system *s, const double *dtp = nullptr, bool trial = false,
-             bool nobreak = false, const int *done = nullptr) {
+             bool nobreak = false, const int *done = nullptr, bool flip = true) {
   Sys64 d{};
   d.N = s->N;
   d.Kc = s->p.contact_capacity;
@@ -2784,7 +2788,7 @@ int forces64(rfd_system *s, const double *dtp = nullptr, bool trial = false,
   k_f64_forces<<<rfd_blocks(s->N, 128), 128, 0, s->ctx->stream>>>(d, m);
   RFD_HIP(s->ctx, hipGetLastError());
   if (e1) RFD_HIP(s->ctx, hipEventRecord(e1, s->ctx->stream));
-  if (!trial) k_flip_state<<<1, 64, 0, s->ctx->stream>>>(s->d_cur, done);
+  if (!trial && flip) k_flip_state<<<1, 64, 0, s->ctx->stream>>>(s->d_cur, done);
   RFD_HIP(s->ctx, hipGetLastError());
   return RFD_OK;
 }
@@ -2855,14 +2859,15 @@ int ensure_work(rfd_system *s) {
   return build_work_fp<64>(s);
 }
 
-// flip = false leaves the flip of the bond-state index to the step's
-// loading control (FP32 explicit steps)
+// flip = false leaves the flip of the bond-state index to the kernel that
+// follows (the loading control of FP32 explicit steps, the FIRE control)
 int forces(rfd_system *s, const float *dtp = nullptr, bool trial = false,
            bool nobreak = false, const AdvDev *adv = nullptr, bool flip = true) {
   if (s->fp64)
     return forces64(s, dtp ? s->fire_dt64 : nullptr, trial, nobreak,
                     dtp ? (const int *)((const char *)s->fire + offsetof(FireDev, done))
-                        : nullptr);
+                        : nullptr,
+                    flip);
   SysDev d = make_dev(s);
   d.dtp = dtp;
   d.trial = trial ? 1 : 0;
@@ -2984,7 +2989,7 @@ int step_plain(rfd_system *s) {
     if (st) return st;
   }
   // the FP32 control flips the bond-state index (the FP64 pass flips it itself)
-  int st = forces(s, nullptr, false, false, nullptr, false);
+  int st = forces(s, nullptr, false, false, nullptr, s->fp64);
   if (!st) st = pass_and_control(s);
   if (st) return st;
   s->step_count++;
@@ -3137,6 +3142,7 @@ void fire_rest(rfd_system *s, hipStream_t q, bool fire_velocity) {
 // one FIRE iteration: force pass at the current configuration, constraint +
 // reductions, control, update
 int fire_iteration(rfd_system *s) {
+  int *flip = nullptr;
   if (s->forces_current) {
     s->forces_current = false;  // the load step's first pass already ran
   } else {
@@ -3144,12 +3150,14 @@ int fire_iteration(rfd_system *s) {
       int st = rebuild_rows(s);
       if (st) return st;
     }
-    int st = forces(s, s->fire_dt, false, true);
+    // the FIRE control flips the bond-state index after this pass
+    int st = forces(s, s->fire_dt, false, true, nullptr, false);
     if (st) return st;
+    flip = s->d_cur;
   }
   hipStream_t q = s->ctx->stream;
   fire_measure(s, q);
-  k_fire_control<<<1, 256, 0, q>>>(s->fire, s->fire_part, (int)s->nblocks_pass, s->ctl);
+  k_fire_control<<<1, 256, 0, q>>>(s->fire, s->fire_part, (int)s->nblocks_pass, s->ctl, flip);
   if (s->fp64)
     k_fire64_update<<<s->nblocks_pass, 256, 0, q>>>(fire_particles64(s));
   else
@@ -3198,7 +3206,8 @@ int begin_pass(rfd_system *s, const FireDev &f, const float *dU, const float *dT
     fm.measure_only = 1;
     RFD_HIP(ctx, hipMemcpyAsync(s->fire, &fm, sizeof(fm), hipMemcpyHostToDevice, q));
     fire_measure(s, q);
-    k_fire_control<<<1, 256, 0, q>>>(s->fire, s->fire_part, (int)s->nblocks_pass, s->ctl);
+    k_fire_control<<<1, 256, 0, q>>>(s->fire, s->fire_part, (int)s->nblocks_pass, s->ctl,
+                                    nullptr);
     RFD_HIP(ctx, hipGetLastError());
     FireDev r;
     RFD_HIP(ctx, hipMemcpyAsync(&r, s->fire, sizeof(r), hipMemcpyDeviceToHost, q));
@@ -4760,7 +4769,8 @@ int rfd_system_residual(rfd_system *s, double out[2]) {
     k_fire_measure<2><<<s->nblocks_pass, 256, 0, q>>>(fp);
   else
     k_fire_measure<0><<<s->nblocks_pass, 256, 0, q>>>(fp);
-  k_fire_control<<<1, 256, 0, q>>>(s->fire, s->fire_part, (int)s->nblocks_pass, s->ctl);
+  k_fire_control<<<1, 256, 0, q>>>(s->fire, s->fire_part, (int)s->nblocks_pass, s->ctl,
+                                    nullptr);
   RFD_HIP(ctx, hipGetLastError());
   RFD_HIP(ctx, hipMemcpyAsync(&f, s->fire, sizeof(f), hipMemcpyDeviceToHost, q));
   RFD_HIP(ctx, hipStreamSynchronize(q));
