@@ -1,7 +1,9 @@
 """Build librfd.so (HIP, gfx950) in-tree.
 
 Every .hip file under rockfit-dem_amd/csrc is compiled for gfx950 and linked
-into rockfit-dem_amd/rfd/librfd.so.  Flags that matter for results:
+into rockfit-dem_amd/rfd/librfd.so; the C++ host driver under
+rockfit-dem_amd/host is linked against it as rockfit-dem_amd/host/rfd_run.
+Flags that matter for results:
 
 * -ffp-contract=off: no FMA contraction, every FP32 product and sum rounds on
   its own, in the order written (the reference-layout kernels are bit-exact
@@ -23,6 +25,11 @@ CSRC = PKG.parent / "csrc"
 INCLUDE = PKG.parent.parent / "include"
 LIB = PKG / "librfd.so"
 OBJ = PKG.parent / "_build"
+HOST = PKG.parent / "host"
+RUN = HOST / "rfd_run"
+# host code: the same no-contraction rule, so that its FP64 parameter
+# arithmetic rounds as written
+HOST_FLAGS = ["-O2", "-std=c++17", "-ffp-contract=off", "-Wall", f"-I{INCLUDE}"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -69,9 +76,23 @@ def build(verbose: bool = False, jobs: int | None = None) -> pathlib.Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
     os.replace(tmp, LIB)
+    build_host()
     if verbose:
-        print(f"built {LIB} from {len(srcs)} sources", file=sys.stderr)
+        print(f"built {LIB} from {len(srcs)} sources, and {RUN}", file=sys.stderr)
     return LIB
+
+
+def build_host() -> pathlib.Path:
+    """Link the C++ host driver (host/*.cpp) against librfd.so."""
+    srcs = sorted(HOST.glob("*.cpp"))
+    tmp = RUN.with_suffix(".tmp")
+    cmd = [HIPCC, *HOST_FLAGS, *map(str, srcs), "-o", str(tmp), f"-L{PKG}", "-lrfd",
+           "-Wl,-rpath,$ORIGIN/../rfd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host driver build failed:\n{r.stderr[-4000:]}")
+    os.replace(tmp, RUN)
+    return RUN
 
 
 if __name__ == "__main__":
