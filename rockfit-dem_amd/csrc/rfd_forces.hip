@@ -1,18 +1,17 @@
 // rfd_forces.hip -- bond and contact forces in the reference slot layout.
 //
-// One thread per particle walks its K-slot row in slot order, so that the
-// per-particle sums F += fs, M += ms are formed in the same order as
-// md_calculateforcesdem3.cu:437-480 and :353-419 and are therefore bit-exact
-// against a sequential restatement.  The row's IL and ILtype are staged in
-// LDS with coalesced loads; slot state (float3 AoS) is read and written only
-// for the slots that are live.
+// One thread per slot evaluates the law of its slot (coalesced loads and
+// stores of the slot state across a row) and leaves the terms the reference
+// adds to the particle's sums in LDS; one thread per particle then adds them
+// in slot order, term by term, so that F += fs, M += ms are formed in the same
+// order as md_calculateforcesdem3.cu:437-480 and :353-419 and are bit-exact
+// against a sequential restatement.
 #include "rfd_internal.h"
 
 using namespace rfd;
 
 namespace {
 
-constexpr int kRowThreads = 64;
 constexpr int kMaxK = 64;
 
 struct f3 {
@@ -27,22 +26,15 @@ __device__ __forceinline__ void st3(float *p, size_t s, float x, float y, float 
   p[3 * s + 2] = z;
 }
 
-// stage the IL / ILtype rows of the block's particles in LDS
-__device__ __forceinline__ void stage_rows(const uint32_t *__restrict__ IL,
-                                           const uint8_t *__restrict__ ILtype,
-                                           uint32_t *s_il, uint8_t *s_ty, uint32_t p0,
-                                           uint32_t np, uint32_t K) {
-  const size_t g0 = (size_t)p0 * K;
-  for (uint32_t e = threadIdx.x; e < np * K; e += blockDim.x) {
-    const uint32_t at = (e / K) * (K + 1) + (e % K);  // padded row stride
-    s_il[at] = IL[g0 + e];
-    s_ty[at] = ILtype[g0 + e];
-  }
-  __syncthreads();
-}
+// d_CalculateForcesDEM_22 (md_calculateforcesdem3.cu:424-483).  One thread
+// per slot evaluates the bond law of its slot and keeps the five terms the
+// reference adds to the particle's sums in LDS; then one thread per particle
+// adds them in slot order, term by term, as the reference's row loop does
+// (same bits as a sequential walk of the row, with every slot's loads and
+// stores coalesced across the row).
+constexpr int kSlotThreads = 256;
 
-// d_CalculateForcesDEM_22 (md_calculateforcesdem3.cu:424-483)
-__global__ void __launch_bounds__(kRowThreads)
+__global__ void __launch_bounds__(kSlotThreads)
 k_forces_bpm(const float *__restrict__ R, const float *__restrict__ V,
              const float *__restrict__ W, const uint32_t *__restrict__ IL,
              uint8_t *__restrict__ ILtype, const float *__restrict__ iL,
@@ -50,233 +42,283 @@ k_forces_bpm(const float *__restrict__ R, const float *__restrict__ V,
              float *__restrict__ Mijt, float *__restrict__ F, float *__restrict__ M,
              uint32_t N, uint32_t K, float dt, float E, float G, float b_r, float scrit,
              float tcrit) {
-  __shared__ uint32_t s_il[kRowThreads * (kMaxK + 1)];
-  __shared__ uint8_t s_ty[kRowThreads * (kMaxK + 1)];
-  const uint32_t p0 = blockIdx.x * kRowThreads;
-  const uint32_t np = min((uint32_t)kRowThreads, N - p0);
-  stage_rows(IL, ILtype, s_il, s_ty, p0, np, K);
-  const uint32_t t = threadIdx.x;
-  if (t >= np) return;
-  const uint32_t i = p0 + t;
+  // per slot: normal force, twisting and bending moments, tangential force,
+  // moment of the tangential force (3 floats each)
+  __shared__ float s_c[15][kSlotThreads];
+  __shared__ uint8_t s_live[kSlotThreads];
+  const uint32_t P = kSlotThreads / K;  // particles per workgroup
+  const uint32_t t = threadIdx.x, lp = t / K, k = t % K;
+  const uint32_t p0 = blockIdx.x * P;
+  const uint32_t i = p0 + lp;
   const size_t N1 = N, N2 = 2 * (size_t)N;
-  const float Rix = R[i], Riy = R[i + N1], Riz = R[i + N2];
-  const float Vix = V[i], Viy = V[i + N1], Viz = V[i + N2];
-  const float Wix = W[i], Wiy = W[i + N1], Wiz = W[i + N2];
   const float area_k = b_r * b_r * kPiF;          // b_r*b_r*MCf_pi
   const float j4 = kPiF * b_r * b_r * b_r * b_r;  // MCf_pi*b_r^4
-  float fsx = 0, fsy = 0, fsz = 0, msx = 0, msy = 0, msz = 0;
-  float Sc = 0, Mc = 0;
-  for (uint32_t k = 0; k < K; ++k) {
-    const uint32_t j = s_il[t * (K + 1) + k];
-    if (j == kSentinel || s_ty[t * (K + 1) + k] != 1) continue;
+  bool live = false;
+  if (lp < P && i < N) {
     const size_t s = (size_t)i * K + k;
-    // dd_Calculate_rijm_nij_2 (:38-52)
-    const float rx = R[j] - Rix, ry = R[j + N1] - Riy, rz = R[j + N2] - Riz;
-    const float rm = norm3(rx, ry, rz);
-    const float inv_rm = 1.0f / rm;
-    const float nx = rx * inv_rm, ny = ry * inv_rm, nz = rz * inv_rm;
-    const float il = iL[s];
-    // dd_Calculate_fijn_bpm_2 (:91-103)
-    float cef = E * (rm - 1.0f / il) * il;
-    Sc = cef;
-    cef = cef * area_k;
-    fsx = fsx + nx * cef;
-    fsy = fsy + ny * cef;
-    fsz = fsz + nz * cef;
-    // dd_Calculate_M123_bpm_2 (:54-89)
-    const f3 np_ = ld3(rij, s);
-    const float tx = np_.y * nz - np_.z * ny;
-    const float ty = np_.z * nx - np_.x * nz;
-    const float tz = np_.x * ny - np_.y * nx;
-    const float Wjx = W[j], Wjy = W[j + N1], Wjz = W[j + N2];
-    const float c2 = ((Wjx + Wix) * nx + (Wjy + Wiy) * ny + (Wjz + Wiz) * nz) * dt * 0.5f;
-    const float phx = nx * c2, phy = ny * c2, phz = nz * c2;
-    st3(rij, s, nx, ny, nz);
-    const float M1x = 1.0f + tz * phz + ty * phy;
-    const float M1y = phz - tz - ty * phx;
-    const float M1z = -phy - tz * phx + ty;
-    const float M2x = tz - phz - tx * phy;
-    const float M2y = tz * phz + 1.0f + tx * phx;
-    const float M2z = -tz * phy + phx - tx;
-    const float M3x = -ty - tx * phz + phy;
-    const float M3y = -ty * phz + tx - phx;
-    const float M3z = ty * phy + tx * phx + 1.0f;
-    // dd_Calculate_mijn_bpm_2 (:105-129)
-    const float dwx = Wjx - Wix, dwy = Wjy - Wiy, dwz = Wjz - Wiz;
-    float c1 = nx * dwx + ny * dwy + nz * dwz;
-    const float dotw = c1;
-    c1 = c1 * (dt * 0.5f * G * il);
-    const f3 mn0 = ld3(Mijn, s);
-    const float mnx = M1x * mn0.x + M1y * mn0.y + M1z * mn0.z + nx * c1;
-    const float mny = M2x * mn0.x + M2y * mn0.y + M2z * mn0.z + ny * c1;
-    const float mnz = M3x * mn0.x + M3y * mn0.y + M3z * mn0.z + nz * c1;
-    msx = msx + mnx * j4;
-    msy = msy + mny * j4;
-    msz = msz + mnz * j4;
-    // dd_Calculate_mijt_bpm_2 (:131-152)
-    float c2t = dt * 0.25f * E * il;
-    const f3 mt0 = ld3(Mijt, s);
-    const float mtx = M1x * mt0.x + M1y * mt0.y + M1z * mt0.z + (dwx - nx * dotw) * c2t;
-    const float mty = M2x * mt0.x + M2y * mt0.y + M2z * mt0.z + (dwy - ny * dotw) * c2t;
-    const float mtz = M3x * mt0.x + M3y * mt0.y + M3z * mt0.z + (dwz - nz * dotw) * c2t;
-    c2t = c2t * j4;
-    msx = msx + mtx * c2t;
-    msy = msy + mty * c2t;
-    msz = msz + mtz * c2t;
-    // dd_Calculate_fijt_mijunsym_bpm_2 (:154-197)
-    const float hx = 0.5f * (Wjx + Wix), hy = 0.5f * (Wjy + Wiy), hz = 0.5f * (Wjz + Wiz);
-    const float vx = V[j] - Vix - hy * rz + hz * ry;
-    const float vy = V[j + N1] - Viy - hz * rx + hx * rz;
-    const float vz = V[j + N2] - Viz - hx * ry + hy * rx;
-    const float cv = nx * vx + ny * vy + nz * vz;
-    const f3 o0 = ld3(Oijt, s);
-    const float ox = (M1x * o0.x + M1y * o0.y + M1z * o0.z) + (vx - nx * cv) * dt;
-    const float oy = (M2x * o0.x + M2y * o0.y + M2z * o0.z) + (vy - ny * cv) * dt;
-    const float oz = (M3x * o0.x + M3y * o0.y + M3z * o0.z) + (vz - nz * cv) * dt;
-    float ct = G * il;
-    Mc = ct;
-    ct = ct * area_k;
-    fsx = fsx + ox * ct;
-    fsy = fsy + oy * ct;
-    fsz = fsz + oz * ct;
-    msx = msx + 0.5f * (ry * oz - rz * oy) * ct;
-    msy = msy + 0.5f * (rz * ox - rx * oz) * ct;
-    msz = msz + 0.5f * (rx * oy - ry * ox) * ct;
-    // dd_CheckBreakConditionDEM_2 (:20-36)
-    Mc = Mc * norm3(ox, oy, oz);
-    Sc = Sc + norm3(mtx, mty, mtz) * 4.0f * b_r;
-    Mc = Mc + norm3(mnx, mny, mnz) * 2.0f * b_r;
-    if (Sc > scrit || Mc > tcrit) {
-      ILtype[s] = 0;
-      st3(Oijt, s, 0.0f, 0.0f, 0.0f);
-      st3(Mijn, s, 0.0f, 0.0f, 0.0f);
-      st3(Mijt, s, 0.0f, 0.0f, 0.0f);
-    } else {
-      st3(Oijt, s, ox, oy, oz);
-      st3(Mijn, s, mnx, mny, mnz);
-      st3(Mijt, s, mtx, mty, mtz);
+    const uint32_t j = IL[s];
+    live = j != kSentinel && ILtype[s] == 1;
+    if (live) {
+      const float Rix = R[i], Riy = R[i + N1], Riz = R[i + N2];
+      const float Vix = V[i], Viy = V[i + N1], Viz = V[i + N2];
+      const float Wix = W[i], Wiy = W[i + N1], Wiz = W[i + N2];
+      // dd_Calculate_rijm_nij_2 (:38-52)
+      const float rx = R[j] - Rix, ry = R[j + N1] - Riy, rz = R[j + N2] - Riz;
+      const float rm = norm3(rx, ry, rz);
+      const float inv_rm = 1.0f / rm;
+      const float nx = rx * inv_rm, ny = ry * inv_rm, nz = rz * inv_rm;
+      const float il = iL[s];
+      // dd_Calculate_fijn_bpm_2 (:91-103)
+      float cef = E * (rm - 1.0f / il) * il;
+      float Sc = cef;
+      cef = cef * area_k;
+      s_c[0][t] = nx * cef;
+      s_c[1][t] = ny * cef;
+      s_c[2][t] = nz * cef;
+      // dd_Calculate_M123_bpm_2 (:54-89)
+      const f3 np_ = ld3(rij, s);
+      const float tx = np_.y * nz - np_.z * ny;
+      const float ty = np_.z * nx - np_.x * nz;
+      const float tz = np_.x * ny - np_.y * nx;
+      const float Wjx = W[j], Wjy = W[j + N1], Wjz = W[j + N2];
+      const float c2 = ((Wjx + Wix) * nx + (Wjy + Wiy) * ny + (Wjz + Wiz) * nz) * dt * 0.5f;
+      const float phx = nx * c2, phy = ny * c2, phz = nz * c2;
+      st3(rij, s, nx, ny, nz);
+      const float M1x = 1.0f + tz * phz + ty * phy;
+      const float M1y = phz - tz - ty * phx;
+      const float M1z = -phy - tz * phx + ty;
+      const float M2x = tz - phz - tx * phy;
+      const float M2y = tz * phz + 1.0f + tx * phx;
+      const float M2z = -tz * phy + phx - tx;
+      const float M3x = -ty - tx * phz + phy;
+      const float M3y = -ty * phz + tx - phx;
+      const float M3z = ty * phy + tx * phx + 1.0f;
+      // dd_Calculate_mijn_bpm_2 (:105-129)
+      const float dwx = Wjx - Wix, dwy = Wjy - Wiy, dwz = Wjz - Wiz;
+      float c1 = nx * dwx + ny * dwy + nz * dwz;
+      const float dotw = c1;
+      c1 = c1 * (dt * 0.5f * G * il);
+      const f3 mn0 = ld3(Mijn, s);
+      const float mnx = M1x * mn0.x + M1y * mn0.y + M1z * mn0.z + nx * c1;
+      const float mny = M2x * mn0.x + M2y * mn0.y + M2z * mn0.z + ny * c1;
+      const float mnz = M3x * mn0.x + M3y * mn0.y + M3z * mn0.z + nz * c1;
+      s_c[3][t] = mnx * j4;
+      s_c[4][t] = mny * j4;
+      s_c[5][t] = mnz * j4;
+      // dd_Calculate_mijt_bpm_2 (:131-152)
+      float c2t = dt * 0.25f * E * il;
+      const f3 mt0 = ld3(Mijt, s);
+      const float mtx = M1x * mt0.x + M1y * mt0.y + M1z * mt0.z + (dwx - nx * dotw) * c2t;
+      const float mty = M2x * mt0.x + M2y * mt0.y + M2z * mt0.z + (dwy - ny * dotw) * c2t;
+      const float mtz = M3x * mt0.x + M3y * mt0.y + M3z * mt0.z + (dwz - nz * dotw) * c2t;
+      c2t = c2t * j4;
+      s_c[6][t] = mtx * c2t;
+      s_c[7][t] = mty * c2t;
+      s_c[8][t] = mtz * c2t;
+      // dd_Calculate_fijt_mijunsym_bpm_2 (:154-197)
+      const float hx = 0.5f * (Wjx + Wix), hy = 0.5f * (Wjy + Wiy), hz = 0.5f * (Wjz + Wiz);
+      const float vx = V[j] - Vix - hy * rz + hz * ry;
+      const float vy = V[j + N1] - Viy - hz * rx + hx * rz;
+      const float vz = V[j + N2] - Viz - hx * ry + hy * rx;
+      const float cv = nx * vx + ny * vy + nz * vz;
+      const f3 o0 = ld3(Oijt, s);
+      const float ox = (M1x * o0.x + M1y * o0.y + M1z * o0.z) + (vx - nx * cv) * dt;
+      const float oy = (M2x * o0.x + M2y * o0.y + M2z * o0.z) + (vy - ny * cv) * dt;
+      const float oz = (M3x * o0.x + M3y * o0.y + M3z * o0.z) + (vz - nz * cv) * dt;
+      float ct = G * il;
+      float Mc = ct;
+      ct = ct * area_k;
+      s_c[9][t] = ox * ct;
+      s_c[10][t] = oy * ct;
+      s_c[11][t] = oz * ct;
+      s_c[12][t] = 0.5f * (ry * oz - rz * oy) * ct;
+      s_c[13][t] = 0.5f * (rz * ox - rx * oz) * ct;
+      s_c[14][t] = 0.5f * (rx * oy - ry * ox) * ct;
+      // dd_CheckBreakConditionDEM_2 (:20-36)
+      Mc = Mc * norm3(ox, oy, oz);
+      Sc = Sc + norm3(mtx, mty, mtz) * 4.0f * b_r;
+      Mc = Mc + norm3(mnx, mny, mnz) * 2.0f * b_r;
+      if (Sc > scrit || Mc > tcrit) {
+        ILtype[s] = 0;
+        st3(Oijt, s, 0.0f, 0.0f, 0.0f);
+        st3(Mijn, s, 0.0f, 0.0f, 0.0f);
+        st3(Mijt, s, 0.0f, 0.0f, 0.0f);
+      } else {
+        st3(Oijt, s, ox, oy, oz);
+        st3(Mijn, s, mnx, mny, mnz);
+        st3(Mijt, s, mtx, mty, mtz);
+      }
     }
   }
-  F[i] += fsx;
-  F[i + N1] += fsy;
-  F[i + N2] += fsz;
-  M[i] += msx;
-  M[i + N1] += msy;
-  M[i + N2] += msz;
+  s_live[t] = live ? 1 : 0;
+  __syncthreads();
+  // the row sums, slot by slot and term by term in the reference's order
+  const uint32_t ip = p0 + t;
+  if (t >= P || ip >= N) return;
+  float fsx = 0, fsy = 0, fsz = 0, msx = 0, msy = 0, msz = 0;
+  for (uint32_t q = t * K; q < (t + 1) * K; ++q) {
+    if (!s_live[q]) continue;
+    fsx = fsx + s_c[0][q];
+    fsy = fsy + s_c[1][q];
+    fsz = fsz + s_c[2][q];
+    msx = msx + s_c[3][q];
+    msy = msy + s_c[4][q];
+    msz = msz + s_c[5][q];
+    msx = msx + s_c[6][q];
+    msy = msy + s_c[7][q];
+    msz = msz + s_c[8][q];
+    fsx = fsx + s_c[9][q];
+    fsy = fsy + s_c[10][q];
+    fsz = fsz + s_c[11][q];
+    msx = msx + s_c[12][q];
+    msy = msy + s_c[13][q];
+    msz = msz + s_c[14][q];
+  }
+  F[ip] += fsx;
+  F[ip + N1] += fsy;
+  F[ip + N2] += fsz;
+  M[ip] += msx;
+  M[ip + N1] += msy;
+  M[ip + N2] += msz;
 }
 
-// d_CalculateForcesDEM_21 (md_calculateforcesdem3.cu:338-422)
-__global__ void __launch_bounds__(kRowThreads)
+// d_CalculateForcesDEM_21 (md_calculateforcesdem3.cu:338-422), slot-parallel
+// like k_forces_bpm: the terms of each contact slot go to LDS, the row sums
+// are formed in slot order, the rolling-moment wipe (:330-335) included.
+__global__ void __launch_bounds__(kSlotThreads)
 k_forces_contact(const float *__restrict__ R, const float *__restrict__ V,
                  const float *__restrict__ W, const uint32_t *__restrict__ IL,
                  const uint8_t *__restrict__ ILtype, float *__restrict__ Oijt,
                  float *__restrict__ F, float *__restrict__ M, uint32_t N, uint32_t K,
                  float dt, float Estar, float Gstar, float pA, float mu, float mur,
                  float m, float r) {
-  __shared__ uint32_t s_il[kRowThreads * (kMaxK + 1)];
-  __shared__ uint8_t s_ty[kRowThreads * (kMaxK + 1)];
-  const uint32_t p0 = blockIdx.x * kRowThreads;
-  const uint32_t np = min((uint32_t)kRowThreads, N - p0);
-  stage_rows(IL, ILtype, s_il, s_ty, p0, np, K);
-  const uint32_t t = threadIdx.x;
-  if (t >= np) return;
-  const uint32_t i = p0 + t;
+  // per slot: normal force, tangential force, its moment, rolling moment
+  __shared__ float s_c[12][kSlotThreads];
+  __shared__ uint8_t s_live[kSlotThreads];  // 0 none, 1 contact, 2 contact that wipes ms
+  const uint32_t P = kSlotThreads / K;
+  const uint32_t t = threadIdx.x, lp = t / K, k = t % K;
+  const uint32_t p0 = blockIdx.x * P;
+  const uint32_t i = p0 + lp;
   const size_t N1 = N, N2 = 2 * (size_t)N;
-  const float Rix = R[i], Riy = R[i + N1], Riz = R[i + N2];
-  const float Vix = V[i], Viy = V[i + N1], Viz = V[i + N2];
-  const float Wix = W[i], Wiy = W[i + N1], Wiz = W[i + N2];
-  float fsx = 0, fsy = 0, fsz = 0, msx = 0, msy = 0, msz = 0;
-  for (uint32_t k = 0; k < K; ++k) {
-    const uint32_t j = s_il[t * (K + 1) + k];
+  uint8_t live = 0;
+  if (lp < P && i < N) {
     const size_t s = (size_t)i * K + k;
+    const uint32_t j = IL[s];
     if (j == kSentinel) {
       st3(Oijt, s, 0.0f, 0.0f, 0.0f);
-      continue;
+    } else if (ILtype[s] == 0) {
+      const float Rix = R[i], Riy = R[i + N1], Riz = R[i + N2];
+      const float rx = R[j] - Rix, ry = R[j + N1] - Riy, rz = R[j + N2] - Riz;
+      const float rm = norm3(rx, ry, rz);
+      const float inv_rm = 1.0f / rm;
+      const float nx = rx * inv_rm, ny = ry * inv_rm, nz = rz * inv_rm;
+      if (!(rm < 2.0f * r)) {
+        st3(Oijt, s, 0.0f, 0.0f, 0.0f);
+      } else {
+        const float Vix = V[i], Viy = V[i + N1], Viz = V[i + N2];
+        const float Wix = W[i], Wiy = W[i + N1], Wiz = W[i + N2];
+        // dd_Calculate_fijn_ppHM_2 (:235-252)
+        const float q = sqrtf(r * fabsf(r - 0.5f * rm));
+        const float Kn = 2.0f * Estar * q;
+        const float Kt = 8.0f * Gstar * q;
+        const float Wjx = W[j], Wjy = W[j + N1], Wjz = W[j + N2];
+        const float wsx = Wjx + Wix, wsy = Wjy + Wiy, wsz = Wjz + Wiz;
+        const float vx = V[j] - Vix + r * (ny * wsz - nz * wsy);
+        const float vy = V[j + N1] - Viy + r * (nz * wsx - nx * wsz);
+        const float vz = V[j + N2] - Viz + r * (nx * wsy - ny * wsx);
+        const float vn = nx * vx + ny * vy + nz * vz;
+        const float fn = (rm - 2.0f * r) * Kn * kTwoThirdsF;
+        const float dmp = -1.8257f * pA * vn * sqrtf(Kn * 0.5f * m);
+        s_c[0][t] = nx * (fn + dmp);
+        s_c[1][t] = ny * (fn + dmp);
+        s_c[2][t] = nz * (fn + dmp);
+        const float vtx = vx - nx * vn, vty = vy - ny * vn, vtz = vz - nz * vn;
+        // dd_Calculate_fijtmijt_ppHM_2 (:255-314)
+        const f3 O = ld3(Oijt, s);
+        float c1 = nx * O.x + ny * O.y + nz * O.z;
+        float ox = O.x - nx * c1, oy = O.y - ny * c1, oz = O.z - nz * c1;
+        c1 = ox * ox + oy * oy + oz * oz;
+        if ((double)c1 > 1e-18) {
+          c1 = 1.0f / c1;
+          c1 = c1 * (O.x * O.x + O.y * O.y + O.z * O.z);
+          c1 = sqrtf(c1);
+          ox = ox * c1;
+          oy = oy * c1;
+          oz = oz * c1;
+        }
+        ox = ox + vtx * dt;
+        oy = oy + vty * dt;
+        oz = oz + vtz * dt;
+        float ftx = ox * Kt, fty = oy * Kt, ftz = oz * Kt;
+        const float ftm = ftx * ftx + fty * fty + ftz * ftz;
+        if (ftm > mu * mu * fn * fn) {
+          c1 = mu * fabsf(fn) * rsqrt_rn(ftm);
+          ftx = ftx * c1;
+          fty = fty * c1;
+          ftz = ftz * c1;
+          c1 = 1.0f / Kt;
+          ox = ftx * c1;
+          oy = fty * c1;
+          oz = ftz * c1;
+        } else {
+          c1 = -1.8257f * pA * sqrtf(Kt * 0.5f * m);
+          ftx = ftx + vtx * c1;
+          fty = fty + vty * c1;
+          ftz = ftz + vtz * c1;
+        }
+        st3(Oijt, s, ox, oy, oz);
+        s_c[3][t] = ftx;
+        s_c[4][t] = fty;
+        s_c[5][t] = ftz;
+        s_c[6][t] = (ny * ftz - nz * fty) * r;
+        s_c[7][t] = (nz * ftx - nx * ftz) * r;
+        s_c[8][t] = (nx * fty - ny * ftx) * r;
+        // dd_Calculate_mijroll_ppHM_2 (:316-336): the else branch clears ms
+        const float wim = Wix * Wix + Wiy * Wiy + Wiz * Wiz;
+        if ((double)wim > 1e-18) {
+          const float cr = -mur * fabsf(fn) * rsqrt_rn(wim);
+          s_c[9][t] = Wix * cr;
+          s_c[10][t] = Wiy * cr;
+          s_c[11][t] = Wiz * cr;
+          live = 1;
+        } else {
+          live = 2;
+        }
+      }
     }
-    if (s_ty[t * (K + 1) + k] != 0) continue;
-    const float rx = R[j] - Rix, ry = R[j + N1] - Riy, rz = R[j + N2] - Riz;
-    const float rm = norm3(rx, ry, rz);
-    const float inv_rm = 1.0f / rm;
-    const float nx = rx * inv_rm, ny = ry * inv_rm, nz = rz * inv_rm;
-    if (!(rm < 2.0f * r)) {
-      st3(Oijt, s, 0.0f, 0.0f, 0.0f);
-      continue;
-    }
-    // dd_Calculate_fijn_ppHM_2 (:235-252)
-    const float q = sqrtf(r * fabsf(r - 0.5f * rm));
-    const float Kn = 2.0f * Estar * q;
-    const float Kt = 8.0f * Gstar * q;
-    const float Wjx = W[j], Wjy = W[j + N1], Wjz = W[j + N2];
-    const float wsx = Wjx + Wix, wsy = Wjy + Wiy, wsz = Wjz + Wiz;
-    const float vx = V[j] - Vix + r * (ny * wsz - nz * wsy);
-    const float vy = V[j + N1] - Viy + r * (nz * wsx - nx * wsz);
-    const float vz = V[j + N2] - Viz + r * (nx * wsy - ny * wsx);
-    const float vn = nx * vx + ny * vy + nz * vz;
-    const float fn = (rm - 2.0f * r) * Kn * kTwoThirdsF;
-    const float dmp = -1.8257f * pA * vn * sqrtf(Kn * 0.5f * m);
-    fsx = fsx + nx * (fn + dmp);
-    fsy = fsy + ny * (fn + dmp);
-    fsz = fsz + nz * (fn + dmp);
-    const float vtx = vx - nx * vn, vty = vy - ny * vn, vtz = vz - nz * vn;
-    // dd_Calculate_fijtmijt_ppHM_2 (:255-314)
-    const f3 O = ld3(Oijt, s);
-    float c1 = nx * O.x + ny * O.y + nz * O.z;
-    float ox = O.x - nx * c1, oy = O.y - ny * c1, oz = O.z - nz * c1;
-    c1 = ox * ox + oy * oy + oz * oz;
-    if ((double)c1 > 1e-18) {
-      c1 = 1.0f / c1;
-      c1 = c1 * (O.x * O.x + O.y * O.y + O.z * O.z);
-      c1 = sqrtf(c1);
-      ox = ox * c1;
-      oy = oy * c1;
-      oz = oz * c1;
-    }
-    ox = ox + vtx * dt;
-    oy = oy + vty * dt;
-    oz = oz + vtz * dt;
-    float ftx = ox * Kt, fty = oy * Kt, ftz = oz * Kt;
-    const float ftm = ftx * ftx + fty * fty + ftz * ftz;
-    if (ftm > mu * mu * fn * fn) {
-      c1 = mu * fabsf(fn) * rsqrt_rn(ftm);
-      ftx = ftx * c1;
-      fty = fty * c1;
-      ftz = ftz * c1;
-      c1 = 1.0f / Kt;
-      ox = ftx * c1;
-      oy = fty * c1;
-      oz = ftz * c1;
-    } else {
-      c1 = -1.8257f * pA * sqrtf(Kt * 0.5f * m);
-      ftx = ftx + vtx * c1;
-      fty = fty + vty * c1;
-      ftz = ftz + vtz * c1;
-    }
-    st3(Oijt, s, ox, oy, oz);
-    fsx = fsx + ftx;
-    fsy = fsy + fty;
-    fsz = fsz + ftz;
-    msx = msx + (ny * ftz - nz * fty) * r;
-    msy = msy + (nz * ftx - nx * ftz) * r;
-    msz = msz + (nx * fty - ny * ftx) * r;
-    // dd_Calculate_mijroll_ppHM_2 (:316-336): the else branch clears ms
-    const float wim = Wix * Wix + Wiy * Wiy + Wiz * Wiz;
-    if ((double)wim > 1e-18) {
-      const float cr = -mur * fabsf(fn) * rsqrt_rn(wim);
-      msx = msx + Wix * cr;
-      msy = msy + Wiy * cr;
-      msz = msz + Wiz * cr;
+  }
+  s_live[t] = live;
+  __syncthreads();
+  const uint32_t ip = p0 + t;
+  if (t >= P || ip >= N) return;
+  float fsx = 0, fsy = 0, fsz = 0, msx = 0, msy = 0, msz = 0;
+  for (uint32_t q = t * K; q < (t + 1) * K; ++q) {
+    if (!s_live[q]) continue;
+    fsx = fsx + s_c[0][q];
+    fsy = fsy + s_c[1][q];
+    fsz = fsz + s_c[2][q];
+    fsx = fsx + s_c[3][q];
+    fsy = fsy + s_c[4][q];
+    fsz = fsz + s_c[5][q];
+    msx = msx + s_c[6][q];
+    msy = msy + s_c[7][q];
+    msz = msz + s_c[8][q];
+    if (s_live[q] == 1) {
+      msx = msx + s_c[9][q];
+      msy = msy + s_c[10][q];
+      msz = msz + s_c[11][q];
     } else {
       msx = 0;
       msy = 0;
       msz = 0;
     }
   }
-  F[i] += fsx;
-  F[i + N1] += fsy;
-  F[i + N2] += fsz;
-  M[i] += msx;
-  M[i + N1] += msy;
-  M[i + N2] += msz;
+  F[ip] += fsx;
+  F[ip + N1] += fsy;
+  F[ip + N2] += fsz;
+  M[ip] += msx;
+  M[ip + N1] += msy;
+  M[ip + N2] += msz;
 }
 
 // d_SumUpForcesDEM (md_sumupforcesdem.cu:19-60)
@@ -340,10 +382,11 @@ int rfd_forces_bpm(rfd_ctx *ctx, const float *R, const float *V, const float *W,
                          M && N > 0 && K > 0 && (uint64_t)N * K < 0xFFFFFFFFull,
                 "rfd_forces_bpm: bad arguments");
   RFD_CHECK_ARG(ctx, K <= (uint32_t)kMaxK, "rfd_forces_bpm: K=%u > %d", K, kMaxK);
-  unsigned blocks = (N + kRowThreads - 1) / kRowThreads;
-  k_forces_bpm<<<blocks, kRowThreads, 0, ctx->stream>>>(R, V, W, IL, ILtype, iL, rij, Oijt,
-                                                         Mijn, Mijt, F, M, N, K, dt, E, G,
-                                                         b_r, sc, tc);
+  const uint32_t P = kSlotThreads / K;
+  const unsigned blocks = (N + P - 1) / P;
+  k_forces_bpm<<<blocks, kSlotThreads, 0, ctx->stream>>>(R, V, W, IL, ILtype, iL, rij, Oijt,
+                                                          Mijn, Mijt, F, M, N, K, dt, E, G,
+                                                          b_r, sc, tc);
   RFD_LAUNCHED(ctx, "rfd_forces_bpm");
   return RFD_OK;
 }
@@ -357,10 +400,11 @@ int rfd_forces_contact(rfd_ctx *ctx, const float *R, const float *V, const float
                          (uint64_t)N * K < 0xFFFFFFFFull,
                 "rfd_forces_contact: bad arguments");
   RFD_CHECK_ARG(ctx, K <= (uint32_t)kMaxK, "rfd_forces_contact: K=%u > %d", K, kMaxK);
-  unsigned blocks = (N + kRowThreads - 1) / kRowThreads;
-  k_forces_contact<<<blocks, kRowThreads, 0, ctx->stream>>>(R, V, W, IL, ILtype, Oijt, F, M, N,
-                                                             K, dt, Estar, Gstar, pA, mu, mur,
-                                                             m, r);
+  const uint32_t P = kSlotThreads / K;
+  const unsigned blocks = (N + P - 1) / P;
+  k_forces_contact<<<blocks, kSlotThreads, 0, ctx->stream>>>(R, V, W, IL, ILtype, Oijt, F, M,
+                                                              N, K, dt, Estar, Gstar, pA, mu,
+                                                              mur, m, r);
   RFD_LAUNCHED(ctx, "rfd_forces_contact");
   return RFD_OK;
 }
