@@ -209,7 +209,9 @@ k_il_reconstruct(const float *__restrict__ R, uint32_t N, const uint32_t *__rest
                  uint8_t *__restrict__ ILtype, uint32_t K, float a, float inv_a,
                  float aacut, uint32_t Nx, uint32_t Ny, uint32_t Nz, int quirk_p0,
                  uint32_t *__restrict__ flags) {
-  __shared__ uint32_t s_row[kRecThreads * (kRecMaxK + 1)];
+  // the block's rows, K + 1 words apart (sized by K at launch, so that short
+  // rows leave room for more resident blocks)
+  extern __shared__ uint32_t s_row[];
   const uint32_t C = Nx * Ny * Nz;
   const uint32_t p0 = blockIdx.x * kRecThreads;
   const uint32_t np = min((uint32_t)kRecThreads, N - p0);
@@ -382,7 +384,8 @@ int rfd_il_reconstruct_bpm(rfd_ctx *ctx, const float *R, uint32_t N,
                 "rfd_il_reconstruct_bpm: bad arguments");
   RFD_CHECK_ARG(ctx, K <= (uint32_t)kRecMaxK, "rfd_il_reconstruct_bpm: K=%u > %d", K, kRecMaxK);
   unsigned blocks = (N + kRecThreads - 1) / kRecThreads;
-  k_il_reconstruct<<<blocks, kRecThreads, 0, ctx->stream>>>(
+  k_il_reconstruct<<<blocks, kRecThreads, sizeof(uint32_t) * kRecThreads * (K + 1),
+                     ctx->stream>>>(
       R, N, CIs, pnC, IL, ILtype, K, a, inv_a, aacut, Nx, Ny, Nz, quirk_p0, ctx->d_flags);
   RFD_LAUNCHED(ctx, "rfd_il_reconstruct_bpm");
   return RFD_OK;
