@@ -200,30 +200,39 @@ __global__ void k_il_construct(const float *__restrict__ R, uint32_t N,
 }
 
 // Rows of one workgroup are staged in LDS (coalesced in and out); each thread
-// then scans its own row for every new candidate.
+// then tests every new candidate against its own row.
 constexpr int kRecThreads = 128;
 constexpr int kRecMaxK = 64;
+
+// LDS row pitch in words: K rounded up to 16 bytes, then an odd multiple of 4
+// words, so that the 16-byte reads of 16 consecutive lanes fall in distinct
+// banks. Padding words hold the sentinel, which no candidate id equals.
+__host__ __device__ constexpr uint32_t rec_pitch(uint32_t K) { return ((K + 3u) & ~3u) | 4u; }
 __global__ void __launch_bounds__(kRecThreads)
 k_il_reconstruct(const float *__restrict__ R, uint32_t N, const uint32_t *__restrict__ CIs,
                  const uint32_t *__restrict__ pnC, uint32_t *__restrict__ IL,
                  uint8_t *__restrict__ ILtype, uint32_t K, float a, float inv_a,
                  float aacut, uint32_t Nx, uint32_t Ny, uint32_t Nz, int quirk_p0,
                  uint32_t *__restrict__ flags) {
-  // the block's rows, K + 1 words apart (sized by K at launch, so that short
-  // rows leave room for more resident blocks)
-  extern __shared__ uint32_t s_row[];
+  // the block's rows, rec_pitch(K) words apart (sized by K at launch, so that
+  // short rows leave room for more resident blocks)
+  extern __shared__ uint4 s_row4[];
+  uint32_t *s_row = reinterpret_cast<uint32_t *>(s_row4);
   const uint32_t C = Nx * Ny * Nz;
   const uint32_t p0 = blockIdx.x * kRecThreads;
   const uint32_t np = min((uint32_t)kRecThreads, N - p0);
-  const uint32_t stride = K + 1;
+  const uint32_t pitch = rec_pitch(K);
   const size_t g0 = (size_t)p0 * K;
-  for (uint32_t e = threadIdx.x; e < np * K; e += kRecThreads)
-    s_row[(e / K) * stride + (e % K)] = IL[g0 + e];
+  for (uint32_t e = threadIdx.x; e < np * pitch; e += kRecThreads) {
+    const uint32_t r = e / pitch, k = e % pitch;
+    s_row[e] = k < K ? IL[g0 + (size_t)r * K + k] : kSentinel;
+  }
   __syncthreads();
   const uint32_t t = threadIdx.x;
   if (t < np) {
     const uint32_t i = p0 + t;
-    uint32_t *row = s_row + t * stride;
+    uint32_t *row = s_row + t * pitch;
+    const uint4 *row4 = s_row4 + t * (pitch / 4);
     uint32_t cells[8];
     if (!search_cells(R, N, i, a, inv_a, Nx, Ny, Nz, cells)) {
       atomicOr(flags, RFD_FLAG_CELL_RANGE);
@@ -236,20 +245,23 @@ k_il_reconstruct(const float *__restrict__ R, uint32_t N, const uint32_t *__rest
           uint32_t j = CIs[N + s0 + u];
           float rr = pair_r2(R, N, i, j);
           if (rr < aacut && (double)rr > 1e-10) {
-            int freeslot = -1;
+            // membership over the whole row, without an early exit: the
+            // reads are independent and issue back to back
             bool found = false;
-            for (uint32_t k = 0; k < K; ++k) {
-              uint32_t v = row[k];
-              if (v == j) {
-                found = true;
-                break;
-              }
-              if (v == kSentinel) {
-                if (freeslot < 0) freeslot = (int)k;
-                else if (quirk_p0 && i == 0 && freeslot == 0) freeslot = (int)k;
-              }
+            for (uint32_t w = 0; w < pitch / 4; ++w) {
+              const uint4 v = row4[w];
+              found |= (v.x == j) | (v.y == j) | (v.z == j) | (v.w == j);
             }
             if (!found) {
+              // the first free slot (the second for particle 0 under the
+              // particle-0 rule), searched only when the candidate is new
+              int freeslot = -1;
+              for (uint32_t k = 0; k < K; ++k) {
+                if (row[k] == kSentinel) {
+                  if (freeslot < 0) freeslot = (int)k;
+                  else if (quirk_p0 && i == 0 && freeslot == 0) freeslot = (int)k;
+                }
+              }
               if (freeslot < 0) {
                 atomicOr(flags, RFD_FLAG_ROW_FULL);
               } else {
@@ -263,8 +275,10 @@ k_il_reconstruct(const float *__restrict__ R, uint32_t N, const uint32_t *__rest
     }
   }
   __syncthreads();
-  for (uint32_t e = threadIdx.x; e < np * K; e += kRecThreads)
-    IL[g0 + e] = s_row[(e / K) * stride + (e % K)];
+  for (uint32_t e = threadIdx.x; e < np * pitch; e += kRecThreads) {
+    const uint32_t r = e / pitch, k = e % pitch;
+    if (k < K) IL[g0 + (size_t)r * K + k] = s_row[e];
+  }
 }
 
 __device__ __forceinline__ void zero3(float *p, size_t s) {
@@ -384,7 +398,7 @@ int rfd_il_reconstruct_bpm(rfd_ctx *ctx, const float *R, uint32_t N,
                 "rfd_il_reconstruct_bpm: bad arguments");
   RFD_CHECK_ARG(ctx, K <= (uint32_t)kRecMaxK, "rfd_il_reconstruct_bpm: K=%u > %d", K, kRecMaxK);
   unsigned blocks = (N + kRecThreads - 1) / kRecThreads;
-  k_il_reconstruct<<<blocks, kRecThreads, sizeof(uint32_t) * kRecThreads * (K + 1),
+  k_il_reconstruct<<<blocks, kRecThreads, sizeof(uint32_t) * kRecThreads * rec_pitch(K),
                      ctx->stream>>>(
       R, N, CIs, pnC, IL, ILtype, K, a, inv_a, aacut, Nx, Ny, Nz, quirk_p0, ctx->d_flags);
   RFD_LAUNCHED(ctx, "rfd_il_reconstruct_bpm");
