@@ -166,8 +166,31 @@ __device__ __forceinline__ float pair_r2(const float *__restrict__ R, uint32_t N
   return dx * dx + dy * dy + dz * dz;
 }
 
+// Positions in cell order, one 16-byte record per cell-list entry: x, y, z and
+// the particle id (as float bits). A candidate test then reads one record
+// instead of the id and three position words in three arrays; the coordinates
+// are copies, so distances round exactly as they do from R. Entries past the
+// binned particles hold whatever the id slot held; they are never read.
+__global__ void k_cell_positions(const float *__restrict__ R, uint32_t N,
+                                 const uint32_t *__restrict__ CIs, float4 *__restrict__ Rc) {
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < N; s += grid_stride()) {
+    const uint32_t j = CIs[N + s];
+    float4 v = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kSentinel));
+    if (j < N) v = make_float4(R[j], R[j + N], R[j + 2 * (size_t)N], __uint_as_float(j));
+    Rc[s] = v;
+  }
+}
+
+// |r_j - r_i|^2 in the same operation order as pair_r2, from a cell-order record
+__device__ __forceinline__ float rec_r2(float4 p, float xi, float yi, float zi) {
+  float dx = p.x - xi;
+  float dy = p.y - yi;
+  float dz = p.z - zi;
+  return dx * dx + dy * dy + dz * dz;
+}
+
 __global__ void k_il_construct(const float *__restrict__ R, uint32_t N,
-                               const uint32_t *__restrict__ CIs,
+                               const float4 *__restrict__ Rc,
                                const uint32_t *__restrict__ pnC,
                                uint32_t *__restrict__ IL, uint32_t K, float a,
                                float inv_a, float aacut, uint32_t Nx, uint32_t Ny,
@@ -180,13 +203,15 @@ __global__ void k_il_construct(const float *__restrict__ R, uint32_t N,
     if (!search_cells(R, N, i, a, inv_a, Nx, Ny, Nz, cells)) {
       atomicOr(flags, RFD_FLAG_CELL_RANGE);
     } else {
+      const float xi = R[i], yi = R[i + N], zi = R[i + 2 * (size_t)N];
       for (int q = 0; q < 8; ++q) {
         uint32_t c = cells[q];
         if (c == kSentinel) continue;
         uint32_t s0 = pnC[c], n = pnC[C + c];
         for (uint32_t t = 0; t < n; ++t) {
-          uint32_t j = CIs[N + s0 + t];
-          float rr = pair_r2(R, N, i, j);
+          const float4 pj = Rc[s0 + t];
+          const uint32_t j = __float_as_uint(pj.w);
+          float rr = rec_r2(pj, xi, yi, zi);
           if (rr < aacut && j != i) {
             if (nk < K) row[nk] = j;
             ++nk;
@@ -209,7 +234,7 @@ constexpr int kRecMaxK = 64;
 // banks. Padding words hold the sentinel, which no candidate id equals.
 __host__ __device__ constexpr uint32_t rec_pitch(uint32_t K) { return ((K + 3u) & ~3u) | 4u; }
 __global__ void __launch_bounds__(kRecThreads)
-k_il_reconstruct(const float *__restrict__ R, uint32_t N, const uint32_t *__restrict__ CIs,
+k_il_reconstruct(const float *__restrict__ R, uint32_t N, const float4 *__restrict__ Rc,
                  const uint32_t *__restrict__ pnC, uint32_t *__restrict__ IL,
                  uint8_t *__restrict__ ILtype, uint32_t K, float a, float inv_a,
                  float aacut, uint32_t Nx, uint32_t Ny, uint32_t Nz, int quirk_p0,
@@ -237,13 +262,15 @@ k_il_reconstruct(const float *__restrict__ R, uint32_t N, const uint32_t *__rest
     if (!search_cells(R, N, i, a, inv_a, Nx, Ny, Nz, cells)) {
       atomicOr(flags, RFD_FLAG_CELL_RANGE);
     } else {
+      const float xi = R[i], yi = R[i + N], zi = R[i + 2 * (size_t)N];
       for (int q = 0; q < 8; ++q) {
         uint32_t c = cells[q];
         if (c == kSentinel) continue;
         uint32_t s0 = pnC[c], n = pnC[C + c];
         for (uint32_t u = 0; u < n; ++u) {
-          uint32_t j = CIs[N + s0 + u];
-          float rr = pair_r2(R, N, i, j);
+          const float4 pj = Rc[s0 + u];
+          const uint32_t j = __float_as_uint(pj.w);
+          float rr = rec_r2(pj, xi, yi, zi);
           if (rr < aacut && (double)rr > 1e-10) {
             // membership over the whole row, without an early exit: the
             // reads are independent and issue back to back
@@ -382,8 +409,13 @@ int rfd_il_construct(rfd_ctx *ctx, const float *R, uint32_t N, const uint32_t *C
   if (!ctx) return RFD_E_ARG;
   RFD_CHECK_ARG(ctx, R && CIs && pnC && IL && N > 0 && K > 0 && slots_fit(N, K),
                 "rfd_il_construct: bad arguments");
+  void *ws = nullptr;
+  int st = rfd_scratch(ctx, sizeof(float4) * (size_t)N, &ws);
+  if (st) return st;
+  float4 *Rc = (float4 *)ws;
+  k_cell_positions<<<rfd_blocks(N, 256), 256, 0, ctx->stream>>>(R, N, CIs, Rc);
   k_il_construct<<<rfd_blocks(N, 128), 128, 0, ctx->stream>>>(
-      R, N, CIs, pnC, IL, K, a, inv_a, aacut, Nx, Ny, Nz, ctx->d_flags);
+      R, N, Rc, pnC, IL, K, a, inv_a, aacut, Nx, Ny, Nz, ctx->d_flags);
   RFD_LAUNCHED(ctx, "rfd_il_construct");
   return RFD_OK;
 }
@@ -397,10 +429,15 @@ int rfd_il_reconstruct_bpm(rfd_ctx *ctx, const float *R, uint32_t N,
   RFD_CHECK_ARG(ctx, R && CIs && pnC && IL && ILtype && N > 0 && K > 0 && slots_fit(N, K),
                 "rfd_il_reconstruct_bpm: bad arguments");
   RFD_CHECK_ARG(ctx, K <= (uint32_t)kRecMaxK, "rfd_il_reconstruct_bpm: K=%u > %d", K, kRecMaxK);
+  void *ws = nullptr;
+  int st = rfd_scratch(ctx, sizeof(float4) * (size_t)N, &ws);
+  if (st) return st;
+  float4 *Rc = (float4 *)ws;
+  k_cell_positions<<<rfd_blocks(N, 256), 256, 0, ctx->stream>>>(R, N, CIs, Rc);
   unsigned blocks = (N + kRecThreads - 1) / kRecThreads;
   k_il_reconstruct<<<blocks, kRecThreads, sizeof(uint32_t) * kRecThreads * rec_pitch(K),
                      ctx->stream>>>(
-      R, N, CIs, pnC, IL, ILtype, K, a, inv_a, aacut, Nx, Ny, Nz, quirk_p0, ctx->d_flags);
+      R, N, Rc, pnC, IL, ILtype, K, a, inv_a, aacut, Nx, Ny, Nz, quirk_p0, ctx->d_flags);
   RFD_LAUNCHED(ctx, "rfd_il_reconstruct_bpm");
   return RFD_OK;
 }
