@@ -54,44 +54,90 @@ __global__ void k_cell_index(const float *__restrict__ R, uint32_t N, float inv_
 // cursor[c] = start, pnC[c] = start for occupied cells, 0 for empty ones
 constexpr int kScanThreads = 1024;
 constexpr int kScanItems = 8;
+constexpr uint32_t kScanChunk = kScanThreads * kScanItems;
+
+// Exclusive scan of n words in chunks of kScanChunk, three launches:
+// k_chunk_sums (one workgroup per chunk) -> k_scan_chunk_sums (one workgroup,
+// carries over the chunk totals in place) -> k_scan_cells (one workgroup per
+// chunk, starting from its chunk's carry). Integer sums: any order gives the
+// same result.
+__device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t *s_sum) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if (lane == 0) s_sum[wid] = v;
+  __syncthreads();
+  uint32_t t = 0;
+  for (int w = 0; w < kScanThreads / 64; ++w) t += s_sum[w];
+  return t;
+}
+
+// exclusive prefix of each thread's `local` inside the workgroup
+__device__ __forceinline__ uint32_t block_prefix(uint32_t local, uint32_t *s_sum) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint32_t inc = local;
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t t = __shfl_up(inc, off, 64);
+    if (lane >= off) inc += t;
+  }
+  if (lane == 63) s_sum[wid] = inc;
+  __syncthreads();
+  uint32_t wave_off = 0;
+  for (int w = 0; w < wid; ++w) wave_off += s_sum[w];
+  return wave_off + inc - local;
+}
+
 __global__ void __launch_bounds__(kScanThreads)
-k_scan_cells(uint32_t *__restrict__ pnC, uint32_t C, uint32_t *__restrict__ cursor) {
+k_chunk_sums(const uint32_t *__restrict__ cnt, uint32_t n, uint32_t *__restrict__ part) {
+  __shared__ uint32_t s_sum[kScanThreads / 64];
+  const uint32_t b0 = blockIdx.x * kScanChunk + threadIdx.x * kScanItems;
+  uint32_t local = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) local += (b0 + k < n) ? cnt[b0 + k] : 0u;
+  const uint32_t t = block_sum(local, s_sum);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// part[0..m) -> exclusive prefix sums, in place (one workgroup)
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_chunk_sums(uint32_t *__restrict__ part, uint32_t m) {
   __shared__ uint32_t s_sum[kScanThreads / 64];
   __shared__ uint32_t s_carry;
-  const uint32_t *cnt = pnC + C;
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  if (tid == 0) s_carry = 0;
+  if (threadIdx.x == 0) s_carry = 0;
   __syncthreads();
-  for (uint32_t base = 0; base < C; base += kScanThreads * kScanItems) {
-    uint32_t v[kScanItems], local = 0;
-    uint32_t b0 = base + tid * kScanItems;
+  for (uint32_t base = 0; base < m; base += kScanThreads) {
+    const uint32_t e = base + threadIdx.x;
+    const uint32_t v = e < m ? part[e] : 0u;
+    const uint32_t carry = s_carry;
+    const uint32_t pre = block_prefix(v, s_sum);
+    if (e < m) part[e] = carry + pre;
+    __syncthreads();
+    if (threadIdx.x == kScanThreads - 1) s_carry = carry + pre + v;
+    __syncthreads();
+  }
+}
+
+// counts in pnC[C..2C) -> starts in pnC[0..C) (0 for an empty cell, as the
+// reference's cell table holds) and a copy of the starts as scatter cursors
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_cells(uint32_t *__restrict__ pnC, uint32_t C, uint32_t *__restrict__ cursor,
+             const uint32_t *__restrict__ part) {
+  __shared__ uint32_t s_sum[kScanThreads / 64];
+  const uint32_t *cnt = pnC + C;
+  const uint32_t b0 = blockIdx.x * kScanChunk + threadIdx.x * kScanItems;
+  uint32_t v[kScanItems], local = 0;
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-      v[k] = (b0 + k < C) ? cnt[b0 + k] : 0u;
-      local += v[k];
-    }
-    // inclusive wave scan of per-thread totals
-    uint32_t inc = local;
-    for (int off = 1; off < 64; off <<= 1) {
-      uint32_t t = __shfl_up(inc, off, 64);
-      if (lane >= off) inc += t;
-    }
-    if (lane == 63) s_sum[wid] = inc;
-    __syncthreads();
-    uint32_t wave_off = 0;
-    for (int w = 0; w < wid; ++w) wave_off += s_sum[w];
-    uint32_t run = s_carry + wave_off + inc - local;
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = (b0 + k < C) ? cnt[b0 + k] : 0u;
+    local += v[k];
+  }
+  uint32_t run = part[blockIdx.x] + block_prefix(local, s_sum);
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-      if (b0 + k < C) {
-        cursor[b0 + k] = run;
-        pnC[b0 + k] = v[k] ? run : 0u;
-      }
-      run += v[k];
+  for (int k = 0; k < kScanItems; ++k) {
+    if (b0 + k < C) {
+      cursor[b0 + k] = run;
+      pnC[b0 + k] = v[k] ? run : 0u;
     }
-    __syncthreads();
-    if (tid == kScanThreads - 1) s_carry = run;
-    __syncthreads();
+    run += v[k];
   }
 }
 
@@ -389,14 +435,17 @@ int rfd_cell_distribution(rfd_ctx *ctx, const float *R, uint32_t N, float inv_a,
                 "rfd_cell_distribution: bad grid %u x %u x %u", Nx, Ny, Nz);
   const uint32_t C = Nx * Ny * Nz;
   void *ws = nullptr;
-  int st = rfd_scratch(ctx, sizeof(uint32_t) * ((size_t)C + N), &ws);
+  const uint32_t chunks = (uint32_t)(((size_t)C + kScanChunk - 1) / kScanChunk);
+  int st = rfd_scratch(ctx, sizeof(uint32_t) * ((size_t)C + N + chunks), &ws);
   if (st) return st;
-  uint32_t *cursor = (uint32_t *)ws, *tmp = cursor + C;
+  uint32_t *cursor = (uint32_t *)ws, *tmp = cursor + C, *part = tmp + N;
   hipStream_t s = ctx->stream;
   k_fill_u32<<<rfd_blocks(2 * (size_t)C, 256, 4096), 256, 0, s>>>(pnC, 2 * (size_t)C, 0u);
   k_cell_index<<<rfd_blocks(N, 256), 256, 0, s>>>(R, N, inv_a, Nx, Ny, Nz, CI, pnC + C,
                                                   ctx->d_flags);
-  k_scan_cells<<<1, kScanThreads, 0, s>>>(pnC, C, cursor);
+  k_chunk_sums<<<chunks, kScanThreads, 0, s>>>(pnC + C, C, part);
+  k_scan_chunk_sums<<<1, kScanThreads, 0, s>>>(part, chunks);
+  k_scan_cells<<<chunks, kScanThreads, 0, s>>>(pnC, C, cursor, part);
   k_scatter<<<rfd_blocks(N, 256), 256, 0, s>>>(CI, N, cursor, tmp);
   k_rank_cells<<<rfd_blocks(C, 4), 256, 0, s>>>(tmp, pnC, C, N, CIs);
   RFD_LAUNCHED(ctx, "rfd_cell_distribution");

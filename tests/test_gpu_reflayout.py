@@ -181,6 +181,29 @@ def test_fire_kernels(ctx):
     del lib
 
 
+def test_cell_distribution_many_chunks_bitexact(ctx):
+    """Cell binning on a grid of 43,200 cells (the scan runs in chunks of 8,192
+    cells, so this spans six of them, the last one partial): fewer particles
+    than cells, so most cells are empty and some hold several."""
+    from rfd.reflayout import Grid
+
+    rng = np.random.default_rng(11)
+    N = 20000
+    hi = np.array([20.0, 20.0, 13.5])
+    pos = rng.uniform(0.0, 1.0, (N, 3)) * hi * 0.999
+    R = np.ascontiguousarray(pos.T.reshape(-1)).astype(np.float32)
+    grid = Grid(0.5, float(np.float32(1.0 / np.float32(0.5))), 40, 40, 27)
+    assert grid.C > 5 * 8192
+    g, o = RefLayout(ctx, R, 4, grid), OracleLayout(R, 4, grid)
+    ctx.flags()
+    g.cell_distribution()
+    o.cell_distribution()
+    assert o.last_status == 0 and ctx.flags() == 0
+    assert_same(g.host(), o.host(), keys=("CI", "CIs", "pnC"), where="cells")
+    counts = o.host()["pnC"][grid.C:]
+    assert counts.sum() == N and (counts == 0).sum() > grid.C // 2 and counts.max() > 1
+
+
 def test_cell_range_flag(ctx):
     """A particle outside the grid raises RFD_FLAG_CELL_RANGE, not a fault."""
     import torch
