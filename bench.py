@@ -165,7 +165,11 @@ def bench_ref(args) -> dict:
     nb_end = s.bond_slots() // 2
     ms = (t1 - t0) * 1e3 / args.steps
     N = case.N
-    alg_bytes = 128 * nb + (12 + 48) * N
+    # k_forces_bpm in the slot layout: IL + ILtype of every slot (5 B); per
+    # particle R, V, W read and F, M read and written (84 B); per bonded slot
+    # (two per bond) iL, rij, Oijt, Mijn, Mijt read (52 B) and rij, Oijt,
+    # Mijn, Mijt written (48 B)
+    alg_bytes = 5 * N * case.K + 84 * N + 100 * 2 * nb
     achieved = alg_bytes / (kern_ms * 1e-3)
     return {
         "value": nb * args.steps / (t1 - t0),
