@@ -156,6 +156,29 @@ def test_driver_trajectory_bitexact(ctx, mode):
     assert cg.cmp.stress == pytest.approx(co.cmp.stress, rel=1e-12)
 
 
+@pytest.mark.parametrize("K", [24, 27, 40])
+def test_driver_trajectory_other_row_lengths(ctx, K):
+    """The same whole steps with K slots per particle other than 32: the
+    force kernels then tile floor(256 / K) particles per workgroup with idle
+    threads at the end (K = 27: 9 particles, 243 of 256 threads), and the list
+    reconstruction pads its LDS rows to 28, 28 and 44 words."""
+    args = (8.0, 14.0)
+    cg = cases.reference_uniaxial(*args, seed=5, K=K)
+    co = cases.reference_uniaxial(*args, seed=5, K=K)
+    for c in (cg, co):
+        c.cmp.v_load *= 50
+    dg = UniaxialDriver(ctx, cg.R, cg.mat, cg.box, cg.cmp, cg.K)
+    do = OracleUniaxialDriver(co.R, co.mat, co.box, co.cmp, co.grid(), co.K)
+    dg.setup()
+    do.setup()
+    for k in range(40):
+        np.testing.assert_allclose(dg.step(), do.step(), rtol=1e-12, atol=1e-300)
+        if k % 20 == 19:
+            assert_same(dg.s.host(), do.s.host(), where=f"K={K} step {k}")
+    assert ctx.flags() == 0 and do.s.last_status == 0
+    assert (do.s.ILtype == 1).sum() > 4 * cg.N
+
+
 def test_fire_kernels(ctx):
     import torch
 
