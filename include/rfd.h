@@ -19,7 +19,12 @@
  *   - Tier-1 pointers are caller-owned device memory; counts are uint32_t.
  *   - Calls are asynchronous on the context's stream, except those that return
  *     host values (platen sums, FIRE power, flags), which synchronise.
- *   - Kernels never allocate caller memory; the context owns its scratch.
+ *   - Kernels never allocate caller memory; the context owns its scratch: a
+ *     grow-only device workspace, allocated by the first call that needs more
+ *     than it holds (cell binning: 4 B per cell and per particle; list
+ *     construct and reconstruct: a 16 B cell-ordered position record per
+ *     particle) and freed by rfd_ctx_destroy.  Such a first call synchronises
+ *     the context's stream.
  *   - One context per host thread.
  */
 #ifndef RFD_H
