@@ -102,6 +102,18 @@ def load_traffic(kernel: str, particles: int, bonds: int):
     return None
 
 
+def cpu_model() -> str:
+    """The host CPU's model name, from /proc/cpuinfo ("unknown" elsewhere)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline_ref(seconds: float, omp: bool = False) -> dict:
     """Oracle driver on a 25 x 50 mm cylinder, the size of the reference's own
     28,593-particle sample (kernel.cu:166): single thread as the reference's
@@ -229,6 +241,21 @@ def bench_native(args) -> dict:
                      "kernel": kernel, "kernel_ms": kern_ms,
                      "alg_bytes_per_launch": alg_bytes},
     }
+
+
+def bench_c2(steps: int = 500) -> dict:
+    """Explicit steps on the D25xH50 cylinder, the size of the CPU baseline's
+    sample: the reference-layout tier on the same monodisperse specimen the
+    CPU loop runs, and the native tier on the polydisperse one."""
+    import argparse
+
+    out = {}
+    for name, fn in (("ref_tier", bench_ref), ("native_tier", bench_native)):
+        r = fn(argparse.Namespace(D=25.0, H=50.0, steps=steps, warmup=20))
+        out[name] = {"bond_updates_per_s": r["value"], "ms_per_step": r["ms_per_step"],
+                     "particles": r["config"]["particles"], "bonds": r["config"]["bonds"],
+                     "workload": r["config"]["workload"]}
+    return out
 
 
 def bench_warmstart(D: float = 61.0, H: float = 122.0, steps: int = 6) -> dict:
@@ -794,7 +821,12 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_ref(args.cpu_seconds)
         out["cpu_baseline"]["host_cores_available"] = os.cpu_count()
+        out["cpu_baseline"]["cpu_model"] = cpu_model()
         out["cpu_baseline_threads"] = cpu_baseline_ref(args.cpu_seconds, omp=True)
+        try:
+            out["cylinder_c2_gpu"] = bench_c2()
+        except Exception as e:  # reported, never hides the main line
+            out["cylinder_c2_gpu"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_warmstart and args.tier == "native":
         try:
             out["peak_strength"] = bench_peak_precision()
