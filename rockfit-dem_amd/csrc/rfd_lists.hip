@@ -6,6 +6,8 @@
 //   cursor -> per-cell rank by id (one wave per cell).
 // Ranking each cell's members by particle id restores index order inside a
 // cell, which is exactly the order a stable sort of (cell, id) produces.
+#include <cmath>
+
 #include "rfd_internal.h"
 
 using namespace rfd;
@@ -422,6 +424,229 @@ __global__ void k_delete_far(const float *__restrict__ R, uint32_t N, uint32_t *
 
 bool slots_fit(uint32_t N, uint32_t K) { return (uint64_t)N * K < 0xFFFFFFFFull; }
 
+
+// ---- fine-grid candidate search for the list reconstruction
+// A second binning of the particles, with a cell edge of 1.01 times the search
+// radius sqrt(aacut), puts every partner that can pass the FP32 range test in
+// the 3 x 3 x 3 fine cells around the particle: along an axis, a partner two
+// fine cells away is more than 0.99 fine cells = 1.0001 radii away, beyond
+// what the FP32 distance and the FP32 fine coordinates (below 40,000 cells)
+// can round across. Of the candidates that pass the range test, those that the
+// caller's cell lists hold in one of the particle's 8 search cells are kept
+// and visited in the order of the 8-cell search: by search cell, then by id
+// (a cell's members are listed by id). So the same partners are inserted, in
+// the same order, as by the scan of the 8 cells, from about 100 candidates per
+// particle instead of 8 coarse cells' worth.
+constexpr uint32_t kFineMaxDim = 40000;
+constexpr int kFineFound = 32;     // in-range partners held (sorted) per particle
+constexpr uint32_t kFineIdBits = 29;  // key = search cell << 29 | id
+
+struct FineGrid {
+  float af, inv_af;     // fine cell edge and its inverse
+  uint32_t nx, ny, nz;  // cells per axis; coordinate x maps to floor((x + af) / af)
+};
+
+__device__ __forceinline__ bool fine_coords(const FineGrid &f, float x, float y, float z,
+                                            uint32_t c[3]) {
+  const float fx = floorf((x + f.af) * f.inv_af);
+  const float fy = floorf((y + f.af) * f.inv_af);
+  const float fz = floorf((z + f.af) * f.inv_af);
+  if (!(fx >= 0.0f && fx < (float)f.nx) || !(fy >= 0.0f && fy < (float)f.ny) ||
+      !(fz >= 0.0f && fz < (float)f.nz))
+    return false;
+  c[0] = (uint32_t)fx;
+  c[1] = (uint32_t)fy;
+  c[2] = (uint32_t)fz;
+  return true;
+}
+
+// coarse cell of every particle the caller's lists hold (others keep the sentinel)
+__global__ void k_cell_of(const uint32_t *__restrict__ CIs, const uint32_t *__restrict__ pnC,
+                          uint32_t C, uint32_t N, uint32_t *__restrict__ cellOf) {
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += grid_stride()) {
+    const uint32_t s0 = pnC[c], n = pnC[C + c];
+    for (uint32_t u = 0; u < n; ++u) {
+      const uint32_t j = CIs[N + s0 + u];
+      if (j < N) cellOf[j] = c;
+    }
+  }
+}
+
+__global__ void k_fine_index(const float *__restrict__ R, uint32_t N, FineGrid f,
+                             uint32_t *__restrict__ fci, uint32_t *__restrict__ cnt) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
+    uint32_t c3[3];
+    if (!fine_coords(f, R[i], R[i + N], R[i + 2 * (size_t)N], c3)) {
+      fci[i] = kSentinel;
+      continue;
+    }
+    const uint32_t c = c3[0] + f.nx * (c3[1] + f.ny * c3[2]);
+    fci[i] = c;
+    atomicAdd(&cnt[c], 1u);
+  }
+}
+
+// exclusive prefix sums of cnt[0..n) into excl[0..n], excl[n] = total (one
+// workgroup per chunk, from the chunk carries of k_scan_chunk_sums)
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_excl(const uint32_t *__restrict__ cnt, uint32_t n, uint32_t *__restrict__ excl,
+            const uint32_t *__restrict__ part) {
+  __shared__ uint32_t s_sum[kScanThreads / 64];
+  const uint32_t b0 = blockIdx.x * kScanChunk + threadIdx.x * kScanItems;
+  uint32_t v[kScanItems], local = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = (b0 + k < n) ? cnt[b0 + k] : 0u;
+    local += v[k];
+  }
+  uint32_t run = part[blockIdx.x] + block_prefix(local, s_sum);
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    if (b0 + k < n) {
+      excl[b0 + k] = run;
+      if (b0 + k == n - 1) excl[n] = run + v[k];
+    }
+    run += v[k];
+  }
+}
+
+// positions in fine-cell order, (x, y, z, id bits); cnt cleared beforehand
+__global__ void k_fine_scatter(const float *__restrict__ R, uint32_t N,
+                               const uint32_t *__restrict__ fci,
+                               const uint32_t *__restrict__ excl, uint32_t *__restrict__ cnt,
+                               float4 *__restrict__ Rf) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
+    const uint32_t c = fci[i];
+    if (c == kSentinel) continue;
+    const uint32_t p = excl[c] + atomicAdd(&cnt[c], 1u);
+    Rf[p] = make_float4(R[i], R[i + N], R[i + 2 * (size_t)N], __uint_as_float(i));
+  }
+}
+
+__global__ void __launch_bounds__(kRecThreads)
+k_il_reconstruct_fine(const float *__restrict__ R, uint32_t N, const uint32_t *__restrict__ CIs,
+                      const uint32_t *__restrict__ pnC, const float4 *__restrict__ Rf,
+                      const uint32_t *__restrict__ excl, const uint32_t *__restrict__ cellOf,
+                      FineGrid f, uint32_t *__restrict__ IL, uint8_t *__restrict__ ILtype,
+                      uint32_t K, float a, float inv_a, float aacut, uint32_t Nx, uint32_t Ny,
+                      uint32_t Nz, int quirk_p0, uint32_t *__restrict__ flags) {
+  extern __shared__ uint4 s_row4[];
+  uint32_t *s_row = reinterpret_cast<uint32_t *>(s_row4);
+  const uint32_t C = Nx * Ny * Nz;
+  const uint32_t p0 = blockIdx.x * kRecThreads;
+  const uint32_t np = min((uint32_t)kRecThreads, N - p0);
+  const uint32_t pitch = rec_pitch(K);
+  const size_t g0 = (size_t)p0 * K;
+  for (uint32_t e = threadIdx.x; e < np * pitch; e += kRecThreads) {
+    const uint32_t r = e / pitch, k = e % pitch;
+    s_row[e] = k < K ? IL[g0 + (size_t)r * K + k] : kSentinel;
+  }
+  __syncthreads();
+  const uint32_t t = threadIdx.x;
+  if (t < np) {
+    const uint32_t i = p0 + t;
+    uint32_t *row = s_row + t * pitch;
+    const uint4 *row4 = s_row4 + t * (pitch / 4);
+    // a partner that passed the range test: insert it unless the row holds it
+    auto consider = [&](uint32_t j) {
+      bool found = false;
+      for (uint32_t w = 0; w < pitch / 4; ++w) {
+        const uint4 v = row4[w];
+        found |= (v.x == j) | (v.y == j) | (v.z == j) | (v.w == j);
+      }
+      if (found) return;
+      int freeslot = -1;
+      for (uint32_t k = 0; k < K; ++k) {
+        if (row[k] == kSentinel) {
+          if (freeslot < 0) freeslot = (int)k;
+          else if (quirk_p0 && i == 0 && freeslot == 0) freeslot = (int)k;
+        }
+      }
+      if (freeslot < 0) {
+        atomicOr(flags, RFD_FLAG_ROW_FULL);
+      } else {
+        row[freeslot] = j;
+        ILtype[(size_t)i * K + freeslot] = 0;
+      }
+    };
+    uint32_t cells[8];
+    if (!search_cells(R, N, i, a, inv_a, Nx, Ny, Nz, cells)) {
+      atomicOr(flags, RFD_FLAG_CELL_RANGE);
+    } else {
+      const float xi = R[i], yi = R[i + N], zi = R[i + 2 * (size_t)N];
+      uint32_t key[kFineFound];  // ascending; unused entries hold 0xFFFFFFFF
+#pragma unroll
+      for (int u = 0; u < kFineFound; ++u) key[u] = 0xFFFFFFFFu;
+      uint32_t nf = 0;
+      uint32_t fc[3];
+      if (fine_coords(f, xi, yi, zi, fc)) {
+        const uint32_t xlo = fc[0] > 0 ? fc[0] - 1 : 0;
+        const uint32_t xhi = fc[0] + 1 < f.nx ? fc[0] + 1 : f.nx - 1;
+        for (int dz = -1; dz <= 1; ++dz) {
+          const int z = (int)fc[2] + dz;
+          if (z < 0 || z >= (int)f.nz) continue;
+          for (int dy = -1; dy <= 1; ++dy) {
+            const int y = (int)fc[1] + dy;
+            if (y < 0 || y >= (int)f.ny) continue;
+            const uint32_t base = f.nx * ((uint32_t)y + f.ny * (uint32_t)z);
+            const uint32_t lo = excl[base + xlo], hi = excl[base + xhi + 1];
+            for (uint32_t e = lo; e < hi; ++e) {
+              const float4 pj = Rf[e];
+              const float rr = rec_r2(pj, xi, yi, zi);
+              if (rr < aacut && (double)rr > 1e-10) {
+                const uint32_t j = __float_as_uint(pj.w);
+                const uint32_t cj = cellOf[j];
+                uint32_t q = 8;
+#pragma unroll
+                for (uint32_t qq = 0; qq < 8; ++qq)
+                  if (cj != kSentinel && cells[qq] == cj) q = qq;
+                if (q < 8) {
+                  uint32_t x = (q << kFineIdBits) | j;
+#pragma unroll
+                  for (int u = 0; u < kFineFound; ++u) {
+                    const uint32_t lo2 = min(key[u], x), hi2 = max(key[u], x);
+                    key[u] = lo2;
+                    x = hi2;
+                  }
+                  ++nf;
+                }
+              }
+            }
+          }
+        }
+      } else {
+        nf = kFineFound + 1;
+      }
+      if (nf <= (uint32_t)kFineFound) {
+        for (uint32_t n = 0; n < nf; ++n) {
+          const uint32_t k0 = key[0];
+#pragma unroll
+          for (int u = 0; u + 1 < kFineFound; ++u) key[u] = key[u + 1];
+          key[kFineFound - 1] = 0xFFFFFFFFu;
+          consider(k0 & ((1u << kFineIdBits) - 1u));
+        }
+      } else {
+        // more partners than the sorted list holds: the 8-cell scan itself
+        for (int q = 0; q < 8; ++q) {
+          const uint32_t c = cells[q];
+          if (c == kSentinel) continue;
+          const uint32_t s0 = pnC[c], n = pnC[C + c];
+          for (uint32_t u = 0; u < n; ++u) {
+            const uint32_t j = CIs[N + s0 + u];
+            const float rr = pair_r2(R, N, i, j);
+            if (rr < aacut && (double)rr > 1e-10) consider(j);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (uint32_t e = threadIdx.x; e < np * pitch; e += kRecThreads) {
+    const uint32_t r = e / pitch, k = e % pitch;
+    if (k < K) IL[g0 + (size_t)r * K + k] = s_row[e];
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -478,15 +703,65 @@ int rfd_il_reconstruct_bpm(rfd_ctx *ctx, const float *R, uint32_t N,
   RFD_CHECK_ARG(ctx, R && CIs && pnC && IL && ILtype && N > 0 && K > 0 && slots_fit(N, K),
                 "rfd_il_reconstruct_bpm: bad arguments");
   RFD_CHECK_ARG(ctx, K <= (uint32_t)kRecMaxK, "rfd_il_reconstruct_bpm: K=%u > %d", K, kRecMaxK);
+  RFD_CHECK_ARG(ctx, Nx && Ny && Nz && (uint64_t)Nx * Ny * Nz < 0x7FFFFFFFull,
+                "rfd_il_reconstruct_bpm: bad grid %u x %u x %u", Nx, Ny, Nz);
+  hipStream_t q = ctx->stream;
+  const unsigned blocks = (N + kRecThreads - 1) / kRecThreads;
+  const size_t lds = sizeof(uint32_t) * kRecThreads * rec_pitch(K);
+  // the fine grid spans the coarse grid plus one fine cell on each side
+  FineGrid f{};
+  bool fine = aacut > 0.0f && std::isfinite(aacut) && N < (1u << kFineIdBits) - 1u;
+  if (fine) {
+    f.af = sqrtf(aacut) * 1.01f;
+    f.inv_af = 1.0f / f.af;
+    const double ext[3] = {(double)Nx * a, (double)Ny * a, (double)Nz * a};
+    uint32_t dims[3];
+    for (int d = 0; d < 3; ++d) {
+      const double n = (ext[d] + 2.0 * f.af) / f.af + 2.0;
+      fine = fine && n < (double)kFineMaxDim;
+      dims[d] = fine ? (uint32_t)n : 0;
+    }
+    f.nx = dims[0];
+    f.ny = dims[1];
+    f.nz = dims[2];
+    fine = fine && (uint64_t)f.nx * f.ny * f.nz < 0x7FFFFFFFull;
+  }
+  if (!fine) {
+    void *ws = nullptr;
+    int st = rfd_scratch(ctx, sizeof(float4) * (size_t)N, &ws);
+    if (st) return st;
+    float4 *Rc = (float4 *)ws;
+    k_cell_positions<<<rfd_blocks(N, 256), 256, 0, q>>>(R, N, CIs, Rc);
+    k_il_reconstruct<<<blocks, kRecThreads, lds, q>>>(R, N, Rc, pnC, IL, ILtype, K, a, inv_a,
+                                                      aacut, Nx, Ny, Nz, quirk_p0,
+                                                      ctx->d_flags);
+    RFD_LAUNCHED(ctx, "rfd_il_reconstruct_bpm");
+    return RFD_OK;
+  }
+  const uint32_t C = Nx * Ny * Nz;
+  const uint32_t Cf = f.nx * f.ny * f.nz;
+  const uint32_t chunks = (uint32_t)(((size_t)Cf + kScanChunk - 1) / kScanChunk);
+  // workspace: fine-ordered positions (16 B each), then cellOf[N], fci[N],
+  // cnt[Cf], excl[Cf + 1], part[chunks]
+  const size_t words = 3 * (size_t)N + 2 * (size_t)Cf + 1 + chunks;
   void *ws = nullptr;
-  int st = rfd_scratch(ctx, sizeof(float4) * (size_t)N, &ws);
+  int st = rfd_scratch(ctx, sizeof(float4) * (size_t)N + sizeof(uint32_t) * words, &ws);
   if (st) return st;
-  float4 *Rc = (float4 *)ws;
-  k_cell_positions<<<rfd_blocks(N, 256), 256, 0, ctx->stream>>>(R, N, CIs, Rc);
-  unsigned blocks = (N + kRecThreads - 1) / kRecThreads;
-  k_il_reconstruct<<<blocks, kRecThreads, sizeof(uint32_t) * kRecThreads * rec_pitch(K),
-                     ctx->stream>>>(
-      R, N, Rc, pnC, IL, ILtype, K, a, inv_a, aacut, Nx, Ny, Nz, quirk_p0, ctx->d_flags);
+  float4 *Rf = (float4 *)ws;
+  uint32_t *cellOf = (uint32_t *)(Rf + N), *fci = cellOf + N, *cnt = fci + N,
+           *excl = cnt + Cf, *part = excl + Cf + 1;
+  k_fill_u32<<<rfd_blocks(N, 256, 4096), 256, 0, q>>>(cellOf, N, kSentinel);
+  k_fill_u32<<<rfd_blocks(Cf, 256, 4096), 256, 0, q>>>(cnt, Cf, 0u);
+  k_cell_of<<<rfd_blocks(C, 256), 256, 0, q>>>(CIs, pnC, C, N, cellOf);
+  k_fine_index<<<rfd_blocks(N, 256), 256, 0, q>>>(R, N, f, fci, cnt);
+  k_chunk_sums<<<chunks, kScanThreads, 0, q>>>(cnt, Cf, part);
+  k_scan_chunk_sums<<<1, kScanThreads, 0, q>>>(part, chunks);
+  k_scan_excl<<<chunks, kScanThreads, 0, q>>>(cnt, Cf, excl, part);
+  k_fill_u32<<<rfd_blocks(Cf, 256, 4096), 256, 0, q>>>(cnt, Cf, 0u);
+  k_fine_scatter<<<rfd_blocks(N, 256), 256, 0, q>>>(R, N, fci, excl, cnt, Rf);
+  k_il_reconstruct_fine<<<blocks, kRecThreads, lds, q>>>(R, N, CIs, pnC, Rf, excl, cellOf, f,
+                                                         IL, ILtype, K, a, inv_a, aacut, Nx,
+                                                         Ny, Nz, quirk_p0, ctx->d_flags);
   RFD_LAUNCHED(ctx, "rfd_il_reconstruct_bpm");
   return RFD_OK;
 }

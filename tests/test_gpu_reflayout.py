@@ -76,6 +76,69 @@ def test_reconstruct_after_motion_bitexact(ctx):
     assert_same(g.host(), o.host(), where="reconstruct")
 
 
+def _relist_twin(ctx, R, K, grid, aacut, seed):
+    """Lists built, cut back to pairs closer than 0.5, particles moved, and
+    rebuilt by the reconstruction, on the GPU and in the oracle."""
+    import torch
+
+    g, o = RefLayout(ctx, R, K, grid), OracleLayout(R, K, grid)
+    ctx.flags()
+    for s in (g, o):
+        s.cell_distribution()
+        s.il_construct(aacut)
+        s.delete_far_links(0.25)
+    d = perturb(len(R) // 3, seed, 0.02)
+    g.R += torch.from_numpy(d).cuda()
+    o.R += d
+    for s in (g, o):
+        s.cell_distribution()
+        s.il_reconstruct_bpm(aacut, quirk_p0=True)
+    assert o.last_status == 0 and ctx.flags() == 0
+    keys = ("R", "CI", "CIs", "pnC", "IL", "ILtype")
+    assert_same(g.host(), o.host(), keys=keys, where="relist")
+    return o.host()["IL"].reshape(-1, K)
+
+
+def test_reconstruct_dense_cluster_bitexact(ctx):
+    """50 particles packed in a ball of radius 0.6 inside a lattice, so that
+    each has 49 partners in reach: more than the 32 the fine-grid search
+    sorts, which sends them through the 8-cell scan."""
+    from rfd.reflayout import Grid
+
+    rng = np.random.default_rng(21)
+    ax = np.arange(1.5, 15.0, 1.0)
+    lat = np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3)
+    c0 = np.array([7.3, 7.3, 7.3])
+    lat = lat[np.linalg.norm(lat - c0, axis=1) > 2.5]
+    u = rng.standard_normal((50, 3))
+    ball = c0 + u / np.linalg.norm(u, axis=1, keepdims=True) * 0.6 * rng.uniform(0, 1, (50, 1)) ** (1 / 3)
+    pos = np.concatenate([lat, ball])
+    R = np.ascontiguousarray(pos.T.reshape(-1)).astype(np.float32)
+    a = np.float32(2.9)
+    grid = Grid(float(a), float(np.float32(1.0) / a), 6, 6, 6)
+    aacut = float(np.float32(1.45) ** 2)
+    IL = _relist_twin(ctx, R, 64, grid, aacut, seed=22)
+    full = (IL != 0xFFFFFFFF).sum(axis=1)
+    assert full[len(lat):].min() > 32
+
+
+def test_reconstruct_wide_grid_bitexact(ctx):
+    """A grid of 25,000 x 1 x 1 cells, too long for the fine binning (more
+    than 40,000 fine cells along x): the reconstruction scans the 8 cells."""
+    from rfd.reflayout import Grid
+
+    rng = np.random.default_rng(31)
+    n = 300
+    pos = np.stack([rng.uniform(50.0, 80.0, n), rng.uniform(0.2, 2.7, n),
+                    rng.uniform(0.2, 2.7, n)], -1)
+    R = np.ascontiguousarray(pos.T.reshape(-1)).astype(np.float32)
+    a = np.float32(2.9)
+    grid = Grid(float(a), float(np.float32(1.0) / a), 25000, 1, 1)
+    aacut = float(np.float32(1.45) ** 2)
+    IL = _relist_twin(ctx, R, 32, grid, aacut, seed=32)
+    assert (IL != 0xFFFFFFFF).sum() > 5 * n
+
+
 def test_forces_bitexact_with_motion(ctx):
     import torch
 
