@@ -22,9 +22,10 @@
  *   - Kernels never allocate caller memory; the context owns its scratch: a
  *     grow-only device workspace, allocated by the first call that needs more
  *     than it holds (cell binning: 4 B per cell and per particle; list
- *     construct and reconstruct: a 16 B cell-ordered position record per
- *     particle) and freed by rfd_ctx_destroy.  Such a first call synchronises
- *     the context's stream.
+ *     construct: a 16 B position record per particle; list reconstruct: that
+ *     record plus 12 B per particle and 8 B per cell of its search grid, whose
+ *     cell edge is 1.01 sqrt(aacut)) and freed by rfd_ctx_destroy.  Such a
+ *     first call synchronises the context's stream.
  *   - One context per host thread.
  */
 #ifndef RFD_H
