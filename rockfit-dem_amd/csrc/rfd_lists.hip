@@ -426,31 +426,36 @@ bool slots_fit(uint32_t N, uint32_t K) { return (uint64_t)N * K < 0xFFFFFFFFull;
 
 
 // ---- fine-grid candidate search for the list reconstruction
-// A second binning of the particles, with a cell edge of 1.01 times the search
-// radius sqrt(aacut), puts every partner that can pass the FP32 range test in
-// the 3 x 3 x 3 fine cells around the particle: along an axis, a partner two
-// fine cells away is more than 0.99 fine cells = 1.0001 radii away, beyond
-// what the FP32 distance and the FP32 fine coordinates (below 40,000 cells)
-// can round across. Of the candidates that pass the range test, those that the
+// A second binning of the particles, with a cell edge of 1.01 / h times the
+// search radius sqrt(aacut), puts every partner that can pass the FP32 range
+// test in the (2h + 1)^3 fine cells around the particle: along an axis, a
+// partner h + 1 fine cells away is more than h + 1 - 0.01 fine cells, over
+// 1.0001 radii, away, beyond what the FP32 distance and the FP32 fine
+// coordinates (below 40,000 cells) can round across. The grid starts h cells
+// below the coarse grid and ends h cells beyond it, so every partner of a
+// particle in the coarse grid is binned. Of the candidates that pass the range test, those that the
 // caller's cell lists hold in one of the particle's 8 search cells are kept
 // and visited in the order of the 8-cell search: by search cell, then by id
 // (a cell's members are listed by id). So the same partners are inserted, in
 // the same order, as by the scan of the 8 cells, from about 100 candidates per
 // particle instead of 8 coarse cells' worth.
 constexpr uint32_t kFineMaxDim = 40000;
+constexpr int kFineHalf = 2;       // fine cells searched on each side
 constexpr int kFineFound = 32;     // in-range partners held (sorted) per particle
 constexpr uint32_t kFineIdBits = 29;  // key = search cell << 29 | id
 
 struct FineGrid {
   float af, inv_af;     // fine cell edge and its inverse
-  uint32_t nx, ny, nz;  // cells per axis; coordinate x maps to floor((x + af) / af)
+  float off;            // h * af: coordinate x maps to cell floor((x + off) / af)
+  uint32_t nx, ny, nz;  // cells per axis
+  int h;                // cells searched on each side (edge = 1.01 radius / h)
 };
 
 __device__ __forceinline__ bool fine_coords(const FineGrid &f, float x, float y, float z,
                                             uint32_t c[3]) {
-  const float fx = floorf((x + f.af) * f.inv_af);
-  const float fy = floorf((y + f.af) * f.inv_af);
-  const float fz = floorf((z + f.af) * f.inv_af);
+  const float fx = floorf((x + f.off) * f.inv_af);
+  const float fy = floorf((y + f.off) * f.inv_af);
+  const float fz = floorf((z + f.off) * f.inv_af);
   if (!(fx >= 0.0f && fx < (float)f.nx) || !(fy >= 0.0f && fy < (float)f.ny) ||
       !(fz >= 0.0f && fz < (float)f.nz))
     return false;
@@ -580,12 +585,13 @@ k_il_reconstruct_fine(const float *__restrict__ R, uint32_t N, const uint32_t *_
       uint32_t nf = 0;
       uint32_t fc[3];
       if (fine_coords(f, xi, yi, zi, fc)) {
-        const uint32_t xlo = fc[0] > 0 ? fc[0] - 1 : 0;
-        const uint32_t xhi = fc[0] + 1 < f.nx ? fc[0] + 1 : f.nx - 1;
-        for (int dz = -1; dz <= 1; ++dz) {
+        const uint32_t hu = (uint32_t)f.h;
+        const uint32_t xlo = fc[0] > hu ? fc[0] - hu : 0;
+        const uint32_t xhi = fc[0] + hu < f.nx ? fc[0] + hu : f.nx - 1;
+        for (int dz = -f.h; dz <= f.h; ++dz) {
           const int z = (int)fc[2] + dz;
           if (z < 0 || z >= (int)f.nz) continue;
-          for (int dy = -1; dy <= 1; ++dy) {
+          for (int dy = -f.h; dy <= f.h; ++dy) {
             const int y = (int)fc[1] + dy;
             if (y < 0 || y >= (int)f.ny) continue;
             const uint32_t base = f.nx * ((uint32_t)y + f.ny * (uint32_t)z);
@@ -712,12 +718,14 @@ int rfd_il_reconstruct_bpm(rfd_ctx *ctx, const float *R, uint32_t N,
   FineGrid f{};
   bool fine = aacut > 0.0f && std::isfinite(aacut) && N < (1u << kFineIdBits) - 1u;
   if (fine) {
-    f.af = sqrtf(aacut) * 1.01f;
+    f.h = kFineHalf;
+    f.af = sqrtf(aacut) * 1.01f / (float)kFineHalf;
     f.inv_af = 1.0f / f.af;
+    f.off = (float)kFineHalf * f.af;
     const double ext[3] = {(double)Nx * a, (double)Ny * a, (double)Nz * a};
     uint32_t dims[3];
     for (int d = 0; d < 3; ++d) {
-      const double n = (ext[d] + 2.0 * f.af) / f.af + 2.0;
+      const double n = (ext[d] + 2.0 * kFineHalf * (double)f.af) / f.af + 2.0;
       fine = fine && n < (double)kFineMaxDim;
       dims[d] = fine ? (uint32_t)n : 0;
     }
