@@ -24,7 +24,7 @@
  *     than it holds (cell binning: 4 B per cell and per particle; list
  *     construct: a 16 B position record per particle; list reconstruct: that
  *     record plus 12 B per particle and 8 B per cell of its search grid, whose
- *     cell edge is 1.01 sqrt(aacut)) and freed by rfd_ctx_destroy.  Such a
+ *     cell edge is 0.505 sqrt(aacut)) and freed by rfd_ctx_destroy.  Such a
  *     first call synchronises the context's stream.
  *   - One context per host thread.
  */
