@@ -6,6 +6,11 @@
 //   cursor -> per-cell rank by id (one wave per cell).
 // Ranking each cell's members by particle id restores index order inside a
 // cell, which is exactly the order a stable sort of (cell, id) produces.
+//
+// List reconstruction finds its candidates in a second, finer binning (cell
+// edge about half the search radius) and visits the ones in reach in the
+// order of the reference's 8-cell scan, so it inserts the same partners into
+// the same slots while testing about a quarter of the candidates.
 #include <cmath>
 
 #include "rfd_internal.h"
