@@ -191,13 +191,15 @@ k_forces_contact(const float *__restrict__ R, const float *__restrict__ V,
                  const uint8_t *__restrict__ ILtype, float *__restrict__ Oijt,
                  float *__restrict__ F, float *__restrict__ M, uint32_t N, uint32_t K,
                  float dt, float Estar, float Gstar, float pA, float mu, float mur,
-                 float m, float r) {
+                 float m, float r, uint32_t nblk, uint32_t per) {
   // per slot: normal force, tangential force, its moment, rolling moment
   __shared__ float s_c[12][kSlotThreads];
   __shared__ uint8_t s_live[kSlotThreads];  // 0 none, 1 contact, 2 contact that wipes ms
   const uint32_t P = kSlotThreads / K;
   const uint32_t t = threadIdx.x, lp = t / K, k = t % K;
-  const uint32_t p0 = blockIdx.x * P;
+  const uint32_t blk = xcd_block(per);
+  if (blk >= nblk) return;
+  const uint32_t p0 = blk * P;
   const uint32_t i = p0 + lp;
   const size_t N1 = N, N2 = 2 * (size_t)N;
   uint8_t live = 0;
@@ -402,9 +404,9 @@ int rfd_forces_contact(rfd_ctx *ctx, const float *R, const float *V, const float
   RFD_CHECK_ARG(ctx, K <= (uint32_t)kMaxK, "rfd_forces_contact: K=%u > %d", K, kMaxK);
   const uint32_t P = kSlotThreads / K;
   const unsigned blocks = (N + P - 1) / P;
-  k_forces_contact<<<blocks, kSlotThreads, 0, ctx->stream>>>(R, V, W, IL, ILtype, Oijt, F, M,
-                                                              N, K, dt, Estar, Gstar, pA, mu,
-                                                              mur, m, r);
+  const uint32_t per = (blocks + kXcds - 1) / kXcds;
+  k_forces_contact<<<per * kXcds, kSlotThreads, 0, ctx->stream>>>(
+      R, V, W, IL, ILtype, Oijt, F, M, N, K, dt, Estar, Gstar, pA, mu, mur, m, r, blocks, per);
   RFD_LAUNCHED(ctx, "rfd_forces_contact");
   return RFD_OK;
 }

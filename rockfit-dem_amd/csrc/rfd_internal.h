@@ -72,6 +72,15 @@ __device__ __forceinline__ uint32_t grid_stride() {
   return blockDim.x * gridDim.x;
 }
 
+// Workgroup b of a launch of kXcds * per workgroups takes block
+// (b % kXcds) * per + b / kXcds. Consecutive workgroups go to the 8 XCDs in
+// turn, so each XCD, with its own L2, walks one contiguous run of blocks and
+// finds its neighbours' data in that L2.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_block(uint32_t per) {
+  return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+}
+
 // wave64 sum of a double across the wavefront (butterfly over all 64 lanes,
 // every lane ends with the same total)
 __device__ __forceinline__ double wave_sum(double v) {

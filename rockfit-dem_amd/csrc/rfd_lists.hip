@@ -539,11 +539,14 @@ k_il_reconstruct_fine(const float *__restrict__ R, uint32_t N, const uint32_t *_
                       const uint32_t *__restrict__ excl, const uint32_t *__restrict__ cellOf,
                       FineGrid f, uint32_t *__restrict__ IL, uint8_t *__restrict__ ILtype,
                       uint32_t K, float a, float inv_a, float aacut, uint32_t Nx, uint32_t Ny,
-                      uint32_t Nz, int quirk_p0, uint32_t *__restrict__ flags) {
+                      uint32_t Nz, int quirk_p0, uint32_t *__restrict__ flags,
+                      uint32_t nblk, uint32_t per) {
   extern __shared__ uint4 s_row4[];
   uint32_t *s_row = reinterpret_cast<uint32_t *>(s_row4);
   const uint32_t C = Nx * Ny * Nz;
-  const uint32_t p0 = blockIdx.x * kRecThreads;
+  const uint32_t blk = xcd_block(per);
+  if (blk >= nblk) return;
+  const uint32_t p0 = blk * kRecThreads;
   const uint32_t np = min((uint32_t)kRecThreads, N - p0);
   const uint32_t pitch = rec_pitch(K);
   const size_t g0 = (size_t)p0 * K;
@@ -772,9 +775,10 @@ int rfd_il_reconstruct_bpm(rfd_ctx *ctx, const float *R, uint32_t N,
   k_scan_excl<<<chunks, kScanThreads, 0, q>>>(cnt, Cf, excl, part);
   k_fill_u32<<<rfd_blocks(Cf, 256, 4096), 256, 0, q>>>(cnt, Cf, 0u);
   k_fine_scatter<<<rfd_blocks(N, 256), 256, 0, q>>>(R, N, fci, excl, cnt, Rf);
-  k_il_reconstruct_fine<<<blocks, kRecThreads, lds, q>>>(R, N, CIs, pnC, Rf, excl, cellOf, f,
-                                                         IL, ILtype, K, a, inv_a, aacut, Nx,
-                                                         Ny, Nz, quirk_p0, ctx->d_flags);
+  const uint32_t per = (blocks + kXcds - 1) / kXcds;
+  k_il_reconstruct_fine<<<per * kXcds, kRecThreads, lds, q>>>(
+      R, N, CIs, pnC, Rf, excl, cellOf, f, IL, ILtype, K, a, inv_a, aacut, Nx, Ny, Nz, quirk_p0,
+      ctx->d_flags, blocks, per);
   RFD_LAUNCHED(ctx, "rfd_il_reconstruct_bpm");
   return RFD_OK;
 }
