@@ -427,6 +427,47 @@ __global__ void k_delete_far(const float *__restrict__ R, uint32_t N, uint32_t *
   }
 }
 
+// The same pruning four slots per thread (K a multiple of 4, IL 16-byte
+// aligned): one 16-byte load of the slot ids and one load of the particle's
+// position serve four distance tests, whose gathers are then in flight
+// together. Every slot is decided on its own, as above.
+__global__ void k_delete_far4(const float *__restrict__ R, uint32_t N, uint32_t *__restrict__ IL,
+                              uint8_t *__restrict__ ILtype, float *__restrict__ rij,
+                              float *__restrict__ Oijt, float *__restrict__ Fijn,
+                              float *__restrict__ Fijt, float *__restrict__ Mijn,
+                              float *__restrict__ Mijt, float *__restrict__ Mijadd,
+                              uint32_t K, float aa_farcut) {
+  const uint32_t Q = N * (K / 4);
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= Q) return;
+  const uint4 v = reinterpret_cast<const uint4 *>(IL)[g];
+  if ((v.x & v.y & v.z & v.w) == kSentinel) return;
+  const uint32_t s0 = 4 * g, i = s0 / K;
+  const float xi = R[i], yi = R[i + N], zi = R[i + 2 * (size_t)N];
+  const uint32_t js[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const uint32_t j = js[u];
+    if (j == kSentinel) continue;
+    const float dx = R[j] - xi;
+    const float dy = R[j + N] - yi;
+    const float dz = R[j + 2 * (size_t)N] - zi;
+    const float rr = dx * dx + dy * dy + dz * dz;
+    if (rr > aa_farcut || (double)rr < 1e-10) {
+      const uint32_t s = s0 + u;
+      IL[s] = kSentinel;
+      ILtype[s] = 0;
+      zero3(rij, s);
+      zero3(Oijt, s);
+      zero3(Fijn, s);
+      zero3(Fijt, s);
+      zero3(Mijn, s);
+      zero3(Mijt, s);
+      zero3(Mijadd, s);
+    }
+  }
+}
+
 bool slots_fit(uint32_t N, uint32_t K) { return (uint64_t)N * K < 0xFFFFFFFFull; }
 
 
@@ -805,8 +846,14 @@ int rfd_delete_far_links(rfd_ctx *ctx, const float *R, uint32_t N, uint32_t *IL,
   RFD_CHECK_ARG(ctx, R && IL && ILtype && rij && Oijt && Fijn && Fijt && Mijn && Mijt &&
                          Mijadd && N > 0 && K > 0 && slots_fit(N, K),
                 "rfd_delete_far_links: bad arguments");
-  k_delete_far<<<rfd_blocks((size_t)N * K, 256), 256, 0, ctx->stream>>>(
-      R, N, IL, ILtype, rij, Oijt, Fijn, Fijt, Mijn, Mijt, Mijadd, K, aa_farcut);
+  if (K % 4 == 0 && reinterpret_cast<uintptr_t>(IL) % 16 == 0) {
+    const size_t quads = (size_t)N * (K / 4);
+    k_delete_far4<<<(unsigned)((quads + 255) / 256), 256, 0, ctx->stream>>>(
+        R, N, IL, ILtype, rij, Oijt, Fijn, Fijt, Mijn, Mijt, Mijadd, K, aa_farcut);
+  } else {
+    k_delete_far<<<rfd_blocks((size_t)N * K, 256), 256, 0, ctx->stream>>>(
+        R, N, IL, ILtype, rij, Oijt, Fijn, Fijt, Mijn, Mijt, Mijadd, K, aa_farcut);
+  }
   RFD_LAUNCHED(ctx, "rfd_delete_far_links");
   return RFD_OK;
 }
