@@ -269,10 +269,19 @@ def test_rolling_torque_and_wipe_quirk():
     np.testing.assert_allclose(M0, expect, rtol=1e-4, atol=1e-12)
     # W_i == 0: the reference's else-branch wipes the whole accumulator
     q = Pair((2 * m.p_r - delta, 0, 0), bonded=False)
-    q.set_w(1, (0, 0, 3e-3))
+    w1 = np.array([0.0, 0.0, 3e-3])
+    q.set_w(1, w1)
     q.contact(m)
     assert np.all(q.vec(q.M, 0) == 0)
-    assert np.any(q.vec(q.M, 1) != 0) or True
+    # particle 1 spins, so its own row keeps the lever moment of its
+    # tangential force (seen along n1 = -x) and gets the rolling torque
+    F1, M1 = q.vec(q.F, 1), q.vec(q.M, 1)
+    n1 = np.array([-1.0, 0, 0])
+    fn1 = abs(F1 @ n1)
+    Ft1 = F1 - n1 * (F1 @ n1)
+    assert np.linalg.norm(Ft1) > 0
+    expect1 = m.p_r * np.cross(n1, Ft1) - m.p_mur * fn1 * w1 / np.linalg.norm(w1)
+    np.testing.assert_allclose(M1, expect1, rtol=1e-4, atol=1e-12)
 
 
 def test_head_on_restitution():
