@@ -74,6 +74,19 @@ int rfd_cell_distribution(rfd_ctx *ctx, const float *R, uint32_t N, float inv_a,
                           uint32_t Nx, uint32_t Ny, uint32_t Nz, uint32_t *CI,
                           uint32_t *CIs, uint32_t *pnC);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): rfd_cell_distribution with the same arguments and
+ * results, whose workspace is a fixed 64 MiB array in the library's own
+ * device memory (one per GPU, held until the process ends) instead of the
+ * context's grow-only workspace; the call allocates nothing.  Limit:
+ * N + C + ceil(C / 8192) <= 16,777,216 (C = Nx Ny Nz), else RFD_E_CAPACITY.
+ * Every call on one GPU shares that array: calls that overlap on one GPU (two
+ * streams, two threads) race and must be ordered by the caller. */
+int rfd_cell_distribution_noalloc(rfd_ctx *ctx, const float *R, uint32_t N, float inv_a,
+                                  uint32_t Nx, uint32_t Ny, uint32_t Nz, uint32_t *CI,
+                                  uint32_t *CIs, uint32_t *pnC);
+
 /* InteractionListConstruct (md_interaction_list_construction.cu:62-67) =
  * memset IL 0xFF + d_ConstructInteractionList (md_interaction_list_construct.cu:35) */
 int rfd_il_construct(rfd_ctx *ctx, const float *R, uint32_t N,

@@ -58,6 +58,11 @@ int rfd_ctx_create(rfd_ctx **out, int device, void *stream) {
     rfd_ctx_destroy(ctx);
     return (int)e;
   }
+  const int st = rfd_fixed_scratch_attach(ctx);
+  if (st != RFD_OK) {
+    rfd_ctx_destroy(ctx);
+    return st;
+  }
   *out = ctx;
   return RFD_OK;
 }

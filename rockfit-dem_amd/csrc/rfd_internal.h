@@ -23,6 +23,7 @@ struct rfd_ctx {
   double *h_red = nullptr;         // pinned host copy of reduction results
   void *scratch = nullptr;         // grow-only workspace
   size_t scratch_bytes = 0;
+  void *fixed_scratch = nullptr;   // device address of the fixed workspace
   char err[512] = {0};
 };
 
@@ -34,6 +35,9 @@ int rfd_set_error(rfd_ctx *ctx, int code, const char *fmt, ...);
 int rfd_hip_status(rfd_ctx *ctx, hipError_t e, const char *where);
 // grow-only workspace on the context's device
 int rfd_scratch(rfd_ctx *ctx, size_t bytes, void **out);
+// look up the device address of the fixed (no-allocation) workspace on the
+// context's device, the device being current
+int rfd_fixed_scratch_attach(rfd_ctx *ctx);
 
 #define RFD_CHECK_ARG(ctx, cond, ...)                  \
   do {                                                 \

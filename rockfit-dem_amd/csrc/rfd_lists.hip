@@ -12,6 +12,7 @@
 // order of the reference's 8-cell scan, so it inserts the same partners into
 // the same slots while testing about a quarter of the candidates.
 #include <cmath>
+#include <mutex>
 
 #include "rfd_internal.h"
 
@@ -702,23 +703,33 @@ k_il_reconstruct_fine(const float *__restrict__ R, uint32_t N, const uint32_t *_
   }
 }
 
-}  // namespace
+// Workspace words of one cell binning: scatter cursors (C), the unranked cell
+// lists (N) and the scan's chunk sums.
+size_t cell_ws_words(uint32_t N, uint32_t C) {
+  return (size_t)C + N + ((size_t)C + kScanChunk - 1) / kScanChunk;
+}
 
-extern "C" {
+// Fixed workspace of rfd_cell_distribution_noalloc: 64 MiB in the code
+// object's own device memory, one copy per GPU, held until the process ends.
+// Host code must not pass this name to a kernel (on the host it is only a
+// stand-in); it uses the device address kept per device in g_fixed_ptr.
+constexpr size_t kFixedWords = (size_t)16 << 20;
+static __device__ __align__(256) uint32_t scratch[kFixedWords];
 
-int rfd_cell_distribution(rfd_ctx *ctx, const float *R, uint32_t N, float inv_a,
-                          uint32_t Nx, uint32_t Ny, uint32_t Nz, uint32_t *CI,
-                          uint32_t *CIs, uint32_t *pnC) {
-  if (!ctx) return RFD_E_ARG;
-  RFD_CHECK_ARG(ctx, R && CI && CIs && pnC && N > 0, "rfd_cell_distribution: null buffer or N=0");
-  RFD_CHECK_ARG(ctx, Nx && Ny && Nz && (uint64_t)Nx * Ny * Nz < 0x7FFFFFFFull,
-                "rfd_cell_distribution: bad grid %u x %u x %u", Nx, Ny, Nz);
+constexpr int kMaxDevices = 64;
+std::mutex g_fixed_mu;
+void *g_fixed_ptr[kMaxDevices] = {};
+
+// Cell binning with the workspace `ws` (cell_ws_words(N, C) words).  Every
+// workspace word a kernel reads was written earlier in the same call (the
+// cursors by the scan, the lists by the scatter, the chunk sums by the first
+// scan pass), so the workspace needs no clearing.
+int cell_distribution_ws(rfd_ctx *ctx, uint32_t *ws, const float *R, uint32_t N, float inv_a,
+                         uint32_t Nx, uint32_t Ny, uint32_t Nz, uint32_t *CI, uint32_t *CIs,
+                         uint32_t *pnC) {
   const uint32_t C = Nx * Ny * Nz;
-  void *ws = nullptr;
   const uint32_t chunks = (uint32_t)(((size_t)C + kScanChunk - 1) / kScanChunk);
-  int st = rfd_scratch(ctx, sizeof(uint32_t) * ((size_t)C + N + chunks), &ws);
-  if (st) return st;
-  uint32_t *cursor = (uint32_t *)ws, *tmp = cursor + C, *part = tmp + N;
+  uint32_t *cursor = ws, *tmp = cursor + C, *part = tmp + N;
   hipStream_t s = ctx->stream;
   k_fill_u32<<<rfd_blocks(2 * (size_t)C, 256, 4096), 256, 0, s>>>(pnC, 2 * (size_t)C, 0u);
   k_cell_index<<<rfd_blocks(N, 256), 256, 0, s>>>(R, N, inv_a, Nx, Ny, Nz, CI, pnC + C,
@@ -730,6 +741,61 @@ int rfd_cell_distribution(rfd_ctx *ctx, const float *R, uint32_t N, float inv_a,
   k_rank_cells<<<rfd_blocks(C, 4), 256, 0, s>>>(tmp, pnC, C, N, CIs);
   RFD_LAUNCHED(ctx, "rfd_cell_distribution");
   return RFD_OK;
+}
+
+}  // namespace
+
+// Device address of the fixed workspace on ctx's device, looked up once per
+// device (with that device current) and kept in g_fixed_ptr; called by
+// rfd_ctx_create.
+int rfd_fixed_scratch_attach(rfd_ctx *ctx) {
+  if (ctx->device < 0 || ctx->device >= kMaxDevices)
+    return rfd_set_error(ctx, RFD_E_ARG, "device %d beyond the fixed-workspace table",
+                         ctx->device);
+  std::lock_guard<std::mutex> lock(g_fixed_mu);
+  if (!g_fixed_ptr[ctx->device]) {
+    void *p = nullptr;
+    RFD_HIP(ctx, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_fixed_ptr[ctx->device] = p;
+  }
+  ctx->fixed_scratch = g_fixed_ptr[ctx->device];
+  return RFD_OK;
+}
+
+extern "C" {
+
+int rfd_cell_distribution(rfd_ctx *ctx, const float *R, uint32_t N, float inv_a,
+                          uint32_t Nx, uint32_t Ny, uint32_t Nz, uint32_t *CI,
+                          uint32_t *CIs, uint32_t *pnC) {
+  if (!ctx) return RFD_E_ARG;
+  RFD_CHECK_ARG(ctx, R && CI && CIs && pnC && N > 0, "rfd_cell_distribution: null buffer or N=0");
+  RFD_CHECK_ARG(ctx, Nx && Ny && Nz && (uint64_t)Nx * Ny * Nz < 0x7FFFFFFFull,
+                "rfd_cell_distribution: bad grid %u x %u x %u", Nx, Ny, Nz);
+  void *ws = nullptr;
+  int st = rfd_scratch(ctx, sizeof(uint32_t) * cell_ws_words(N, Nx * Ny * Nz), &ws);
+  if (st) return st;
+  return cell_distribution_ws(ctx, (uint32_t *)ws, R, N, inv_a, Nx, Ny, Nz, CI, CIs, pnC);
+}
+
+int rfd_cell_distribution_noalloc(rfd_ctx *ctx, const float *R, uint32_t N, float inv_a,
+                                  uint32_t Nx, uint32_t Ny, uint32_t Nz, uint32_t *CI,
+                                  uint32_t *CIs, uint32_t *pnC) {
+  if (!ctx) return RFD_E_ARG;
+  RFD_CHECK_ARG(ctx, R && CI && CIs && pnC && N > 0,
+                "rfd_cell_distribution_noalloc: null buffer or N=0");
+  RFD_CHECK_ARG(ctx, Nx && Ny && Nz && (uint64_t)Nx * Ny * Nz < 0x7FFFFFFFull,
+                "rfd_cell_distribution_noalloc: bad grid %u x %u x %u", Nx, Ny, Nz);
+  const size_t words = cell_ws_words(N, Nx * Ny * Nz);
+  if (words > kFixedWords)
+    return rfd_set_error(ctx, RFD_E_CAPACITY,
+                         "rfd_cell_distribution_noalloc: N + cells = %zu words of workspace, "
+                         "the fixed workspace holds %zu (N + cells + ceil(cells / 8192) <= "
+                         "16777216)",
+                         words, kFixedWords);
+  if (!ctx->fixed_scratch)
+    return rfd_set_error(ctx, RFD_E_STATE, "rfd_cell_distribution_noalloc: no fixed workspace");
+  return cell_distribution_ws(ctx, (uint32_t *)ctx->fixed_scratch, R, N, inv_a, Nx, Ny, Nz, CI,
+                              CIs, pnC);
 }
 
 int rfd_il_construct(rfd_ctx *ctx, const float *R, uint32_t N, const uint32_t *CIs,
