@@ -298,6 +298,11 @@ int rfd_system_get_loading(rfd_system *sys, rfd_loading *L);
 /* host copies; any pointer may be NULL */
 int rfd_system_get_state(rfd_system *sys, double *R, float *V, float *W, float *F,
                          float *M);
+/* the same in double: the FP64 path's own values without rounding (the state
+ * a step-by-step comparison with an FP64 restatement starts each step from),
+ * the FP32 path's widened */
+int rfd_system_get_state64(rfd_system *sys, double *R, double *V, double *W, double *F,
+                           double *M);
 /* one force evaluation (bonds, broken-bond contacts, contact rows) into F, M;
  * commits the bond / contact state like a step does */
 int rfd_system_compute_forces(rfd_system *sys);

@@ -47,7 +47,7 @@ _SIG = {
     "orc_native_bonds_f64": ([_P] * 3 + [_u32] + [_P] * 8 + [_u32, C.c_double, C.c_double,
                                                              C.c_double, _int, _P, _P], None),
     "orc_native_contacts_f64": ([_P] * 4 + [_u32] + [_P] * 4 + [_u32] + [C.c_double] * 7
-                                + [_P, _P], None),
+                                + [_P, _P, _P], None),
 }
 
 _libs: dict = {}
@@ -235,11 +235,13 @@ class OracleUniaxialDriver:
         return sums
 
 
-def native_contacts(R, V, W, rad, pairs, reach, O, dt, Estar, Gstar, pA, mu, mur, rho43pi):
+def native_contacts(R, V, W, rad, pairs, reach, O, dt, Estar, Gstar, pA, mu, mur, rho43pi,
+                    cap=None):
     """Native-tier contact law per directed pair (orc_native_contacts_f64):
     R, V, W are (N, 3), pairs (P, 2) with i = pairs[:, 0] seeing j =
     pairs[:, 1], reach (P,), O (P, 3) updated in place.  Returns F, M (N, 3)
-    of i's load in float64."""
+    of i's load in float64; cap, a float64 (P,) array if given, receives each
+    pair's Coulomb-test margin."""
     R, V, W = (np.ascontiguousarray(np.asarray(a, np.float64).T) for a in (R, V, W))
     N = R.shape[1]
     rad = np.ascontiguousarray(rad, np.float64)
@@ -249,9 +251,11 @@ def native_contacts(R, V, W, rad, pairs, reach, O, dt, Estar, Gstar, pA, mu, mur
     assert O.dtype == np.float64 and O.flags["C_CONTIGUOUS"] and O.shape == (len(pi), 3)
     F = np.zeros((3, N))
     M = np.zeros((3, N))
+    if cap is not None:
+        assert cap.dtype == np.float64 and cap.flags["C_CONTIGUOUS"] and cap.shape == (len(pi),)
     lib().orc_native_contacts_f64(ptr(R), ptr(V), ptr(W), ptr(rad), N, ptr(pi), ptr(pj),
                                   ptr(reach), ptr(O), len(pi), dt, Estar, Gstar, pA, mu, mur,
-                                  rho43pi, ptr(F), ptr(M))
+                                  rho43pi, ptr(F), ptr(M), ptr(cap))
     return F.T.copy(), M.T.copy()
 
 

@@ -853,9 +853,10 @@ void orc_native_contacts_f64(const double *R, const double *V, const double *W,
                              const uint32_t *pj, const double *reach, double *O,
                              uint32_t P, double dt, double Estar, double Gstar,
                              double pA, double mu, double mur, double rho43pi,
-                             double *F, double *M) {
+                             double *F, double *M, double *cap) {
   memset(F, 0, sizeof(double) * 3 * (size_t)N);
   memset(M, 0, sizeof(double) * 3 * (size_t)N);
+  if (cap) memset(cap, 0, sizeof(double) * (size_t)P);
   for (uint32_t k = 0; k < P; ++k) {
     const uint32_t i = pi[k], j = pj[k];
     double r[3], Vi[3], Vj[3], Wi[3], Wj[3];
@@ -903,6 +904,7 @@ void orc_native_contacts_f64(const double *R, const double *V, const double *W,
       ft[d] = ot[d] * Kt;
     }
     const double ftm = dot3(ft, ft);
+    if (cap) cap[k] = ftm / (mu * mu * fn * fn) - 1.0;
     if (ftm > mu * mu * fn * fn) {
       const double s = mu * fabs(fn) / sqrt(ftm);
       for (int d = 0; d < 3; ++d) {

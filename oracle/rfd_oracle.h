@@ -120,13 +120,16 @@ void orc_native_bonds_f64(const double *R, const double *V, const double *W,
 /* Native-tier contact law per directed pair (i sees j), double precision:
  * pairs (pi[k], pj[k]) overlap below reach[k]; O[3k] is the pair's shear
  * spring (updated); F, M (zeroed here) receive i's load.  Per-particle radii
- * rad[N]; masses rho43pi r^3. */
+ * rad[N]; masses rho43pi r^3.  cap (may be NULL) receives per pair the
+ * Coulomb test's relative margin |Ft|^2 / (mu fn)^2 - 1 (0 for a pair not in
+ * contact): the law switches from the damped spring to the capped force at
+ * 0, so a margin near 0 is decided by rounding. */
 void orc_native_contacts_f64(const double *R, const double *V, const double *W,
                              const double *rad, uint32_t N, const uint32_t *pi,
                              const uint32_t *pj, const double *reach, double *O,
                              uint32_t P, double dt, double Estar, double Gstar,
                              double pA, double mu, double mur, double rho43pi,
-                             double *F, double *M);
+                             double *F, double *M, double *cap);
 
 #ifdef __cplusplus
 }

@@ -142,18 +142,21 @@ struct EndData {
 // Returns the load on i and the contact moments on i and j.
 // reach: the centre distance at which the pair starts to overlap (ri + rj,
 // or less for a broken bond made while overlapping)
-__device__ __forceinline__ void contact_law(const Mat &mt, v3 n, float L, float ri, float rj,
-                                            float reach, v3 Vi, v3 Vj, v3 Wi, v3 Wj, v3 &O,
-                                            v3 &Fi, v3 &Mi, v3 &Mj) {
+// dl is the overlap reach - L (> 0), formed by the caller from the FP64
+// positions: for a pair that just touches it is far smaller than either
+// length, and its FP32 difference would lose most of its digits
+__device__ __forceinline__ void contact_law(const Mat &mt, v3 n, float dl, float ri, float rj,
+                                            v3 Vi, v3 Vj, v3 Wi, v3 Wj, v3 &O, v3 &Fi,
+                                            v3 &Mi, v3 &Mj) {
   const float Rs = ri * rj / (ri + rj);
   const float mi = mt.rho43pi * ri * ri * ri, mj = mt.rho43pi * rj * rj * rj;
   const float ms = mi * mj / (mi + mj);
-  const float q = sqrtf(Rs * (reach - L));
+  const float q = sqrtf(Rs * dl);
   const float Kn = 2.0f * mt.Estar * q;
   const float Kt = 8.0f * mt.Gstar * q;
   const v3 v = Vj - Vi + cross(n, Wi * ri + Wj * rj);
   const float vn = dot(n, v);
-  const float fn = (L - reach) * Kn * kTwoThirdsF;
+  const float fn = -dl * Kn * kTwoThirdsF;
   const float dmp = -1.8257f * mt.pA * vn * sqrtf(Kn * ms);
   Fi = n * (fn + dmp);
   const v3 vt = v - n * vn;
@@ -278,10 +281,12 @@ __device__ __forceinline__ void eval_bond(const Mat &mt, const EndData &A, const
     // a broken bond meets the contact law at its rest length when it was made
     // overlapping, so the overlap it was made with does not spring apart
     // (reference mode: at ri + rj, as md_calculateforcesdem3.cu does)
-    const float reach = mt.refmode ? ri + rj : fminf(ri + rj, 1.0f / c.il);
-    if (L < reach) {
+    const double reach = mt.refmode ? A.P.r + B.P.r : fmin(A.P.r + B.P.r, 1.0 / (double)c.il);
+    const double L64 = __builtin_sqrt(r2);
+    if (L64 < reach) {
       v3 Fc;
-      contact_law(mt, n, L, ri, rj, reach, A.V, B.V, A.W, B.W, O, Fc, o.Mci, o.Mcj);
+      contact_law(mt, n, (float)(reach - L64), ri, rj, A.V, B.V, A.W, B.W, O, Fc, o.Mci,
+                  o.Mcj);
       o.Fi = o.Fi + Fc;
       o.hasc = true;
     } else {
@@ -465,13 +470,14 @@ k_nat_forces(SysDev s, Mat mt, uint32_t ntiles, uint32_t per) {
     const double rx = other.P.x - me.P.x, ry = other.P.y - me.P.y, rz = other.P.z - me.P.z;
     const double L64 = __builtin_sqrt(rx * rx + ry * ry + rz * rz);
     const float ri = (float)me.P.r, rj = (float)other.P.r;
-    if ((float)L64 < ri + rj && L64 > 1e-5) {
+    const double reach = me.P.r + other.P.r;
+    if (L64 < reach && L64 > 1e-5) {
       const double inv = 1.0 / L64;
       const v3 n = {(float)(rx * inv), (float)(ry * inv), (float)(rz * inv)};
       v3 O = xyz(s.cO[sl]);
       v3 Fi, Mi, Mj;
-      contact_law(mt, n, (float)L64, ri, rj, ri + rj, me.V, other.V, me.W, other.W, O, Fi, Mi,
-                  Mj);
+      contact_law(mt, n, (float)(reach - L64), ri, rj, me.V, other.V, me.W, other.W, O, Fi,
+                  Mi, Mj);
       F = F + Fi;
       Mc = Mc + Mi;
       anyc = true;
@@ -802,13 +808,14 @@ k_nat_forces1(SysDev s, Mat mt, WorkDev w, uint32_t ntiles, uint32_t per, uint32
       const double rx = other.P.x - me.P.x, ry = other.P.y - me.P.y, rz = other.P.z - me.P.z;
       const double L64 = __builtin_sqrt(rx * rx + ry * ry + rz * rz);
       const float ri = (float)me.P.r, rj = (float)other.P.r;
-      if ((float)L64 < ri + rj && L64 > 1e-5) {
+      const double reach = me.P.r + other.P.r;
+      if (L64 < reach && L64 > 1e-5) {
         const double inv = 1.0 / L64;
         const v3 n = {(float)(rx * inv), (float)(ry * inv), (float)(rz * inv)};
         v3 O = xyz(s.cO[sl]);
         v3 Fi, Mi, Mj;
-        contact_law(mt, n, (float)L64, ri, rj, ri + rj, me.V, other.V, me.W, other.W, O, Fi, Mi,
-                    Mj);
+        contact_law(mt, n, (float)(reach - L64), ri, rj, me.V, other.V, me.W, other.W, O, Fi,
+                    Mi, Mj);
         F = F + Fi;
         Mc = Mc + Mi;
         if (!s.trial) s.cO[sl] = f4(O);
@@ -4078,6 +4085,42 @@ int rfd_system_get_state(rfd_system *s, double *R, float *V, float *W, float *F,
   if (!st && W) st = download_vec3(ctx, W, s->W, N);
   if (!st && F) st = download_vec3(ctx, F, s->F, N);
   if (!st && M) st = download_vec3(ctx, M, s->M, N);
+  return st;
+}
+
+// device vec3 records (float4 or double4) to host SoA doubles
+extern "C++" {
+template <class T4>
+int download_soa64(rfd_ctx *ctx, double *soa, const T4 *src, uint32_t N) {
+  std::vector<T4> h(N);
+  RFD_HIP(ctx, hipMemcpy(h.data(), src, sizeof(T4) * N, hipMemcpyDeviceToHost));
+  for (uint32_t i = 0; i < N; ++i) {
+    soa[i] = (double)h[i].x;
+    soa[i + (size_t)N] = (double)h[i].y;
+    soa[i + 2 * (size_t)N] = (double)h[i].z;
+  }
+  return RFD_OK;
+}
+}
+
+int rfd_system_get_state64(rfd_system *s, double *R, double *V, double *W, double *F,
+                           double *M) {
+  if (!valid(s)) return RFD_E_ARG;
+  int st = rfd_system_get_state(s, R, nullptr, nullptr, nullptr, nullptr);
+  if (st) return st;
+  rfd_ctx *ctx = s->ctx;
+  const uint32_t N = s->N;
+  if (s->fp64) {
+    if (V) st = download_soa64(ctx, V, s->V64, N);
+    if (!st && W) st = download_soa64(ctx, W, s->W64, N);
+    if (!st && F) st = download_soa64(ctx, F, s->F64, N);
+    if (!st && M) st = download_soa64(ctx, M, s->M64, N);
+    return st;
+  }
+  if (V) st = download_soa64(ctx, V, s->V, N);
+  if (!st && W) st = download_soa64(ctx, W, s->W, N);
+  if (!st && F) st = download_soa64(ctx, F, s->F, N);
+  if (!st && M) st = download_soa64(ctx, M, s->M, N);
   return st;
 }
 

@@ -111,6 +111,7 @@ NATIVE_SIGNATURES = {
     "rfd_system_set_loading": [_vp, C.POINTER(Loading)],
     "rfd_system_get_loading": [_vp, C.POINTER(Loading)],
     "rfd_system_get_state": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "rfd_system_get_state64": [_vp, _vp, _vp, _vp, _vp, _vp],
     "rfd_system_compute_forces": [_vp],
     "rfd_system_step": [_vp, _u32],
     "rfd_system_get_history": [_vp, _vp, _u32, _u32p],
@@ -222,7 +223,11 @@ class NativeSystem:
 
     def close(self):
         if getattr(self, "h", None):
-            self.L.rfd_system_destroy(self.h)
+            # a system outliving its context (e.g. held by a traceback until
+            # interpreter exit) is left to the process end: its destroy call
+            # would use the freed context
+            if getattr(self.ctx, "h", None):
+                self.L.rfd_system_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -326,6 +331,14 @@ class NativeSystem:
         self._call("rfd_system_get_state", self.h, *(a.ctypes.data_as(C.c_void_p)
                                                       for a in (R, V, W, F, M)))
         return {"R": R, "V": V, "W": W, "F": F, "M": M}
+
+    def state64(self):
+        """R, V, W, F, M as (N, 3) float64 arrays: the FP64 path's own values
+        (the FP32 path's widened)."""
+        out = {k: np.zeros(3 * self.N, np.float64) for k in ("R", "V", "W", "F", "M")}
+        self._call("rfd_system_get_state64", self.h,
+                   *(out[k].ctypes.data_as(C.c_void_p) for k in ("R", "V", "W", "F", "M")))
+        return {k: v.reshape(3, self.N).T.copy() for k, v in out.items()}
 
     def history(self) -> np.ndarray:
         n = C.c_uint32()
