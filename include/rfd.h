@@ -64,6 +64,12 @@ int rfd_read_flags(rfd_ctx *ctx, uint32_t *flags);
 /* library build description, e.g. "rfd gfx950 ..." */
 const char *rfd_version(void);
 
+/* No counterpart in the reference: the measured HBM ceiling that roofline
+ * figures are read against.  Copies `bytes` (a multiple of 16, both buffers
+ * 16-byte aligned) from src to dst with 16-byte non-temporal loads and
+ * stores, `blocks` workgroups of 256 threads (0: 4096).  Asynchronous. */
+int rfd_stream_copy(rfd_ctx *ctx, void *dst, const void *src, size_t bytes, uint32_t blocks);
+
 /* ========================= tier 1: reference layout ===================== */
 
 /* CellDistribution (md_cell_distribution.cu:52-101): d_CalculateCellIndex

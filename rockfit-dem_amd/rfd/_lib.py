@@ -26,6 +26,7 @@ SIGNATURES: dict[str, list] = {
     "rfd_sync": [_vp],
     "rfd_read_flags": [_vp, C.POINTER(_u32)],
     "rfd_version": [],
+    "rfd_stream_copy": [_vp, _vp, _vp, C.c_size_t, _u32],
     "rfd_cell_distribution": [_vp, _vp, _u32, _f32, _u32, _u32, _u32, _vp, _vp, _vp],
     "rfd_cell_distribution_noalloc": [_vp, _vp, _u32, _f32, _u32, _u32, _u32, _vp, _vp, _vp],
     "rfd_il_construct": [_vp, _vp, _u32, _vp, _vp, _vp, _u32, _f32, _f32, _f32, _u32, _u32, _u32],
