@@ -195,6 +195,40 @@ int rfd_fire_halfstep(rfd_ctx *ctx, const float *V, float *R, uint32_t N,
 int rfd_fire_power(rfd_ctx *ctx, const float *F, const float *V,
                    const float *M, const float *W, uint32_t N, double out[4]);
 
+/* The FIRE relaxation loop of the reference (CalculateGPUStepsContractRelaxFIRE,
+ * md_firerelaxation.cu:201-304) over the three kernels above: per iteration
+ * the caller's force callback fills F, P = F.V is summed (in FP64 here; the
+ * reference sums FP32 block partials on the host in FP32), then
+ *   P > 0:  ++n_positive, n_negative = 0; after n_delay positive iterations
+ *           dt = min(dt dtgrow, dtmax) and alpha *= alphashrink;
+ *   else:   n_positive = 0, ++n_negative (stop once above n_negative_max);
+ *           after the first n_delay iterations dt = max(dt dtshrink, dtmin)
+ *           and alpha = alpha0; R -= V dt/2 with the new dt, V = 0;
+ * then d_CalculateIncrementsFIRE with dt_m (the potential's dt/m, fixed) and the
+ * current dt and alpha, and the caller's restriction callback (the reference
+ * calls d_CylinderRestrictionZ there).  dt and alpha are float, as in
+ * firerelax_data (md_data_types.h:296-301).  The state persists in rfd_fire_ref
+ * across calls; rfd_fire_ref_init sets SetFIREData's values
+ * (md_firerelaxation.cu:72-84: dtmax 100 dt0, dtmin 1e-3 dt0, dtgrow 1.1,
+ * dtshrink 0.5, alphashrink 0.99, n_negative_max 30000, n_delay 20) and the
+ * initial alpha, which the reference leaves unset (0.25 is its alpha0; its
+ * packing uses 0.1, md_createcylindersample.cu:266).  A callback returns 0 on
+ * success; its non-zero return ends the call with that status. */
+typedef struct rfd_fire_ref {
+  float dt0, dtmax, dtmin, dtgrow, dtshrink, alpha0, alphashrink;
+  uint32_t n_negative_max, n_delay;
+  float dt, alpha;            /* current time step and mixing weight        */
+  uint32_t n_positive, n_negative;
+  uint32_t steps;             /* iterations taken (the loop counter)        */
+  int stopped;                /* 1 once n_negative exceeded n_negative_max  */
+  double power;               /* F.V of the last iteration                  */
+} rfd_fire_ref;
+typedef int (*rfd_callback)(void *user);
+void rfd_fire_ref_init(rfd_fire_ref *f, float dt0, float alpha0);
+int rfd_fire_relax_reference(rfd_ctx *ctx, const float *F, float *V, float *R, uint32_t N,
+                             float dt_m, rfd_fire_ref *f, uint32_t iterations,
+                             rfd_callback forces, rfd_callback restriction, void *user);
+
 /* =========================== tier 2: native system ======================
  * Particles: R in FP64 (position accumulation), V, W, F, M in FP32, SoA by
  * component; per-particle radius (polydisperse).  Bonds: one record per

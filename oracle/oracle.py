@@ -295,3 +295,64 @@ def pack_iteration(R, rad, beta, cx, cy, rc, z0, z1, reach_pairs=None):
     out[:, 2] = np.minimum(out[:, 2], z1 - rad)
     out[:, 2] = np.maximum(out[:, 2], z0 + rad)
     return out, mo
+
+
+class FireRefState:
+    """The reference FIRE loop's settings and state (firerelax_data,
+    md_data_types.h:296-301, float fields), as SetFIREData leaves them
+    (md_firerelaxation.cu:72-84) with the given dt0 and initial alpha."""
+
+    def __init__(self, dt0, alpha0):
+        f32 = np.float32
+        self.dt0 = f32(dt0)
+        self.dtmax = f32(100.0 * float(self.dt0))
+        self.dtmin = f32(0.001 * float(self.dt0))
+        self.dtgrow, self.dtshrink = f32(1.1), f32(0.5)
+        self.alpha0, self.alphashrink = f32(alpha0), f32(0.99)
+        self.n_negative_max, self.n_delay = 30000, 20
+        self.dt, self.alpha = self.dt0, self.alpha0
+        self.n_positive = self.n_negative = self.steps = 0
+        self.stopped = False
+        self.power = 0.0
+
+
+def fire_relax_reference(F, V, R, N, dt_m, f, iterations, forces=None, restriction=None):
+    """The reference's FIRE loop (md_firerelaxation.cu:201-304) on host arrays
+    (float32, SoA by component): forces() fills F, then the power F.V (summed
+    in double), the P > 0 / P <= 0 rules on the float dt and alpha, the
+    half-step back-off and velocity reset, d_CalculateIncrementsFIRE and
+    restriction()."""
+    f32 = np.float32
+    out = (C.c_double * 4)()
+    for _ in range(iterations):
+        if f.stopped:
+            break
+        if forces is not None:
+            forces()
+        lib().orc_fire_power(ptr(F), ptr(V), None, None, N, out)
+        f.power = out[0]
+        if out[0] > 0:
+            f.n_positive += 1
+            f.n_negative = 0
+            if f.n_positive > f.n_delay:
+                g = f32(f.dt * f.dtgrow)
+                f.dt = g if g < f.dtmax else f.dtmax
+                f.alpha = f32(f.alpha * f.alphashrink)
+        else:
+            f.n_positive = 0
+            f.n_negative += 1
+            if f.n_negative > f.n_negative_max:
+                f.stopped = True
+                break
+            if f.steps > f.n_delay:
+                g = f32(f.dt * f.dtshrink)
+                f.dt = g if g > f.dtmin else f.dtmin
+                f.alpha = f.alpha0
+            lib().orc_fire_halfstep(ptr(V), ptr(R), N, f32(0.5 * float(f.dt)))
+            V[:] = 0
+        lib().orc_fire_increments(ptr(F), ptr(V), ptr(R), N, f32(dt_m), f.dt, f.alpha)
+        if restriction is not None:
+            restriction()
+        f.steps += 1
+    return f
+

@@ -81,6 +81,13 @@ __global__ void k_zero_fm(float *__restrict__ F, float *__restrict__ M, size_t n
   }
 }
 
+// cudaMemset(P.d_V, 0, ...) of the FIRE back-off (md_firerelaxation.cu:292)
+__global__ void k_zero_f(float *__restrict__ a, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)blockDim.x * gridDim.x)
+    a[i] = 0.0f;
+}
+
 // d_CalculateIncrementsDEM (md_calculateincrementsdem.cu:19-49)
 __global__ void k_increments(const float *__restrict__ F, float *__restrict__ V,
                              float *__restrict__ R, const float *__restrict__ M,
@@ -384,6 +391,79 @@ int rfd_fire_halfstep(rfd_ctx *ctx, const float *V, float *R, uint32_t N, float 
   k_fire_halfstep<<<rfd_blocks(3 * (size_t)N, 256), 256, 0, ctx->stream>>>(V, R, 3 * (size_t)N,
                                                                            dt_d2);
   RFD_LAUNCHED(ctx, "rfd_fire_halfstep");
+  return RFD_OK;
+}
+
+void rfd_fire_ref_init(rfd_fire_ref *f, float dt0, float alpha0) {
+  if (!f) return;
+  *f = rfd_fire_ref{};
+  // SetFIREData (md_firerelaxation.cu:72-84), with the float fields of
+  // firerelax_data (md_data_types.h:296-301)
+  f->dt0 = dt0;
+  f->dtmax = (float)(100.0 * (double)dt0);
+  f->dtmin = (float)(0.001 * (double)dt0);
+  f->dtgrow = 1.1f;
+  f->dtshrink = 0.5f;
+  f->alpha0 = alpha0;
+  f->alphashrink = 0.99f;
+  f->n_negative_max = 30000;
+  f->n_delay = 20;
+  f->dt = dt0;
+  f->alpha = alpha0;
+}
+
+int rfd_fire_relax_reference(rfd_ctx *ctx, const float *F, float *V, float *R, uint32_t N,
+                             float dt_m, rfd_fire_ref *f, uint32_t iterations,
+                             rfd_callback forces, rfd_callback restriction, void *user) {
+  if (!ctx) return RFD_E_ARG;
+  RFD_CHECK_ARG(ctx, F && V && R && N > 0 && f, "rfd_fire_relax_reference: bad arguments");
+  RFD_CHECK_ARG(ctx, f->dt > 0.f && f->dtgrow > 0.f && f->dtshrink > 0.f,
+                "rfd_fire_relax_reference: state not initialised (rfd_fire_ref_init)");
+  for (uint32_t it = 0; it < iterations && !f->stopped; ++it) {
+    if (forces) {
+      const int st = forces(user);
+      if (st) return rfd_set_error(ctx, st, "rfd_fire_relax_reference: force callback returned %d",
+                                   st);
+    }
+    // d_FdotVEntire + the host sum (md_firerelaxation.cu:265-272), in FP64
+    double pw[4];
+    int st = rfd_fire_power(ctx, F, V, nullptr, nullptr, N, pw);
+    if (st) return st;
+    f->power = pw[0];
+    // md_firerelaxation.cu:274-299
+    if (pw[0] > 0.0) {
+      ++f->n_positive;
+      f->n_negative = 0;
+      if (f->n_positive > f->n_delay) {
+        f->dt = (f->dt * f->dtgrow < f->dtmax) ? f->dt * f->dtgrow : f->dtmax;
+        f->alpha *= f->alphashrink;
+      }
+    } else {
+      f->n_positive = 0;
+      ++f->n_negative;
+      if (f->n_negative > f->n_negative_max) {
+        f->stopped = 1;
+        break;
+      }
+      if (f->steps > f->n_delay) {
+        f->dt = (f->dt * f->dtshrink > f->dtmin) ? f->dt * f->dtshrink : f->dtmin;
+        f->alpha = f->alpha0;
+      }
+      st = rfd_fire_halfstep(ctx, V, R, N, (float)(0.5 * (double)f->dt));
+      if (st) return st;
+      k_zero_f<<<rfd_blocks(3 * (size_t)N, 256), 256, 0, ctx->stream>>>(V, 3 * (size_t)N);
+      RFD_LAUNCHED(ctx, "rfd_fire_relax_reference");
+    }
+    st = rfd_fire_increments(ctx, F, V, R, N, dt_m, f->dt, f->alpha);
+    if (st) return st;
+    if (restriction) {
+      st = restriction(user);
+      if (st)
+        return rfd_set_error(ctx, st, "rfd_fire_relax_reference: restriction callback returned %d",
+                             st);
+    }
+    ++f->steps;
+  }
   return RFD_OK;
 }
 

@@ -52,8 +52,23 @@ SIGNATURES: dict[str, list] = {
     "rfd_fire_increments": [_vp, _vp, _vp, _vp, _u32, _f32, _f32, _f32],
     "rfd_fire_halfstep": [_vp, _vp, _vp, _u32, _f32],
     "rfd_fire_power": [_vp, _vp, _vp, _vp, _vp, _u32, _d2],
+    "rfd_fire_ref_init": [_vp, _f32, _f32],
+    "rfd_fire_relax_reference": [_vp, _vp, _vp, _vp, _u32, _f32, _vp, _u32, _vp, _vp, _vp],
 }
-_RESTYPE = {"rfd_last_error": C.c_char_p, "rfd_version": C.c_char_p}
+_RESTYPE = {"rfd_last_error": C.c_char_p, "rfd_version": C.c_char_p, "rfd_fire_ref_init": None}
+
+# int (*)(void *user): the force and restriction callbacks of the FIRE loop
+CALLBACK = C.CFUNCTYPE(C.c_int, C.c_void_p)
+
+
+class FireRef(C.Structure):
+    """rfd_fire_ref: SetFIREData's settings and the loop state."""
+
+    _fields_ = [("dt0", _f32), ("dtmax", _f32), ("dtmin", _f32), ("dtgrow", _f32),
+                ("dtshrink", _f32), ("alpha0", _f32), ("alphashrink", _f32),
+                ("n_negative_max", _u32), ("n_delay", _u32), ("dt", _f32), ("alpha", _f32),
+                ("n_positive", _u32), ("n_negative", _u32), ("steps", _u32), ("stopped", _int),
+                ("power", _f64)]
 
 _lib: C.CDLL | None = None
 

@@ -208,6 +208,50 @@ class RefLayout:
         return int((self.ILtype == 1).sum().item())
 
 
+class ReferenceFire:
+    """The reference's FIRE relaxation loop (rfd_fire_relax_reference,
+    md_firerelaxation.cu:201-304) on a RefLayout's F, V and R: per iteration
+    forces() fills lay.F, then the power, the dt / alpha rules, the back-off
+    and d_CalculateIncrementsFIRE run in the library, then restriction().
+    The settings and the loop state (float dt and alpha, positive and
+    negative counters, iteration count) persist between run() calls, as the
+    reference keeps them in firerelax_data."""
+
+    def __init__(self, dt0: float, alpha0: float = 0.25):
+        from ._lib import FireRef
+
+        self.state = FireRef()
+        lib().rfd_fire_ref_init(C.byref(self.state), dt0, alpha0)
+
+    def run(self, lay: "RefLayout", dt_m: float, iterations: int, forces=None,
+            restriction=None) -> None:
+        from ._lib import CALLBACK
+
+        errors = []
+
+        def wrap(fn):
+            if fn is None:
+                return None
+
+            def cb(_user):
+                try:
+                    fn()
+                    return 0
+                except Exception as e:  # reported after the call returns
+                    errors.append(e)
+                    return 1
+            return CALLBACK(cb)
+
+        cf, cr = wrap(forces), wrap(restriction)
+        try:
+            lay.ctx.call("rfd_fire_relax_reference", _p(lay.F), _p(lay.V), _p(lay.R), lay.N,
+                         dt_m, C.byref(self.state), iterations, cf, cr, None)
+        except RfdError:
+            if errors:
+                raise errors[0]
+            raise
+
+
 class UniaxialDriver:
     """h_CalculationUniaxialCompression (md_calculationuniaxialcompression.cu:113-197)
     on the reference layout.  Setup (:131-143): CellDistribution,

@@ -466,30 +466,28 @@ def test_fracture_detector_on_synthetic_trace():
 
 
 def test_reference_fire_relaxes_quadratic_bowl():
-    """Reference FIRE kernels + control (md_firerelaxation.cu:263-304) on
-    F = -k (R - R0): converges to the analytic minimum R0."""
+    """Reference FIRE kernels + control (md_firerelaxation.cu:201-304) on
+    F = -k (R - R0): converges to the analytic minimum R0 (its first
+    iteration, at rest, takes the back-off branch: P = 0)."""
+    from oracle.oracle import FireRefState, fire_relax_reference
+
     g = np.random.default_rng(7)
     N = 64
     R0 = g.uniform(0, 5, 3 * N).astype(np.float32)
     R = (R0 + g.normal(0, 0.1, 3 * N)).astype(np.float32)
     V = np.zeros_like(R)
-    k, mass, dt0 = 2.0, 1.0, 0.05
-    dt, alpha, npos = dt0, 0.25, 0
-    for step in range(3000):
-        F = (-k * (R - R0)).astype(np.float32)
-        out = (C.c_double * 4)()
-        lib().orc_fire_power(ptr(F), ptr(V), None, None, N, out)
-        if out[0] > 0:
-            npos += 1
-            if npos > 20:
-                dt, alpha = min(dt * 1.1, 100 * dt0), alpha * 0.99
-        else:
-            npos = 0
-            if step > 20:
-                dt, alpha = max(dt * 0.5, 1e-3 * dt0), 0.25
-            lib().orc_fire_halfstep(ptr(V), ptr(R), N, 0.5 * dt)
-            V[:] = 0
-        lib().orc_fire_increments(ptr(F), ptr(V), ptr(R), N, dt / mass, dt, alpha)
+    F = np.zeros_like(R)
+    k = np.float32(2.0)
+
+    def forces():
+        F[:] = -k * (R - R0)
+
+    f = FireRefState(0.05, 0.25)
+    fire_relax_reference(F, V, R, N, 0.05, f, 1, forces)
+    assert f.power == 0 and f.n_negative == 1 and f.dt == np.float32(0.05)
+    fire_relax_reference(F, V, R, N, 0.05, f, 2999, forces)
+    assert not f.stopped and f.steps == 3000
+    assert f.dt == f.dtmax  # grown by 1.1 per positive step after the first 20
     assert np.abs(R - R0).max() < 1e-4
 
 
