@@ -109,14 +109,22 @@ class OracleNativeStep:
     def _soa(self, a):
         return np.ascontiguousarray(a.T).reshape(-1)
 
-    def forces(self):
+    def forces(self, dt=None, quasi=False, nobreak=False):
+        """One force evaluation.  dt: the time step the increments take
+        (default the system's); quasi: a quasi-static pass of a FIRE load step
+        (no viscous damping, no rolling resistance); nobreak: bonds are not
+        judged."""
         N = self.N
+        dt = self.dt if dt is None else dt
         R, V, W = self._soa(self.R), self._soa(self.V), self._soa(self.W)
         Fb, Mb = np.zeros(3 * N), np.zeros(3 * N)
         before = self.act.copy()
+        sc, tc = self.sc, self.tc
+        if nobreak:
+            sc = tc = np.full(self.Nb, np.inf)
         lib().orc_native_bonds_f64(ptr(R), ptr(V), ptr(W), N, ptr(self.bi), ptr(self.bj),
-                                   ptr(self.iL), ptr(self.br), ptr(self.sc), ptr(self.tc),
-                                   ptr(self.state), ptr(self.act), self.Nb, self.dt, self.E,
+                                   ptr(self.iL), ptr(self.br), ptr(sc), ptr(tc),
+                                   ptr(self.state), ptr(self.act), self.Nb, dt, self.E,
                                    self.G, self.refmode, ptr(Fb), ptr(Mb))
         newly = np.nonzero((before == 1) & (self.act == 0))[0]
         self.broke_last = newly
@@ -160,9 +168,9 @@ class OracleNativeStep:
             O2 = np.ascontiguousarray(np.concatenate([O, -O]))
             cap = np.zeros(2 * P)
             Fc, Mc = native_contacts(self.R, self.V, self.W, self.rad, pairs.astype(np.uint32),
-                                     np.concatenate([reach, reach]), O2, self.dt, self.Estar,
-                                     self.Gstar, self.pA, self.mu, self.mur, self.rho43pi,
-                                     cap=cap)
+                                     np.concatenate([reach, reach]), O2, dt, self.Estar,
+                                     self.Gstar, 0.0 if quasi else self.pA, self.mu,
+                                     0.0 if quasi else self.mur, self.rho43pi, cap=cap)
             self.cap_tie = float(np.abs(cap).min())
             nb = int(touch_b.sum())
             self.Ob[brk[touch_b]] = O2[:nb]
@@ -179,8 +187,8 @@ class OracleNativeStep:
         return F, M
 
     # ------------------------------------------------------- particle pass
-    def _platen(self, R, V, F):
-        L = self.L
+    def _platen(self, R, V, F, friction=True, L=None):
+        L = self.L if L is None else L
         mode = L["mode"]
         sb = st = 0.0
         a = 2 if mode == 1 else 1
@@ -195,7 +203,7 @@ class OracleNativeStep:
                 f = F[i, a]
                 if f > 0:
                     st += f
-                    if mode == 2:
+                    if mode == 2 and friction:
                         vv = V[i, 0] * V[i, 0] + V[i, 2] * V[i, 2]
                         if vv > 1e-9:
                             vv = 1.0 / math.sqrt(vv)
@@ -207,7 +215,7 @@ class OracleNativeStep:
                 f = F[i, a]
                 if f < 0:
                     sb += f
-                    if mode == 2:
+                    if mode == 2 and friction:
                         vv = V[i, 0] * V[i, 0] + V[i, 2] * V[i, 2]
                         if vv > 1e-9:
                             vv = 1.0 / math.sqrt(vv)
