@@ -54,6 +54,9 @@ def parse():
     p.add_argument("--fire-only", action="store_true",
                    help="only the plain-FIRE load steps on the large cylinder and the "
                         "FP32 / FP64 FIRE comparison")
+    p.add_argument("--c2-peak-only", action="store_true",
+                   help="only the FP32 / FP64 quasi-static run of the D25xH50 cylinder to past "
+                        "its peak")
     p.add_argument("--packed-only", action="store_true",
                    help="only the random-packing cylinder measurement")
     p.add_argument("--copy-calib", action="store_true",
@@ -277,12 +280,16 @@ def bench_c2(steps: int = 500) -> dict:
 
 def bench_warmstart(D: float = 61.0, H: float = 122.0, steps: int = 6) -> dict:
     """Quasi-static load steps (strain increment -1e-5, FIRE to |F|max <
-    1e-6 F_ref) on a ~500k-particle cylinder, plain GPU FIRE against FIRE
-    warm-started by the message-passing network, which is first trained on
-    the device on plain-FIRE relaxations of two smaller cylinders (~20k and
-    ~50k particles) and validated on a third.  Both runs start from the same
-    seeded specimen; the first two steps (seating, first increment) are plain
-    in both and not timed."""
+    1e-6 F_ref) on a ~500k-particle cylinder from three starts: plain GPU
+    FIRE (the platen move alone), the skip path (the last step's increment
+    repeated: the network with a zero correction head) and the trained
+    network (skip path + learned correction).  The network is trained on the
+    device on plain-FIRE relaxations of four cylinders of 19k to 207k
+    particles and validated on a fifth; its correction head is kept only if
+    it beats a zero correction on the validation steps, so with the head off
+    the "learned" start is the skip path, and the line says so.  All runs
+    start from the same seeded specimen; the first two steps (seating, first
+    increment) are plain and not timed."""
     from rfd import native, native_cases, warmstart as ws
     from rfd.reflayout import Context
 
@@ -301,15 +308,20 @@ def bench_warmstart(D: float = 61.0, H: float = 122.0, steps: int = 6) -> dict:
                   f"converged {r.converged}, warm {r.warm_used}", file=sys.stderr, flush=True)
         return f
 
-    tr = [spec(21.0, 42.0, 101), spec(28.0, 56.0, 102)]
-    fp = native.fire_params(tr[0].mat.E, 0.5, tr[0].mat.dt, rel_tol=1e-6, max_iter=100000,
-                            check_every=200)
-    samples = []
-    for i, c in enumerate(tr):
-        s_, _ = ws.collect_samples(c.sys, dstrain, fp, 10, log=log(f"train{i} N={c.sys.N}"))
+    train_specs = ((21.0, 101), (28.0, 102), (36.0, 105), (46.0, 106))
+    samples, sizes = [], []
+    fp = None
+    for d, seed in train_specs:
+        c = spec(d, 2 * d, seed)
+        if fp is None:
+            fp = native.fire_params(c.mat.E, 0.5, c.mat.dt, rel_tol=1e-6, max_iter=100000,
+                                    check_every=200)
+        s_, _ = ws.collect_samples(c.sys, dstrain, fp, 8, log=log(f"train N={c.sys.N}"))
         samples += s_
-    vs = spec(21.0, 42.0, 103)
-    val, _ = ws.collect_samples(vs.sys, dstrain, fp, 6, log=log("val"))
+        sizes.append(c.sys.N)
+        del c
+    vs = spec(28.0, 56.0, 103)
+    val, _ = ws.collect_samples(vs.sys, dstrain, fp, 8, log=log("val"))
     net = ws.WarmStartNet(ctx, seed=7)
     t1 = time.perf_counter()
     hist = ws.train(net, samples, epochs=30, lr=1e-3, val=val)
@@ -317,10 +329,11 @@ def bench_warmstart(D: float = 61.0, H: float = 122.0, steps: int = 6) -> dict:
     t_train = time.perf_counter() - t1
     print(f"[bench] warm start: {len(samples)} samples, trained in {t_train:.1f}s, "
           f"head {'on' if hist['head'] else 'off'}", file=sys.stderr, flush=True)
+    skip = ws.WarmStartNet(ctx, seed=7)  # zero decoder: repeats the last increment
     out = {}
-    for name in ("plain", "learned"):
+    for name, nt in (("plain", None), ("skip", skip), ("learned", net)):
         c = spec(D, H, 104)
-        warm = (lambda s: ws.predict(net, s, dstrain)) if name == "learned" else None
+        warm = (lambda s_, nt=nt: ws.predict(nt, s_, dstrain)) if nt is not None else None
         _, res = ws.collect_samples(c.sys, dstrain, fp, 2 + steps, warm=warm,
                                     log=log(f"{name} N={c.sys.N}"))
         timed = res[2:]
@@ -333,15 +346,21 @@ def bench_warmstart(D: float = 61.0, H: float = 122.0, steps: int = 6) -> dict:
                      "warm_starts_taken": sum(r.warm_used for r in timed)}
         print(f"[bench] warm start {name}: {out[name]}", file=sys.stderr, flush=True)
         del c
+    head = bool(hist["head"])
+    if not head:
+        out["learned"]["note"] = ("correction head off (it did not beat a zero correction on "
+                                  "the validation steps): this start is the skip path")
     return {"metric": "converged load-steps/s", "unit": "load-steps/s",
             "workload": f"uniaxial cylinder D{D:g}xH{H:g} mm, strain step {dstrain:g}, "
                         "FIRE to |F|max < 1e-6 E pi r_mean^2",
-            "ftol": fp.ftol, "plain": out["plain"], "learned": out["learned"],
+            "ftol": fp.ftol, **out,
             "speedup_wall": out["plain"]["s_per_step"] / out["learned"]["s_per_step"],
-            "training": {"specimens": "D21xH42 (seed 101), D28xH56 (seed 102); "
-                                      "validation D21xH42 (seed 103)",
-                         "samples": len(samples), "epochs": 30, "seconds": t_train,
-                         "correction_head": hist["head"],
+            "speedup_skip_vs_plain": out["plain"]["s_per_step"] / out["skip"]["s_per_step"],
+            "speedup_learned_vs_skip": out["skip"]["s_per_step"] / out["learned"]["s_per_step"],
+            "training": {"specimens": "D21xH42 (101), D28xH56 (102), D36xH72 (105), "
+                                      "D46xH92 (106); validation D28xH56 (103)",
+                         "particles": sizes, "samples": len(samples), "epochs": 30,
+                         "seconds": t_train, "correction_head": head,
                          "val_loss": min(hist["val"]) if hist["val"] else None,
                          "val_loss_zero_correction": hist["val_zero"]},
             "total_seconds": time.perf_counter() - t0}
@@ -487,6 +506,62 @@ def bench_fire_fp64(D: float = 25.0, H: float = 50.0, steps: int = 4) -> dict:
     return {"workload": f"uniaxial cylinder D{D:g}xH{H:g} mm, load steps 2-{steps + 1} of "
                         "strain -1e-5, FIRE to |F|max < 1e-6 E pi r_mean^2",
             **out, "max_relative_reaction_difference": diff}
+
+
+def bench_c2_peak(D: float = 25.0, H: float = 50.0, dstrain: float = -1e-5,
+                  max_steps: int = 600, budget_s: float = 150.0) -> dict:
+    """Quasi-static uniaxial compression of the D25 x H50 mm cylinder (lattice
+    of spacing 1 mm, r ~ U[0.4, 0.6] mm, strengths N(20,4)/N(40,8) MPa) to past
+    its peak: load steps of `dstrain`, each FIRE-relaxed to |F|max < 1e-6 F_ref
+    with bonds judged at equilibrium, on the FP32 path and on the FP64 GPU path
+    from the same seeded specimen.  A run stops once the stress has fallen
+    below 90 % of its peak, after max_steps steps or after budget_s seconds.
+    Reports both stress-strain curves, peak stress and strain, and the
+    relative difference of the peaks."""
+    from rfd import native, native_cases
+    from rfd.reflayout import Context
+
+    ctx = Context(0)
+    out = {}
+    for name, fp64 in (("fp32", False), ("fp64", True)):
+        nc = native_cases.uniaxial_cylinder(ctx, D, H, flat_ends=True, fp64=fp64)
+        s = nc.sys
+        s.seat_platens(preload=0.0)
+        L0 = s.loading()
+        area_inv = L0.inv_area
+        fp = native.fire_params(nc.mat.E, 0.5, nc.mat.dt, rel_tol=1e-6, max_iter=100000,
+                                check_every=200)
+        curve, iters, breaks, conv = [], 0, 0, True
+        peak, peak_strain, t0 = 0.0, 0.0, time.perf_counter()
+        for k in range(max_steps):
+            r = s.fire_load_step(dstrain, fp)
+            stress = 0.5 * (abs(r.sum_bottom) + abs(r.sum_top)) * area_inv
+            strain = -r.strain
+            curve.append((strain, stress * 1e3))
+            iters += r.iterations
+            breaks += r.breaks
+            conv = conv and bool(r.converged)
+            if stress > peak:
+                peak, peak_strain = stress, strain
+            if k % 10 == 0:
+                print(f"[bench] C2 {name} step {k}: strain {strain:.2e} stress "
+                      f"{stress * 1e3:.2f} MPa, {r.iterations} iterations, {r.breaks} breaks",
+                      file=sys.stderr, flush=True)
+            if stress < 0.9 * peak or time.perf_counter() - t0 > budget_s:
+                break
+        sec = time.perf_counter() - t0
+        out[name] = {"peak_stress_MPa": peak * 1e3, "peak_strain": peak_strain,
+                     "steps": len(curve), "past_peak": curve[-1][1] < 0.9 * peak * 1e3,
+                     "fire_iterations": iters, "bonds_broken": breaks,
+                     "all_converged": conv, "seconds": sec,
+                     "load_steps_per_s": len(curve) / sec, "particles": s.N,
+                     "bonds": s.bonds.n, "curve_strain_stressMPa": curve}
+        s.close()
+    out["peak_relative_difference"] = abs(out["fp32"]["peak_stress_MPa"] /
+                                          out["fp64"]["peak_stress_MPa"] - 1.0)
+    out["workload"] = (f"uniaxial cylinder D{D:g}xH{H:g} mm, quasi-static load steps of strain "
+                       f"{dstrain:g} to past the peak, FIRE to |F|max < 1e-6 E pi r_mean^2")
+    return out
 
 
 def bench_c1(steps: int = 5000, cpu_seconds: float = 5.0, cpu: bool = True) -> dict:
@@ -800,6 +875,9 @@ def main():
     if args.batch_only:
         print(json.dumps({"batch": bench_batch()}))
         return
+    if args.c2_peak_only:
+        print(json.dumps({"c2_peak": bench_c2_peak()}))
+        return
     if args.packed_only:
         print(json.dumps({"packed": bench_packed()}))
         return
@@ -859,6 +937,13 @@ def main():
             out["fire_fp64"] = bench_fire_fp64()
         except Exception as e:  # reported, never hides the main line
             out["fire_fp64"] = {"error": f"{type(e).__name__}: {e}"}
+        try:
+            c2 = bench_c2_peak()
+            for k in ("fp32", "fp64"):  # every 4th point of the curves in the line
+                c2[k]["curve_strain_stressMPa"] = c2[k]["curve_strain_stressMPa"][::4]
+            out["c2_peak"] = c2
+        except Exception as e:  # reported, never hides the main line
+            out["c2_peak"] = {"error": f"{type(e).__name__}: {e}"}
         try:
             out["warm_start"] = bench_warmstart()
         except Exception as e:  # reported, never hides the main line

@@ -518,7 +518,7 @@ int rfd_system_fire_buffers(rfd_system *sys, float **du_last, float **du_prev,
                             float **dth_last, float **dth_prev);
 /* bond graph on the device, for callers that build per-bond features in HBM:
  * hoff[N+1] row offsets; half[h] = {j, b} (uint32 pairs: other end, bond);
- * bond[b] = {L0, br, sc, tc} (floats); state[b] = {O.xyz, mn, Mt.xyz, flag}
+ * bond[b] = {1/L0, br, sc, tc} (floats); state[b] = {O.xyz, mn, Mt.xyz, flag}
  * (7 floats + uint32 flag: 1 bond, 0 broken, 2 removed), the latest state;
  * *H = number of halves (2 Nb).  Rows list lower partners first, each row by
  * partner id. */
@@ -543,8 +543,11 @@ int rfd_system_device_arrays(rfd_system *sys, double **P, float **V, float **W,
  * other end's row), H entries */
 int rfd_system_half_mirror(rfd_system *sys, const uint32_t **hm);
 /* network inputs at the start of a load step: X[N x 32] = last two step
- * displacements / su and rotations / sth (12 used, rest 0); E[H x 4] per half
- * = (1e3 * bond strain if intact else 0, unit direction to the other end) */
+ * displacements / su and rotations / sth (12 columns), then the utilisation of
+ * the particle's intact bonds, E strain / tensile strength (largest, mean)
+ * and the share of its bond records broken (3 columns; the rest 0);
+ * E[H x 4] per half = (1e3 * bond strain if intact else 0, unit direction to
+ * the other end) */
 int rfd_system_mpnn_inputs(rfd_system *sys, double su, double sth, float *X, float *E);
 
 /* Y[M x N] = X[M x K] B[K x N] + bias (+ Y if accumulate), B(k, c) = W[k*wsk + c*wsc]

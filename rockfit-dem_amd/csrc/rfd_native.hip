@@ -2006,7 +2006,7 @@ __global__ void k_mpnn_inputs(const P4 *__restrict__ P, uint32_t N,
                               const BR *__restrict__ brc, const uint8_t *__restrict__ fl8,
                               const float *__restrict__ du0, const float *__restrict__ du1,
                               const float *__restrict__ dth0, const float *__restrict__ dth1,
-                              float isu, float isth, float *__restrict__ X,
+                              float isu, float isth, float E_mod, float *__restrict__ X,
                               float *__restrict__ E) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
     float *x = X + 32 * (size_t)i;
@@ -2018,17 +2018,34 @@ __global__ void k_mpnn_inputs(const P4 *__restrict__ P, uint32_t N,
     }
     for (int d = 12; d < 32; ++d) x[d] = 0.f;
     const P4 A = P[i];
+    // utilisation of the particle's bonds: the tensile stress of the stretch
+    // over the bond's tensile strength (the first term of the break
+    // criterion, md_calculateforcesdem3.cu:23), its largest and mean value,
+    // and the share of its bond records already broken
+    float umax = 0.f, usum = 0.f, nint = 0.f, nbrk = 0.f;
     for (uint32_t h = hoff[i]; h < hoff[i + 1]; ++h) {
       const HR q = hr[h];
       const P4 B = P[q.j];
       const double rx = B.x - A.x, ry = B.y - A.y, rz = B.z - A.z;
       const double L = __builtin_sqrt(rx * rx + ry * ry + rz * rz);
       const double il = L > 0 ? 1.0 / L : 0.0;
-      const float eps =
-          fl8[q.b] == kBond ? (float)((L * (double)brc[q.b].il - 1.0) * 1e3) : 0.f;
+      const bool intact = fl8[q.b] == kBond;
+      const double strain = L * (double)brc[q.b].il - 1.0;
+      const float eps = intact ? (float)(strain * 1e3) : 0.f;
+      if (intact) {
+        const float u = (float)((double)E_mod * strain / (double)brc[q.b].sc);
+        umax = fmaxf(umax, u);
+        usum += u;
+        nint += 1.f;
+      } else {
+        nbrk += 1.f;
+      }
       *reinterpret_cast<float4 *>(E + 4 * (size_t)h) =
           make_float4(eps, (float)(rx * il), (float)(ry * il), (float)(rz * il));
     }
+    x[12] = umax;
+    x[13] = nint > 0.f ? usum / nint : 0.f;
+    x[14] = nint + nbrk > 0.f ? nbrk / (nint + nbrk) : 0.f;
   }
 }
 
@@ -4851,7 +4868,7 @@ int rfd_system_mpnn_inputs(rfd_system *s, double su, double sth, float *X, float
   const int a = s->dcur, b = 1 - s->dcur;
   k_mpnn_inputs<<<rfd_blocks(s->N, 256), 256, 0, ctx->stream>>>(
       s->P, s->N, s->hoff, s->hr, s->brc, s->fl8, s->du[a], s->du[b], s->dth[a], s->dth[b],
-      (float)(1.0 / su), (float)(1.0 / sth), X, E);
+      (float)(1.0 / su), (float)(1.0 / sth), (float)s->p.E, X, E);
   RFD_HIP(ctx, hipGetLastError());
   return RFD_OK;
 }

@@ -387,7 +387,7 @@ class NativeSystem:
 
     def bond_graph(self):
         """Device views: hoff[N+1] int32, half[H, 2] int32 (other end, bond),
-        bond[Nb, 4] float32 (L0, br, sc, tc), state[Nb, 8] float32 (O, mn, Mt,
+        bond[Nb, 4] float32 (1/L0, br, sc, tc), state[Nb, 8] float32 (O, mn, Mt,
         flag as uint32 bits: read column 7 through .view(torch.int32)).  The
         state view is the buffer current at the time of the call."""
         a, b, c, d, H = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_uint32()

@@ -8,7 +8,10 @@ Every arithmetic operation runs in the HIP kernels of csrc/rfd_mpnn.hip
 csrc/rfd_native.hip (inputs); torch only allocates device memory.
 
 Inputs per particle: the last two step displacements and rotations, scaled by
-the platen displacement of one step (su) and su / r_mean (sth).  Inputs per
+the platen displacement of one step (su) and su / r_mean (sth); the largest
+and the mean utilisation of its intact bonds (E strain / tensile strength,
+how close the stretch alone brings each to breaking) and the share of its
+bonds already broken.  Inputs per
 half-bond: bond strain x 1e3 (0 once broken) and the unit direction.  The
 network predicts corrections: du = du_last + su * y[0:3], dth = dth_last +
 sth * y[3:6], so an untrained (near zero) output repeats the last increment.
