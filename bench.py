@@ -531,7 +531,7 @@ def bench_c2_peak(D: float = 25.0, H: float = 50.0, dstrain: float = -1e-5,
         area_inv = L0.inv_area
         fp = native.fire_params(nc.mat.E, 0.5, nc.mat.dt, rel_tol=1e-6, max_iter=100000,
                                 check_every=200)
-        curve, iters, breaks, conv = [], 0, 0, True
+        curve, iters, breaks, conv, unconv = [], 0, 0, True, []
         peak, peak_strain, t0 = 0.0, 0.0, time.perf_counter()
         for k in range(max_steps):
             r = s.fire_load_step(dstrain, fp)
@@ -541,6 +541,8 @@ def bench_c2_peak(D: float = 25.0, H: float = 50.0, dstrain: float = -1e-5,
             iters += r.iterations
             breaks += r.breaks
             conv = conv and bool(r.converged)
+            if not r.converged:
+                unconv.append(strain)
             if stress > peak:
                 peak, peak_strain = stress, strain
             if k % 10 == 0:
@@ -553,7 +555,10 @@ def bench_c2_peak(D: float = 25.0, H: float = 50.0, dstrain: float = -1e-5,
         out[name] = {"peak_stress_MPa": peak * 1e3, "peak_strain": peak_strain,
                      "steps": len(curve), "past_peak": curve[-1][1] < 0.9 * peak * 1e3,
                      "fire_iterations": iters, "bonds_broken": breaks,
-                     "all_converged": conv, "seconds": sec,
+                     "all_converged": conv, "unconverged_steps": len(unconv),
+                     "unconverged_before_peak": sum(1 for e in unconv if e <= peak_strain),
+                     "first_unconverged_strain": unconv[0] if unconv else None,
+                     "seconds": sec,
                      "load_steps_per_s": len(curve) / sec, "particles": s.N,
                      "bonds": s.bonds.n, "curve_strain_stressMPa": curve}
         s.close()
