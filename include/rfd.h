@@ -463,6 +463,19 @@ int rfd_system_get_tiles(rfd_system *sys, int *force_tile, int *nbr_block, int *
  * rfd_system_get_fusion reports whether steps currently run fused. */
 int rfd_system_set_fusion(rfd_system *sys, int on);
 int rfd_system_get_fusion(rfd_system *sys, int *fused);
+/* Multi-step launches.  Replaces the per-step launch sequence of the host
+ * loops (md_calculationuniaxialcompression.cu:169-189,
+ * md_calculationbraziltest.cu:169-190) for small specimens and batches, where
+ * launch latency bounds a step: with it on, rfd_system_step runs the fused
+ * steps between two contact-row rebuilds as one cooperative launch with a
+ * grid barrier per step and the loading control inside.  It applies when
+ * steps run fused (rfd_system_get_fusion) and every force tile gets a
+ * resident workgroup (a few tens of thousands of particles), and not under
+ * graph capture; otherwise steps run as before.  Off by default; the results
+ * are the same bits either way.  rfd_system_get_persistent reports the
+ * setting and whether the current system qualifies. */
+int rfd_system_set_persistent(rfd_system *sys, int on);
+int rfd_system_get_persistent(rfd_system *sys, int *on, int *applies);
 int rfd_tune_cache_save(const char *path);
 int rfd_tune_cache_load(const char *path);
 /* ---- quasi-static load steps: axial strain increment + FIRE relaxation ----

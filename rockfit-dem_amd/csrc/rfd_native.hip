@@ -43,6 +43,8 @@
 #include <string>
 #include <vector>
 
+#include <hip/hip_cooperative_groups.h>
+
 #include "rfd_internal.h"
 
 // The native tier is compared with its oracles at a stated tolerance, not bit
@@ -657,6 +659,8 @@ struct AdvDev {
   // specimens' first particles (S + 1); every specimen holds at least 128
   // particles, so a tile meets at most two
   const uint32_t *spec32, *spec_first;
+  // ctl holds the loadings of specimens ctl_base, ctl_base + 1, ... (0: all)
+  uint32_t ctl_base;
 };
 
 // Force pass of the physical mode with every bond whose ends share a tile
@@ -666,10 +670,13 @@ struct AdvDev {
 // precomputed records, so no lane waits on a load another lane needs.
 // Per-particle sums in row order, then the contact rows: the same bits as
 // k_nat_forces.
+// One tile of the pass below: `cur` is the index of the bond-state buffer
+// holding the latest state.  Every thread of the workgroup calls it; threads
+// return from it only after its last workgroup barrier.
 template <int FP, int FT, bool ADV>
-__global__ void __launch_bounds__(FT) __attribute__((amdgpu_waves_per_eu(5, 8)))
-k_nat_forces1(SysDev s, Mat mt, WorkDev w, uint32_t ntiles, uint32_t per, uint32_t cap,
-              AdvDev a) {
+__device__ __forceinline__ void forces1_tile(const SysDev &s, const Mat &mt, const WorkDev &w,
+                                             uint32_t tile, uint32_t cap, const AdvDev &a,
+                                             int cur) {
   static_assert(FP < FT && FP <= 128, "tile shape");
   static_assert(2 * FP <= FT, "one sum task per thread");
   // per-half loads: F (3 x cap floats) then M (3 x cap), cap >= 2 FP
@@ -681,12 +688,8 @@ k_nat_forces1(SysDev s, Mat mt, WorkDev w, uint32_t ntiles, uint32_t per, uint32
   __shared__ float4 sV[FP], sW[FP];
   // per-particle sums, kept in the slot memory once the slots are read
   float (*const sAcc)[3][FP] = reinterpret_cast<float (*)[3][FP]>(sDyn);
-  const uint32_t tile = xcd_tile(per);
-  if (tile >= ntiles) return;
-  if (s.done && *s.done) return;
-  if (s.dtp) mt.dt = *s.dtp;
-  BSt *const st_in = *s.curp ? s.bst1 : s.bst0;
-  BSt *const st_out = *s.curp ? s.bst0 : s.bst1;
+  BSt *const st_in = cur ? s.bst1 : s.bst0;
+  BSt *const st_out = cur ? s.bst0 : s.bst1;
   const uint32_t tid = threadIdx.x;
   const uint32_t p0 = tile * FP;
   const uint32_t np = min((uint32_t)FP, s.N - p0);
@@ -839,7 +842,7 @@ k_nat_forces1(SysDev s, Mat mt, WorkDev w, uint32_t ntiles, uint32_t per, uint32
     sp = sp0 + (i >= a.spec_first[sp0 + 1] ? 1u : 0u);
   }
   if (act) {
-    const rfd_loading L = a.ctl[sp];
+    const rfd_loading L = a.ctl[sp - a.ctl_base];
     const float2 iv = a.inv[i];
     if (a.mode == 1)
       advance_regs<1>(L, me.P, me.V, me.W, F, M, iv, a.dt, qb, qt);
@@ -864,6 +867,17 @@ k_nat_forces1(SysDev s, Mat mt, WorkDev w, uint32_t ntiles, uint32_t per, uint32
       fx_wave_add(a.acc + 2 * (sp0 + 1), hi ? qb : 0, hi ? qt : 0);
     }
   }
+}
+
+template <int FP, int FT, bool ADV>
+__global__ void __launch_bounds__(FT) __attribute__((amdgpu_waves_per_eu(5, 8)))
+k_nat_forces1(SysDev s, Mat mt, WorkDev w, uint32_t ntiles, uint32_t per, uint32_t cap,
+              AdvDev a) {
+  const uint32_t tile = xcd_tile(per);
+  if (tile >= ntiles) return;
+  if (s.done && *s.done) return;
+  if (s.dtp) mt.dt = *s.dtp;
+  forces1_tile<FP, FT, ADV>(s, mt, w, tile, cap, a, *s.curp);
 }
 
 // work records of the single-evaluation pass: per tile, count the in-tile
@@ -1037,9 +1051,11 @@ __device__ __forceinline__ void control_update(rfd_loading &L, double s0, double
   L.sum_bottom = s0;
   L.sum_top = s1;
   if (L.hist_len < hist_cap) {
-    hist[3 * (size_t)L.hist_len] = (L.size - L.size0) / L.size0;
-    hist[3 * (size_t)L.hist_len + 1] = s0 * L.inv_area;
-    hist[3 * (size_t)L.hist_len + 2] = s1 * L.inv_area;
+    if (hist) {  // null: the record is counted but written by another workgroup
+      hist[3 * (size_t)L.hist_len] = (L.size - L.size0) / L.size0;
+      hist[3 * (size_t)L.hist_len + 1] = s0 * L.inv_area;
+      hist[3 * (size_t)L.hist_len + 2] = s1 * L.inv_area;
+    }
     L.hist_len += 1;
   }
   L.stress = 0.5 * (fabs(s0) + fabs(s1)) * L.inv_area;
@@ -1109,6 +1125,108 @@ __global__ void __launch_bounds__(64) k_nat_control_fx(rfd_loading *ctls,
   rfd_loading L = ctls[sp];
   control_update(L, s0, s1, hist + 3 * (size_t)hist_cap * sp, hist_cap, dt);
   ctls[sp] = L;
+}
+
+// ------------------------------------------------- multi-step launch
+// Explicit FP32 steps of the fused pass in one cooperative launch, for
+// specimens and batches small enough that every force tile has a resident
+// workgroup: step j reads particle set j & 1 and writes the other, a grid
+// barrier ends each step, and each workgroup then runs the loading control of
+// the step for the (at most two) specimens its tile meets.  Those inputs are
+// the same for every workgroup, so every copy of a specimen's loading holds
+// the same bits; the workgroup whose tile holds the specimen's first particle
+// writes its history record and, at the end, its loading.  The fixed-point
+// platen sums rotate through three buffers: step j adds into buffer j % 3,
+// which every workgroup reads after the barrier; the owner clears buffer
+// (j + 2) % 3 then (read after the previous barrier, added into two steps
+// on).  The buffers are zero on entry and on exit.  The arithmetic of a step
+// is that of k_nat_forces1<.., true> followed by k_nat_control_fx.
+struct StepsDev {
+  P4 *P[2];
+  float4 *V[2], *W[2];
+  rfd_loading *ctl;             // per specimen
+  unsigned long long *acc3;     // 3 x 2 x S
+  double *hist;                 // per specimen, hist_cap records of 3
+  uint32_t hist_cap, S, nsteps;
+  int *curp;
+  double dt;
+};
+
+template <int FP, int FT>
+__global__ void __launch_bounds__(FT) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_nat_steps(SysDev s, Mat mt, WorkDev w, uint32_t ntiles, uint32_t per, uint32_t cap,
+            AdvDev a, StepsDev q) {
+  __shared__ rfd_loading sL[2];
+  __shared__ uint32_t sSp[2];
+  __shared__ int sOwn[2];
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  const uint32_t tile = xcd_tile(per);
+  const bool live = tile < ntiles;
+  const int cur0 = *q.curp;
+  const uint32_t S = q.S;
+  if (threadIdx.x == 0) {
+    uint32_t sp0 = 0, sp1 = 0;
+    int own0 = 0, own1 = 0;
+    if (live) {
+      const uint32_t p0 = tile * FP, pl = min(p0 + FP, s.N) - 1;
+      if (a.spec32) {
+        sp0 = a.spec32[p0 >> 5];
+        sp1 = sp0 + 1 < S && pl >= a.spec_first[sp0 + 1] ? sp0 + 1 : sp0;
+        own0 = a.spec_first[sp0] >= p0;
+        own1 = sp1 != sp0;  // the second specimen starts inside this tile
+      } else {
+        own0 = tile == 0;
+      }
+    }
+    sSp[0] = sp0;
+    sSp[1] = sp1;
+    sOwn[0] = own0;
+    sOwn[1] = own1;
+    sL[0] = q.ctl[sp0];
+    sL[1] = q.ctl[sp1];
+  }
+  __syncthreads();
+  AdvDev b = a;
+  b.ctl = sL;
+  b.ctl_base = sSp[0];
+  for (uint32_t j = 0; j < q.nsteps; ++j) {
+    const int in = j & 1;  // (selects, not indexing: no copy of q to scratch)
+    s.P = in ? q.P[1] : q.P[0];
+    s.V = in ? q.V[1] : q.V[0];
+    s.W = in ? q.W[1] : q.W[0];
+    b.P = in ? q.P[0] : q.P[1];
+    b.V = in ? q.V[0] : q.V[1];
+    b.W = in ? q.W[0] : q.W[1];
+    b.acc = q.acc3 + 2 * (size_t)S * (j % 3);
+    if (live) forces1_tile<FP, FT, true>(s, mt, w, tile, cap, b, cur0 ^ (int)in);
+    grid.sync();
+    if (live && threadIdx.x < 2 && (threadIdx.x == 0 || sSp[1] != sSp[0])) {
+      const uint32_t k = threadIdx.x, sp = sSp[k];
+      const bool own = sOwn[k] != 0;
+      unsigned long long *acc = q.acc3 + 2 * (size_t)S * (j % 3) + 2 * sp;
+      const double s0 = (double)(long long)acc[0] * kFxInv;
+      const double s1 = (double)(long long)acc[1] * kFxInv;
+      control_update(sL[k], s0, s1, own ? q.hist + 3 * (size_t)q.hist_cap * sp : nullptr,
+                     q.hist_cap, q.dt);
+      if (own) {
+        unsigned long long *old = q.acc3 + 2 * (size_t)S * ((j + 2) % 3) + 2 * sp;
+        old[0] = 0ull;
+        old[1] = 0ull;
+      }
+    }
+    __syncthreads();
+  }
+  grid.sync();
+  // every workgroup has read the last step's sums: clear them, hand out the
+  // loadings and the bond-state index
+  if (live && threadIdx.x < 2 && (threadIdx.x == 0 || sSp[1] != sSp[0]) && sOwn[threadIdx.x]) {
+    const uint32_t sp = sSp[threadIdx.x];
+    unsigned long long *last = q.acc3 + 2 * (size_t)S * ((q.nsteps + 2) % 3) + 2 * sp;
+    last[0] = 0ull;
+    last[1] = 0ull;
+    q.ctl[sp] = sL[threadIdx.x];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *q.curp = cur0 ^ (int)(q.nsteps & 1u);
 }
 
 // batched pass: workgroup b advances particles [blk_first[b], blk_end[b]) of
@@ -2586,6 +2704,11 @@ struct rfd_system {
   uint32_t *spec32 = nullptr;           // specimen of particle 32 k (fused batch steps)
   uint32_t *spec_first_d = nullptr;     // S + 1 first particles, on the device
   bool batch_fusable = false;           // every specimen holds at least 128 particles
+  // multi-step launches (k_nat_steps): on request, when every force tile gets
+  // a resident workgroup; their three fixed-point sum buffers per specimen
+  bool persist = false;
+  unsigned long long *acc3 = nullptr;
+  uint32_t acc3_S = 0;
   // step graphs: 2 x rebuild_every steps captured once, replayed
   uint64_t gen = 1;                   // bumped by every change a captured step depends on
   hipStream_t gstream = nullptr;
@@ -3044,17 +3167,142 @@ int step_fused(rfd_system *s) {
   return st;
 }
 
+// ------------------------------------------------- multi-step launches
+// the launch shape of k_nat_steps for the current tile shape
+struct StepsShape {
+  uint32_t nb, per, cap;
+  size_t lds;
+};
+
+template <int FP>
+StepsShape steps_shape(const rfd_system *s) {
+  StepsShape h;
+  h.nb = (s->N + FP - 1) / FP;
+  h.per = (h.nb + kXcd - 1) / kXcd;
+  const int q = FP == 32 ? 0 : (FP == 64 ? 1 : 2);
+  h.cap = std::max((s->tile_halves[q] + 31u) & ~31u, 2u * FP);
+  h.lds = sizeof(float) * 6 * h.cap;
+  return h;
+}
+
+// every workgroup of a cooperative launch must be resident at once
+template <int FP, int FT>
+bool steps_fit_fp(const rfd_system *s) {
+  const StepsShape h = steps_shape<FP>(s);
+  int bpc = 0, dev = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, (const void *)k_nat_steps<FP, FT>, FT,
+                                                   h.lds) != hipSuccess ||
+      hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return false;
+  return (uint64_t)bpc * (uint64_t)cus >= (uint64_t)h.per * kXcd;
+}
+
+bool steps_fit(const rfd_system *s) {
+  if (!can_fuse(s)) return false;
+  if (s->force_fp == 32) return steps_fit_fp<32, 128>(s);
+  if (s->force_fp == 128) return steps_fit_fp<128, 256>(s);
+  return steps_fit_fp<kFP, kFT>(s);
+}
+
+template <int FP, int FT>
+int launch_steps(rfd_system *s, SysDev d, Mat m, AdvDev a, StepsDev q) {
+  const StepsShape h = steps_shape<FP>(s);
+  WorkDev w{s->wrec, s->wofs, s->wnin};
+  uint32_t nb = h.nb, per = h.per, cap = h.cap;
+  void *args[] = {&d, &m, &w, &nb, &per, &cap, &a, &q};
+  RFD_HIP(s->ctx, hipLaunchCooperativeKernel((const void *)k_nat_steps<FP, FT>,
+                                             dim3(h.per * kXcd), dim3(FT), args,
+                                             (unsigned)h.lds, s->ctx->stream));
+  return RFD_OK;
+}
+
+// n (even) fused steps in one launch; no contact-row rebuild falls inside
+int steps_persistent(rfd_system *s, uint32_t n) {
+  const uint32_t Ssp = s->S ? s->S : 1u;
+  if (s->acc3_S != Ssp) {
+    int st = dalloc(s->ctx, &s->acc3, 6 * (size_t)Ssp);
+    if (st) return st;
+    RFD_HIP(s->ctx, hipMemsetAsync(s->acc3, 0, sizeof(unsigned long long) * 6 * Ssp,
+                                   s->ctx->stream));
+    s->acc3_S = Ssp;
+  }
+
+  SysDev d = make_dev(s);
+  Mat m = make_mat(s->p);
+  AdvDev a{s->P2, s->V2, s->W2, s->inv, s->ctl, s->acc, (float)s->p.dt, s->mode,
+           nullptr, nullptr, 0u};
+  StepsDev q{};
+  q.P[0] = s->P;
+  q.P[1] = s->P2;
+  q.V[0] = s->V;
+  q.V[1] = s->V2;
+  q.W[0] = s->W;
+  q.W[1] = s->W2;
+  q.ctl = s->ctl;
+  q.acc3 = s->acc3;
+  q.hist = s->hist;
+  q.hist_cap = (uint32_t)kHistCap;
+  q.S = Ssp;
+  q.nsteps = n;
+  q.curp = s->d_cur;
+  q.dt = s->p.dt;
+
+  if (s->S) {
+    a.spec32 = s->spec32;
+    a.spec_first = s->spec_first_d;
+    q.ctl = s->ctls;
+    q.hist = s->hist_b;
+    q.hist_cap = s->hist_cap_b;
+  }
+  int st;
+  if (s->force_fp == 32)
+    st = launch_steps<32, 128>(s, d, m, a, q);
+  else if (s->force_fp == 128)
+    st = launch_steps<128, 256>(s, d, m, a, q);
+  else
+    st = launch_steps<kFP, kFT>(s, d, m, a, q);
+  if (!st) s->step_count += n;
+  return st;
+}
+
+bool capturing(const rfd_system *s) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(s->ctx->stream, &cs) != hipSuccess ||
+         cs != hipStreamCaptureStatusNone;
+}
+
 // n explicit steps; fused ones run in pairs, so that P, V, W hold the state
 // when this returns (the particle update is the same arithmetic either way,
-// and the two orders of steps give the same bits)
+// and the two orders of steps give the same bits).  With multi-step launches
+// on, the steps between two contact-row rebuilds run as one launch (not
+// under graph capture).
 int run_steps(rfd_system *s, uint32_t n) {
   uint32_t k = 0;
   if (can_fuse(s)) {
-    for (; k + 2 <= n; k += 2) {
+    bool multi = s->persist && n >= 2 && !capturing(s) && steps_fit(s);
+    while (k + 2 <= n) {
+      uint32_t seg = 0;
+      if (multi) {
+        if (!s->rows_valid || s->step_count % s->p.rebuild_every == 0) {
+          const int st = rebuild_rows(s);
+          if (st) return st;
+        }
+        const uint32_t left =
+            s->p.rebuild_every - (uint32_t)(s->step_count % s->p.rebuild_every);
+        seg = std::min(n - k, left) & ~1u;
+      }
+      if (seg >= 2) {
+        const int st = steps_persistent(s, seg);
+        if (st) return st;
+        k += seg;
+        continue;
+      }
       const int st = step_fused(s);
       const int st2 = step_fused(s);
       if (st) return st;
       if (st2) return st2;
+      k += 2;
     }
   }
   for (; k < n; ++k) {
@@ -3422,6 +3670,7 @@ int rfd_system_destroy(rfd_system *s) {
   dfree(s->blk_first);
   dfree(s->blk_end);
   dfree(s->acc_b);
+  dfree(s->acc3);
   dfree(s->spec32);
   dfree(s->spec_first_d);
   if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
@@ -4169,6 +4418,19 @@ int rfd_system_set_fusion(rfd_system *s, int on) {
 int rfd_system_get_fusion(rfd_system *s, int *fused) {
   if (!valid(s) || !fused) return RFD_E_ARG;
   *fused = can_fuse(s) ? 1 : 0;
+  return RFD_OK;
+}
+
+int rfd_system_set_persistent(rfd_system *s, int on) {
+  if (!valid(s)) return RFD_E_ARG;
+  s->persist = on != 0;
+  return RFD_OK;
+}
+
+int rfd_system_get_persistent(rfd_system *s, int *on, int *applies) {
+  if (!valid(s) || !on || !applies) return RFD_E_ARG;
+  *on = s->persist ? 1 : 0;
+  *applies = steps_fit(s) ? 1 : 0;
   return RFD_OK;
 }
 

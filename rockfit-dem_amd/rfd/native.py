@@ -140,6 +140,8 @@ NATIVE_SIGNATURES = {
     "rfd_system_get_tiles": [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "rfd_system_set_fusion": [_vp, C.c_int],
     "rfd_system_get_fusion": [_vp, C.POINTER(C.c_int)],
+    "rfd_system_set_persistent": [_vp, C.c_int],
+    "rfd_system_get_persistent": [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "rfd_tune_cache_save": [C.c_char_p],
     "rfd_system_pack": [_vp, C.POINTER(PackParams), C.POINTER(PackResult)],
     "rfd_tune_cache_load": [C.c_char_p],
@@ -492,6 +494,18 @@ class NativeSystem:
         f = C.c_int()
         self._call("rfd_system_get_fusion", self.h, C.byref(f))
         return bool(f.value)
+
+    def set_persistent(self, on: bool = True):
+        """Run the fused steps between two contact-row rebuilds as one
+        cooperative launch where the system qualifies (off by default; the
+        same bits either way)."""
+        self._call("rfd_system_set_persistent", self.h, int(on))
+
+    def persistent(self) -> tuple:
+        """(multi-step launches requested, the current system qualifies)."""
+        a, b = C.c_int(), C.c_int()
+        self._call("rfd_system_get_persistent", self.h, C.byref(a), C.byref(b))
+        return bool(a.value), bool(b.value)
 
     def force_kernel(self) -> str:
         """Name of the force-pass kernel the next explicit step launches."""
