@@ -102,7 +102,10 @@ int rfd_il_construct(rfd_ctx *ctx, const float *R, uint32_t N,
 
 /* InteractionListReConstructBPM (md_interaction_list_construction.cu:75-79) =
  * d_ReConstructInteractionListbpm (md_interaction_list_reconstructbpm.cu:80).
- * quirk_p0 != 0 keeps the reference's particle-0 free-slot rule (:32). */
+ * quirk_p0 != 0 keeps the reference's particle-0 free-slot rule (:32).  New
+ * partners are inserted in the order of the 8-cell scan over the caller's
+ * lists: search cell by search cell, each cell's members in the order CIs
+ * holds them (any order, not only ascending id). */
 int rfd_il_reconstruct_bpm(rfd_ctx *ctx, const float *R, uint32_t N,
                            const uint32_t *CIs, const uint32_t *pnC,
                            uint32_t *IL, uint8_t *ILtype, uint32_t K, float a,

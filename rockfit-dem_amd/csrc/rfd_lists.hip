@@ -482,14 +482,15 @@ bool slots_fit(uint32_t N, uint32_t K) { return (uint64_t)N * K < 0xFFFFFFFFull;
 // below the coarse grid and ends h cells beyond it, so every partner of a
 // particle in the coarse grid is binned. Of the candidates that pass the range test, those that the
 // caller's cell lists hold in one of the particle's 8 search cells are kept
-// and visited in the order of the 8-cell search: by search cell, then by id
-// (a cell's members are listed by id). So the same partners are inserted, in
-// the same order, as by the scan of the 8 cells, from about 100 candidates per
-// particle instead of 8 coarse cells' worth.
+// and visited in the order of the 8-cell search: by search cell, then by the
+// candidate's place in that cell's list (CIs), whatever order the caller's
+// lists hold. So the same partners are inserted, in the same order, as by the
+// scan of the 8 cells, from about 100 candidates per particle instead of 8
+// coarse cells' worth.
 constexpr uint32_t kFineMaxDim = 40000;
 constexpr int kFineHalf = 2;       // fine cells searched on each side
 constexpr int kFineFound = 32;     // in-range partners held (sorted) per particle
-constexpr uint32_t kFineIdBits = 29;  // key = search cell << 29 | id
+constexpr uint32_t kFineIdBits = 29;  // key = search cell << 29 | place in CIs
 
 struct FineGrid {
   float af, inv_af;     // fine cell edge and its inverse
@@ -513,13 +514,19 @@ __device__ __forceinline__ bool fine_coords(const FineGrid &f, float x, float y,
 }
 
 // coarse cell of every particle the caller's lists hold (others keep the sentinel)
+// the cell of every particle the caller's lists hold, and its place in them
+// (j = CIs[N + placeOf[j]])
 __global__ void k_cell_of(const uint32_t *__restrict__ CIs, const uint32_t *__restrict__ pnC,
-                          uint32_t C, uint32_t N, uint32_t *__restrict__ cellOf) {
+                          uint32_t C, uint32_t N, uint32_t *__restrict__ cellOf,
+                          uint32_t *__restrict__ placeOf) {
   for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += grid_stride()) {
     const uint32_t s0 = pnC[c], n = pnC[C + c];
     for (uint32_t u = 0; u < n; ++u) {
       const uint32_t j = CIs[N + s0 + u];
-      if (j < N) cellOf[j] = c;
+      if (j < N) {
+        cellOf[j] = c;
+        placeOf[j] = s0 + u;
+      }
     }
   }
 }
@@ -579,7 +586,8 @@ __global__ void __launch_bounds__(kRecThreads)
 k_il_reconstruct_fine(const float *__restrict__ R, uint32_t N, const uint32_t *__restrict__ CIs,
                       const uint32_t *__restrict__ pnC, const float4 *__restrict__ Rf,
                       const uint32_t *__restrict__ excl, const uint32_t *__restrict__ cellOf,
-                      FineGrid f, uint32_t *__restrict__ IL, uint8_t *__restrict__ ILtype,
+                      const uint32_t *__restrict__ placeOf, FineGrid f,
+                      uint32_t *__restrict__ IL, uint8_t *__restrict__ ILtype,
                       uint32_t K, float a, float inv_a, float aacut, uint32_t Nx, uint32_t Ny,
                       uint32_t Nz, int quirk_p0, uint32_t *__restrict__ flags,
                       uint32_t nblk, uint32_t per) {
@@ -657,7 +665,7 @@ k_il_reconstruct_fine(const float *__restrict__ R, uint32_t N, const uint32_t *_
                 for (uint32_t qq = 0; qq < 8; ++qq)
                   if (cj != kSentinel && cells[qq] == cj) q = qq;
                 if (q < 8) {
-                  uint32_t x = (q << kFineIdBits) | j;
+                  uint32_t x = (q << kFineIdBits) | placeOf[j];
 #pragma unroll
                   for (int u = 0; u < kFineFound; ++u) {
                     const uint32_t lo2 = min(key[u], x), hi2 = max(key[u], x);
@@ -679,7 +687,7 @@ k_il_reconstruct_fine(const float *__restrict__ R, uint32_t N, const uint32_t *_
 #pragma unroll
           for (int u = 0; u + 1 < kFineFound; ++u) key[u] = key[u + 1];
           key[kFineFound - 1] = 0xFFFFFFFFu;
-          consider(k0 & ((1u << kFineIdBits) - 1u));
+          consider(CIs[N + (k0 & ((1u << kFineIdBits) - 1u))]);
         }
       } else {
         // more partners than the sorted list holds: the 8-cell scan itself
@@ -829,7 +837,7 @@ int rfd_il_reconstruct_bpm(rfd_ctx *ctx, const float *R, uint32_t N,
   hipStream_t q = ctx->stream;
   const unsigned blocks = (N + kRecThreads - 1) / kRecThreads;
   const size_t lds = sizeof(uint32_t) * kRecThreads * rec_pitch(K);
-  // the fine grid spans the coarse grid plus one fine cell on each side
+  // the fine grid spans the coarse grid plus kFineHalf fine cells on each side
   FineGrid f{};
   bool fine = aacut > 0.0f && std::isfinite(aacut) && N < (1u << kFineIdBits) - 1u;
   if (fine) {
@@ -864,18 +872,19 @@ int rfd_il_reconstruct_bpm(rfd_ctx *ctx, const float *R, uint32_t N,
   const uint32_t C = Nx * Ny * Nz;
   const uint32_t Cf = f.nx * f.ny * f.nz;
   const uint32_t chunks = (uint32_t)(((size_t)Cf + kScanChunk - 1) / kScanChunk);
-  // workspace: fine-ordered positions (16 B each), then cellOf[N], fci[N],
-  // cnt[Cf], excl[Cf + 1], part[chunks]
+  // workspace: fine-ordered positions (16 B each), then cellOf[N], placeOf[N],
+  // fci[N], cnt[Cf], excl[Cf + 1], part[chunks]
   const size_t words = 3 * (size_t)N + 2 * (size_t)Cf + 1 + chunks;
   void *ws = nullptr;
   int st = rfd_scratch(ctx, sizeof(float4) * (size_t)N + sizeof(uint32_t) * words, &ws);
   if (st) return st;
   float4 *Rf = (float4 *)ws;
-  uint32_t *cellOf = (uint32_t *)(Rf + N), *fci = cellOf + N, *cnt = fci + N,
+  uint32_t *cellOf = (uint32_t *)(Rf + N), *placeOf = cellOf + N, *fci = placeOf + N,
+           *cnt = fci + N,
            *excl = cnt + Cf, *part = excl + Cf + 1;
   k_fill_u32<<<rfd_blocks(N, 256, 4096), 256, 0, q>>>(cellOf, N, kSentinel);
   k_fill_u32<<<rfd_blocks(Cf, 256, 4096), 256, 0, q>>>(cnt, Cf, 0u);
-  k_cell_of<<<rfd_blocks(C, 256), 256, 0, q>>>(CIs, pnC, C, N, cellOf);
+  k_cell_of<<<rfd_blocks(C, 256), 256, 0, q>>>(CIs, pnC, C, N, cellOf, placeOf);
   k_fine_index<<<rfd_blocks(N, 256), 256, 0, q>>>(R, N, f, fci, cnt);
   k_chunk_sums<<<chunks, kScanThreads, 0, q>>>(cnt, Cf, part);
   k_scan_chunk_sums<<<1, kScanThreads, 0, q>>>(part, chunks);
@@ -884,7 +893,8 @@ int rfd_il_reconstruct_bpm(rfd_ctx *ctx, const float *R, uint32_t N,
   k_fine_scatter<<<rfd_blocks(N, 256), 256, 0, q>>>(R, N, fci, excl, cnt, Rf);
   const uint32_t per = (blocks + kXcds - 1) / kXcds;
   k_il_reconstruct_fine<<<per * kXcds, kRecThreads, lds, q>>>(
-      R, N, CIs, pnC, Rf, excl, cellOf, f, IL, ILtype, K, a, inv_a, aacut, Nx, Ny, Nz, quirk_p0,
+      R, N, CIs, pnC, Rf, excl, cellOf, placeOf, f, IL, ILtype, K, a, inv_a, aacut, Nx, Ny, Nz,
+      quirk_p0,
       ctx->d_flags, blocks, per);
   RFD_LAUNCHED(ctx, "rfd_il_reconstruct_bpm");
   return RFD_OK;

@@ -99,6 +99,46 @@ def _relist_twin(ctx, R, K, grid, aacut, seed):
     return o.host()["IL"].reshape(-1, K)
 
 
+def test_reconstruct_follows_caller_list_order(ctx):
+    """The reconstruction inserts partners in the order of the caller's cell
+    lists, whatever that order is: with every cell's members listed in
+    reverse id order (CIs), the GPU's fine-grid path gives the rows of the
+    oracle's 8-cell scan over the same lists, and those differ from the rows
+    the id-ordered lists give."""
+    import torch
+
+    case = cases.reference_uniaxial(8.0, 12.0, seed=5)
+    aacut = float(np.float32(case.mat.a_farcut) ** 2)
+    rows = {}
+    for order in ("id", "reversed"):
+        g, o = twin(ctx, case)
+        for s in (g, o):
+            s.cell_distribution()
+            s.il_construct(aacut)
+            s.delete_far_links(0.25)
+        d = perturb(case.N, 13, 0.02)
+        g.R += torch.from_numpy(d).cuda()
+        o.R += d
+        for s in (g, o):
+            s.cell_distribution()
+        if order == "reversed":
+            N, C = case.N, case.grid().C
+            CIs = o.CIs.copy()
+            for c in np.nonzero(o.pnC[C:])[0]:
+                s0, n = int(o.pnC[c]), int(o.pnC[C + c])
+                CIs[N + s0:N + s0 + n] = CIs[N + s0:N + s0 + n][::-1]
+            o.CIs[:] = CIs
+            g.CIs.copy_(torch.from_numpy(CIs.view(np.int32)).cuda())
+        for s in (g, o):
+            s.il_reconstruct_bpm(aacut, quirk_p0=True)
+        assert o.last_status == 0 and ctx.flags() == 0
+        assert_same(g.host(), o.host(), keys=("CIs", "IL", "ILtype"), where=order)
+        rows[order] = o.host()["IL"]
+    changed = (rows["id"] != rows["reversed"]).sum()
+    print(f"slots placed differently by the reversed lists: {changed}", flush=True)
+    assert changed > 0
+
+
 def test_reconstruct_dense_cluster_bitexact(ctx):
     """50 particles packed in a ball of radius 0.6 inside a lattice, so that
     each has 49 partners in reach: more than the 32 the fine-grid search
