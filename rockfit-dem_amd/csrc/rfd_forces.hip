@@ -41,14 +41,16 @@ k_forces_bpm(const float *__restrict__ R, const float *__restrict__ V,
              float *__restrict__ rij, float *__restrict__ Oijt, float *__restrict__ Mijn,
              float *__restrict__ Mijt, float *__restrict__ F, float *__restrict__ M,
              uint32_t N, uint32_t K, float dt, float E, float G, float b_r, float scrit,
-             float tcrit) {
+             float tcrit, uint32_t nblk, uint32_t per) {
   // per slot: normal force, twisting and bending moments, tangential force,
   // moment of the tangential force (3 floats each)
   __shared__ float s_c[15][kSlotThreads];
   __shared__ uint8_t s_live[kSlotThreads];
   const uint32_t P = kSlotThreads / K;  // particles per workgroup
   const uint32_t t = threadIdx.x, lp = t / K, k = t % K;
-  const uint32_t p0 = blockIdx.x * P;
+  const uint32_t blk = xcd_block(per);
+  if (blk >= nblk) return;
+  const uint32_t p0 = blk * P;
   const uint32_t i = p0 + lp;
   const size_t N1 = N, N2 = 2 * (size_t)N;
   const float area_k = b_r * b_r * kPiF;          // b_r*b_r*MCf_pi
@@ -386,9 +388,10 @@ int rfd_forces_bpm(rfd_ctx *ctx, const float *R, const float *V, const float *W,
   RFD_CHECK_ARG(ctx, K <= (uint32_t)kMaxK, "rfd_forces_bpm: K=%u > %d", K, kMaxK);
   const uint32_t P = kSlotThreads / K;
   const unsigned blocks = (N + P - 1) / P;
-  k_forces_bpm<<<blocks, kSlotThreads, 0, ctx->stream>>>(R, V, W, IL, ILtype, iL, rij, Oijt,
-                                                          Mijn, Mijt, F, M, N, K, dt, E, G,
-                                                          b_r, sc, tc);
+  const uint32_t per = (blocks + kXcds - 1) / kXcds;
+  k_forces_bpm<<<per * kXcds, kSlotThreads, 0, ctx->stream>>>(
+      R, V, W, IL, ILtype, iL, rij, Oijt, Mijn, Mijt, F, M, N, K, dt, E, G, b_r, sc, tc, blocks,
+      per);
   RFD_LAUNCHED(ctx, "rfd_forces_bpm");
   return RFD_OK;
 }

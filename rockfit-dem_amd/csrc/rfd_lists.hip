@@ -513,21 +513,13 @@ __device__ __forceinline__ bool fine_coords(const FineGrid &f, float x, float y,
   return true;
 }
 
-// coarse cell of every particle the caller's lists hold (others keep the sentinel)
-// the cell of every particle the caller's lists hold, and its place in them
-// (j = CIs[N + placeOf[j]])
-__global__ void k_cell_of(const uint32_t *__restrict__ CIs, const uint32_t *__restrict__ pnC,
-                          uint32_t C, uint32_t N, uint32_t *__restrict__ cellOf,
-                          uint32_t *__restrict__ placeOf) {
-  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += grid_stride()) {
-    const uint32_t s0 = pnC[c], n = pnC[C + c];
-    for (uint32_t u = 0; u < n; ++u) {
-      const uint32_t j = CIs[N + s0 + u];
-      if (j < N) {
-        cellOf[j] = c;
-        placeOf[j] = s0 + u;
-      }
-    }
+// the place of every particle in the caller's lists, j = CIs[N + placeOf[j]]
+// (particles the lists do not hold keep the sentinel)
+__global__ void k_place_of(const uint32_t *__restrict__ CIs, uint32_t N,
+                           uint32_t *__restrict__ placeOf) {
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += grid_stride()) {
+    const uint32_t j = CIs[N + p];
+    if (j < N) placeOf[j] = p;
   }
 }
 
@@ -569,24 +561,25 @@ k_scan_excl(const uint32_t *__restrict__ cnt, uint32_t n, uint32_t *__restrict__
   }
 }
 
-// positions in fine-cell order, (x, y, z, id bits); cnt cleared beforehand
+// positions in fine-cell order, (x, y, z, bits of the particle's place in the
+// caller's lists); cnt cleared beforehand
 __global__ void k_fine_scatter(const float *__restrict__ R, uint32_t N,
                                const uint32_t *__restrict__ fci,
+                               const uint32_t *__restrict__ placeOf,
                                const uint32_t *__restrict__ excl, uint32_t *__restrict__ cnt,
                                float4 *__restrict__ Rf) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
     const uint32_t c = fci[i];
     if (c == kSentinel) continue;
     const uint32_t p = excl[c] + atomicAdd(&cnt[c], 1u);
-    Rf[p] = make_float4(R[i], R[i + N], R[i + 2 * (size_t)N], __uint_as_float(i));
+    Rf[p] = make_float4(R[i], R[i + N], R[i + 2 * (size_t)N], __uint_as_float(placeOf[i]));
   }
 }
 
 __global__ void __launch_bounds__(kRecThreads)
 k_il_reconstruct_fine(const float *__restrict__ R, uint32_t N, const uint32_t *__restrict__ CIs,
                       const uint32_t *__restrict__ pnC, const float4 *__restrict__ Rf,
-                      const uint32_t *__restrict__ excl, const uint32_t *__restrict__ cellOf,
-                      const uint32_t *__restrict__ placeOf, FineGrid f,
+                      const uint32_t *__restrict__ excl, FineGrid f,
                       uint32_t *__restrict__ IL, uint8_t *__restrict__ ILtype,
                       uint32_t K, float a, float inv_a, float aacut, uint32_t Nx, uint32_t Ny,
                       uint32_t Nz, int quirk_p0, uint32_t *__restrict__ flags,
@@ -637,6 +630,14 @@ k_il_reconstruct_fine(const float *__restrict__ R, uint32_t N, const uint32_t *_
       atomicOr(flags, RFD_FLAG_CELL_RANGE);
     } else {
       const float xi = R[i], yi = R[i + N], zi = R[i + 2 * (size_t)N];
+      // the search cells' ranges of places in the lists (empty when outside)
+      uint32_t cfirst[8], ccount[8];
+#pragma unroll
+      for (int qq = 0; qq < 8; ++qq) {
+        const bool in = cells[qq] != kSentinel;
+        cfirst[qq] = in ? pnC[cells[qq]] : 0u;
+        ccount[qq] = in ? pnC[C + cells[qq]] : 0u;
+      }
       uint32_t key[kFineFound];  // ascending; unused entries hold 0xFFFFFFFF
 #pragma unroll
       for (int u = 0; u < kFineFound; ++u) key[u] = 0xFFFFFFFFu;
@@ -658,14 +659,14 @@ k_il_reconstruct_fine(const float *__restrict__ R, uint32_t N, const uint32_t *_
               const float4 pj = Rf[e];
               const float rr = rec_r2(pj, xi, yi, zi);
               if (rr < aacut && (double)rr > 1e-10) {
-                const uint32_t j = __float_as_uint(pj.w);
-                const uint32_t cj = cellOf[j];
+                // kept if its place lies in one of the 8 search cells' ranges
+                const uint32_t pl = __float_as_uint(pj.w);
                 uint32_t q = 8;
 #pragma unroll
                 for (uint32_t qq = 0; qq < 8; ++qq)
-                  if (cj != kSentinel && cells[qq] == cj) q = qq;
+                  if (pl - cfirst[qq] < ccount[qq]) q = qq;
                 if (q < 8) {
-                  uint32_t x = (q << kFineIdBits) | placeOf[j];
+                  uint32_t x = (q << kFineIdBits) | pl;
 #pragma unroll
                   for (int u = 0; u < kFineFound; ++u) {
                     const uint32_t lo2 = min(key[u], x), hi2 = max(key[u], x);
@@ -682,12 +683,16 @@ k_il_reconstruct_fine(const float *__restrict__ R, uint32_t N, const uint32_t *_
         nf = kFineFound + 1;
       }
       if (nf <= (uint32_t)kFineFound) {
+        // the partners' ids from their places in the lists, all loads in
+        // flight at once, then inserted in key order
+#pragma unroll
+        for (int u = 0; u < kFineFound; ++u)
+          if ((uint32_t)u < nf) key[u] = CIs[N + (key[u] & ((1u << kFineIdBits) - 1u))];
         for (uint32_t n = 0; n < nf; ++n) {
-          const uint32_t k0 = key[0];
+          const uint32_t j = key[0];
 #pragma unroll
           for (int u = 0; u + 1 < kFineFound; ++u) key[u] = key[u + 1];
-          key[kFineFound - 1] = 0xFFFFFFFFu;
-          consider(CIs[N + (k0 & ((1u << kFineIdBits) - 1u))]);
+          consider(j);
         }
       } else {
         // more partners than the sorted list holds: the 8-cell scan itself
@@ -872,29 +877,27 @@ int rfd_il_reconstruct_bpm(rfd_ctx *ctx, const float *R, uint32_t N,
   const uint32_t C = Nx * Ny * Nz;
   const uint32_t Cf = f.nx * f.ny * f.nz;
   const uint32_t chunks = (uint32_t)(((size_t)Cf + kScanChunk - 1) / kScanChunk);
-  // workspace: fine-ordered positions (16 B each), then cellOf[N], placeOf[N],
-  // fci[N], cnt[Cf], excl[Cf + 1], part[chunks]
+  // workspace: fine-ordered positions (16 B each), then placeOf[N], fci[N],
+  // cnt[Cf], excl[Cf + 1], part[chunks]
   const size_t words = 3 * (size_t)N + 2 * (size_t)Cf + 1 + chunks;
   void *ws = nullptr;
   int st = rfd_scratch(ctx, sizeof(float4) * (size_t)N + sizeof(uint32_t) * words, &ws);
   if (st) return st;
   float4 *Rf = (float4 *)ws;
-  uint32_t *cellOf = (uint32_t *)(Rf + N), *placeOf = cellOf + N, *fci = placeOf + N,
-           *cnt = fci + N,
+  uint32_t *placeOf = (uint32_t *)(Rf + N), *fci = placeOf + N, *cnt = fci + N,
            *excl = cnt + Cf, *part = excl + Cf + 1;
-  k_fill_u32<<<rfd_blocks(N, 256, 4096), 256, 0, q>>>(cellOf, N, kSentinel);
+  k_fill_u32<<<rfd_blocks(N, 256, 4096), 256, 0, q>>>(placeOf, N, kSentinel);
   k_fill_u32<<<rfd_blocks(Cf, 256, 4096), 256, 0, q>>>(cnt, Cf, 0u);
-  k_cell_of<<<rfd_blocks(C, 256), 256, 0, q>>>(CIs, pnC, C, N, cellOf, placeOf);
+  k_place_of<<<rfd_blocks(N, 256), 256, 0, q>>>(CIs, N, placeOf);
   k_fine_index<<<rfd_blocks(N, 256), 256, 0, q>>>(R, N, f, fci, cnt);
   k_chunk_sums<<<chunks, kScanThreads, 0, q>>>(cnt, Cf, part);
   k_scan_chunk_sums<<<1, kScanThreads, 0, q>>>(part, chunks);
   k_scan_excl<<<chunks, kScanThreads, 0, q>>>(cnt, Cf, excl, part);
   k_fill_u32<<<rfd_blocks(Cf, 256, 4096), 256, 0, q>>>(cnt, Cf, 0u);
-  k_fine_scatter<<<rfd_blocks(N, 256), 256, 0, q>>>(R, N, fci, excl, cnt, Rf);
+  k_fine_scatter<<<rfd_blocks(N, 256), 256, 0, q>>>(R, N, fci, placeOf, excl, cnt, Rf);
   const uint32_t per = (blocks + kXcds - 1) / kXcds;
   k_il_reconstruct_fine<<<per * kXcds, kRecThreads, lds, q>>>(
-      R, N, CIs, pnC, Rf, excl, cellOf, placeOf, f, IL, ILtype, K, a, inv_a, aacut, Nx, Ny, Nz,
-      quirk_p0,
+      R, N, CIs, pnC, Rf, excl, f, IL, ILtype, K, a, inv_a, aacut, Nx, Ny, Nz, quirk_p0,
       ctx->d_flags, blocks, per);
   RFD_LAUNCHED(ctx, "rfd_il_reconstruct_bpm");
   return RFD_OK;
