@@ -150,6 +150,23 @@ int rfd_forces_contact(rfd_ctx *ctx, const float *R, const float *V,
                        float Gstar, float pA, float mu, float mur, float m,
                        float r);
 
+/* The superseded per-slot force path, kept for A/B comparisons:
+ * d_CalculateForcesDEM (md_calculateforcesdem.cu:328) for mode 0 (bond slots;
+ * empty slots zeroed; contact slots untouched, as its contact branch is off),
+ * d_CalculateForcesDEM_1 (:501) for mode 1 (contact slots within 2 rP get the
+ * Hertz-Mindlin law with E, G = the contact moduli; the others and empty slots
+ * are zeroed), d_CalculateForcesDEM_2 (:581) for mode 2 (bond slots only).
+ * Every slot's loads go to Fijn, Fijt, Mijn, Mijt, Mijadd (and its state to
+ * rij, Oijt); rfd_sumup_forces then forms F, M and rfd_check_break judges the
+ * bonds.  float3 arrays are passed as float* (3 floats per slot). */
+int rfd_forces_slots(rfd_ctx *ctx, int mode, const float *R, const float *V,
+                     const float *W, const uint32_t *IL, const uint8_t *ILtype,
+                     const float *iL, float *rij, float *Oijt, float *Fijn,
+                     float *Fijt, float *Mijn, float *Mijt, float *Mijadd,
+                     uint32_t N, uint32_t K, float dt, float E, float G,
+                     float b_r, float pA, float mu, float mur, float mP,
+                     float rP);
+
 /* d_SumUpForcesDEM (md_sumupforcesdem.cu:19): F,M = row sums of slot terms */
 int rfd_sumup_forces(rfd_ctx *ctx, const float *Fijn, const float *Fijt,
                      const float *Mijn, const float *Mijt, const float *Mijadd,
@@ -174,6 +191,17 @@ int rfd_platen_uniaxial(rfd_ctx *ctx, float *R, float *V, float *F, uint32_t N,
                         float cx, float cy, float RR, float Zt, float Zb,
                         float Ztr, float Zbr, float vt, float Zcut,
                         double out_sums[2]);
+
+/* d_UniaxialCompression_simple (md_uniaxialcompression_simple.cu:29), the
+ * superseded penalty platens: a particle inside the footprint (RR, below
+ * Zcut) past z = cz + Hd2 or cz - Hd2 gets F.z += C times its penetration,
+ * with Coulomb friction mu |F.z| against its in-plane velocity
+ * (dd_FrictionForce :19).  out_sums = (bottom, top) reaction sums in FP64
+ * (the reference adds FP32 block partials on the host); synchronises. */
+int rfd_platen_uniaxial_penalty(rfd_ctx *ctx, const float *R, const float *V,
+                                float *F, uint32_t N, float cx, float cy,
+                                float cz, float RR, float Hd2, float C,
+                                float mu, float Zcut, double out_sums[2]);
 
 /* d_braziltest_simple2 (md_braziltest_simple2.cu:19) + partial sums */
 int rfd_platen_brazil(rfd_ctx *ctx, float *R, float *V, float *F, uint32_t N,

@@ -32,6 +32,8 @@ _SIG = {
     "orc_delete_far_links": ([_P, _u32] + [_P] * 9 + [_u32, _f32], None),
     "orc_forces_bpm": ([_P] * 12 + [_u32, _u32] + [_f32] * 6, None),
     "orc_forces_contact": ([_P] * 8 + [_u32, _u32] + [_f32] * 8, None),
+    "orc_forces_slots": ([C.c_int] + [_P] * 13 + [_u32, _u32] + [_f32] * 9, None),
+    "orc_platen_uniaxial_penalty": ([_P, _P, _P, _u32] + [_f32] * 8 + [_D], None),
     "orc_sumup_forces": ([_P] * 7 + [_u32, _u32], None),
     "orc_check_break": ([_P] * 7 + [_u32, _f32, _f32, _f32], None),
     "orc_increments": ([_P] * 5 + [_u32, _f32, _f32, _f32], None),
@@ -156,6 +158,22 @@ class OracleLayout:
                                  ptr(self.ILtype), ptr(self.slot["Oijt"]), ptr(self.F),
                                  ptr(self.M), self.N, self.K, m.dt, m.hm_E, m.hm_G, m.p_A,
                                  m.p_mu, m.p_mur, m.p_m, m.p_r)
+
+    def forces_slots(self, mode, m):
+        E, G = (m.hm_E, m.hm_G) if mode == 1 else (m.E, m.G)
+        self._lib.orc_forces_slots(int(mode), ptr(self.R), ptr(self.V), ptr(self.W), ptr(self.IL),
+                                   ptr(self.ILtype), ptr(self.iL),
+                                   *self._s("rij", "Oijt", "Fijn", "Fijt", "Mijn", "Mijt",
+                                            "Mijadd"),
+                                   self.N, self.K, m.dt, E, G, m.b_r, m.p_A, m.p_mu, m.p_mur,
+                                   m.p_m, m.p_r)
+
+    def platen_uniaxial_penalty(self, center, RR, Hd2, stiffness, mu, zcut):
+        out = (C.c_double * 2)()
+        self._lib.orc_platen_uniaxial_penalty(ptr(self.R), ptr(self.V), ptr(self.F), self.N,
+                                              center[0], center[1], center[2], RR, Hd2,
+                                              stiffness, mu, zcut, out)
+        return out[0], out[1]
 
     def sumup_forces(self):
         self._lib.orc_sumup_forces(*self._s("Fijn", "Fijt", "Mijn", "Mijt", "Mijadd"), ptr(self.F),

@@ -538,6 +538,241 @@ void orc_sumup_forces(const float *Fijn, const float *Fijt, const float *Mijn,
   }
 }
 
+/* ---- the superseded per-slot force path (md_calculateforcesdem.cu) ----
+ * Each slot's law goes to the slot arrays; d_SumUpForcesDEM
+ * (orc_sumup_forces) then adds them per particle and d_CheckBreakConditionDEM
+ * (orc_check_break) judges the bonds.  mode 0: d_CalculateForcesDEM (:328):
+ * bonds, empty slots zeroed, contact slots untouched (its contact branch is
+ * switched off by `&& false`, :361); mode 1: d_CalculateForcesDEM_1 (:501):
+ * contact slots, those beyond 2 rP and empty slots zeroed; mode 2:
+ * d_CalculateForcesDEM_2 (:581): bonds only. */
+static void zero_slot(float *Fijn, float *Fijt, float *Oijt, float *Mijn, float *Mijt,
+                      float *Mijadd, size_t s) {
+  zero3(Fijn, s);
+  zero3(Fijt, s);
+  zero3(Oijt, s);
+  zero3(Mijn, s);
+  zero3(Mijt, s);
+  zero3(Mijadd, s);
+}
+
+/* the bond law of one slot: dd_Calculate_fijn_bpm (:73), dd_Calculate_M123_bpm
+ * (:36), dd_Calculate_mijn_bpm (:112), dd_Calculate_mijt_bpm (:129),
+ * dd_Calculate_fijt_mijunsym_bpm (:147), in the order d_CalculateForcesDEM_2
+ * calls them */
+static void slot_bond(const float *R, const float *V, const float *W, const float *iL,
+                      float *rij, float *Oijt, float *Fijn, float *Fijt, float *Mijn,
+                      float *Mijt, float *Mijadd, size_t N, uint32_t i, uint32_t j, size_t s,
+                      float dt, float E, float G, float b_r) {
+  const size_t N1 = N, N2 = 2 * N;
+  /* dd_Calculate_rijm_nij (:20-34) */
+  float rx = R[j] - R[i], ry = R[j + N1] - R[i + N1], rz = R[j + N2] - R[i + N2];
+  float rm = orc_norm3(rx, ry, rz);
+  float inv = 1.0f / rm;
+  float nx = rx * inv, ny = ry * inv, nz = rz * inv;
+  float il = iL[s];
+  /* fijn: b_r^2 pi E (|r| - L0) / L0 along n */
+  float cef = b_r * b_r * PI_F * E * (rm - 1.0f / il) * il;
+  Fijn[3 * s] = nx * cef;
+  Fijn[3 * s + 1] = ny * cef;
+  Fijn[3 * s + 2] = nz * cef;
+  /* M123: the first-order rotation from the previous direction and the spin */
+  float px = rij[3 * s], py = rij[3 * s + 1], pz = rij[3 * s + 2];
+  float tx = py * nz - pz * ny, ty = pz * nx - px * nz, tz = px * ny - py * nx;
+  float c2 = ((W[j] + W[i]) * nx + (W[j + N1] + W[i + N1]) * ny + (W[j + N2] + W[i + N2]) * nz) *
+             dt * 0.5f;
+  float phx = nx * c2, phy = ny * c2, phz = nz * c2;
+  rij[3 * s] = nx;
+  rij[3 * s + 1] = ny;
+  rij[3 * s + 2] = nz;
+  float M1x = 1.0f + tz * phz + ty * phy, M1y = phz - tz - ty * phx, M1z = -phy - tz * phx + ty;
+  float M2x = tz - phz - tx * phy, M2y = tz * phz + 1.0f + tx * phx, M2z = -tz * phy + phx - tx;
+  float M3x = -ty - tx * phz + phy, M3y = -ty * phz + tx - phx, M3z = ty * phy + tx * phx + 1.0f;
+  /* mijn: twist, dt pi b_r^4 G / (2 L0) per unit relative spin along n */
+  float dwx = W[j] - W[i], dwy = W[j + N1] - W[i + N1], dwz = W[j + N2] - W[i + N2];
+  float c1 = nx * dwx + ny * dwy + nz * dwz;
+  float k2 = dt * PI_F * b_r * b_r * b_r * b_r * 0.5f * G * il;
+  float mx = Mijn[3 * s], my = Mijn[3 * s + 1], mz = Mijn[3 * s + 2];
+  Mijn[3 * s] = M1x * mx + M1y * my + M1z * mz + nx * c1 * k2;
+  Mijn[3 * s + 1] = M2x * mx + M2y * my + M2z * mz + ny * c1 * k2;
+  Mijn[3 * s + 2] = M3x * mx + M3y * my + M3z * mz + nz * c1 * k2;
+  /* mijt: bending, dt pi b_r^4 E / (4 L0) per unit relative spin across n */
+  k2 = dt * PI_F * b_r * b_r * b_r * b_r * 0.25f * E * il;
+  mx = Mijt[3 * s];
+  my = Mijt[3 * s + 1];
+  mz = Mijt[3 * s + 2];
+  Mijt[3 * s] = M1x * mx + M1y * my + M1z * mz + (dwx - nx * c1) * k2;
+  Mijt[3 * s + 1] = M2x * mx + M2y * my + M2z * mz + (dwy - ny * c1) * k2;
+  Mijt[3 * s + 2] = M3x * mx + M3y * my + M3z * mz + (dwz - nz * c1) * k2;
+  /* fijt and the unsymmetric moment */
+  float hx = 0.5f * (W[j] + W[i]), hy = 0.5f * (W[j + N1] + W[i + N1]),
+        hz = 0.5f * (W[j + N2] + W[i + N2]);
+  float vx = V[j] - V[i] - hy * rz + hz * ry;
+  float vy = V[j + N1] - V[i + N1] - hz * rx + hx * rz;
+  float vz = V[j + N2] - V[i + N2] - hx * ry + hy * rx;
+  float cv = nx * vx + ny * vy + nz * vz;
+  float Ox = Oijt[3 * s], Oy = Oijt[3 * s + 1], Oz = Oijt[3 * s + 2];
+  float ox = (M1x * Ox + M1y * Oy + M1z * Oz) + (vx - nx * cv) * dt;
+  float oy = (M2x * Ox + M2y * Oy + M2z * Oz) + (vy - ny * cv) * dt;
+  float oz = (M3x * Ox + M3y * Oy + M3z * Oz) + (vz - nz * cv) * dt;
+  float ct = G * b_r * b_r * PI_F * il;
+  Fijt[3 * s] = ox * ct;
+  Fijt[3 * s + 1] = oy * ct;
+  Fijt[3 * s + 2] = oz * ct;
+  Oijt[3 * s] = ox;
+  Oijt[3 * s + 1] = oy;
+  Oijt[3 * s + 2] = oz;
+  Mijadd[3 * s] = 0.5f * (ry * oz - rz * oy) * ct;
+  Mijadd[3 * s + 1] = 0.5f * (rz * ox - rx * oz) * ct;
+  Mijadd[3 * s + 2] = 0.5f * (rx * oy - ry * ox) * ct;
+}
+
+/* the contact law of one slot within 2 rP: dd_Calculate_fijn_ppHM (:189),
+ * dd_Calculate_fijtmijt_ppHM (:245), Mijn zeroed, dd_Calculate_mijroll_ppHM
+ * (:306) */
+static void slot_contact(const float *V, const float *W, float *Oijt, float *Fijn, float *Fijt,
+                         float *Mijn, float *Mijt, float *Mijadd, size_t N, uint32_t i,
+                         uint32_t j, size_t s, float nx, float ny, float nz, float rm, float dt,
+                         float E, float G, float pA, float mu, float mur, float mP, float rP) {
+  const size_t N1 = N, N2 = 2 * N;
+  float q = sqrtf(rP * fabsf(rP - 0.5f * rm));
+  float Kn = 2.0f * E * q, Kt = 8.0f * G * q;
+  float vx = V[j] - V[i] + rP * (ny * (W[j + N2] + W[i + N2]) - nz * (W[j + N1] + W[i + N1]));
+  float vy = V[j + N1] - V[i + N1] + rP * (nz * (W[j] + W[i]) - nx * (W[j + N2] + W[i + N2]));
+  float vz = V[j + N2] - V[i + N2] + rP * (nx * (W[j + N1] + W[i + N1]) - ny * (W[j] + W[i]));
+  float vn = nx * vx + ny * vy + nz * vz;
+  float fn = (rm - 2.0f * rP) * Kn * TWO_THIRDS_F;
+  float dmp = -1.8257f * pA * vn * sqrtf(Kn * 0.5f * mP);
+  Fijn[3 * s] = nx * (fn + dmp);
+  Fijn[3 * s + 1] = ny * (fn + dmp);
+  Fijn[3 * s + 2] = nz * (fn + dmp);
+  float vtx = vx - nx * vn, vty = vy - ny * vn, vtz = vz - nz * vn;
+  float Ox = Oijt[3 * s], Oy = Oijt[3 * s + 1], Oz = Oijt[3 * s + 2];
+  float c1 = nx * Ox + ny * Oy + nz * Oz;
+  float ox = Ox - nx * c1, oy = Oy - ny * c1, oz = Oz - nz * c1;
+  c1 = ox * ox + oy * oy + oz * oz;
+  if ((double)c1 > 1e-18) {
+    c1 = 1.0f / c1;
+    c1 = c1 * (Ox * Ox + Oy * Oy + Oz * Oz);
+    c1 = sqrtf(c1);
+    ox = ox * c1;
+    oy = oy * c1;
+    oz = oz * c1;
+  }
+  ox = ox + vtx * dt;
+  oy = oy + vty * dt;
+  oz = oz + vtz * dt;
+  float fx = ox * Kt, fy = oy * Kt, fz = oz * Kt;
+  float ftm = fx * fx + fy * fy + fz * fz;
+  if (ftm > mu * mu * fn * fn) {
+    c1 = mu * fabsf(fn) * orc_rsqrt(ftm);
+    fx = fx * c1;
+    fy = fy * c1;
+    fz = fz * c1;
+    c1 = 1.0f / Kt;
+    ox = fx * c1;
+    oy = fy * c1;
+    oz = fz * c1;
+  } else {
+    c1 = -1.8257f * pA * sqrtf(Kt * 0.5f * mP);
+    fx = fx + vtx * c1;
+    fy = fy + vty * c1;
+    fz = fz + vtz * c1;
+  }
+  Oijt[3 * s] = ox;
+  Oijt[3 * s + 1] = oy;
+  Oijt[3 * s + 2] = oz;
+  Fijt[3 * s] = fx;
+  Fijt[3 * s + 1] = fy;
+  Fijt[3 * s + 2] = fz;
+  Mijt[3 * s] = (ny * fz - nz * fy) * rP;
+  Mijt[3 * s + 1] = (nz * fx - nx * fz) * rP;
+  Mijt[3 * s + 2] = (nx * fy - ny * fx) * rP;
+  zero3(Mijn, s);
+  float wx = W[i], wy = W[i + N1], wz = W[i + N2];
+  float wim = wx * wx + wy * wy + wz * wz;
+  if ((double)wim > 1e-18) {
+    float cr = -mur * fabsf(fn) * orc_rsqrt(wim);
+    Mijadd[3 * s] = wx * cr;
+    Mijadd[3 * s + 1] = wy * cr;
+    Mijadd[3 * s + 2] = wz * cr;
+  } else {
+    zero3(Mijadd, s);
+  }
+}
+
+void orc_forces_slots(int mode, const float *R, const float *V, const float *W,
+                      const uint32_t *IL, const uint8_t *ILtype, const float *iL, float *rij,
+                      float *Oijt, float *Fijn, float *Fijt, float *Mijn, float *Mijt,
+                      float *Mijadd, uint32_t N, uint32_t K, float dt, float E, float G,
+                      float b_r, float pA, float mu, float mur, float mP, float rP) {
+  const size_t N1 = N, N2 = 2 * (size_t)N;
+  for (uint32_t i = 0; i < N; ++i) {
+    for (uint32_t k = 0; k < K; ++k) {
+      size_t s = (size_t)i * K + k;
+      uint32_t j = IL[s];
+      if (j == ORC_SENTINEL) {
+        if (mode != 2) zero_slot(Fijn, Fijt, Oijt, Mijn, Mijt, Mijadd, s);
+        continue;
+      }
+      uint8_t type = ILtype[s];
+      if (type == 1 && mode != 1) {
+        slot_bond(R, V, W, iL, rij, Oijt, Fijn, Fijt, Mijn, Mijt, Mijadd, N, i, j, s, dt, E, G,
+                  b_r);
+      } else if (type == 0 && mode == 1) {
+        float rx = R[j] - R[i], ry = R[j + N1] - R[i + N1], rz = R[j + N2] - R[i + N2];
+        float rm = orc_norm3(rx, ry, rz);
+        float inv = 1.0f / rm;
+        if (rm < 2.0f * rP)
+          slot_contact(V, W, Oijt, Fijn, Fijt, Mijn, Mijt, Mijadd, N, i, j, s, rx * inv,
+                       ry * inv, rz * inv, rm, dt, E, G, pA, mu, mur, mP, rP);
+        else
+          zero_slot(Fijn, Fijt, Oijt, Mijn, Mijt, Mijadd, s);
+      }
+    }
+  }
+}
+
+/* md_uniaxialcompression_simple.cu:19-29 dd_FrictionForce, :29-81
+ * d_UniaxialCompression_simple: penalty platens at z = c.z +- Hd2 with
+ * stiffness C and Coulomb friction mu against the in-plane velocity; the
+ * reaction sums (bottom, top) are accumulated in double */
+static void penalty_friction(const float *V, float *F, size_t N, uint32_t i, float fz,
+                             float mu) {
+  float vv = V[i] * V[i] + V[i + N] * V[i + N];
+  if ((double)vv > 1e-10) {
+    float iv = orc_rsqrt(vv);
+    F[i] -= fabsf(fz) * mu * V[i] * iv;
+    F[i + N] -= fabsf(fz) * mu * V[i + N] * iv;
+  }
+}
+
+void orc_platen_uniaxial_penalty(const float *R, const float *V, float *F, uint32_t N,
+                                 float cx, float cy, float cz, float RR, float Hd2, float C,
+                                 float mu, float Zcut, double sums[2]) {
+  const size_t N1 = N, N2 = 2 * (size_t)N;
+  double sfb = 0, sft = 0;
+  for (uint32_t i = 0; i < N; ++i) {
+    float rx = R[i] - cx, ry = R[i + N1] - cy, rz = R[i + N2] - cz;
+    if (rx * rx + ry * ry < RR && R[i + N2] < Zcut) {
+      if (rz - Hd2 > 0) {
+        float fz = (Hd2 - rz) * C;
+        penalty_friction(V, F, N, i, fz, mu);
+        F[i + N2] += fz;
+        sft -= fz;
+      }
+      if (rz + Hd2 < 0) {
+        float fz = (Hd2 + rz) * C;
+        penalty_friction(V, F, N, i, fz, mu);
+        F[i + N2] -= fz;
+        sfb += fz;
+      }
+    }
+  }
+  sums[0] = sfb;
+  sums[1] = sft;
+}
+
 /* md_checkbreakconditiondem.cu:20-55 */
 void orc_check_break(const float *rij, float *Oijt, const float *Fijn,
                      const float *Fijt, const float *Mijn, const float *Mijt,

@@ -70,6 +70,17 @@ void orc_forces_contact(const float *R, const float *V, const float *W,
 void orc_sumup_forces(const float *Fijn, const float *Fijt, const float *Mijn,
                       const float *Mijt, const float *Mijadd, float *F, float *M,
                       uint32_t N, uint32_t K);
+/* f4 superseded per-slot force path (md_calculateforcesdem.cu:328, :501,
+ * :581 by mode 0, 1, 2) and penalty platens
+ * (md_uniaxialcompression_simple.cu:29) */
+void orc_forces_slots(int mode, const float *R, const float *V, const float *W,
+                      const uint32_t *IL, const uint8_t *ILtype, const float *iL, float *rij,
+                      float *Oijt, float *Fijn, float *Fijt, float *Mijn, float *Mijt,
+                      float *Mijadd, uint32_t N, uint32_t K, float dt, float E, float G,
+                      float b_r, float pA, float mu, float mur, float mP, float rP);
+void orc_platen_uniaxial_penalty(const float *R, const float *V, float *F, uint32_t N,
+                                 float cx, float cy, float cz, float RR, float Hd2, float C,
+                                 float mu, float Zcut, double sums[2]);
 /* a12 stand-alone per-slot break check */
 void orc_check_break(const float *rij, float *Oijt, const float *Fijn,
                      const float *Fijt, const float *Mijn, const float *Mijt,

@@ -373,6 +373,167 @@ __global__ void k_check_break(const float *__restrict__ rij, float *__restrict__
   }
 }
 
+// ---- the superseded per-slot force path (md_calculateforcesdem.cu) ----
+// Every slot's law goes to the slot arrays (Fijn, Fijt, Oijt, Mijn, Mijt,
+// Mijadd, and the bond direction rij); d_SumUpForcesDEM (k_sumup) adds them
+// per particle and d_CheckBreakConditionDEM (k_check_break) judges the bonds.
+// Slots are independent, so one thread per slot.  MODE 0: d_CalculateForcesDEM
+// (:328-499): bonds, empty slots zeroed, contact slots left alone (its contact
+// branch is switched off by `&& false`, :361); MODE 1: d_CalculateForcesDEM_1
+// (:501-579): contact slots, those beyond 2 rP and empty slots zeroed;
+// MODE 2: d_CalculateForcesDEM_2 (:581-628): bonds only.
+struct SlotArrays {
+  float *rij, *Oijt, *Fijn, *Fijt, *Mijn, *Mijt, *Mijadd;
+};
+
+__device__ __forceinline__ void slot_zero(const SlotArrays &a, size_t s) {
+  st3(a.Fijn, s, 0.f, 0.f, 0.f);
+  st3(a.Fijt, s, 0.f, 0.f, 0.f);
+  st3(a.Oijt, s, 0.f, 0.f, 0.f);
+  st3(a.Mijn, s, 0.f, 0.f, 0.f);
+  st3(a.Mijt, s, 0.f, 0.f, 0.f);
+  st3(a.Mijadd, s, 0.f, 0.f, 0.f);
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256)
+k_forces_slots(const float *__restrict__ R, const float *__restrict__ V,
+               const float *__restrict__ W, const uint32_t *__restrict__ IL,
+               const uint8_t *__restrict__ ILtype, const float *__restrict__ iL, SlotArrays a,
+               uint32_t N, uint32_t K, float dt, float E, float G, float b_r, float pA, float mu,
+               float mur, float mP, float rP) {
+  const size_t S = (size_t)N * K, N1 = N, N2 = 2 * (size_t)N;
+  for (size_t s = blockIdx.x * (size_t)blockDim.x + threadIdx.x; s < S;
+       s += (size_t)blockDim.x * gridDim.x) {
+    const uint32_t i = (uint32_t)(s / K), j = IL[s];
+    if (j == kSentinel) {
+      if (MODE != 2) slot_zero(a, s);
+      continue;
+    }
+    const uint8_t type = ILtype[s];
+    // dd_Calculate_rijm_nij (:20-34)
+    const float rx = R[j] - R[i], ry = R[j + N1] - R[i + N1], rz = R[j + N2] - R[i + N2];
+    const float rm = norm3(rx, ry, rz);
+    const float inv = 1.0f / rm;
+    const float nx = rx * inv, ny = ry * inv, nz = rz * inv;
+    if (type == 1 && MODE != 1) {
+      const float il = iL[s];
+      // dd_Calculate_fijn_bpm (:73-79)
+      const float cef = b_r * b_r * kPiF * E * (rm - 1.0f / il) * il;
+      st3(a.Fijn, s, nx * cef, ny * cef, nz * cef);
+      // dd_Calculate_M123_bpm (:36-71)
+      const f3 p = ld3(a.rij, s);
+      const float tx = p.y * nz - p.z * ny, ty = p.z * nx - p.x * nz, tz = p.x * ny - p.y * nx;
+      const float c2 = ((W[j] + W[i]) * nx + (W[j + N1] + W[i + N1]) * ny +
+                        (W[j + N2] + W[i + N2]) * nz) *
+                       dt * 0.5f;
+      const float phx = nx * c2, phy = ny * c2, phz = nz * c2;
+      st3(a.rij, s, nx, ny, nz);
+      const float M1x = 1.0f + tz * phz + ty * phy, M1y = phz - tz - ty * phx,
+                  M1z = -phy - tz * phx + ty;
+      const float M2x = tz - phz - tx * phy, M2y = tz * phz + 1.0f + tx * phx,
+                  M2z = -tz * phy + phx - tx;
+      const float M3x = -ty - tx * phz + phy, M3y = -ty * phz + tx - phx,
+                  M3z = ty * phy + tx * phx + 1.0f;
+      // dd_Calculate_mijn_bpm (:112-127)
+      const float dwx = W[j] - W[i], dwy = W[j + N1] - W[i + N1], dwz = W[j + N2] - W[i + N2];
+      const float c1 = nx * dwx + ny * dwy + nz * dwz;
+      float k2 = dt * kPiF * b_r * b_r * b_r * b_r * 0.5f * G * il;
+      f3 m = ld3(a.Mijn, s);
+      st3(a.Mijn, s, M1x * m.x + M1y * m.y + M1z * m.z + nx * c1 * k2,
+          M2x * m.x + M2y * m.y + M2z * m.z + ny * c1 * k2,
+          M3x * m.x + M3y * m.y + M3z * m.z + nz * c1 * k2);
+      // dd_Calculate_mijt_bpm (:129-145)
+      k2 = dt * kPiF * b_r * b_r * b_r * b_r * 0.25f * E * il;
+      m = ld3(a.Mijt, s);
+      st3(a.Mijt, s, M1x * m.x + M1y * m.y + M1z * m.z + (dwx - nx * c1) * k2,
+          M2x * m.x + M2y * m.y + M2z * m.z + (dwy - ny * c1) * k2,
+          M3x * m.x + M3y * m.y + M3z * m.z + (dwz - nz * c1) * k2);
+      // dd_Calculate_fijt_mijunsym_bpm (:147-187)
+      const float hx = 0.5f * (W[j] + W[i]), hy = 0.5f * (W[j + N1] + W[i + N1]),
+                  hz = 0.5f * (W[j + N2] + W[i + N2]);
+      const float vx = V[j] - V[i] - hy * rz + hz * ry;
+      const float vy = V[j + N1] - V[i + N1] - hz * rx + hx * rz;
+      const float vz = V[j + N2] - V[i + N2] - hx * ry + hy * rx;
+      const float cv = nx * vx + ny * vy + nz * vz;
+      const f3 o0 = ld3(a.Oijt, s);
+      const float ox = (M1x * o0.x + M1y * o0.y + M1z * o0.z) + (vx - nx * cv) * dt;
+      const float oy = (M2x * o0.x + M2y * o0.y + M2z * o0.z) + (vy - ny * cv) * dt;
+      const float oz = (M3x * o0.x + M3y * o0.y + M3z * o0.z) + (vz - nz * cv) * dt;
+      const float ct = G * b_r * b_r * kPiF * il;
+      st3(a.Fijt, s, ox * ct, oy * ct, oz * ct);
+      st3(a.Oijt, s, ox, oy, oz);
+      st3(a.Mijadd, s, 0.5f * (ry * oz - rz * oy) * ct, 0.5f * (rz * ox - rx * oz) * ct,
+          0.5f * (rx * oy - ry * ox) * ct);
+    } else if (type == 0 && MODE == 1) {
+      if (!(rm < 2.0f * rP)) {
+        slot_zero(a, s);
+        continue;
+      }
+      // dd_Calculate_fijn_ppHM (:189-243)
+      const float q = sqrtf(rP * fabsf(rP - 0.5f * rm));
+      const float Kn = 2.0f * E * q, Kt = 8.0f * G * q;
+      const float vx =
+          V[j] - V[i] + rP * (ny * (W[j + N2] + W[i + N2]) - nz * (W[j + N1] + W[i + N1]));
+      const float vy =
+          V[j + N1] - V[i + N1] + rP * (nz * (W[j] + W[i]) - nx * (W[j + N2] + W[i + N2]));
+      const float vz =
+          V[j + N2] - V[i + N2] + rP * (nx * (W[j + N1] + W[i + N1]) - ny * (W[j] + W[i]));
+      const float vn = nx * vx + ny * vy + nz * vz;
+      const float fn = (rm - 2.0f * rP) * Kn * kTwoThirdsF;
+      const float dmp = -1.8257f * pA * vn * sqrtf(Kn * 0.5f * mP);
+      st3(a.Fijn, s, nx * (fn + dmp), ny * (fn + dmp), nz * (fn + dmp));
+      const float vtx = vx - nx * vn, vty = vy - ny * vn, vtz = vz - nz * vn;
+      // dd_Calculate_fijtmijt_ppHM (:245-304)
+      const f3 O = ld3(a.Oijt, s);
+      float c1 = nx * O.x + ny * O.y + nz * O.z;
+      float ox = O.x - nx * c1, oy = O.y - ny * c1, oz = O.z - nz * c1;
+      c1 = ox * ox + oy * oy + oz * oz;
+      if ((double)c1 > 1e-18) {
+        c1 = 1.0f / c1;
+        c1 = c1 * (O.x * O.x + O.y * O.y + O.z * O.z);
+        c1 = sqrtf(c1);
+        ox = ox * c1;
+        oy = oy * c1;
+        oz = oz * c1;
+      }
+      ox = ox + vtx * dt;
+      oy = oy + vty * dt;
+      oz = oz + vtz * dt;
+      float fx = ox * Kt, fy = oy * Kt, fz = oz * Kt;
+      const float ftm = fx * fx + fy * fy + fz * fz;
+      if (ftm > mu * mu * fn * fn) {
+        c1 = mu * fabsf(fn) * rsqrt_rn(ftm);
+        fx = fx * c1;
+        fy = fy * c1;
+        fz = fz * c1;
+        c1 = 1.0f / Kt;
+        ox = fx * c1;
+        oy = fy * c1;
+        oz = fz * c1;
+      } else {
+        c1 = -1.8257f * pA * sqrtf(Kt * 0.5f * mP);
+        fx = fx + vtx * c1;
+        fy = fy + vty * c1;
+        fz = fz + vtz * c1;
+      }
+      st3(a.Oijt, s, ox, oy, oz);
+      st3(a.Fijt, s, fx, fy, fz);
+      st3(a.Mijt, s, (ny * fz - nz * fy) * rP, (nz * fx - nx * fz) * rP, (nx * fy - ny * fx) * rP);
+      st3(a.Mijn, s, 0.f, 0.f, 0.f);
+      // dd_Calculate_mijroll_ppHM (:306-326)
+      const float wx = W[i], wy = W[i + N1], wz = W[i + N2];
+      const float wim = wx * wx + wy * wy + wz * wz;
+      if ((double)wim > 1e-18) {
+        const float cr = -mur * fabsf(fn) * rsqrt_rn(wim);
+        st3(a.Mijadd, s, wx * cr, wy * cr, wz * cr);
+      } else {
+        st3(a.Mijadd, s, 0.f, 0.f, 0.f);
+      }
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -411,6 +572,33 @@ int rfd_forces_contact(rfd_ctx *ctx, const float *R, const float *V, const float
   k_forces_contact<<<per * kXcds, kSlotThreads, 0, ctx->stream>>>(
       R, V, W, IL, ILtype, Oijt, F, M, N, K, dt, Estar, Gstar, pA, mu, mur, m, r, blocks, per);
   RFD_LAUNCHED(ctx, "rfd_forces_contact");
+  return RFD_OK;
+}
+
+int rfd_forces_slots(rfd_ctx *ctx, int mode, const float *R, const float *V, const float *W,
+                     const uint32_t *IL, const uint8_t *ILtype, const float *iL, float *rij,
+                     float *Oijt, float *Fijn, float *Fijt, float *Mijn, float *Mijt,
+                     float *Mijadd, uint32_t N, uint32_t K, float dt, float E, float G,
+                     float b_r, float pA, float mu, float mur, float mP, float rP) {
+  if (!ctx) return RFD_E_ARG;
+  RFD_CHECK_ARG(ctx, R && V && W && IL && ILtype && iL && rij && Oijt && Fijn && Fijt && Mijn &&
+                         Mijt && Mijadd && N > 0 && K > 0 &&
+                         (uint64_t)N * K < 0xFFFFFFFFull,
+                "rfd_forces_slots: bad arguments");
+  RFD_CHECK_ARG(ctx, mode >= 0 && mode <= 2, "rfd_forces_slots: mode %d not 0, 1 or 2", mode);
+  const SlotArrays a{rij, Oijt, Fijn, Fijt, Mijn, Mijt, Mijadd};
+  const unsigned nb = rfd_blocks((size_t)N * K, 256, 65536);
+  hipStream_t q = ctx->stream;
+  if (mode == 0)
+    k_forces_slots<0><<<nb, 256, 0, q>>>(R, V, W, IL, ILtype, iL, a, N, K, dt, E, G, b_r, pA, mu,
+                                          mur, mP, rP);
+  else if (mode == 1)
+    k_forces_slots<1><<<nb, 256, 0, q>>>(R, V, W, IL, ILtype, iL, a, N, K, dt, E, G, b_r, pA, mu,
+                                          mur, mP, rP);
+  else
+    k_forces_slots<2><<<nb, 256, 0, q>>>(R, V, W, IL, ILtype, iL, a, N, K, dt, E, G, b_r, pA, mu,
+                                          mur, mP, rP);
+  RFD_LAUNCHED(ctx, "rfd_forces_slots");
   return RFD_OK;
 }
 

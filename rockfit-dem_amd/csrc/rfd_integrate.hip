@@ -143,6 +143,53 @@ k_platen_uniaxial(float *__restrict__ R, float *__restrict__ V, float *__restric
   }
 }
 
+// d_UniaxialCompression_simple (md_uniaxialcompression_simple.cu:29-81), the
+// superseded penalty platens: a particle past z = c.z +- Hd2 is pushed back
+// by the stiffness C, with Coulomb friction mu against its in-plane velocity
+// (dd_FrictionForce :19-28); the reactions are summed in double
+__device__ __forceinline__ void penalty_friction(const float *__restrict__ V,
+                                                 float *__restrict__ F, size_t N, uint32_t i,
+                                                 float fz, float mu) {
+  const float vv = V[i] * V[i] + V[i + N] * V[i + N];
+  if ((double)vv > 1e-10) {
+    const float iv = rsqrt_rn(vv);
+    F[i] -= fabsf(fz) * mu * V[i] * iv;
+    F[i + N] -= fabsf(fz) * mu * V[i + N] * iv;
+  }
+}
+
+__global__ void __launch_bounds__(kRedThreads)
+k_platen_uniaxial_penalty(const float *__restrict__ R, const float *__restrict__ V,
+                          float *__restrict__ F, uint32_t N, float cx, float cy, float cz,
+                          float RR, float Hd2, float C, float mu, float Zcut,
+                          double *__restrict__ part) {
+  __shared__ double s_part[(kRedThreads / 64) * 2];
+  const size_t N1 = N, N2 = 2 * (size_t)N;
+  double v[2] = {0.0, 0.0};  // bottom, top
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
+    const float rx = R[i] - cx, ry = R[i + N1] - cy, rz = R[i + N2] - cz;
+    if (rx * rx + ry * ry < RR && R[i + N2] < Zcut) {
+      if (rz - Hd2 > 0) {
+        const float fz = (Hd2 - rz) * C;
+        penalty_friction(V, F, N, i, fz, mu);
+        F[i + N2] += fz;
+        v[1] -= fz;
+      }
+      if (rz + Hd2 < 0) {
+        const float fz = (Hd2 + rz) * C;
+        penalty_friction(V, F, N, i, fz, mu);
+        F[i + N2] -= fz;
+        v[0] += fz;
+      }
+    }
+  }
+  block_sum<2>(v, s_part);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x * 2] = v[0];
+    part[blockIdx.x * 2 + 1] = v[1];
+  }
+}
+
 // d_braziltest_simple2 (md_braziltest_simple2.cu:19-92)
 __global__ void __launch_bounds__(kRedThreads)
 k_platen_brazil(float *__restrict__ R, float *__restrict__ V, float *__restrict__ F,
@@ -346,6 +393,21 @@ int rfd_platen_uniaxial(rfd_ctx *ctx, float *R, float *V, float *F, uint32_t N, 
   k_finish<2><<<1, kRedThreads, 0, ctx->stream>>>(ctx->d_red, nb, 0u,
                                                    ctx->d_red + (size_t)kRedBlocks * 4);
   RFD_LAUNCHED(ctx, "rfd_platen_uniaxial");
+  return fetch(ctx, 2, out_sums);
+}
+
+int rfd_platen_uniaxial_penalty(rfd_ctx *ctx, const float *R, const float *V, float *F,
+                                uint32_t N, float cx, float cy, float cz, float RR, float Hd2,
+                                float C, float mu, float Zcut, double out_sums[2]) {
+  if (!ctx) return RFD_E_ARG;
+  RFD_CHECK_ARG(ctx, R && V && F && N > 0 && out_sums,
+                "rfd_platen_uniaxial_penalty: bad arguments");
+  const unsigned nb = red_blocks(N);
+  k_platen_uniaxial_penalty<<<nb, kRedThreads, 0, ctx->stream>>>(R, V, F, N, cx, cy, cz, RR, Hd2,
+                                                                  C, mu, Zcut, ctx->d_red);
+  k_finish<2><<<1, kRedThreads, 0, ctx->stream>>>(ctx->d_red, nb, 0u,
+                                                   ctx->d_red + (size_t)kRedBlocks * 4);
+  RFD_LAUNCHED(ctx, "rfd_platen_uniaxial_penalty");
   return fetch(ctx, 2, out_sums);
 }
 

@@ -41,6 +41,7 @@ SIGNATURES: dict[str, list] = {
                        _u32, _f32, _f32, _f32, _f32, _f32, _f32],
     "rfd_forces_contact": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _u32, _f32, _f32,
                            _f32, _f32, _f32, _f32, _f32, _f32],
+    "rfd_forces_slots": [_vp, C.c_int] + [_vp] * 13 + [_u32, _u32] + [_f32] * 9,
     "rfd_sumup_forces": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _u32],
     "rfd_check_break": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _f32, _f32, _f32],
     "rfd_increments": [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _f32, _f32, _f32],
@@ -48,6 +49,7 @@ SIGNATURES: dict[str, list] = {
                             _f32, _f32, _d2],
     "rfd_platen_brazil": [_vp, _vp, _vp, _vp, _u32, _f32, _f32, _f32, _f32, _f32, _f32, _f32,
                           _f32, _f32, _d2],
+    "rfd_platen_uniaxial_penalty": [_vp, _vp, _vp, _vp, _u32] + [_f32] * 8 + [_d2],
     "rfd_cut_restriction": [_vp, _vp, _vp, _vp, _u32, _f3, _f3, _f3],
     "rfd_fire_increments": [_vp, _vp, _vp, _vp, _u32, _f32, _f32, _f32],
     "rfd_fire_halfstep": [_vp, _vp, _vp, _u32, _f32],

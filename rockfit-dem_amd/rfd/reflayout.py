@@ -149,6 +149,18 @@ class RefLayout:
                       _p(self.ILtype), _p(self.slot["Oijt"]), _p(self.F), _p(self.M), self.N,
                       self.K, m.dt, m.hm_E, m.hm_G, m.p_A, m.p_mu, m.p_mur, m.p_m, m.p_r)
 
+    def forces_slots(self, mode: int, m: Material):
+        """The superseded per-slot force path (md_calculateforcesdem.cu): every
+        slot's law to the slot arrays; mode 0 d_CalculateForcesDEM (bonds, empty
+        slots zeroed), 1 d_CalculateForcesDEM_1 (contacts, Hertz-Mindlin moduli),
+        2 d_CalculateForcesDEM_2 (bonds only).  F and M follow from
+        sumup_forces."""
+        E, G = (m.hm_E, m.hm_G) if mode == 1 else (m.E, m.G)
+        self.ctx.call("rfd_forces_slots", int(mode), _p(self.R), _p(self.V), _p(self.W),
+                      _p(self.IL), _p(self.ILtype), _p(self.iL),
+                      *self._slots("rij", "Oijt", "Fijn", "Fijt", "Mijn", "Mijt", "Mijadd"),
+                      self.N, self.K, m.dt, E, G, m.b_r, m.p_A, m.p_mu, m.p_mur, m.p_m, m.p_r)
+
     def sumup_forces(self):
         """Stand-alone sum-up (d_SumUpForcesDEM): F, M from the slot loads."""
         self.ctx.call("rfd_sumup_forces", *self._slots("Fijn", "Fijt", "Mijn", "Mijt", "Mijadd"),
@@ -169,6 +181,15 @@ class RefLayout:
         self.ctx.call("rfd_platen_uniaxial", _p(self.R), _p(self.V), _p(self.F), self.N,
                       cmp.center_d[0], cmp.center_d[1], cmp.RR, cmp.top, cmp.bottom, cmp.top_r,
                       cmp.bottom_r, cmp.V, zcut, out)
+        return out[0], out[1]
+
+    def platen_uniaxial_penalty(self, center, RR: float, Hd2: float, stiffness: float,
+                                mu: float, zcut: float) -> tuple[float, float]:
+        """The superseded penalty platens (d_UniaxialCompression_simple): returns
+        the (bottom, top) reaction sums."""
+        out = (C.c_double * 2)()
+        self.ctx.call("rfd_platen_uniaxial_penalty", _p(self.R), _p(self.V), _p(self.F), self.N,
+                      center[0], center[1], center[2], RR, Hd2, stiffness, mu, zcut, out)
         return out[0], out[1]
 
     def platen_brazil(self, cmp: Compression, zcut: float) -> tuple[float, float]:
