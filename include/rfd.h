@@ -490,7 +490,10 @@ int rfd_system_get_tiles(rfd_system *sys, int *force_tile, int *nbr_block, int *
  * specimen in the FP32 physical mode and the single-evaluation force pass,
  * rfd_system_step runs the platens, integration and cut box of each particle
  * inside the force pass, pairs of steps alternating between two particle
- * buffers.  On by default; the results are the same bits with it off.
+ * buffers.  On by default; the results are the same bits with it off.  It
+ * also moves the FIRE measure of rfd_system_fire_load_step (platens at rest,
+ * power, norms, reactions) into the epilogue of the quasi-static force pass;
+ * there only the order of the FP64 partial sums differs.
  * rfd_system_get_fusion reports whether steps currently run fused. */
 int rfd_system_set_fusion(rfd_system *sys, int on);
 int rfd_system_get_fusion(rfd_system *sys, int *fused);

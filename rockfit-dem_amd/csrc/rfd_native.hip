@@ -670,13 +670,27 @@ struct AdvDev {
 // precomputed records, so no lane waits on a load another lane needs.
 // Per-particle sums in row order, then the contact rows: the same bits as
 // k_nat_forces.
+// The FIRE measure fused into a quasi-static force pass: each tile's summed
+// loads meet the platens at rest without friction (on copies of R and the
+// FIRE velocity: the clamp of positions and velocities is applied by the
+// update that follows, with the same decisions), and each wave writes its
+// partial power, norms, maxima and platen sums (kFireVals values) to
+// part[(tile * waves per tile + wave) * kFireVals].
+struct MeasDev {
+  const rfd_loading *ctl;
+  const float4 *Vf;
+  double *part;
+  int mode;
+};
+constexpr int kMeasVals = 9;  // P, |F|^2, |V|^2, |M|^2, |W|^2, max|F|^2, max(|M|/r)^2, sb, st
+
 // One tile of the pass below: `cur` is the index of the bond-state buffer
 // holding the latest state.  Every thread of the workgroup calls it; threads
 // return from it only after its last workgroup barrier.
-template <int FP, int FT, bool ADV>
+template <int FP, int FT, bool ADV, bool MEAS = false>
 __device__ __forceinline__ void forces1_tile(const SysDev &s, const Mat &mt, const WorkDev &w,
                                              uint32_t tile, uint32_t cap, const AdvDev &a,
-                                             int cur) {
+                                             int cur, const MeasDev *fm = nullptr) {
   static_assert(FP < FT && FP <= 128, "tile shape");
   static_assert(2 * FP <= FT, "one sum task per thread");
   // per-half loads: F (3 x cap floats) then M (3 x cap), cap >= 2 FP
@@ -789,8 +803,8 @@ __device__ __forceinline__ void forces1_tile(const SysDev &s, const Mat &mt, con
     sAcc[tg][2][tp] = z;
   }
   __syncthreads();
-  // the fused update keeps whole waves for its wave-wide sums
-  if (tid >= (ADV ? (FP + 63u) & ~63u : np)) return;
+  // the fused update and measure keep whole waves for their wave-wide sums
+  if (tid >= ((ADV || MEAS) ? (FP + 63u) & ~63u : np)) return;
   const bool act = tid < np;
   const uint32_t i = p0 + tid, N = s.N;
   v3 F = kZero, M = kZero;
@@ -827,6 +841,41 @@ __device__ __forceinline__ void forces1_tile(const SysDev &s, const Mat &mt, con
       }
     }
     M = Mb + Mc;
+  }
+  if (MEAS) {
+    // k_fire_measure's arithmetic for this particle; the sums in wave order
+    double v[kMeasVals] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (act) {
+      rfd_loading L = *fm->ctl;
+      L.v = 0.0;
+      v3 Vf = xyz(fm->Vf[i]);
+      P4 R = me.P;
+      if (fm->mode == 1)
+        platen<1>(L, R, Vf, F, v[7], v[8], false);
+      else if (fm->mode == 2)
+        platen<2>(L, R, Vf, F, v[7], v[8], false);
+      s.F[i] = f4(F);
+      s.M[i] = f4(M);
+      const v3 W = me.W;
+      v[0] = (double)dot(F, Vf) + (double)dot(M, W);
+      const double f2 = dot(F, F), m2 = dot(M, M);
+      v[1] = f2;
+      v[2] = (double)dot(Vf, Vf);
+      v[3] = m2;
+      v[4] = (double)dot(W, W);
+      v[5] = f2;
+      v[6] = m2 / (R.r * R.r);
+    }
+#pragma unroll
+    for (int q = 0; q < kMeasVals; ++q)
+      v[q] = (q == 5 || q == 6) ? wave_max(v[q]) : wave_sum(v[q]);
+    if ((tid & 63) == 0) {
+      constexpr uint32_t wpt = (FP + 63) / 64;
+      double *dst = fm->part + ((size_t)tile * wpt + (tid >> 6)) * kMeasVals;
+#pragma unroll
+      for (int q = 0; q < kMeasVals; ++q) dst[q] = v[q];
+    }
+    return;
   }
   if (!ADV) {
     s.F[i] = f4(F);
@@ -878,6 +927,18 @@ k_nat_forces1(SysDev s, Mat mt, WorkDev w, uint32_t ntiles, uint32_t per, uint32
   if (s.done && *s.done) return;
   if (s.dtp) mt.dt = *s.dtp;
   forces1_tile<FP, FT, ADV>(s, mt, w, tile, cap, a, *s.curp);
+}
+
+// a quasi-static force pass with the FIRE measure in its epilogue
+template <int FP, int FT>
+__global__ void __launch_bounds__(FT) __attribute__((amdgpu_waves_per_eu(5, 8)))
+k_nat_forces1_fire(SysDev s, Mat mt, WorkDev w, uint32_t ntiles, uint32_t per, uint32_t cap,
+                   MeasDev fm) {
+  const uint32_t tile = xcd_tile(per);
+  if (tile >= ntiles) return;
+  if (s.done && *s.done) return;
+  if (s.dtp) mt.dt = *s.dtp;
+  forces1_tile<FP, FT, false, true>(s, mt, w, tile, cap, AdvDev{}, *s.curp, &fm);
 }
 
 // work records of the single-evaluation pass: per tile, count the in-tile
@@ -1916,18 +1977,65 @@ __global__ void __launch_bounds__(256) k_fire_control(FireDev *fire, const doubl
   *fire = f;
 }
 
-// velocity mixing (or back-off), semi-implicit Euler kick and move
+static_assert(kMeasVals == kFireVals, "measure partials");
+
+// second stage of the fused measure's reduction: workgroup b adds (or takes
+// the maxima of) a contiguous run of the per-wave partials, in a fixed order,
+// into the partial pair slot b that k_fire_control reduces
+__global__ void __launch_bounds__(256) k_fire_reduce(const FireDev *fire,
+                                                     const double *__restrict__ tpart,
+                                                     uint32_t nparts, double *__restrict__ part) {
+  __shared__ double s_part[4 * kFireVals];
+  if (fire->done) return;
+  const uint32_t chunk = (nparts + gridDim.x - 1) / gridDim.x;
+  const uint32_t b0 = blockIdx.x * chunk, b1 = min(b0 + chunk, nparts);
+  double v[kFireVals] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint32_t b = b0 + threadIdx.x; b < b1; b += blockDim.x)
+    for (int q = 0; q < kFireVals; ++q) {
+      const double x = tpart[(size_t)b * kFireVals + q];
+      v[q] = (q == 5 || q == 6) ? fmax(v[q], x) : v[q] + x;
+    }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < kFireVals; ++q) v[q] = (q == 5 || q == 6) ? wave_max(v[q]) : wave_sum(v[q]);
+  if (lane == 0)
+    for (int q = 0; q < kFireVals; ++q) s_part[wid * kFireVals + q] = v[q];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < kFireVals; ++q) {
+      double a = 0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
+        a = (q == 5 || q == 6) ? fmax(a, s_part[w * kFireVals + q]) : a + s_part[w * kFireVals + q];
+      part[blockIdx.x * kFireVals + q] = a;
+    }
+  }
+}
+
+// velocity mixing (or back-off), semi-implicit Euler kick and move.  The
+// platen clamp of positions and FIRE velocities (platens at rest) runs first:
+// k_fire_measure has applied it already (it is idempotent), the measure fused
+// into the force pass has not.
 __global__ void __launch_bounds__(256) k_fire_update(FireParticles p) {
   const FireDev f = *p.fire;
   if (f.done) return;
+  rfd_loading L = *p.ctl;
+  L.v = 0.0;
   const float a1 = (float)(1.0 - f.alpha), cV = (float)f.cV, cW = (float)f.cW;
   const float scale = (float)(f.dt / p.dt_phys);  // inv2 holds dt_phys/m, dt_phys/I
   const float dtf = (float)f.dt;
   const double inv_dt = 1.0 / (double)dtf;  // the force pass multiplies by dt_f
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.N; i += grid_stride()) {
-    const P4 R0 = p.P[i];
-    P4 R = R0;
+    P4 R0 = p.P[i];
     v3 V = xyz(p.Vf[i]), W = xyz(p.W[i]);
+    if (L.mode) {
+      v3 Fd = kZero;
+      double d0 = 0, d1 = 0;
+      if (L.mode == 1)
+        platen<1>(L, R0, V, Fd, d0, d1, false);
+      else
+        platen<2>(L, R0, V, Fd, d0, d1, false);
+    }
+    P4 R = R0;
     const v3 F = xyz(p.F[i]), M = xyz(p.M[i]);
     if (f.backoff) {
       const double h = 0.5 * f.dt_last;
@@ -2670,6 +2778,8 @@ struct rfd_system {
   FireDev *fire = nullptr;
   float *fire_dt = nullptr;  // &fire->dt_f, handed to the force pass
   double *fire_part = nullptr;
+  double *fire_tpart = nullptr;  // per-wave partials of the measure fused into the force pass
+  uint32_t fire_tpart_n = 0;
   P4 *P0 = nullptr;          // positions at the start of the load step
   float4 *Vf = nullptr;      // FIRE velocity
   uint32_t *d_nbreak = nullptr;  // bonds broken by the last break-check pass
@@ -3411,25 +3521,96 @@ void fire_rest(rfd_system *s, hipStream_t q, bool fire_velocity) {
   }
 }
 
+// the FIRE measure runs in the epilogue of the quasi-static force pass (FP32
+// physical mode, one specimen, single-evaluation pass for the current tiles)
+// unless fusion is switched off
+bool can_measure_fused(const rfd_system *s) {
+  return s->fuse && !s->fp64 && !s->S && !s->p.refmode && !s->once_off &&
+         s->work_fp == s->force_fp;
+}
+
+template <int FP, int FT>
+void launch_forces_fire(rfd_system *s, const SysDev &d, const Mat &m, const MeasDev &fm) {
+  const uint32_t nb = (s->N + FP - 1) / FP;
+  const uint32_t per = (nb + kXcd - 1) / kXcd;
+  WorkDev w{s->wrec, s->wofs, s->wnin};
+  const int q = FP == 32 ? 0 : (FP == 64 ? 1 : 2);
+  const uint32_t cap = std::max((s->tile_halves[q] + 31u) & ~31u, 2u * FP);
+  k_nat_forces1_fire<FP, FT><<<per * kXcd, FT, sizeof(float) * 6 * cap, s->ctx->stream>>>(
+      d, m, w, nb, per, cap, fm);
+}
+
+// first-stage workgroups of the fused measure's reduction (the partial pairs
+// k_fire_control then reduces)
+uint32_t fire_reduce_blocks(const rfd_system *s) { return std::min(s->nblocks_pass, 64u); }
+
+// quasi-static force pass with the measure fused in, then the two-stage
+// reduction into fire_part (the FIRE control follows)
+int forces_fire_measured(rfd_system *s) {
+  const uint32_t wpt = s->force_fp == 128 ? 2u : 1u;
+  const uint32_t nparts = (s->N + s->force_fp - 1) / s->force_fp * wpt;
+  if (s->fire_tpart_n < nparts) {
+    int st = dalloc(s->ctx, &s->fire_tpart, (size_t)kFireVals * nparts);
+    if (st) return st;
+    s->fire_tpart_n = nparts;
+  }
+  SysDev d = make_dev(s);
+  d.dtp = s->fire_dt;
+  d.nbreak = s->d_nbreak;
+  d.done = (const int *)((const char *)s->fire + offsetof(FireDev, done));
+  Mat m = make_mat(s->p);
+  m.pA = 0.f;
+  m.mur = 0.f;
+  m.nobreak = 1;
+  const MeasDev fm{s->ctl, s->Vf, s->fire_tpart, s->mode};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (s->timing && s->ev_used + 2 <= s->ev.size()) {
+    e0 = s->ev[s->ev_used];
+    e1 = s->ev[s->ev_used + 1];
+    s->ev_used += 2;
+    RFD_HIP(s->ctx, hipEventRecord(e0, s->ctx->stream));
+  }
+  if (s->force_fp == 32)
+    launch_forces_fire<32, 128>(s, d, m, fm);
+  else if (s->force_fp == 128)
+    launch_forces_fire<128, 256>(s, d, m, fm);
+  else
+    launch_forces_fire<kFP, kFT>(s, d, m, fm);
+  RFD_HIP(s->ctx, hipGetLastError());
+  if (e1) RFD_HIP(s->ctx, hipEventRecord(e1, s->ctx->stream));
+  k_fire_reduce<<<fire_reduce_blocks(s), 256, 0, s->ctx->stream>>>(s->fire, s->fire_tpart,
+                                                                   nparts, s->fire_part);
+  RFD_HIP(s->ctx, hipGetLastError());
+  return RFD_OK;
+}
+
 // one FIRE iteration: force pass at the current configuration, constraint +
 // reductions, control, update
 int fire_iteration(rfd_system *s) {
   int *flip = nullptr;
+  uint32_t nparts = s->nblocks_pass;  // partial pairs for the control
+  hipStream_t q = s->ctx->stream;
   if (s->forces_current) {
     s->forces_current = false;  // the load step's first pass already ran
+    fire_measure(s, q);
   } else {
     if (!s->rows_valid || s->step_count % s->p.rebuild_every == 0) {
       int st = rebuild_rows(s);
       if (st) return st;
     }
     // the FIRE control flips the bond-state index after this pass
-    int st = forces(s, s->fire_dt, false, true, nullptr, false);
-    if (st) return st;
+    if (can_measure_fused(s)) {
+      int st = forces_fire_measured(s);
+      if (st) return st;
+      nparts = fire_reduce_blocks(s);
+    } else {
+      int st = forces(s, s->fire_dt, false, true, nullptr, false);
+      if (st) return st;
+      fire_measure(s, q);
+    }
     flip = s->d_cur;
   }
-  hipStream_t q = s->ctx->stream;
-  fire_measure(s, q);
-  k_fire_control<<<1, 256, 0, q>>>(s->fire, s->fire_part, (int)s->nblocks_pass, s->ctl, flip);
+  k_fire_control<<<1, 256, 0, q>>>(s->fire, s->fire_part, (int)nparts, s->ctl, flip);
   if (s->fp64)
     k_fire64_update<<<s->nblocks_pass, 256, 0, q>>>(fire_particles64(s));
   else
@@ -3653,6 +3834,7 @@ int rfd_system_destroy(rfd_system *s) {
   dfree(s->scan_off);
   dfree(s->fire);
   dfree(s->fire_part);
+  dfree(s->fire_tpart);
   dfree(s->P0);
   dfree(s->Th);
   for (int q = 0; q < 2; ++q) {

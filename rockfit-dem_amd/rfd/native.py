@@ -486,7 +486,9 @@ class NativeSystem:
 
     def set_fusion(self, on: bool = True):
         """Fuse the particle update into the force pass of explicit steps
-        (on by default; the same bits either way)."""
+        (on by default; the same bits either way) and the FIRE measure into
+        the quasi-static force pass (there only the order of the FP64 partial
+        sums differs)."""
         self._call("rfd_system_set_fusion", self.h, int(on))
 
     def fused(self) -> bool:
