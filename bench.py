@@ -278,7 +278,7 @@ def bench_c2(steps: int = 500) -> dict:
     return out
 
 
-def bench_warmstart(D: float = 61.0, H: float = 122.0, steps: int = 6) -> dict:
+def bench_warmstart(D: float = 61.0, H: float = 122.0, steps: int = 12) -> dict:
     """Quasi-static load steps (strain increment -1e-5, FIRE to |F|max <
     1e-6 F_ref) on a ~500k-particle cylinder from three starts: plain GPU
     FIRE (the platen move alone), the skip path (the last step's increment
@@ -341,6 +341,7 @@ def bench_warmstart(D: float = 61.0, H: float = 122.0, steps: int = 6) -> dict:
         out[name] = {"particles": c.sys.N, "bonds": c.sys.bonds.n,
                      "load_steps_per_s": len(timed) / sec, "s_per_step": sec / len(timed),
                      "fire_iterations_per_step": sum(r.iterations for r in timed) / len(timed),
+                     "fire_iterations": [int(r.iterations) for r in timed],
                      "all_converged": all(bool(r.converged) for r in timed),
                      "max_final_residual": max(r.fmax for r in timed),
                      "warm_starts_taken": sum(r.warm_used for r in timed)}
@@ -410,6 +411,7 @@ def bench_warmstart_damage(pre: int = 18, steps: int = 26) -> dict:
         sec = sum(r.seconds for r in res)
         out[name] = {"s_per_step": sec / len(res),
                      "fire_iterations_per_step": sum(r.iterations for r in res) / len(res),
+                     "fire_iterations": [int(r.iterations) for r in res],
                      "bonds_broken": sum(r.breaks for r in res),
                      "all_converged": all(bool(r.converged) for r in res),
                      "warm_starts_taken": sum(r.warm_used for r in res)}

@@ -2016,6 +2016,10 @@ __global__ void __launch_bounds__(256) k_fire_reduce(const FireDev *fire,
 // k_fire_measure has applied it already (it is idempotent), the measure fused
 // into the force pass has not.
 __global__ void __launch_bounds__(256) k_fire_update(FireParticles p) {
+  // no contraction here: the rounding of the move does not depend on how the
+  // compiler schedules this kernel (warm-started relaxations near equilibrium
+  // take visibly different iteration counts from last-bit differences)
+#pragma clang fp contract(off)
   const FireDev f = *p.fire;
   if (f.done) return;
   rfd_loading L = *p.ctl;
