@@ -103,11 +103,14 @@ def stream_copy(reps: int = 5) -> dict:
     return out
 
 
-def force_alg_bytes(kernel: str, bonds: int, particles: int) -> int:
+def force_alg_bytes(kernel: str, bonds: int, particles: int, contacts: int = 0) -> int:
     """Algorithmic bytes of one native force launch: 128 B per bond and 72 B
     per particle; the fused particle update adds 1/m, 1/I (8 B) read and the
-    next R, V, W (48 B) written per particle."""
-    return 128 * bonds + (72 + (56 if kernel.endswith("<fused>") else 0)) * particles
+    next R, V, W (48 B) written per particle; each contact-row entry (a
+    never-bonded pair within reach) its partner id (4 B) and spring read and
+    written (32 B)."""
+    return (128 * bonds + (72 + (56 if kernel.endswith("<fused>") else 0)) * particles +
+            36 * contacts)
 
 
 def load_traffic(kernel: str, particles: int, bonds: int):
@@ -255,7 +258,8 @@ def bench_native(args) -> dict:
                    "particles": N, "bonds": Nb, "bonds_active_end": active,
                    "bonds_broken_end": broken, "tier": "native",
                    "precision": "f32 bond/contact law; f64 positions, stretch, break check",
-                   "contact_rebuild_every": 20},
+                   "contact_rebuild_every": 20,
+                   "contact_entries_per_particle": s.contact_entries() / N},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": None,
                      "kernel": kernel, "kernel_ms": kern_ms,
@@ -685,7 +689,9 @@ def bench_packed(D: float = 100.0, H: float = 200.0, steps: int = 50, warmup: in
     kern_ms /= max(launches, 1)
     active, broken, removed = s.bond_counts()
     kernel = s.force_kernel()
+    contacts = s.contact_entries()
     alg = force_alg_bytes(kernel, Nb, N)
+    alg_c = force_alg_bytes(kernel, Nb, N, contacts)
     out = {
         "workload": f"uniaxial cylinder D{D:g}xH{H:g} mm, seeded random packing phi=0.6, "
                     "r~U[0.4,0.6] mm, bonds at gap < 0.05 r_mean",
@@ -695,6 +701,8 @@ def bench_packed(D: float = 100.0, H: float = 200.0, steps: int = 50, warmup: in
         "ms_per_step": (t2 - t1) * 1e3 / steps,
         "force_kernel": kernel, "force_ms": kern_ms,
         "force_roofline_frac": alg / (kern_ms * 1e-3) / PEAK_HBM,
+        "contact_entries_per_particle": contacts / N,
+        "force_roofline_frac_with_contacts": alg_c / (kern_ms * 1e-3) / PEAK_HBM,
     }
     assert ctx.flags() == 0, "device flags raised in the packed run"
     s.close()

@@ -386,6 +386,9 @@ int rfd_system_get_history(rfd_system *sys, double *hist, uint32_t cap, uint32_t
 int rfd_system_bond_counts(rfd_system *sys, uint32_t out[3]);
 /* number of particles and bond records */
 int rfd_system_sizes(rfd_system *sys, uint32_t *N, uint32_t *Nb);
+/* entries in the current contact rows (never-bonded pairs within reach plus
+ * skin, as last rebuilt); synchronises */
+int rfd_system_contact_entries(rfd_system *sys, uint64_t *entries);
 /* HIP-event timing of the force kernel launched by rfd_system_step /
  * rfd_system_compute_forces, recorded on the context stream: enable (also
  * clears), then read the summed kernel time in ms and the launch count */

@@ -5095,6 +5095,21 @@ int rfd_system_bond_counts(rfd_system *s, uint32_t out[3]) {
   return RFD_OK;
 }
 
+int rfd_system_contact_entries(rfd_system *s, uint64_t *entries) {
+  if (!valid(s) || !entries) return RFD_E_ARG;
+  rfd_ctx *ctx = s->ctx;
+  *entries = 0;
+  if (!s->rows_valid || !s->ccnt[s->ccur]) return RFD_OK;
+  std::vector<uint32_t> c(s->N);
+  RFD_HIP(ctx, hipMemcpyAsync(c.data(), s->ccnt[s->ccur], sizeof(uint32_t) * s->N,
+                              hipMemcpyDeviceToHost, ctx->stream));
+  RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  uint64_t n = 0;
+  for (uint32_t v : c) n += v;
+  *entries = n;
+  return RFD_OK;
+}
+
 int rfd_system_sizes(rfd_system *s, uint32_t *N, uint32_t *Nb) {
   if (!valid(s)) return RFD_E_ARG;
   if (N) *N = s->N;

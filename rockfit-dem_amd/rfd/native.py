@@ -116,6 +116,7 @@ NATIVE_SIGNATURES = {
     "rfd_system_step": [_vp, _u32],
     "rfd_system_get_history": [_vp, _vp, _u32, _u32p],
     "rfd_system_bond_counts": [_vp, _u32p],
+    "rfd_system_contact_entries": [_vp, C.POINTER(C.c_uint64)],
     "rfd_system_sizes": [_vp, _u32p, _u32p],
     "rfd_system_device_arrays": [_vp, _vp, _vp, _vp, _vp, _vp],
     "rfd_system_timing": [_vp, C.c_int],
@@ -531,6 +532,12 @@ class NativeSystem:
         out = (C.c_uint32 * 3)()
         self._call("rfd_system_bond_counts", self.h, out)
         return tuple(out)
+
+    def contact_entries(self) -> int:
+        """Entries in the current contact rows (never-bonded candidates)."""
+        n = C.c_uint64()
+        self._call("rfd_system_contact_entries", self.h, C.byref(n))
+        return n.value
 
 
 def tune_cache_save(path: str) -> None:
