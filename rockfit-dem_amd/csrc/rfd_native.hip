@@ -1625,7 +1625,7 @@ constexpr int kRowHB = kRowCB + 2;              // halo block edge
 constexpr uint32_t kRowHC = kRowHB * kRowHB * kRowHB;
 constexpr uint32_t kRowCand = 512;              // candidates held in LDS
 constexpr uint32_t kRowStage = 16;              // bonded partners held per particle
-constexpr uint32_t kRowFound = 32;              // in-reach candidates listed per particle
+constexpr uint32_t kRowFound = 16;              // in-reach candidates listed per particle
 
 __global__ void __launch_bounds__(64) k_nat_rows_cells(NbrDev s, uint32_t nblocks, uint32_t per,
                                                        uint32_t nbx, uint32_t nby) {
