@@ -1623,7 +1623,7 @@ __device__ __forceinline__ void row_entry(const NbrDev &s, uint32_t i, uint32_t 
 constexpr int kRowCB = 2;                       // home block edge (cells)
 constexpr int kRowHB = kRowCB + 2;              // halo block edge
 constexpr uint32_t kRowHC = kRowHB * kRowHB * kRowHB;
-constexpr uint32_t kRowCand = 512;              // candidates held in LDS
+constexpr uint32_t kRowCand = 384;              // candidates held in LDS
 constexpr uint32_t kRowStage = 16;              // bonded partners held per particle
 constexpr uint32_t kRowFound = 16;              // in-reach candidates listed per particle
 
