@@ -26,13 +26,21 @@ __device__ __forceinline__ void st3(float *p, size_t s, float x, float y, float 
   p[3 * s + 2] = z;
 }
 
-// d_CalculateForcesDEM_22 (md_calculateforcesdem3.cu:424-483).  One thread
-// per slot evaluates the bond law of its slot and keeps the five terms the
-// reference adds to the particle's sums in LDS; then one thread per particle
-// adds them in slot order, term by term, as the reference's row loop does
-// (same bits as a sequential walk of the row, with every slot's loads and
-// stores coalesced across the row).
+// d_CalculateForcesDEM_22 (md_calculateforcesdem3.cu:424-483).  A workgroup
+// takes the rows of P particles (P K <= 512 slots), lists their live bond
+// slots in slot order (a ballot per wave), evaluates the bond law of each
+// listed slot on its own lane and keeps the five terms the reference adds to
+// the particle's sums in LDS; then one thread per particle adds them in slot
+// order, term by term, as the reference's row loop does (same bits as a
+// sequential walk of the row).  Listing the live slots first keeps every lane
+// of the law busy: about a third of a row's slots hold bonds.
 constexpr int kSlotThreads = 256;
+constexpr uint32_t kBpmSlots = 512;  // slots of a workgroup's rows
+// particles per workgroup: as many rows as fit in kBpmSlots, at most one per
+// thread (the row sums take one thread each)
+__host__ __device__ inline uint32_t bpm_rows(uint32_t K) {
+  return min(kBpmSlots / K, (uint32_t)kSlotThreads);
+}
 
 __global__ void __launch_bounds__(kSlotThreads)
 k_forces_bpm(const float *__restrict__ R, const float *__restrict__ V,
@@ -42,124 +50,161 @@ k_forces_bpm(const float *__restrict__ R, const float *__restrict__ V,
              float *__restrict__ Mijt, float *__restrict__ F, float *__restrict__ M,
              uint32_t N, uint32_t K, float dt, float E, float G, float b_r, float scrit,
              float tcrit, uint32_t nblk, uint32_t per) {
-  // per slot: normal force, twisting and bending moments, tangential force,
-  // moment of the tangential force (3 floats each)
-  __shared__ float s_c[15][kSlotThreads];
-  __shared__ uint8_t s_live[kSlotThreads];
-  const uint32_t P = kSlotThreads / K;  // particles per workgroup
-  const uint32_t t = threadIdx.x, lp = t / K, k = t % K;
+  // per listed slot: normal force, twisting and bending moments, tangential
+  // force, moment of the tangential force (3 floats each)
+  __shared__ float s_c[15][kBpmSlots];
+  __shared__ uint16_t s_slot[kBpmSlots];  // row-local slot (lp K + k) of each listed slot
+  __shared__ uint32_t s_wave[kSlotThreads / 64];
+  const uint32_t P = bpm_rows(K);
+  const uint32_t t = threadIdx.x, lane = t & 63, wid = t >> 6;
   const uint32_t blk = xcd_block(per);
   if (blk >= nblk) return;
   const uint32_t p0 = blk * P;
-  const uint32_t i = p0 + lp;
+  const uint32_t np = min(P, N - p0);
+  const uint32_t nsl = np * K;
+  const size_t g0 = (size_t)p0 * K;
   const size_t N1 = N, N2 = 2 * (size_t)N;
   const float area_k = b_r * b_r * kPiF;          // b_r*b_r*MCf_pi
   const float j4 = kPiF * b_r * b_r * b_r * b_r;  // MCf_pi*b_r^4
-  bool live = false;
-  if (lp < P && i < N) {
-    const size_t s = (size_t)i * K + k;
+  // 1. the live bond slots, in slot order
+  uint32_t nlive = 0;
+  for (uint32_t r0 = 0; r0 < nsl; r0 += kSlotThreads) {
+    const uint32_t ls = r0 + t;
+    bool live = false;
+    if (ls < nsl) {
+      const size_t s = g0 + ls;
+      live = IL[s] != kSentinel && ILtype[s] == 1;
+    }
+    const uint64_t bal = __ballot(live);
+    if (lane == 0) s_wave[wid] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (uint32_t w = 0; w < kSlotThreads / 64; ++w) {
+      before += w < wid ? s_wave[w] : 0u;
+      all += s_wave[w];
+    }
+    if (live)
+      s_slot[nlive + before + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = (uint16_t)ls;
+    nlive += all;
+    __syncthreads();
+  }
+  // 2. the law of each listed slot
+  for (uint32_t q = t; q < nlive; q += kSlotThreads) {
+    const uint32_t ls = s_slot[q];
+    const uint32_t i = p0 + ls / K;
+    const size_t s = g0 + ls;
     const uint32_t j = IL[s];
-    live = j != kSentinel && ILtype[s] == 1;
-    if (live) {
-      const float Rix = R[i], Riy = R[i + N1], Riz = R[i + N2];
-      const float Vix = V[i], Viy = V[i + N1], Viz = V[i + N2];
-      const float Wix = W[i], Wiy = W[i + N1], Wiz = W[i + N2];
-      // dd_Calculate_rijm_nij_2 (:38-52)
-      const float rx = R[j] - Rix, ry = R[j + N1] - Riy, rz = R[j + N2] - Riz;
-      const float rm = norm3(rx, ry, rz);
-      const float inv_rm = 1.0f / rm;
-      const float nx = rx * inv_rm, ny = ry * inv_rm, nz = rz * inv_rm;
-      const float il = iL[s];
-      // dd_Calculate_fijn_bpm_2 (:91-103)
-      float cef = E * (rm - 1.0f / il) * il;
-      float Sc = cef;
-      cef = cef * area_k;
-      s_c[0][t] = nx * cef;
-      s_c[1][t] = ny * cef;
-      s_c[2][t] = nz * cef;
-      // dd_Calculate_M123_bpm_2 (:54-89)
-      const f3 np_ = ld3(rij, s);
-      const float tx = np_.y * nz - np_.z * ny;
-      const float ty = np_.z * nx - np_.x * nz;
-      const float tz = np_.x * ny - np_.y * nx;
-      const float Wjx = W[j], Wjy = W[j + N1], Wjz = W[j + N2];
-      const float c2 = ((Wjx + Wix) * nx + (Wjy + Wiy) * ny + (Wjz + Wiz) * nz) * dt * 0.5f;
-      const float phx = nx * c2, phy = ny * c2, phz = nz * c2;
-      st3(rij, s, nx, ny, nz);
-      const float M1x = 1.0f + tz * phz + ty * phy;
-      const float M1y = phz - tz - ty * phx;
-      const float M1z = -phy - tz * phx + ty;
-      const float M2x = tz - phz - tx * phy;
-      const float M2y = tz * phz + 1.0f + tx * phx;
-      const float M2z = -tz * phy + phx - tx;
-      const float M3x = -ty - tx * phz + phy;
-      const float M3y = -ty * phz + tx - phx;
-      const float M3z = ty * phy + tx * phx + 1.0f;
-      // dd_Calculate_mijn_bpm_2 (:105-129)
-      const float dwx = Wjx - Wix, dwy = Wjy - Wiy, dwz = Wjz - Wiz;
-      float c1 = nx * dwx + ny * dwy + nz * dwz;
-      const float dotw = c1;
-      c1 = c1 * (dt * 0.5f * G * il);
-      const f3 mn0 = ld3(Mijn, s);
-      const float mnx = M1x * mn0.x + M1y * mn0.y + M1z * mn0.z + nx * c1;
-      const float mny = M2x * mn0.x + M2y * mn0.y + M2z * mn0.z + ny * c1;
-      const float mnz = M3x * mn0.x + M3y * mn0.y + M3z * mn0.z + nz * c1;
-      s_c[3][t] = mnx * j4;
-      s_c[4][t] = mny * j4;
-      s_c[5][t] = mnz * j4;
-      // dd_Calculate_mijt_bpm_2 (:131-152)
-      float c2t = dt * 0.25f * E * il;
-      const f3 mt0 = ld3(Mijt, s);
-      const float mtx = M1x * mt0.x + M1y * mt0.y + M1z * mt0.z + (dwx - nx * dotw) * c2t;
-      const float mty = M2x * mt0.x + M2y * mt0.y + M2z * mt0.z + (dwy - ny * dotw) * c2t;
-      const float mtz = M3x * mt0.x + M3y * mt0.y + M3z * mt0.z + (dwz - nz * dotw) * c2t;
-      c2t = c2t * j4;
-      s_c[6][t] = mtx * c2t;
-      s_c[7][t] = mty * c2t;
-      s_c[8][t] = mtz * c2t;
-      // dd_Calculate_fijt_mijunsym_bpm_2 (:154-197)
-      const float hx = 0.5f * (Wjx + Wix), hy = 0.5f * (Wjy + Wiy), hz = 0.5f * (Wjz + Wiz);
-      const float vx = V[j] - Vix - hy * rz + hz * ry;
-      const float vy = V[j + N1] - Viy - hz * rx + hx * rz;
-      const float vz = V[j + N2] - Viz - hx * ry + hy * rx;
-      const float cv = nx * vx + ny * vy + nz * vz;
-      const f3 o0 = ld3(Oijt, s);
-      const float ox = (M1x * o0.x + M1y * o0.y + M1z * o0.z) + (vx - nx * cv) * dt;
-      const float oy = (M2x * o0.x + M2y * o0.y + M2z * o0.z) + (vy - ny * cv) * dt;
-      const float oz = (M3x * o0.x + M3y * o0.y + M3z * o0.z) + (vz - nz * cv) * dt;
-      float ct = G * il;
-      float Mc = ct;
-      ct = ct * area_k;
-      s_c[9][t] = ox * ct;
-      s_c[10][t] = oy * ct;
-      s_c[11][t] = oz * ct;
-      s_c[12][t] = 0.5f * (ry * oz - rz * oy) * ct;
-      s_c[13][t] = 0.5f * (rz * ox - rx * oz) * ct;
-      s_c[14][t] = 0.5f * (rx * oy - ry * ox) * ct;
-      // dd_CheckBreakConditionDEM_2 (:20-36)
-      Mc = Mc * norm3(ox, oy, oz);
-      Sc = Sc + norm3(mtx, mty, mtz) * 4.0f * b_r;
-      Mc = Mc + norm3(mnx, mny, mnz) * 2.0f * b_r;
-      if (Sc > scrit || Mc > tcrit) {
-        ILtype[s] = 0;
-        st3(Oijt, s, 0.0f, 0.0f, 0.0f);
-        st3(Mijn, s, 0.0f, 0.0f, 0.0f);
-        st3(Mijt, s, 0.0f, 0.0f, 0.0f);
-      } else {
-        st3(Oijt, s, ox, oy, oz);
-        st3(Mijn, s, mnx, mny, mnz);
-        st3(Mijt, s, mtx, mty, mtz);
-      }
+    const float Rix = R[i], Riy = R[i + N1], Riz = R[i + N2];
+    const float Vix = V[i], Viy = V[i + N1], Viz = V[i + N2];
+    const float Wix = W[i], Wiy = W[i + N1], Wiz = W[i + N2];
+    // dd_Calculate_rijm_nij_2 (:38-52)
+    const float rx = R[j] - Rix, ry = R[j + N1] - Riy, rz = R[j + N2] - Riz;
+    const float rm = norm3(rx, ry, rz);
+    const float inv_rm = 1.0f / rm;
+    const float nx = rx * inv_rm, ny = ry * inv_rm, nz = rz * inv_rm;
+    const float il = iL[s];
+    // dd_Calculate_fijn_bpm_2 (:91-103)
+    float cef = E * (rm - 1.0f / il) * il;
+    float Sc = cef;
+    cef = cef * area_k;
+    s_c[0][q] = nx * cef;
+    s_c[1][q] = ny * cef;
+    s_c[2][q] = nz * cef;
+    // dd_Calculate_M123_bpm_2 (:54-89)
+    const f3 np_ = ld3(rij, s);
+    const float tx = np_.y * nz - np_.z * ny;
+    const float ty = np_.z * nx - np_.x * nz;
+    const float tz = np_.x * ny - np_.y * nx;
+    const float Wjx = W[j], Wjy = W[j + N1], Wjz = W[j + N2];
+    const float c2 = ((Wjx + Wix) * nx + (Wjy + Wiy) * ny + (Wjz + Wiz) * nz) * dt * 0.5f;
+    const float phx = nx * c2, phy = ny * c2, phz = nz * c2;
+    st3(rij, s, nx, ny, nz);
+    const float M1x = 1.0f + tz * phz + ty * phy;
+    const float M1y = phz - tz - ty * phx;
+    const float M1z = -phy - tz * phx + ty;
+    const float M2x = tz - phz - tx * phy;
+    const float M2y = tz * phz + 1.0f + tx * phx;
+    const float M2z = -tz * phy + phx - tx;
+    const float M3x = -ty - tx * phz + phy;
+    const float M3y = -ty * phz + tx - phx;
+    const float M3z = ty * phy + tx * phx + 1.0f;
+    // dd_Calculate_mijn_bpm_2 (:105-129)
+    const float dwx = Wjx - Wix, dwy = Wjy - Wiy, dwz = Wjz - Wiz;
+    float c1 = nx * dwx + ny * dwy + nz * dwz;
+    const float dotw = c1;
+    c1 = c1 * (dt * 0.5f * G * il);
+    const f3 mn0 = ld3(Mijn, s);
+    const float mnx = M1x * mn0.x + M1y * mn0.y + M1z * mn0.z + nx * c1;
+    const float mny = M2x * mn0.x + M2y * mn0.y + M2z * mn0.z + ny * c1;
+    const float mnz = M3x * mn0.x + M3y * mn0.y + M3z * mn0.z + nz * c1;
+    s_c[3][q] = mnx * j4;
+    s_c[4][q] = mny * j4;
+    s_c[5][q] = mnz * j4;
+    // dd_Calculate_mijt_bpm_2 (:131-152)
+    float c2t = dt * 0.25f * E * il;
+    const f3 mt0 = ld3(Mijt, s);
+    const float mtx = M1x * mt0.x + M1y * mt0.y + M1z * mt0.z + (dwx - nx * dotw) * c2t;
+    const float mty = M2x * mt0.x + M2y * mt0.y + M2z * mt0.z + (dwy - ny * dotw) * c2t;
+    const float mtz = M3x * mt0.x + M3y * mt0.y + M3z * mt0.z + (dwz - nz * dotw) * c2t;
+    c2t = c2t * j4;
+    s_c[6][q] = mtx * c2t;
+    s_c[7][q] = mty * c2t;
+    s_c[8][q] = mtz * c2t;
+    // dd_Calculate_fijt_mijunsym_bpm_2 (:154-197)
+    const float hx = 0.5f * (Wjx + Wix), hy = 0.5f * (Wjy + Wiy), hz = 0.5f * (Wjz + Wiz);
+    const float vx = V[j] - Vix - hy * rz + hz * ry;
+    const float vy = V[j + N1] - Viy - hz * rx + hx * rz;
+    const float vz = V[j + N2] - Viz - hx * ry + hy * rx;
+    const float cv = nx * vx + ny * vy + nz * vz;
+    const f3 o0 = ld3(Oijt, s);
+    const float ox = (M1x * o0.x + M1y * o0.y + M1z * o0.z) + (vx - nx * cv) * dt;
+    const float oy = (M2x * o0.x + M2y * o0.y + M2z * o0.z) + (vy - ny * cv) * dt;
+    const float oz = (M3x * o0.x + M3y * o0.y + M3z * o0.z) + (vz - nz * cv) * dt;
+    float ct = G * il;
+    float Mc = ct;
+    ct = ct * area_k;
+    s_c[9][q] = ox * ct;
+    s_c[10][q] = oy * ct;
+    s_c[11][q] = oz * ct;
+    s_c[12][q] = 0.5f * (ry * oz - rz * oy) * ct;
+    s_c[13][q] = 0.5f * (rz * ox - rx * oz) * ct;
+    s_c[14][q] = 0.5f * (rx * oy - ry * ox) * ct;
+    // dd_CheckBreakConditionDEM_2 (:20-36)
+    Mc = Mc * norm3(ox, oy, oz);
+    Sc = Sc + norm3(mtx, mty, mtz) * 4.0f * b_r;
+    Mc = Mc + norm3(mnx, mny, mnz) * 2.0f * b_r;
+    if (Sc > scrit || Mc > tcrit) {
+      ILtype[s] = 0;
+      st3(Oijt, s, 0.0f, 0.0f, 0.0f);
+      st3(Mijn, s, 0.0f, 0.0f, 0.0f);
+      st3(Mijt, s, 0.0f, 0.0f, 0.0f);
+    } else {
+      st3(Oijt, s, ox, oy, oz);
+      st3(Mijn, s, mnx, mny, mnz);
+      st3(Mijt, s, mtx, mty, mtz);
     }
   }
-  s_live[t] = live ? 1 : 0;
   __syncthreads();
-  // the row sums, slot by slot and term by term in the reference's order
+  // 3. the row sums, slot by slot and term by term in the reference's order
+  if (t >= np) return;
+  uint32_t q0 = 0, q1 = nlive;  // first listed slot of row t, then of row t + 1
+  {
+    uint32_t lo = 0, hi = nlive;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (s_slot[mid] < t * K) lo = mid + 1; else hi = mid;
+    }
+    q0 = lo;
+    hi = nlive;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (s_slot[mid] < (t + 1) * K) lo = mid + 1; else hi = mid;
+    }
+    q1 = lo;
+  }
   const uint32_t ip = p0 + t;
-  if (t >= P || ip >= N) return;
   float fsx = 0, fsy = 0, fsz = 0, msx = 0, msy = 0, msz = 0;
-  for (uint32_t q = t * K; q < (t + 1) * K; ++q) {
-    if (!s_live[q]) continue;
+  for (uint32_t q = q0; q < q1; ++q) {
     fsx = fsx + s_c[0][q];
     fsy = fsy + s_c[1][q];
     fsz = fsz + s_c[2][q];
@@ -547,7 +592,7 @@ int rfd_forces_bpm(rfd_ctx *ctx, const float *R, const float *V, const float *W,
                          M && N > 0 && K > 0 && (uint64_t)N * K < 0xFFFFFFFFull,
                 "rfd_forces_bpm: bad arguments");
   RFD_CHECK_ARG(ctx, K <= (uint32_t)kMaxK, "rfd_forces_bpm: K=%u > %d", K, kMaxK);
-  const uint32_t P = kSlotThreads / K;
+  const uint32_t P = bpm_rows(K);
   const unsigned blocks = (N + P - 1) / P;
   const uint32_t per = (blocks + kXcds - 1) / kXcds;
   k_forces_bpm<<<per * kXcds, kSlotThreads, 0, ctx->stream>>>(

@@ -207,6 +207,61 @@ def test_forces_bitexact_with_motion(ctx):
     assert np.abs(o.F).max() > 0
 
 
+@pytest.mark.parametrize("k", [1, 2, 5, 64])
+def test_forces_short_and_long_rows_bitexact(ctx, k):
+    """The force kernels at row lengths far from 32: the rows of a K = 32
+    bond construction cut to their first k slots (k = 1 puts 256 rows in a
+    bond workgroup, one per thread) or padded with empty slots to 64, then
+    a few force passes with growing motion that breaks most bonds."""
+    import torch
+
+    case = cases.reference_uniaxial(10.0, 14.0, seed=12)  # 1221 particles: 5 workgroups at k = 1
+    m = case.mat
+    base = OracleLayout(case.R, case.K, case.grid())
+    base.cell_distribution()
+    base.il_construct(float(np.float32(m.a_farcut) ** 2))
+    base.bonds_construct(m.b_dd)
+    N, K0 = case.N, case.K
+
+    def rows(a, width):
+        a = a.reshape(N, K0, width)
+        if k <= K0:
+            return a[:, :k].reshape(-1)
+        pad = np.zeros((N, k - K0, width), a.dtype)
+        if a.dtype == np.uint32:
+            pad[:] = 0xFFFFFFFF
+        return np.concatenate([a, pad], axis=1).reshape(-1)
+
+    g, o = RefLayout(ctx, case.R, k, case.grid()), OracleLayout(case.R, k, case.grid())
+    o.IL[:] = rows(base.IL, 1)
+    o.ILtype[:] = rows(base.ILtype, 1)
+    o.iL[:] = rows(base.iL, 1)
+    for name in ("rij", "Oijt", "Mijn", "Mijt"):
+        o.slot[name][:] = rows(base.slot[name], 3)
+        g.slot[name].copy_(torch.from_numpy(o.slot[name]).cuda())
+    g.IL.copy_(torch.from_numpy(o.IL.view(np.int32)).cuda())
+    g.ILtype.copy_(torch.from_numpy(o.ILtype).cuda())
+    g.iL.copy_(torch.from_numpy(o.iL).cuda())
+    n0 = int((o.ILtype == 1).sum())
+    assert n0 > 0
+    dv = perturb(N, 41, 2e-2)
+    dw = perturb(N, 42, 2e-2)
+    for it in range(4):
+        g.V.copy_(torch.from_numpy(dv * (it + 1)).cuda())
+        g.W.copy_(torch.from_numpy(dw * (it + 1)).cuda())
+        o.V[:] = dv * (it + 1)
+        o.W[:] = dw * (it + 1)
+        for s in (g, o):
+            s.zero_fm()
+            s.forces_bpm(m)
+            s.forces_contact(m)
+            s.increments(m)
+        assert_same(g.host(), o.host(), where=f"k={k} iteration {it}")
+    n1 = int((o.ILtype == 1).sum())
+    print(f"k={k}: bond slots {n0} -> {n1}", flush=True)
+    assert 0 < n1 < n0
+
+
 def test_bond_break_bitexact(ctx):
     """Weak bonds break under the same motion on both sides."""
     import dataclasses
@@ -262,8 +317,9 @@ def test_driver_trajectory_bitexact(ctx, mode):
 @pytest.mark.parametrize("K", [24, 27, 40])
 def test_driver_trajectory_other_row_lengths(ctx, K):
     """The same whole steps with K slots per particle other than 32: the
-    force kernels then tile floor(256 / K) particles per workgroup with idle
-    threads at the end (K = 27: 9 particles, 243 of 256 threads), and the list
+    contact kernel then tiles floor(256 / K) particles per workgroup with idle
+    threads at the end (K = 27: 9 particles, 243 of 256 threads), the bond
+    kernel floor(512 / K) (18 rows, 486 slots), and the list
     reconstruction pads its LDS rows to 28, 28 and 44 words."""
     args = (8.0, 14.0)
     cg = cases.reference_uniaxial(*args, seed=5, K=K)
