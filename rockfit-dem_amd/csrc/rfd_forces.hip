@@ -1,9 +1,9 @@
 // rfd_forces.hip -- bond and contact forces in the reference slot layout.
 //
-// One thread per slot evaluates the law of its slot (coalesced loads and
-// stores of the slot state across a row) and leaves the terms the reference
-// adds to the particle's sums in LDS; one thread per particle then adds them
-// in slot order, term by term, so that F += fs, M += ms are formed in the same
+// The law of each slot (of each listed bond slot in the bond kernel) is
+// evaluated by a thread of its own, which leaves the terms the reference adds
+// to the particle's sums in LDS; one thread per particle then adds them in
+// slot order, term by term, so that F += fs, M += ms are formed in the same
 // order as md_calculateforcesdem3.cu:437-480 and :353-419 and are bit-exact
 // against a sequential restatement.
 #include "rfd_internal.h"
@@ -42,6 +42,50 @@ __host__ __device__ inline uint32_t bpm_rows(uint32_t K) {
   return min(kBpmSlots / K, (uint32_t)kSlotThreads);
 }
 
+// Lists, in slot order, the row-local slots ls < nsl of a tile for which
+// keep(ls) holds (one ballot per wave and sweep of kSlotThreads slots) and
+// returns how many.  Every thread of the workgroup calls it.
+template <class Keep>
+__device__ __forceinline__ uint32_t list_slots(uint32_t nsl, uint16_t *s_slot,
+                                               uint32_t *s_wave, Keep keep) {
+  const uint32_t t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  uint32_t n = 0;
+  for (uint32_t r0 = 0; r0 < nsl; r0 += kSlotThreads) {
+    const uint32_t ls = r0 + t;
+    const bool kept = ls < nsl && keep(ls);
+    const uint64_t bal = __ballot(kept);
+    if (lane == 0) s_wave[wid] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (uint32_t w = 0; w < kSlotThreads / 64; ++w) {
+      before += w < wid ? s_wave[w] : 0u;
+      all += s_wave[w];
+    }
+    if (kept)
+      s_slot[n + before + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = (uint16_t)ls;
+    n += all;
+    __syncthreads();
+  }
+  return n;
+}
+
+// The listed entries [q0, q1) that belong to row lp (slots lp K .. lp K + K - 1).
+__device__ __forceinline__ void row_range(const uint16_t *s_slot, uint32_t n, uint32_t lp,
+                                          uint32_t K, uint32_t &q0, uint32_t &q1) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (s_slot[mid] < lp * K) lo = mid + 1; else hi = mid;
+  }
+  q0 = lo;
+  hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (s_slot[mid] < (lp + 1) * K) lo = mid + 1; else hi = mid;
+  }
+  q1 = lo;
+}
+
 __global__ void __launch_bounds__(kSlotThreads)
 k_forces_bpm(const float *__restrict__ R, const float *__restrict__ V,
              const float *__restrict__ W, const uint32_t *__restrict__ IL,
@@ -56,7 +100,7 @@ k_forces_bpm(const float *__restrict__ R, const float *__restrict__ V,
   __shared__ uint16_t s_slot[kBpmSlots];  // row-local slot (lp K + k) of each listed slot
   __shared__ uint32_t s_wave[kSlotThreads / 64];
   const uint32_t P = bpm_rows(K);
-  const uint32_t t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const uint32_t t = threadIdx.x;
   const uint32_t blk = xcd_block(per);
   if (blk >= nblk) return;
   const uint32_t p0 = blk * P;
@@ -67,27 +111,9 @@ k_forces_bpm(const float *__restrict__ R, const float *__restrict__ V,
   const float area_k = b_r * b_r * kPiF;          // b_r*b_r*MCf_pi
   const float j4 = kPiF * b_r * b_r * b_r * b_r;  // MCf_pi*b_r^4
   // 1. the live bond slots, in slot order
-  uint32_t nlive = 0;
-  for (uint32_t r0 = 0; r0 < nsl; r0 += kSlotThreads) {
-    const uint32_t ls = r0 + t;
-    bool live = false;
-    if (ls < nsl) {
-      const size_t s = g0 + ls;
-      live = IL[s] != kSentinel && ILtype[s] == 1;
-    }
-    const uint64_t bal = __ballot(live);
-    if (lane == 0) s_wave[wid] = (uint32_t)__popcll(bal);
-    __syncthreads();
-    uint32_t before = 0, all = 0;
-    for (uint32_t w = 0; w < kSlotThreads / 64; ++w) {
-      before += w < wid ? s_wave[w] : 0u;
-      all += s_wave[w];
-    }
-    if (live)
-      s_slot[nlive + before + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = (uint16_t)ls;
-    nlive += all;
-    __syncthreads();
-  }
+  const uint32_t nlive = list_slots(nsl, s_slot, s_wave, [&](uint32_t ls) {
+    return IL[g0 + ls] != kSentinel && ILtype[g0 + ls] == 1;
+  });
   // 2. the law of each listed slot
   for (uint32_t q = t; q < nlive; q += kSlotThreads) {
     const uint32_t ls = s_slot[q];
@@ -187,21 +213,8 @@ k_forces_bpm(const float *__restrict__ R, const float *__restrict__ V,
   __syncthreads();
   // 3. the row sums, slot by slot and term by term in the reference's order
   if (t >= np) return;
-  uint32_t q0 = 0, q1 = nlive;  // first listed slot of row t, then of row t + 1
-  {
-    uint32_t lo = 0, hi = nlive;
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (s_slot[mid] < t * K) lo = mid + 1; else hi = mid;
-    }
-    q0 = lo;
-    hi = nlive;
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (s_slot[mid] < (t + 1) * K) lo = mid + 1; else hi = mid;
-    }
-    q1 = lo;
-  }
+  uint32_t q0, q1;
+  row_range(s_slot, nlive, t, K, q0, q1);
   const uint32_t ip = p0 + t;
   float fsx = 0, fsy = 0, fsz = 0, msx = 0, msy = 0, msz = 0;
   for (uint32_t q = q0; q < q1; ++q) {
