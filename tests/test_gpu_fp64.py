@@ -117,14 +117,17 @@ def test_fire_brazil_fp32_vs_fp64(ctx):
     """The same comparison on a Brazilian disc (platens along y, the FIRE
     start, measure and update of loading mode 2, frictionless platens), with
     bonds too strong to break, relaxed to |F|max < 1e-6 F_ref: every step
-    converges on both paths and the reactions agree to 1e-3 of the largest
-    (the spread of where two relaxations stop, N tol F_ref, against the
-    reaction of a point-loaded disc)."""
+    converges on both paths.  Where both relaxations run the same number of
+    iterations the reactions agree to 1e-5 of the largest (FP32 round-off);
+    where they stop at different iterations they agree to 3e-3.  That spread
+    is where two relaxations stop on the same approach to equilibrium, not
+    FP32 arithmetic: it shrinks with the tolerance (measured 1.4e-3 of the
+    largest at 1e-6 F_ref, 3.5e-4 at 3e-7, 1.5e-4 at 1e-7)."""
     from rfd import native
 
     case = native_cases.brazil_batch(ctx, S=1, seed=13, n_lo=1500, n_hi=1600,
                                      mu_range=(1e3, 1e3), hist_cap=4096)
-    res = {}
+    res, its = {}, {}
     for fp64 in (False, True):
         s = native_cases.batch_member(ctx, case, 0, fp64=fp64)
         fp = native.fire_params(case.mat.E, 0.5, case.mat.dt, rel_tol=1e-6, max_iter=50000,
@@ -134,10 +137,15 @@ def test_fire_brazil_fp32_vs_fp64(ctx):
               f"fmax {[r.fmax for r in rr]}", flush=True)
         assert all(r.converged and r.fmax < fp.ftol for r in rr), [r.as_dict() for r in rr]
         res[fp64] = np.array([[r.sum_bottom, r.sum_top] for r in rr])
+        its[fp64] = [r.iterations for r in rr]
         s.close()
     scale = np.abs(res[True]).max()
-    print(f"reactions: FP64 {res[True][:, 1]}, FP32 - FP64 max "
-          f"{np.abs(res[False] - res[True]).max() / scale:.2e} of the largest", flush=True)
+    diff = np.abs(res[False] - res[True]).max(axis=1) / scale
+    print(f"reactions: FP64 {res[True][:, 1]}, FP32 - FP64 per step {diff} of the largest",
+          flush=True)
     assert scale > 0
-    np.testing.assert_allclose(res[False], res[True], rtol=0, atol=1e-3 * scale)
+    same = [k for k in range(len(diff)) if its[False][k] == its[True][k]]
+    assert same, "no step ran the same iterations on both paths"
+    assert diff[same].max() <= 1e-5, (same, diff)
+    assert diff.max() <= 3e-3, diff
     assert ctx.flags() == 0
