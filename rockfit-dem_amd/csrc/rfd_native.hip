@@ -1158,12 +1158,15 @@ __device__ __forceinline__ void reduce_pairs(const double *part, int b0, int b1,
 }
 
 // loading control of the FP64 path: the platen partial sums are reduced by
-// one workgroup in a fixed order, then one thread applies the control
+// one workgroup in a fixed order, then one thread applies the control and
+// flips the bond-state index that the step's force pass left
 __global__ void __launch_bounds__(256) k_nat_control(rfd_loading *ctl, const double *part,
-                                                     int nblocks, double *hist, double dt) {
+                                                     int nblocks, double *hist, double dt,
+                                                     int *curp) {
   double s0, s1;
   reduce_pairs(part, 0, nblocks, s0, s1);
   if (threadIdx.x != 0) return;
+  curp[0] = 1 - curp[0];
   rfd_loading L = *ctl;
   control_update(L, s0, s1, hist, (uint32_t)kHistCap, dt);
   *ctl = L;
@@ -2385,133 +2388,200 @@ struct Sys64 {
   uint32_t *nbreak;   // when set, counts the bonds this pass breaks
 };
 
-__global__ void __launch_bounds__(128) k_f64_forces(Sys64 s, Mat64 mt) {
+// Lanes per particle in the FP64 force pass: the lanes of a group evaluate a
+// row's entries side by side, and every lane then adds the group's terms in
+// lane order, which is row order.
+constexpr uint32_t kG64 = 8;
+
+__device__ __forceinline__ double grp_get(double v, uint32_t k) {
+  return __shfl(v, (int)k, (int)kG64);
+}
+__device__ __forceinline__ d3 grp_get(d3 v, uint32_t k) {
+  return {grp_get(v.x, k), grp_get(v.y, k), grp_get(v.z, k)};
+}
+
+// One bond half of row i in the FP64 pass (md_calculateforcesdem3.cu:424-483
+// in FP64; the crossing of the far cut removes the bond): its load on i (dF,
+// and the bond moments dMb), the contact terms (dMc) once the bond is broken
+// and the pair touches (touch).  The owner half (i < j) writes the bond
+// state.  Returns false when the entry adds nothing.
+__device__ __forceinline__ bool f64_half(const Sys64 &s, const Mat64 &mt, BSt64 *st_in,
+                                         BSt64 *st_out, uint32_t i, const P4 &A, const d3 &Vi,
+                                         const d3 &Wi, uint32_t h, d3 &dF, d3 &dMb, d3 &dMc,
+                                         bool &touch) {
+  const HR q = s.hr[h];
+  const BSt64 old = st_in[q.b];
+  if (old.fl == kRemoved) return false;
+  const bool own = i < q.j;
+  const double sg = own ? 1.0 : -1.0;
+  const P4 B = s.P[q.j];
+  const d3 Vj = xyz(s.V[q.j]), Wj = xyz(s.W[q.j]);
+  const BR c = s.brc[q.b];
+  const d3 r = {B.x - A.x, B.y - A.y, B.z - A.z};
+  const double r2 = dot(r, r);
+  const double lim = mt.farcut * (A.r + B.r);
+  BSt64 nw = old;
+  if (r2 > lim * lim || !(r2 > 1e-10)) {
+    if (own) {
+      st_out[q.b].fl = kRemoved;
+      if (!s.trial) {
+        st_in[q.b].fl = kRemoved;
+        s.fl8[q.b] = kRemoved;
+      }
+    }
+    return false;
+  }
+  const double L = __builtin_sqrt(r2);
+  const d3 n = r * (1.0 / L);
+  d3 O = d3{old.ox, old.oy, old.oz} * sg;
+  bool contact_now = old.fl == kBroken;
+  bool any = false;
+  if (old.fl == kBond) {
+    const d3 rp = r - (Vj - Vi) * mt.dt;
+    const double pp = dot(rp, rp);
+    const d3 np = pp > 0.0 ? rp * (1.0 / __builtin_sqrt(pp)) : n;
+    const d3 Mt0 = d3{old.mx, old.my, old.mz} * sg;
+    const double il = (double)c.il, br = (double)c.br;
+    const double A2 = 3.14159265358979323846 * br * br, J = A2 * br * br;
+    const double eps = L * il - 1.0;
+    const double sig = mt.E * eps;
+    const d3 Fn = n * (sig * A2);
+    const d3 ws = Wi + Wj;
+    Rot64 M;
+    M.build(cross(np, n), n * (dot(ws, n) * mt.dt * 0.5));
+    const d3 dw = Wj - Wi;
+    const double dwn = dot(n, dw);
+    const double mn = old.mn + dwn * (mt.dt * 0.5 * mt.G * il);
+    const double cb = mt.dt * 0.25 * mt.E * il;
+    const d3 Mt = M.apply(Mt0) + (dw - n * dwn) * cb;
+    const d3 v = Vj - Vi - cross(ws * 0.5, r);
+    const double vn = dot(n, v);
+    O = M.apply(O) + (v - n * vn) * mt.dt;
+    const d3 Ft = O * (mt.G * il * A2);
+    const double mtf = mt.refmode ? cb * J : J;
+    dF = Fn + Ft;
+    dMb = n * (mn * J) + Mt * mtf + cross(r, Ft) * 0.5;
+    any = true;
+    const double Sc = sig + __builtin_sqrt(dot(Mt, Mt)) * 4.0 * br;
+    const double Mcr = mt.G * il * __builtin_sqrt(dot(O, O)) + fabs(mn) * 2.0 * br;
+    if (!mt.nobreak && (Sc > (double)c.sc || Mcr > (double)c.tc)) {
+      nw.fl = kBroken;
+      O = kZero64;
+      nw.mn = nw.mx = nw.my = nw.mz = 0.0;
+      contact_now = true;
+    } else {
+      const d3 Os = O * sg, Ms = Mt * sg;
+      nw.ox = Os.x;
+      nw.oy = Os.y;
+      nw.oz = Os.z;
+      nw.mn = mn;
+      nw.mx = Ms.x;
+      nw.my = Ms.y;
+      nw.mz = Ms.z;
+    }
+  }
+  if (contact_now) {
+    const double reach = mt.refmode ? A.r + B.r : fmin(A.r + B.r, 1.0 / (double)c.il);
+    if (L < reach) {
+      d3 Fc, Mci, Mcj;
+      contact_law64(mt, n, L, A.r, B.r, reach, Vi, Vj, Wi, Wj, O, Fc, Mci, Mcj);
+      dF = dF + Fc;
+      dMc = Mci;
+      touch = true;
+      any = true;
+    } else {
+      O = kZero64;
+    }
+    const d3 Os = O * sg;
+    nw.ox = Os.x;
+    nw.oy = Os.y;
+    nw.oz = Os.z;
+  }
+  if (own) {
+    st_out[q.b] = nw;
+    if (!s.trial && nw.fl != old.fl) {
+      s.fl8[q.b] = (uint8_t)nw.fl;
+      if (s.nbreak && nw.fl == kBroken) atomicAdd(s.nbreak, 1u);
+    }
+  }
+  return any;
+}
+
+// Contact entry k of particle i's contact row (md_calculateforcesdem3.cu:
+// 338-422 in FP64); writes the entry's spring.  Returns whether it touches.
+__device__ __forceinline__ bool f64_contact(const Sys64 &s, const Mat64 &mt, uint32_t i,
+                                            const P4 &A, const d3 &Vi, const d3 &Wi,
+                                            uint32_t k, d3 &dF, d3 &dMc) {
+  const size_t sl = (size_t)k * s.N + i;
+  const uint32_t j = s.cj[sl];
+  const P4 B = s.P[j];
+  const d3 r = {B.x - A.x, B.y - A.y, B.z - A.z};
+  const double L = __builtin_sqrt(dot(r, r));
+  if (L < A.r + B.r && L > 1e-5) {
+    d3 O = xyz(s.cO[sl]);
+    d3 Mj;
+    contact_law64(mt, r * (1.0 / L), L, A.r, B.r, A.r + B.r, Vi, xyz(s.V[j]), Wi,
+                  xyz(s.W[j]), O, dF, dMc, Mj);
+    if (!s.trial) s.cO[sl] = d4(O);
+    return true;
+  }
+  if (!s.trial) s.cO[sl] = make_double4(0.0, 0.0, 0.0, 0.0);
+  return false;
+}
+
+// FP64 force pass: kG64 lanes per particle take its bond halves, then its
+// contact entries, kG64 at a time; each lane adds the group's terms in row
+// order, so every lane holds the row sums and the first one writes them.
+__global__ void __launch_bounds__(256) k_f64_forces(Sys64 s, Mat64 mt) {
   if (s.done && *s.done) return;
   if (s.dtp) mt.dt = *s.dtp;
   BSt64 *const st_in = *s.curp ? s.b1 : s.b0;
   BSt64 *const st_out = *s.curp ? s.b0 : s.b1;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < s.N; i += grid_stride()) {
+  const uint32_t g = threadIdx.x % kG64;
+  const uint32_t stride = grid_stride() / kG64;
+  for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / kG64; i < s.N; i += stride) {
     const P4 A = s.P[i];
     const d3 Vi = xyz(s.V[i]), Wi = xyz(s.W[i]);
     d3 F = kZero64, Mb = kZero64, Mc = kZero64;
     bool anyc = false;
-    for (uint32_t h = s.hoff[i]; h < s.hoff[i + 1]; ++h) {
-      const HR q = s.hr[h];
-      const BSt64 old = st_in[q.b];
-      if (old.fl == kRemoved) continue;
-      const bool own = i < q.j;
-      const double sg = own ? 1.0 : -1.0;
-      const P4 B = s.P[q.j];
-      const d3 Vj = xyz(s.V[q.j]), Wj = xyz(s.W[q.j]);
-      const BR c = s.brc[q.b];
-      const d3 r = {B.x - A.x, B.y - A.y, B.z - A.z};
-      const double r2 = dot(r, r);
-      const double lim = mt.farcut * (A.r + B.r);
-      BSt64 nw = old;
-      if (r2 > lim * lim || !(r2 > 1e-10)) {
-        if (own) {
-          st_out[q.b].fl = kRemoved;
-          if (!s.trial) {
-            st_in[q.b].fl = kRemoved;
-            s.fl8[q.b] = kRemoved;
-          }
+    const uint32_t h1 = s.hoff[i + 1];
+    for (uint32_t h = s.hoff[i]; h < h1; h += kG64) {
+      d3 dF = kZero64, dMb = kZero64, dMc = kZero64;
+      bool touch = false;
+      const bool on = h + g < h1 &&
+                      f64_half(s, mt, st_in, st_out, i, A, Vi, Wi, h + g, dF, dMb, dMc, touch);
+      const int flags = (on ? 1 : 0) | (touch ? 2 : 0);
+      for (uint32_t k = 0; k < kG64; ++k) {
+        const int fk = __shfl(flags, (int)k, (int)kG64);
+        const d3 a = grp_get(dF, k), b = grp_get(dMb, k), c = grp_get(dMc, k);
+        if (fk & 1) {
+          F = F + a;
+          Mb = Mb + b;
+          Mc = Mc + c;
         }
-        continue;
-      }
-      const double L = __builtin_sqrt(r2);
-      const d3 n = r * (1.0 / L);
-      d3 O = d3{old.ox, old.oy, old.oz} * sg;
-      bool contact_now = old.fl == kBroken;
-      if (old.fl == kBond) {
-        const d3 rp = r - (Vj - Vi) * mt.dt;
-        const double pp = dot(rp, rp);
-        const d3 np = pp > 0.0 ? rp * (1.0 / __builtin_sqrt(pp)) : n;
-        const d3 Mt0 = d3{old.mx, old.my, old.mz} * sg;
-        const double il = (double)c.il, br = (double)c.br;
-        const double A2 = 3.14159265358979323846 * br * br, J = A2 * br * br;
-        const double eps = L * il - 1.0;
-        const double sig = mt.E * eps;
-        const d3 Fn = n * (sig * A2);
-        const d3 ws = Wi + Wj;
-        Rot64 M;
-        M.build(cross(np, n), n * (dot(ws, n) * mt.dt * 0.5));
-        const d3 dw = Wj - Wi;
-        const double dwn = dot(n, dw);
-        const double mn = old.mn + dwn * (mt.dt * 0.5 * mt.G * il);
-        const double cb = mt.dt * 0.25 * mt.E * il;
-        const d3 Mt = M.apply(Mt0) + (dw - n * dwn) * cb;
-        const d3 v = Vj - Vi - cross(ws * 0.5, r);
-        const double vn = dot(n, v);
-        O = M.apply(O) + (v - n * vn) * mt.dt;
-        const d3 Ft = O * (mt.G * il * A2);
-        const double mtf = mt.refmode ? cb * J : J;
-        F = F + Fn + Ft;
-        Mb = Mb + n * (mn * J) + Mt * mtf + cross(r, Ft) * 0.5;
-        const double Sc = sig + __builtin_sqrt(dot(Mt, Mt)) * 4.0 * br;
-        const double Mcr = mt.G * il * __builtin_sqrt(dot(O, O)) + fabs(mn) * 2.0 * br;
-        if (!mt.nobreak && (Sc > (double)c.sc || Mcr > (double)c.tc)) {
-          nw.fl = kBroken;
-          O = kZero64;
-          nw.mn = nw.mx = nw.my = nw.mz = 0.0;
-          contact_now = true;
-        } else {
-          const d3 Os = O * sg, Ms = Mt * sg;
-          nw.ox = Os.x;
-          nw.oy = Os.y;
-          nw.oz = Os.z;
-          nw.mn = mn;
-          nw.mx = Ms.x;
-          nw.my = Ms.y;
-          nw.mz = Ms.z;
-        }
-      }
-      if (contact_now) {
-        const double reach = mt.refmode ? A.r + B.r : fmin(A.r + B.r, 1.0 / (double)c.il);
-        if (L < reach) {
-          d3 Fc, Mci, Mcj;
-          contact_law64(mt, n, L, A.r, B.r, reach, Vi, Vj, Wi, Wj, O, Fc, Mci, Mcj);
-          F = F + Fc;
-          Mc = Mc + Mci;
-          anyc = true;
-        } else {
-          O = kZero64;
-        }
-        const d3 Os = O * sg;
-        nw.ox = Os.x;
-        nw.oy = Os.y;
-        nw.oz = Os.z;
-      }
-      if (own) {
-        st_out[q.b] = nw;
-        if (!s.trial && nw.fl != old.fl) {
-          s.fl8[q.b] = (uint8_t)nw.fl;
-          if (s.nbreak && nw.fl == kBroken) atomicAdd(s.nbreak, 1u);
-        }
+        anyc = anyc || (fk & 2);
       }
     }
     const uint32_t nc = s.ccnt[i];
-    for (uint32_t k = 0; k < nc; ++k) {
-      const size_t sl = (size_t)k * s.N + i;
-      const uint32_t j = s.cj[sl];
-      const P4 B = s.P[j];
-      const d3 r = {B.x - A.x, B.y - A.y, B.z - A.z};
-      const double L = __builtin_sqrt(dot(r, r));
-      if (L < A.r + B.r && L > 1e-5) {
-        d3 O = xyz(s.cO[sl]);
-        d3 Fi, Mi, Mj;
-        contact_law64(mt, r * (1.0 / L), L, A.r, B.r, A.r + B.r, Vi, xyz(s.V[j]), Wi,
-                      xyz(s.W[j]), O, Fi, Mi, Mj);
-        F = F + Fi;
-        Mc = Mc + Mi;
-        anyc = true;
-        if (!s.trial) s.cO[sl] = d4(O);
-      } else if (!s.trial) {
-        s.cO[sl] = make_double4(0.0, 0.0, 0.0, 0.0);
+    for (uint32_t k0 = 0; k0 < nc; k0 += kG64) {
+      d3 dF = kZero64, dMc = kZero64;
+      const bool on = k0 + g < nc && f64_contact(s, mt, i, A, Vi, Wi, k0 + g, dF, dMc);
+      const int flag = on ? 1 : 0;
+      for (uint32_t k = 0; k < kG64; ++k) {
+        const int fk = __shfl(flag, (int)k, (int)kG64);
+        const d3 a = grp_get(dF, k), c = grp_get(dMc, k);
+        if (fk) {
+          F = F + a;
+          Mc = Mc + c;
+          anyc = true;
+        }
       }
     }
     if (mt.refmode && anyc && !(dot(Wi, Wi) > 1e-18)) Mc = kZero64;
-    s.F[i] = d4(F);
-    s.M[i] = d4(Mb + Mc);
+    if (g == 0) {
+      s.F[i] = d4(F);
+      s.M[i] = d4(Mb + Mc);
+    }
   }
 }
 
@@ -3046,7 +3116,7 @@ int forces64(rfd_system *s, const double *dtp = nullptr, bool trial = false,
     s->ev_used += 2;
     RFD_HIP(s->ctx, hipEventRecord(e0, s->ctx->stream));
   }
-  k_f64_forces<<<rfd_blocks(s->N, 128), 128, 0, s->ctx->stream>>>(d, m);
+  k_f64_forces<<<rfd_blocks((size_t)s->N * kG64, 256), 256, 0, s->ctx->stream>>>(d, m);
   RFD_HIP(s->ctx, hipGetLastError());
   if (e1) RFD_HIP(s->ctx, hipEventRecord(e1, s->ctx->stream));
   if (!trial && flip) k_flip_state<<<1, 64, 0, s->ctx->stream>>>(s->d_cur, done);
@@ -3196,7 +3266,8 @@ int pass_and_control(rfd_system *s) {
       k_f64_pass<2><<<s->nblocks_pass, 256, 0, st64>>>(q);
     else
       k_f64_pass<0><<<s->nblocks_pass, 256, 0, st64>>>(q);
-    k_nat_control<<<1, 256, 0, st64>>>(s->ctl, s->part, (int)s->nblocks_pass, s->hist, s->p.dt);
+    k_nat_control<<<1, 256, 0, st64>>>(s->ctl, s->part, (int)s->nblocks_pass, s->hist, s->p.dt,
+                                       s->d_cur);
     RFD_HIP(s->ctx, hipGetLastError());
     return RFD_OK;
   }
@@ -3249,8 +3320,8 @@ int step_plain(rfd_system *s) {
     int st = rebuild_rows(s);
     if (st) return st;
   }
-  // the FP32 control flips the bond-state index (the FP64 pass flips it itself)
-  int st = forces(s, nullptr, false, false, nullptr, s->fp64);
+  // the loading control flips the bond-state index
+  int st = forces(s, nullptr, false, false, nullptr, false);
   if (!st) st = pass_and_control(s);
   if (st) return st;
   s->step_count++;
