@@ -302,6 +302,84 @@ def test_head_on_restitution():
     assert e == pytest.approx(m.p_e, abs=0.06)
 
 
+def _beta(e):
+    # damping ratio of the restitution coefficient e (Tsuji, Tanaka and
+    # Ishida 1992): ln e / sqrt(ln^2 e + pi^2), negative for e < 1
+    le = math.log(e)
+    return le / math.sqrt(le * le + math.pi * math.pi)
+
+
+@pytest.mark.parametrize("u", [2e-2, -2e-2])
+def test_normal_damping_tsuji_closed_form(u):
+    """Hertz spring plus Tsuji's viscous normal force for equal spheres:
+    F = -(4/3) E* sqrt(R*) d^1.5 - 2 sqrt(5/6) |beta| sqrt(S_n m*) u along the
+    line of centres, S_n = 2 E* sqrt(R* d), R* = r/2, m* = m/2, u the approach
+    speed (negative: separating).  The damping opposes the relative motion."""
+    m = mat()
+    delta = 2.0 ** -7  # 2 r - delta is exact in FP32
+    p = Pair((2 * m.p_r - delta, 0, 0), bonded=False)
+    p.set_v(1, (-u, 0, 0))  # j moves toward i for u > 0
+    p.contact(m)
+    E_star = m.E / (2.0 * (1.0 - m.nu ** 2))
+    Rs = m.p_r / 2.0
+    Sn = 2.0 * E_star * math.sqrt(Rs * delta)
+    spring = (4.0 / 3.0) * E_star * math.sqrt(Rs) * delta ** 1.5
+    damp = 2.0 * math.sqrt(5.0 / 6.0) * abs(_beta(m.p_e)) * math.sqrt(Sn * m.p_m / 2.0) * u
+    assert abs(damp) / spring > 0.1  # the damping term is not lost in round-off
+    F0 = p.vec(p.F, 0)
+    # md_calculateforcesdem3.cu:245 writes 2 sqrt(5/6) = 1.825742 as 1.8257
+    # (2.3e-5 relative); FP32 round-off adds about 1e-6 of each term
+    assert F0[0] == pytest.approx(-(spring + damp), abs=5e-5 * (spring + abs(damp)))
+    assert F0[1] == 0 and F0[2] == 0
+    assert np.array_equal(p.vec(p.F, 1), -F0)
+
+
+def _rotation(seed):
+    if seed is None:
+        return np.eye(3)
+    q, r = np.linalg.qr(np.random.default_rng(seed).standard_normal((3, 3)))
+    q = q * np.sign(np.diag(r))
+    return q if np.linalg.det(q) > 0 else -q
+
+
+@pytest.mark.parametrize("seed", [None, 1, 2, 3])
+@pytest.mark.parametrize("wj_sign,slip", [(-1.0, 0.0), (0.0, 1.0), (1.0, 2.0)])
+def test_contact_point_slip_kinematics(wj_sign, slip, seed):
+    """Sphere i at the origin spins at w about z; j touches it along +x with
+    spin wj_sign * w.  The surface of i at the contact point moves at r w (+y),
+    that of j at -wj_sign r w (+y), so counter-rotating spheres roll on each
+    other without slip, and otherwise j's surface slips past i's at slip * r w
+    along -y.  One call from a zero spring: the tangential force on i is
+    Mindlin's spring over dt of that slip plus Tsuji's tangential damping,
+    S_t = 8 G* sqrt(R* d), along -y, and its moment r n x F_t opposes the spin.
+    The same holds with the whole configuration turned by a rotation Q (seeded;
+    None: no turn), which puts every component of the slip velocity to use."""
+    m = mat()
+    delta = 2.0 ** -7
+    w = 3e-3
+    Q = _rotation(seed)
+    p = Pair(tuple(Q @ np.array([2 * m.p_r - delta, 0.0, 0.0])), bonded=False)
+    p.set_w(0, Q @ np.array([0.0, 0.0, w]))
+    p.set_w(1, Q @ np.array([0.0, 0.0, wj_sign * w]))
+    p.contact(m)
+    G_star = (m.E / (2.0 * (1.0 + m.nu))) / (2.0 * (2.0 - m.nu))
+    St = 8.0 * G_star * math.sqrt(m.p_r / 2.0 * delta)
+    vs = slip * m.p_r * w
+    ft = St * vs * m.dt + 2.0 * math.sqrt(5.0 / 6.0) * abs(_beta(m.p_e)) * math.sqrt(
+        St * m.p_m / 2.0) * vs
+    F0, M0 = Q.T @ p.vec(p.F, 0), Q.T @ p.vec(p.M, 0)  # back in the unturned frame
+    # FP32 positions of a turned pair carry round-off of about 1e-7 of r
+    tol = 0.0 if seed is None else 2e-4 * abs(F0[0])
+    if slip == 0.0:
+        assert abs(F0[1]) <= tol and abs(F0[2]) <= tol
+    else:
+        # md_calculateforcesdem3.cu:299 writes 2 sqrt(5/6) as 1.8257, as above
+        assert F0[1] == pytest.approx(-ft, rel=5e-5 if seed is None else 2e-3)
+        assert abs(F0[2]) <= tol
+        # the friction moment on i works against its spin
+        assert M0[2] < -m.p_r * ft * 0.99
+
+
 # ------------------------------------------------------------ integration
 def test_increments_exact_affine():
     g = np.random.default_rng(5)
@@ -671,3 +749,55 @@ def test_cut_restriction_moves_outside_particles():
             assert np.all(got == hp) and V[i::N].max() == 0 and F[i::N].max() == 0
         else:
             assert np.all(got == P[i]) and V[i::N].min() == 1 and F[i::N].min() == 2
+
+
+def test_platen_brazil_friction_opposes_sliding_on_both_platens():
+    """md_braziltest_simple2.cu:19-92: a particle pressed on either platen
+    (top: F_y > 0, bottom: F_y < 0) and sliding in the (x, z) plane gets a
+    Coulomb friction force of 0.45 |F_y| against its sliding direction."""
+    P = np.array([[0, 9.9, 0], [0, 0.1, 0]], np.float32)  # top band, bottom band
+    N = len(P)
+    R = np.ascontiguousarray(P.T).reshape(-1).copy()
+    V = np.zeros(3 * N, np.float32)
+    F = np.zeros(3 * N, np.float32)
+    F[N + 0], F[N + 1] = 2.0, -4.0
+    v = [(0.6, 0.8), (-0.6, 0.8)]  # unit in-plane velocities
+    for i, (vx, vz) in enumerate(v):
+        V[i], V[2 * N + i] = vx, vz
+    sums = (C.c_double * 2)()
+    lib().orc_platen_brazil(ptr(R), ptr(V), ptr(F), N, 0.0, 0.0, 1.0, 10.0, 0.0, 9.5, 0.5,
+                            -1e-3, 100.0, sums)
+    for i, (vx, vz) in enumerate(v):
+        fy = 2.0 if i == 0 else -4.0
+        fric = np.array([F[i], F[2 * N + i]])
+        np.testing.assert_allclose(fric, -0.45 * abs(fy) * np.array([vx, vz]), rtol=1e-6)
+
+
+def test_delete_far_links_threshold():
+    """md_interaction_list_reconstructbpm.cu:141-186: a link whose squared
+    length exceeds aa_farcut is removed from the row, one just inside stays."""
+    aa = np.float32(2.25)
+    for scale, gone in ((1.0 - 1e-3, False), (1.0 + 1e-3, True)):
+        x = np.float32(math.sqrt(float(aa) * scale))
+        R = np.array([0.0, x, 0, 0, 0, 0], np.float32)
+        N, K = 2, 1
+        IL = np.array([1, 0], np.uint32)
+        ty = np.array([1, 1], np.uint8)
+        st = [np.ones(6, np.float32) for _ in range(7)]
+        lib().orc_delete_far_links(ptr(R), N, ptr(IL), ptr(ty), *[ptr(a) for a in st], K, aa)
+        if gone:
+            assert np.all(IL == 0xFFFFFFFF) and np.all(ty == 0)
+            assert all(np.all(a == 0) for a in st)
+        else:
+            assert list(IL) == [1, 0] and np.all(ty == 1)
+            assert all(np.all(a == 1) for a in st)
+
+
+def test_fire_halfstep_moves_back_along_the_velocity():
+    """md_firerelaxation.cu:58-70: the back-off after a negative power returns
+    every particle half a step along its path, R <- R - V dt/2."""
+    R = np.array([1.0, 2.0, 3.0, -1.0, 0.5, 4.0], np.float32)
+    V = np.array([2.0, -4.0, 6.0, 8.0, 0.0, -2.0], np.float32)
+    want = R - V * np.float32(0.25)
+    lib().orc_fire_halfstep(ptr(V), ptr(R), 2, np.float32(0.25))
+    np.testing.assert_array_equal(R, want)
