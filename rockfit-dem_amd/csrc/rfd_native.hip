@@ -68,6 +68,9 @@ constexpr uint32_t kMaxBreakRounds = 200;  // relax-and-break rounds per load st
 // (own L2) works through one contiguous run of Morton-ordered particle tiles,
 // which keeps the neighbour gathers of adjacent tiles in the same L2
 constexpr uint32_t kXcd = 8;
+// force passes of at most this many tiles (one per CU) run the loading
+// control themselves, and load each tile's first records ahead of its barrier
+constexpr uint32_t kCtlInPassTiles = 256;
 __device__ __forceinline__ uint32_t xcd_tile(uint32_t per) {
   return (blockIdx.x % kXcd) * per + blockIdx.x / kXcd;
 }
@@ -661,6 +664,16 @@ struct AdvDev {
   const uint32_t *spec32, *spec_first;
   // ctl holds the loadings of specimens ctl_base, ctl_base + 1, ... (0: all)
   uint32_t ctl_base;
+  // when set, the workgroup that finishes the pass last runs the loading
+  // control of every specimen (the step's second launch otherwise): done
+  // counts the finished workgroups and is zero between passes
+  uint32_t *done;
+  rfd_loading *ctl_w;   // the loadings it updates (ctl, writable)
+  uint32_t nspec;       // specimens
+  double *hist;         // history records, hist_cap per specimen
+  uint32_t hist_cap;
+  double dt64;
+  int *curp;            // bond-state index to flip
 };
 
 // Force pass of the physical mode with every bond whose ends share a tile
@@ -687,7 +700,7 @@ constexpr int kMeasVals = 9;  // P, |F|^2, |V|^2, |M|^2, |W|^2, max|F|^2, max(|M
 // One tile of the pass below: `cur` is the index of the bond-state buffer
 // holding the latest state.  Every thread of the workgroup calls it; threads
 // return from it only after its last workgroup barrier.
-template <int FP, int FT, bool ADV, bool MEAS = false>
+template <int FP, int FT, bool ADV, bool MEAS = false, bool PIPE = false>
 __device__ __forceinline__ void forces1_tile(const SysDev &s, const Mat &mt, const WorkDev &w,
                                              uint32_t tile, uint32_t cap, const AdvDev &a,
                                              int cur, const MeasDev *fm = nullptr) {
@@ -707,15 +720,6 @@ __device__ __forceinline__ void forces1_tile(const SysDev &s, const Mat &mt, con
   const uint32_t tid = threadIdx.x;
   const uint32_t p0 = tile * FP;
   const uint32_t np = min((uint32_t)FP, s.N - p0);
-  if (tid <= np) sOff[tid] = s.hoff[p0 + tid];
-  if (tid < np) {
-    sP[tid] = s.P[p0 + tid];
-    sV[tid] = s.V[p0 + tid];
-    sW[tid] = s.W[p0 + tid];
-  }
-  __syncthreads();
-  const uint32_t h0 = sOff[0];
-  const uint32_t w0 = w.ofs[tile], w1 = w.ofs[tile + 1], wx = w0 + w.n_in[tile];
   // owner side of a bond: state out, flag copy, break count
   auto commit = [&](uint32_t bnd, const BSt &old, const BondOut &o) {
     if (o.st.fl == kRemoved) {
@@ -737,50 +741,140 @@ __device__ __forceinline__ void forces1_tile(const SysDev &s, const Mat &mt, con
     sM1[e] = M.y;
     sM2[e] = M.z;
   };
-  // 1. bonds inside the tile, once each (the lower end owns them)
-  for (uint32_t k = w0 + tid; k < wx; k += FT) {
-    const WorkRec r = w.rec[k];
-    const uint32_t e = r.a & 0x7FFu, lo = (r.a >> 11) & 0x7Fu, jl = (r.a >> 18) & 0x7Fu;
-    const BSt old = st_in[r.b];
-    BondOut o;
-    if (old.fl == kRemoved) {
-      o.Fi = o.Mbi = o.Mbj = o.Mci = o.Mcj = kZero;
-    } else {
-      EndData A, B;
-      A.P = sP[lo];
-      A.V = xyz(sV[lo]);
-      A.W = xyz(sW[lo]);
-      B.P = sP[jl];
-      B.V = xyz(sV[jl]);
-      B.W = xyz(sW[jl]);
-      eval_bond(mt, A, B, s.brc[r.b], old, o);
-      commit(r.b, old, o);
+  uint32_t h0;
+  if constexpr (!PIPE) {
+    if (tid <= np) sOff[tid] = s.hoff[p0 + tid];
+    if (tid < np) {
+      sP[tid] = s.P[p0 + tid];
+      sV[tid] = s.V[p0 + tid];
+      sW[tid] = s.W[p0 + tid];
     }
-    put(e, o.Fi, o.Mbi + o.Mci);
-    put(r.c, -o.Fi, o.Mbj + o.Mcj);
-  }
-  // 2. halves whose partner is outside the tile, each evaluated by its end
-  for (uint32_t k = wx + tid; k < w1; k += FT) {
-    const WorkRec r = w.rec[k];
-    const uint32_t e = r.a & 0x7FFu, lo = (r.a >> 11) & 0x7Fu, j = r.c;
-    const bool own = p0 + lo < j;
-    const BSt old = st_in[r.b];
-    BondOut o;
-    if (old.fl == kRemoved) {
-      o.Fi = o.Mbi = o.Mbj = o.Mci = o.Mcj = kZero;
-    } else {
-      EndData me;
-      me.P = sP[lo];
-      me.V = xyz(sV[lo]);
-      me.W = xyz(sW[lo]);
-      const EndData other = load_end(s, j);
-      eval_bond(mt, own ? me : other, own ? other : me, s.brc[r.b], old, o);
-      if (own) commit(r.b, old, o);
-    }
-    if (own)
+    __syncthreads();
+    h0 = sOff[0];
+    const uint32_t w0 = w.ofs[tile], w1 = w.ofs[tile + 1], wx = w0 + w.n_in[tile];
+    // 1. bonds inside the tile, once each (the lower end owns them)
+    for (uint32_t k = w0 + tid; k < wx; k += FT) {
+      const WorkRec r = w.rec[k];
+      const uint32_t e = r.a & 0x7FFu, lo = (r.a >> 11) & 0x7Fu, jl = (r.a >> 18) & 0x7Fu;
+      const BSt old = st_in[r.b];
+      BondOut o;
+      if (old.fl == kRemoved) {
+        o.Fi = o.Mbi = o.Mbj = o.Mci = o.Mcj = kZero;
+      } else {
+        EndData A, B;
+        A.P = sP[lo];
+        A.V = xyz(sV[lo]);
+        A.W = xyz(sW[lo]);
+        B.P = sP[jl];
+        B.V = xyz(sV[jl]);
+        B.W = xyz(sW[jl]);
+        eval_bond(mt, A, B, s.brc[r.b], old, o);
+        commit(r.b, old, o);
+      }
       put(e, o.Fi, o.Mbi + o.Mci);
-    else
-      put(e, -o.Fi, o.Mbj + o.Mcj);
+      put(r.c, -o.Fi, o.Mbj + o.Mcj);
+    }
+    // 2. halves whose partner is outside the tile, each evaluated by its end
+    for (uint32_t k = wx + tid; k < w1; k += FT) {
+      const WorkRec r = w.rec[k];
+      const uint32_t e = r.a & 0x7FFu, lo = (r.a >> 11) & 0x7Fu, j = r.c;
+      const bool own = p0 + lo < j;
+      const BSt old = st_in[r.b];
+      BondOut o;
+      if (old.fl == kRemoved) {
+        o.Fi = o.Mbi = o.Mbj = o.Mci = o.Mcj = kZero;
+      } else {
+        EndData me;
+        me.P = sP[lo];
+        me.V = xyz(sV[lo]);
+        me.W = xyz(sW[lo]);
+        const EndData other = load_end(s, j);
+        eval_bond(mt, own ? me : other, own ? other : me, s.brc[r.b], old, o);
+        if (own) commit(r.b, old, o);
+      }
+      if (own)
+        put(e, o.Fi, o.Mbi + o.Mci);
+      else
+        put(e, -o.Fi, o.Mbj + o.Mcj);
+    }
+  } else {
+    // Small systems, where a tile's chain of dependent loads bounds the pass
+    // (one tile per CU): the records run as one list, bonds inside the tile
+    // first, then the leaving halves; a thread takes records tid, tid + FT,
+    // ...; its first two records and their bond data are loaded together with
+    // the tile's particles, before the barrier that publishes those.  The
+    // same arithmetic as the two loops above, on more registers.
+    const uint32_t w0 = w.ofs[tile], w1 = w.ofs[tile + 1], wx = w0 + w.n_in[tile];
+    if (tid <= np) sOff[tid] = s.hoff[p0 + tid];
+    if (tid < np) {
+      sP[tid] = s.P[p0 + tid];
+      sV[tid] = s.V[p0 + tid];
+      sW[tid] = s.W[p0 + tid];
+    }
+    struct Fetched {
+      WorkRec r;
+      BSt old;
+      BR c;
+      EndData x;  // the partner of a leaving half
+    };
+    auto fetch = [&](uint32_t k, Fetched &f) {
+      f.r = w.rec[k];
+      f.old = st_in[f.r.b];
+      f.c = s.brc[f.r.b];
+      if (k >= wx) f.x = load_end(s, f.r.c);
+    };
+    auto eval = [&](uint32_t k, const Fetched &f) {
+      const WorkRec &r = f.r;
+      const uint32_t e = r.a & 0x7FFu, lo = (r.a >> 11) & 0x7Fu;
+      BondOut o;
+      if (k < wx) {
+        const uint32_t jl = (r.a >> 18) & 0x7Fu;
+        if (f.old.fl == kRemoved) {
+          o.Fi = o.Mbi = o.Mbj = o.Mci = o.Mcj = kZero;
+        } else {
+          EndData A, B;
+          A.P = sP[lo];
+          A.V = xyz(sV[lo]);
+          A.W = xyz(sW[lo]);
+          B.P = sP[jl];
+          B.V = xyz(sV[jl]);
+          B.W = xyz(sW[jl]);
+          eval_bond(mt, A, B, f.c, f.old, o);
+          commit(r.b, f.old, o);
+        }
+        put(e, o.Fi, o.Mbi + o.Mci);
+        put(r.c, -o.Fi, o.Mbj + o.Mcj);
+      } else {
+        const bool own = p0 + lo < r.c;
+        if (f.old.fl == kRemoved) {
+          o.Fi = o.Mbi = o.Mbj = o.Mci = o.Mcj = kZero;
+        } else {
+          EndData me;
+          me.P = sP[lo];
+          me.V = xyz(sV[lo]);
+          me.W = xyz(sW[lo]);
+          eval_bond(mt, own ? me : f.x, own ? f.x : me, f.c, f.old, o);
+          if (own) commit(r.b, f.old, o);
+        }
+        if (own)
+          put(e, o.Fi, o.Mbi + o.Mci);
+        else
+          put(e, -o.Fi, o.Mbj + o.Mcj);
+      }
+    };
+    const uint32_t k1 = w0 + tid, k2 = k1 + FT;
+    Fetched f1, f2;
+    if (k1 < w1) fetch(k1, f1);
+    if (k2 < w1) fetch(k2, f2);
+    __syncthreads();
+    h0 = sOff[0];
+    if (k1 < w1) eval(k1, f1);
+    if (k2 < w1) eval(k2, f2);
+    for (uint32_t k = k2 + FT; k < w1; k += FT) {
+      Fetched f;
+      fetch(k, f);
+      eval(k, f);
+    }
   }
   __syncthreads();
   // 3. per-particle sums in row order: task (group g, particle p), one per
@@ -918,6 +1012,40 @@ __device__ __forceinline__ void forces1_tile(const SysDev &s, const Mat &mt, con
   }
 }
 
+// The loading control in the force pass (explicit steps of small systems,
+// where the control's own launch is a large part of a step): every workgroup,
+// once its platen reactions have been added to the fixed-point sums, counts
+// itself done, and the last one reads and clears the sums and runs the
+// control of every specimen, as k_nat_control_fx does.  The sums and the
+// counter are only ever read, added to or cleared with device-scope atomics,
+// which the GPU performs at one point per address for all of its XCDs, and
+// each workgroup's additions are complete (s_waitcnt vmcnt(0)) before it
+// counts itself; so the last workgroup sees every addition without a fence
+// that writes back an L2.  Nothing else the control reads is written during
+// the pass (each specimen's loading is read by the tiles and written only
+// here, after every tile is done), and it leaves the counter at zero.
+__device__ __forceinline__ void control_update(rfd_loading &L, double s0, double s1,
+                                               double *hist, uint32_t hist_cap, double dt);
+__device__ __forceinline__ void last_tile_control(const AdvDev &a, uint32_t ntiles) {
+  __shared__ uint32_t sLast;
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  if (threadIdx.x == 0) sLast = atomicAdd(a.done, 1u) == ntiles - 1 ? 1u : 0u;
+  __syncthreads();
+  if (!sLast) return;
+  for (uint32_t sp = threadIdx.x; sp < a.nspec; sp += blockDim.x) {
+    const double s0 = (double)(long long)atomicExch(a.acc + 2 * sp, 0ull) * kFxInv;
+    const double s1 = (double)(long long)atomicExch(a.acc + 2 * sp + 1, 0ull) * kFxInv;
+    rfd_loading L = a.ctl_w[sp];
+    control_update(L, s0, s1, a.hist + 3 * (size_t)a.hist_cap * sp, a.hist_cap, a.dt64);
+    a.ctl_w[sp] = L;
+  }
+  if (threadIdx.x == 0) {
+    a.curp[0] = 1 - a.curp[0];
+    atomicExch(a.done, 0u);
+  }
+}
+
 template <int FP, int FT, bool ADV>
 __global__ void __launch_bounds__(FT) __attribute__((amdgpu_waves_per_eu(5, 8)))
 k_nat_forces1(SysDev s, Mat mt, WorkDev w, uint32_t ntiles, uint32_t per, uint32_t cap,
@@ -927,6 +1055,19 @@ k_nat_forces1(SysDev s, Mat mt, WorkDev w, uint32_t ntiles, uint32_t per, uint32
   if (s.done && *s.done) return;
   if (s.dtp) mt.dt = *s.dtp;
   forces1_tile<FP, FT, ADV>(s, mt, w, tile, cap, a, *s.curp);
+}
+
+// The fused explicit step of a small system (at most kCtlInPassTiles tiles,
+// one per CU): the pipelined tile, which needs more registers than one CU
+// can give five workgroups, and the loading control by the last workgroup
+template <int FP, int FT>
+__global__ void __launch_bounds__(FT) __attribute__((amdgpu_waves_per_eu(2, 8)))
+k_nat_forces1_small(SysDev s, Mat mt, WorkDev w, uint32_t ntiles, uint32_t per,
+                    uint32_t cap, AdvDev a) {
+  const uint32_t tile = xcd_tile(per);
+  if (tile >= ntiles) return;
+  forces1_tile<FP, FT, true, false, true>(s, mt, w, tile, cap, a, *s.curp);
+  last_tile_control(a, ntiles);
 }
 
 // a quasi-static force pass with the FIRE measure in its epilogue
@@ -2836,6 +2977,7 @@ struct rfd_system {
   rfd_loading *ctl = nullptr;
   double *part = nullptr;               // FP64 path: per-workgroup platen partials
   unsigned long long *acc = nullptr;    // FP32 path: fixed-point platen sums
+  uint32_t *done_cnt = nullptr;         // workgroups done, for the control in the pass
   double *hist = nullptr;
   // explicit steps with the particle update fused into the force pass write
   // positions, velocities and spins to these and swap them with P, V, W; the
@@ -2843,6 +2985,7 @@ struct rfd_system {
   P4 *P2 = nullptr;
   float4 *V2 = nullptr, *W2 = nullptr;
   bool fuse = true;
+  bool ctl_in_pass = true;  // small systems: the loading control in the force pass
   uint32_t nblocks_pass = 0;
   uint64_t step_count = 0;
   int mode = 0;  // host copy of the loading mode (selects the pass kernel)
@@ -3137,7 +3280,10 @@ void launch_forces(rfd_system *s, const SysDev &d, const Mat &m, const AdvDev *a
     // slots for the system's largest tile, at least the per-particle sums
     const uint32_t cap = std::max((s->tile_halves[q] + 31u) & ~31u, 2u * FP);
     const size_t lds = sizeof(float) * 6 * cap;
-    if (adv)
+    if (adv && adv->done)
+      k_nat_forces1_small<FP, FT><<<per * kXcd, FT, lds, s->ctx->stream>>>(d, m, w, nb, per,
+                                                                           cap, *adv);
+    else if (adv)
       k_nat_forces1<FP, FT, true><<<per * kXcd, FT, lds, s->ctx->stream>>>(d, m, w, nb, per,
                                                                             cap, *adv);
     else
@@ -3330,21 +3476,43 @@ int step_plain(rfd_system *s) {
 
 // one fused step: reads P, V, W, writes P2, V2, W2 and swaps the two sets
 // (also after an error, so that a pair of calls always restores them)
+// the loading control runs in the force pass when the pass has few enough
+// tiles that every tile's workgroup is resident at once (one round of
+// workgroups): then the wait of each workgroup for its done count does not
+// hold back other tiles, and the control's launch is saved
+bool control_in_pass(const rfd_system *s) {
+  if (!s->ctl_in_pass) return false;
+  const uint32_t fp = s->force_fp == 32 ? 32u : (s->force_fp == 128 ? 128u : (uint32_t)kFP);
+  return (s->N + fp - 1) / fp <= kCtlInPassTiles;
+}
+
 int step_fused(rfd_system *s) {
   int st = RFD_OK;
+  bool ctl_in_pass = false;
   if (!s->rows_valid || s->step_count % s->p.rebuild_every == 0) st = rebuild_rows(s);
   if (!st) {
     AdvDev a{s->P2, s->V2, s->W2, s->inv, s->ctl, s->acc, (float)s->p.dt, s->mode,
              nullptr, nullptr};
+    a.ctl_w = s->ctl;
+    a.nspec = 1;
+    a.hist = s->hist;
+    a.hist_cap = (uint32_t)kHistCap;
     if (s->S) {
-      a.ctl = s->ctls;
+      a.ctl = a.ctl_w = s->ctls;
       a.acc = s->acc_b;
       a.spec32 = s->spec32;
       a.spec_first = s->spec_first_d;
+      a.nspec = s->S;
+      a.hist = s->hist_b;
+      a.hist_cap = s->hist_cap_b;
     }
+    a.dt64 = s->p.dt;
+    a.curp = s->d_cur;
+    ctl_in_pass = control_in_pass(s);
+    if (ctl_in_pass) a.done = s->done_cnt;
     st = forces(s, nullptr, false, false, &a, false);
   }
-  if (!st) st = control(s);
+  if (!st && !ctl_in_pass) st = control(s);
   std::swap(s->P, s->P2);
   std::swap(s->V, s->V2);
   std::swap(s->W, s->W2);
@@ -3836,6 +4004,7 @@ int rfd_system_create(rfd_ctx *ctx, const rfd_system_params *p, uint32_t N, rfd_
   s->nblocks_pass = rfd_blocks(N, 256, 2048);
   if (!st) st = dalloc(ctx, &s->part, 2 * (size_t)s->nblocks_pass);
   if (!st) st = dalloc(ctx, &s->acc, 2);
+  if (!st) st = dalloc(ctx, &s->done_cnt, 1);
   if (!st) st = dalloc(ctx, &s->hist, 3 * (size_t)kHistCap);
   if (!st && !s->fp64) {
     st = dalloc(ctx, &s->P2, N);
@@ -3846,6 +4015,7 @@ int rfd_system_create(rfd_ctx *ctx, const rfd_system_params *p, uint32_t N, rfd_
     rfd_loading L{};
     hipError_t e = hipMemcpy(s->ctl, &L, sizeof(L), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(s->acc, 0, 2 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(s->done_cnt, 0, sizeof(uint32_t));
     for (int q = 0; q < 2 && e == hipSuccess; ++q)
       e = hipMemset(s->ccnt[q], 0, sizeof(uint32_t) * N);
     if (e != hipSuccess) st = rfd_hip_status(ctx, e, "rfd_system_create");
@@ -3899,6 +4069,7 @@ int rfd_system_destroy(rfd_system *s) {
   }
   dfree(s->part);
   dfree(s->acc);
+  dfree(s->done_cnt);
   dfree(s->Vf64);
   dfree(s->Th64);
   dfree(s->P2);
