@@ -1896,24 +1896,37 @@ __global__ void __launch_bounds__(64) k_nat_rows_cells(NbrDev s, uint32_t nblock
     };
     uint32_t nfound = 0;
     if (staged) {
-      // 1. in-reach candidates in scan order (box indices), per lane
+      // 1. in-reach candidates in scan order (box indices), per lane: the
+      //    lane's 27 cells are 9 x-rows of three cells, each a contiguous
+      //    range of the box copy, visited in (z, y) order as the per-particle
+      //    search visits them
       uint32_t nf = 0;
-      uint32_t e = 0;
-      for (; e + 4 <= total; e += 4) {
-        const float4 q0 = sC[e], q1 = sC[e + 1], q2 = sC[e + 2], q3 = sC[e + 3];
-        const bool h0 = reach(q0), h1 = reach(q1), h2 = reach(q2), h3 = reach(q3);
-        if (live && (h0 || h1 || h2 || h3)) {
-          if (h0) { if (nf < kRowFound) sF[nf][tid] = (uint16_t)e; ++nf; }
-          if (h1) { if (nf < kRowFound) sF[nf][tid] = (uint16_t)(e + 1); ++nf; }
-          if (h2) { if (nf < kRowFound) sF[nf][tid] = (uint16_t)(e + 2); ++nf; }
-          if (h3) { if (nf < kRowFound) sF[nf][tid] = (uint16_t)(e + 3); ++nf; }
+      if (live) {
+        // the home cell in box coordinates: the home x-row's first cell, or
+        // the next one along x when the particle lies past the first's members
+        uint32_t r = 0;
+        while (r + 1 < kRowCB * kRowCB && sHome[r + 1] <= k) ++r;
+        const int hy = y0 + (int)r % kRowCB - hy0, hz = z0 + (int)r / kRowCB - hz0;
+        const int c0 = (x0 - hx0) + hy * hnx + hz * hnx * hny;
+        const uint32_t eg = sHomeG[r] + (k - sHome[r]);
+        const int hxb = (x0 - hx0) + (eg >= sG[c0] + (sS[c0 + 1] - sS[c0]) ? 1 : 0);
+        const int xa = max(hxb - 1, 0), xb = min(hxb + 1, hnx - 1);
+        for (int dz = -1; dz <= 1; ++dz) {
+          const int z = hz + dz;
+          if (z < 0 || z >= hnz) continue;
+          for (int dy = -1; dy <= 1; ++dy) {
+            const int y = hy + dy;
+            if (y < 0 || y >= hny) continue;
+            const int row = y * hnx + z * hnx * hny;
+            const uint32_t e1 = sS[row + xb + 1];
+            for (uint32_t e = sS[row + xa]; e < e1; ++e)
+              if (reach(sC[e])) {
+                if (nf < kRowFound) sF[nf][tid] = (uint16_t)e;
+                ++nf;
+              }
+          }
         }
       }
-      for (; e < total; ++e)
-        if (live && reach(sC[e])) {
-          if (nf < kRowFound) sF[nf][tid] = (uint16_t)e;
-          ++nf;
-        }
       // 2. drop self and bonded partners, write the row in scan order
       if (live && nf <= kRowFound) {
         for (uint32_t f = 0; f < nf; ++f) {
