@@ -2783,6 +2783,109 @@ __global__ void __launch_bounds__(256) k_f64_pass(Pass64 p) {
   write_partials(p.part, sb, stp);
 }
 
+// The FP64 particle pass of a small system with the loading control of
+// k_nat_control run by its last workgroup (see last_tile_control): each
+// workgroup's platen partials are written, and then read, with device-scope
+// atomics, so that the last workgroup to count itself done sees every one; it
+// reduces them in k_nat_control's fixed order (the same 256 threads), applies
+// the control, flips the bond-state index and resets the counter.
+__device__ __forceinline__ double atomic_read_f64(double *p) {
+  return __longlong_as_double(
+      (long long)atomicAdd(reinterpret_cast<unsigned long long *>(p), 0ull));
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_f64_pass_ctl(Pass64 p, uint32_t *done, double *hist,
+                                                      int *curp) {
+  __shared__ double s_part[8];
+  __shared__ uint32_t sLast;
+  {
+    const rfd_loading L = *p.ctl;
+    double sb = 0.0, stp = 0.0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.N; i += grid_stride()) {
+      P4 R = p.P[i];
+      d3 V = xyz(p.V[i]), F = xyz(p.F[i]);
+      if (MODE) platen<MODE>(L, R, V, F, sb, stp);
+      const double m = p.rho43pi * R.r * R.r * R.r, I = 0.4 * m * R.r * R.r;
+      V = V + F * (p.dt / m);
+      R.x += V.x * p.dt;
+      R.y += V.y * p.dt;
+      R.z += V.z * p.dt;
+      const d3 W = xyz(p.W[i]) + xyz(p.M[i]) * (p.dt / I);
+      if (L.cut) {
+        const double dx = R.x - L.box_c[0], dy = R.y - L.box_c[1], dz = R.z - L.box_c[2];
+        if (dx - L.box_h[0] > 0 || dx + L.box_h[0] < 0 || dy - L.box_h[1] > 0 ||
+            dy + L.box_h[1] < 0 || dz - L.box_h[2] > 0 || dz + L.box_h[2] < 0) {
+          R.x = L.hidden[0];
+          R.y = L.hidden[1];
+          R.z = L.hidden[2];
+          V = kZero64;
+          F = kZero64;
+        }
+      }
+      p.P[i] = R;
+      p.V[i] = d4(V);
+      p.W[i] = d4(W);
+      p.F[i] = d4(F);
+    }
+    if (MODE) {
+      // write_partials' sums, stored with atomics
+      sb = wave_sum(sb);
+      stp = wave_sum(stp);
+      const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+      if (lane == 0) {
+        s_part[2 * wid] = sb;
+        s_part[2 * wid + 1] = stp;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double a = 0, c = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+          a += s_part[2 * w];
+          c += s_part[2 * w + 1];
+        }
+        atomicExch(reinterpret_cast<unsigned long long *>(p.part + 2 * blockIdx.x),
+                   (unsigned long long)__double_as_longlong(a));
+        atomicExch(reinterpret_cast<unsigned long long *>(p.part + 2 * blockIdx.x + 1),
+                   (unsigned long long)__double_as_longlong(c));
+      }
+    }
+  }
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  if (threadIdx.x == 0) sLast = atomicAdd(done, 1u) == gridDim.x - 1 ? 1u : 0u;
+  __syncthreads();
+  if (!sLast) return;
+  // reduce_pairs' order over the gridDim.x partial pairs
+  double a = 0, c = 0;
+  if (MODE) {
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
+      a += atomic_read_f64(p.part + 2 * b);
+      c += atomic_read_f64(p.part + 2 * b + 1);
+    }
+  }
+  a = wave_sum(a);
+  c = wave_sum(c);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+    s_part[2 * wid] = a;
+    s_part[2 * wid + 1] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double s0 = 0, s1 = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+    s0 += s_part[2 * w];
+    s1 += s_part[2 * w + 1];
+  }
+  curp[0] = 1 - curp[0];
+  rfd_loading L = *p.ctl;
+  control_update(L, s0, s1, hist, (uint32_t)kHistCap, p.dt);
+  *p.ctl = L;
+  atomicExch(done, 0u);
+}
+
 __global__ void k_f32_to_f64(const float *__restrict__ soa, uint32_t N,
                              double4 *__restrict__ out) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride())
@@ -3419,6 +3522,17 @@ int pass_and_control(rfd_system *s) {
     q.rho43pi = s->p.rho * 4.0 / 3.0 * M_PI;
     q.ctl = s->ctl;
     q.part = s->part;
+    if (s->ctl_in_pass && s->nblocks_pass <= kCtlInPassTiles) {
+      // small system: the control in the pass (one launch less per step)
+      if (s->mode == 1)
+        k_f64_pass_ctl<1><<<s->nblocks_pass, 256, 0, st64>>>(q, s->done_cnt, s->hist, s->d_cur);
+      else if (s->mode == 2)
+        k_f64_pass_ctl<2><<<s->nblocks_pass, 256, 0, st64>>>(q, s->done_cnt, s->hist, s->d_cur);
+      else
+        k_f64_pass_ctl<0><<<s->nblocks_pass, 256, 0, st64>>>(q, s->done_cnt, s->hist, s->d_cur);
+      RFD_HIP(s->ctx, hipGetLastError());
+      return RFD_OK;
+    }
     if (s->mode == 1)
       k_f64_pass<1><<<s->nblocks_pass, 256, 0, st64>>>(q);
     else if (s->mode == 2)
