@@ -279,7 +279,49 @@ def bench_c2(steps: int = 500) -> dict:
         out[name] = {"bond_updates_per_s": r["value"], "ms_per_step": r["ms_per_step"],
                      "particles": r["config"]["particles"], "bonds": r["config"]["bonds"],
                      "workload": r["config"]["workload"]}
+    out["ref_tier"]["driver"] = "Python (ctypes) calls, one per reference launch site"
+    out["ref_tier_cpp"] = bench_ref_cpp(25.0, 50.0, steps, out["ref_tier"]["bonds"])
     return out
+
+
+def bench_ref_cpp(D: float, H: float, steps: int, bonds: int) -> dict:
+    """The reference-layout tier driven by the C++ host driver (host/rfd_run),
+    as the reference's own C++ test loop would call the library: one ABI call
+    per reference launch site, the blocking platen-sum read each step
+    (md_results.cu:36).  Same monodisperse specimen as the Python-driven line."""
+    import re
+    import subprocess
+    import tempfile
+
+    import numpy as np
+
+    from rfd import cases, formats
+    from rfd.build import RUN
+
+    c = cases.reference_uniaxial(D, H)
+    N, K = c.N, c.K
+    S = N * K
+    z = np.zeros(3 * N, np.float32)
+    slots = {k: np.zeros((S, 3), np.float32) for k in formats.SLOT_VEC}
+    ck = formats.Checkpoint({k: np.zeros(3) for k in formats.SAMPLE_FIELDS}, z, z, c.R, z, z,
+                            np.zeros(4 * N, np.float32), K, 0.0,
+                            np.full(S, 0xFFFFFFFF, np.uint32), np.zeros(S, np.uint8),
+                            np.zeros(S, np.float32), slots)
+    with tempfile.TemporaryDirectory() as td:
+        formats.write_checkpoint(f"{td}/in.dat", ck)
+        r = subprocess.run([str(RUN), "--mode", "uniaxial", "--D", str(D), "--H", str(H),
+                            "--in", f"{td}/in.dat", "--out", td, "--steps", str(steps)],
+                           capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        raise RuntimeError(f"rfd_run failed: {r.stderr[-2000:]}")
+    m = re.search(r"us_per_step=([0-9.]+)", r.stdout)
+    us = float(m.group(1))
+    return {"driver": "C++ host driver (host/rfd_run), one ABI call per reference launch site",
+            "ms_per_step": us / 1e3, "particles": N, "bonds": bonds, "steps": steps,
+            "device_flags_clear": "flags=0" in r.stdout,
+            "bond_updates_per_s": bonds * 1e6 / us,
+            "workload": f"uniaxial cylinder D{D:g}xH{H:g} mm, reference slot layout K={K}, "
+                        "monodisperse r=0.5 mm"}
 
 
 def bench_warmstart(D: float = 61.0, H: float = 122.0, steps: int = 12) -> dict:

@@ -8,6 +8,10 @@
 //
 //   rfd_run --mode uniaxial|brazil --D <mm> --H <mm> --in DATA.dat --out <dir>
 //           [--steps N] [--vload-scale s] [--until-fracture]
+//
+// The summary line ends with the wall time of the step loop (after set-up,
+// to the end of its last step's GPU work) per step.
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -66,18 +70,24 @@ int main(int argc, char **argv) {
   {
     Driver d(ctx, c0.R, m, box, cmp, K);
     d.setup();
+    check(ctx, rfd_sync(ctx), "rfd_sync");
+    const auto t0 = std::chrono::steady_clock::now();
     const uint32_t n = until_fracture ? d.run(steps) : ([&]() {
       for (unsigned k = 0; k < steps; ++k) d.step();
       return (uint32_t)steps;
     })();
     check(ctx, rfd_sync(ctx), "rfd_sync");
+    const double us = std::chrono::duration<double, std::micro>(
+                          std::chrono::steady_clock::now() - t0).count();
     uint32_t flags = 0;
     check(ctx, rfd_read_flags(ctx, &flags), "rfd_read_flags");
     save_sum_forces_loading(out + (uni ? "/Uniaxial.txt" : "/Brazil.txt"), d.cmp);
     save_all_data(out + "/DATA.dat", checkpoint_of(d.s, (float)m.a_farcut));
-    std::printf("%s: N=%u K=%u steps=%u stress=%g Pa peak=%g Pa strain=%g fracture=%d flags=%u\n",
+    std::printf("%s: N=%u K=%u steps=%u stress=%g Pa peak=%g Pa strain=%g fracture=%d flags=%u "
+                "us_per_step=%.3f\n",
                 mode.c_str(), d.s.N, K, n, d.cmp.stress * kStress,
-                d.cmp.fracture_stress * kStress, d.cmp.strain, (int)d.cmp.fracture, flags);
+                d.cmp.fracture_stress * kStress, d.cmp.strain, (int)d.cmp.fracture, flags,
+                n ? us / n : 0.0);
   }
   rfd_ctx_destroy(ctx);
   return 0;
