@@ -2739,6 +2739,168 @@ __global__ void __launch_bounds__(256) k_f64_forces(Sys64 s, Mat64 mt) {
   }
 }
 
+// The FP64 explicit step of a small system in one launch: k_f64_forces with,
+// once a particle's group has its sums, the first lane running k_f64_pass's
+// arithmetic on them in registers (platens, symplectic Euler, cut box) and
+// writing the next position, velocity and spin to the second buffers (other
+// groups still read the current ones); then the loading control of
+// k_nat_control by the last workgroup, as k_f64_pass_ctl does.  The platen
+// partials are per workgroup of this launch, so the reaction sums can differ
+// from the separate pass's in their last bits.
+struct Adv64 {
+  P4 *P;
+  double4 *V, *W;     // next positions, velocities, spins
+  rfd_loading *ctl;
+  double *part;       // platen partials, two per workgroup
+  uint32_t *done;     // workgroups done (zero between steps)
+  double *hist;
+  int *curp;          // bond-state index to flip
+  double dt, rho43pi;
+};
+
+__device__ __forceinline__ double atomic_read_f64(double *p) {
+  return __longlong_as_double(
+      (long long)atomicAdd(reinterpret_cast<unsigned long long *>(p), 0ull));
+}
+
+// the partials of a workgroup, stored with atomics; true in the last
+// workgroup to count itself done, whose threads then hold the fixed-order
+// reduction of every workgroup's partials (k_nat_control's order) in s0, s1
+__device__ __forceinline__ bool last_block_sums(double *part, uint32_t *done, double sb,
+                                                double stp, double &s0, double &s1) {
+  __shared__ double s_part[8];
+  __shared__ uint32_t sLast;
+  sb = wave_sum(sb);
+  stp = wave_sum(stp);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    s_part[2 * wid] = sb;
+    s_part[2 * wid + 1] = stp;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, c = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+      a += s_part[2 * w];
+      c += s_part[2 * w + 1];
+    }
+    atomicExch(reinterpret_cast<unsigned long long *>(part + 2 * blockIdx.x),
+               (unsigned long long)__double_as_longlong(a));
+    atomicExch(reinterpret_cast<unsigned long long *>(part + 2 * blockIdx.x + 1),
+               (unsigned long long)__double_as_longlong(c));
+  }
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  if (threadIdx.x == 0) sLast = atomicAdd(done, 1u) == gridDim.x - 1 ? 1u : 0u;
+  __syncthreads();
+  if (!sLast) return false;
+  double a = 0, c = 0;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
+    a += atomic_read_f64(part + 2 * b);
+    c += atomic_read_f64(part + 2 * b + 1);
+  }
+  a = wave_sum(a);
+  c = wave_sum(c);
+  __syncthreads();
+  if (lane == 0) {
+    s_part[2 * wid] = a;
+    s_part[2 * wid + 1] = c;
+  }
+  __syncthreads();
+  s0 = s1 = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+    s0 += s_part[2 * w];
+    s1 += s_part[2 * w + 1];
+  }
+  return true;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_f64_step(Sys64 s, Mat64 mt, Adv64 a) {
+  BSt64 *const st_in = *s.curp ? s.b1 : s.b0;
+  BSt64 *const st_out = *s.curp ? s.b0 : s.b1;
+  const rfd_loading L = *a.ctl;
+  double sb = 0.0, stp = 0.0;
+  const uint32_t g = threadIdx.x % kG64;
+  const uint32_t stride = grid_stride() / kG64;
+  for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / kG64; i < s.N; i += stride) {
+    const P4 A = s.P[i];
+    const d3 Vi = xyz(s.V[i]), Wi = xyz(s.W[i]);
+    d3 F = kZero64, Mb = kZero64, Mc = kZero64;
+    bool anyc = false;
+    const uint32_t h1 = s.hoff[i + 1];
+    for (uint32_t h = s.hoff[i]; h < h1; h += kG64) {
+      d3 dF = kZero64, dMb = kZero64, dMc = kZero64;
+      bool touch = false;
+      const bool on = h + g < h1 &&
+                      f64_half(s, mt, st_in, st_out, i, A, Vi, Wi, h + g, dF, dMb, dMc, touch);
+      const int flags = (on ? 1 : 0) | (touch ? 2 : 0);
+      for (uint32_t k = 0; k < kG64; ++k) {
+        const int fk = __shfl(flags, (int)k, (int)kG64);
+        const d3 x = grp_get(dF, k), y = grp_get(dMb, k), z = grp_get(dMc, k);
+        if (fk & 1) {
+          F = F + x;
+          Mb = Mb + y;
+          Mc = Mc + z;
+        }
+        anyc = anyc || (fk & 2);
+      }
+    }
+    const uint32_t nc = s.ccnt[i];
+    for (uint32_t k0 = 0; k0 < nc; k0 += kG64) {
+      d3 dF = kZero64, dMc = kZero64;
+      const bool on = k0 + g < nc && f64_contact(s, mt, i, A, Vi, Wi, k0 + g, dF, dMc);
+      const int flag = on ? 1 : 0;
+      for (uint32_t k = 0; k < kG64; ++k) {
+        const int fk = __shfl(flag, (int)k, (int)kG64);
+        const d3 x = grp_get(dF, k), z = grp_get(dMc, k);
+        if (fk) {
+          F = F + x;
+          Mc = Mc + z;
+          anyc = true;
+        }
+      }
+    }
+    if (mt.refmode && anyc && !(dot(Wi, Wi) > 1e-18)) Mc = kZero64;
+    if (g == 0) {
+      const d3 M = Mb + Mc;
+      P4 R = A;
+      d3 V = Vi;
+      if (MODE) platen<MODE>(L, R, V, F, sb, stp);
+      const double m = a.rho43pi * R.r * R.r * R.r, I = 0.4 * m * R.r * R.r;
+      V = V + F * (a.dt / m);
+      R.x += V.x * a.dt;
+      R.y += V.y * a.dt;
+      R.z += V.z * a.dt;
+      const d3 W = Wi + M * (a.dt / I);
+      if (L.cut) {
+        const double dx = R.x - L.box_c[0], dy = R.y - L.box_c[1], dz = R.z - L.box_c[2];
+        if (dx - L.box_h[0] > 0 || dx + L.box_h[0] < 0 || dy - L.box_h[1] > 0 ||
+            dy + L.box_h[1] < 0 || dz - L.box_h[2] > 0 || dz + L.box_h[2] < 0) {
+          R.x = L.hidden[0];
+          R.y = L.hidden[1];
+          R.z = L.hidden[2];
+          V = kZero64;
+          F = kZero64;
+        }
+      }
+      a.P[i] = R;
+      a.V[i] = d4(V);
+      a.W[i] = d4(W);
+      s.F[i] = d4(F);
+      s.M[i] = d4(M);
+    }
+  }
+  double s0, s1;
+  if (!last_block_sums(a.part, a.done, sb, stp, s0, s1)) return;
+  if (threadIdx.x != 0) return;
+  a.curp[0] = 1 - a.curp[0];
+  rfd_loading Lc = *a.ctl;
+  control_update(Lc, s0, s1, a.hist, (uint32_t)kHistCap, a.dt);
+  *a.ctl = Lc;
+  atomicExch(a.done, 0u);
+}
+
 struct Pass64 {
   uint32_t N;
   P4 *P;
@@ -2789,11 +2951,6 @@ __global__ void __launch_bounds__(256) k_f64_pass(Pass64 p) {
 // atomics, so that the last workgroup to count itself done sees every one; it
 // reduces them in k_nat_control's fixed order (the same 256 threads), applies
 // the control, flips the bond-state index and resets the counter.
-__device__ __forceinline__ double atomic_read_f64(double *p) {
-  return __longlong_as_double(
-      (long long)atomicAdd(reinterpret_cast<unsigned long long *>(p), 0ull));
-}
-
 template <int MODE>
 __global__ void __launch_bounds__(256) k_f64_pass_ctl(Pass64 p, uint32_t *done, double *hist,
                                                       int *curp) {
@@ -3100,6 +3257,9 @@ struct rfd_system {
   // steps run in pairs, so that P, V, W hold the state between calls
   P4 *P2 = nullptr;
   float4 *V2 = nullptr, *W2 = nullptr;
+  // the same for the FP64 path's one-launch steps (small systems)
+  double4 *V64b = nullptr, *W64b = nullptr;
+  double *part_step = nullptr;  // their platen partials, two per workgroup
   bool fuse = true;
   bool ctl_in_pass = true;  // small systems: the loading control in the force pass
   uint32_t nblocks_pass = 0;
@@ -3338,8 +3498,7 @@ Mat64 make_mat64(const rfd_system_params &p) {
 }
 
 // FP64 force pass; dtp, trial, nobreak and done as in forces() (FIRE passes)
-int forces64(rfd_system *s, const double *dtp = nullptr, bool trial = false,
-             bool nobreak = false, const int *done = nullptr, bool flip = true) {
+Sys64 make_sys64(rfd_system *s, const double *dtp, bool trial, const int *done) {
   Sys64 d{};
   d.N = s->N;
   d.Kc = s->p.contact_capacity;
@@ -3362,6 +3521,12 @@ int forces64(rfd_system *s, const double *dtp = nullptr, bool trial = false,
   d.done = done;
   d.trial = trial ? 1 : 0;
   d.nbreak = s->d_nbreak;
+  return d;
+}
+
+int forces64(rfd_system *s, const double *dtp = nullptr, bool trial = false,
+             bool nobreak = false, const int *done = nullptr, bool flip = true) {
+  Sys64 d = make_sys64(s, dtp, trial, done);
   Mat64 m = make_mat64(s->p);
   if (dtp) {
     m.pA = 0.0;
@@ -3757,8 +3922,64 @@ bool capturing(const rfd_system *s) {
 // and the two orders of steps give the same bits).  With multi-step launches
 // on, the steps between two contact-row rebuilds run as one launch (not
 // under graph capture).
+// FP64 explicit steps in one launch each (k_f64_step): small systems whose
+// force grid fits the one-round limit, one specimen
+bool can_fuse64(const rfd_system *s) {
+  return s->fuse && s->fp64 && !s->S && s->ctl_in_pass && s->P2 && s->V64b && s->W64b &&
+         rfd_blocks((size_t)s->N * kG64, 256) <= kCtlInPassTiles;
+}
+
+int step_fused64(rfd_system *s) {
+  if (!s->rows_valid || s->step_count % s->p.rebuild_every == 0) {
+    const int st = rebuild_rows(s);
+    if (st) return st;
+  }
+  const Sys64 d = make_sys64(s, nullptr, false, nullptr);
+  const Mat64 m = make_mat64(s->p);
+  Adv64 a{};
+  a.P = s->P2;
+  a.V = s->V64b;
+  a.W = s->W64b;
+  a.ctl = s->ctl;
+  a.part = s->part_step;
+  a.done = s->done_cnt;
+  a.hist = s->hist;
+  a.curp = s->d_cur;
+  a.dt = s->p.dt;
+  a.rho43pi = s->p.rho * 4.0 / 3.0 * M_PI;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (s->timing && s->ev_used + 2 <= s->ev.size()) {
+    e0 = s->ev[s->ev_used];
+    e1 = s->ev[s->ev_used + 1];
+    s->ev_used += 2;
+    RFD_HIP(s->ctx, hipEventRecord(e0, s->ctx->stream));
+  }
+  const unsigned grid = rfd_blocks((size_t)s->N * kG64, 256);
+  if (s->mode == 1)
+    k_f64_step<1><<<grid, 256, 0, s->ctx->stream>>>(d, m, a);
+  else if (s->mode == 2)
+    k_f64_step<2><<<grid, 256, 0, s->ctx->stream>>>(d, m, a);
+  else
+    k_f64_step<0><<<grid, 256, 0, s->ctx->stream>>>(d, m, a);
+  RFD_HIP(s->ctx, hipGetLastError());
+  if (e1) RFD_HIP(s->ctx, hipEventRecord(e1, s->ctx->stream));
+  std::swap(s->P, s->P2);
+  std::swap(s->V64, s->V64b);
+  std::swap(s->W64, s->W64b);
+  s->step_count++;
+  return RFD_OK;
+}
+
 int run_steps(rfd_system *s, uint32_t n) {
   uint32_t k = 0;
+  if (can_fuse64(s)) {
+    // in pairs, so that P, V, W hold the state between calls
+    for (; k + 2 <= n; k += 2) {
+      const int st = step_fused64(s);
+      const int st2 = st ? st : step_fused64(s);
+      if (st2) return st2;
+    }
+  }
   if (can_fuse(s)) {
     bool multi = s->persist && n >= 2 && !capturing(s) && steps_fit(s);
     while (k + 2 <= n) {
@@ -4138,6 +4359,12 @@ int rfd_system_create(rfd_ctx *ctx, const rfd_system_params *p, uint32_t N, rfd_
     if (!st) st = dalloc(ctx, &s->V2, N);
     if (!st) st = dalloc(ctx, &s->W2, N);
   }
+  if (!st && s->fp64 && rfd_blocks((size_t)N * kG64, 256) <= kCtlInPassTiles) {
+    st = dalloc(ctx, &s->P2, N);
+    if (!st) st = dalloc(ctx, &s->V64b, N);
+    if (!st) st = dalloc(ctx, &s->W64b, N);
+    if (!st) st = dalloc(ctx, &s->part_step, 2 * (size_t)kCtlInPassTiles);
+  }
   if (!st) {
     rfd_loading L{};
     hipError_t e = hipMemcpy(s->ctl, &L, sizeof(L), hipMemcpyHostToDevice);
@@ -4200,6 +4427,9 @@ int rfd_system_destroy(rfd_system *s) {
   dfree(s->Vf64);
   dfree(s->Th64);
   dfree(s->P2);
+  dfree(s->V64b);
+  dfree(s->W64b);
+  dfree(s->part_step);
   dfree(s->V2);
   dfree(s->W2);
   dfree(s->hist);

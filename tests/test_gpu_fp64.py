@@ -149,3 +149,38 @@ def test_fire_brazil_fp32_vs_fp64(ctx):
     assert diff[same].max() <= 1e-5, (same, diff)
     assert diff.max() <= 3e-3, diff
     assert ctx.flags() == 0
+
+
+def test_fp64_one_launch_steps_match_separate_pass(ctx):
+    """A small FP64 system runs its explicit steps in pairs of one launch each
+    (k_f64_step: the particle update on the force pass's sums in registers,
+    the loading control by the last workgroup).  Against the same system
+    stepped one step per call (the force pass, then the particle pass with the
+    control), with weak bonds breaking and the platens loading the specimen:
+    the state and the bond flags are the same bits, and the loading history
+    agrees to 1e-12 of its largest stress (the platen reactions are summed per
+    workgroup of the launch that forms them, so their last bits may differ)."""
+    from test_gpu_batch import _same_state
+
+    def build():
+        s = native_cases.uniaxial_cylinder(ctx, 8.0, 16.0, seed=9, strength=(4e6, 8e6),
+                                           v_load=-0.05, fp64=True).sys
+        s.seat_platens(0.05)
+        return s
+
+    a, b = build(), build()
+    n0 = a.bond_counts()[0]
+    for n in (2, 40, 38, 60, 80):
+        print(f"fp64: {n} steps", flush=True)
+        a.step(n)
+        for _ in range(n):
+            b.step(1)
+        _same_state(a, b, label=f"fp64 +{n}")
+        assert np.array_equal(a.bond_state()[1], b.bond_state()[1])
+    ha, hb = a.history(), b.history()
+    assert ha.shape == hb.shape and len(ha) == 220
+    np.testing.assert_allclose(ha, hb, rtol=0, atol=1e-12 * np.abs(hb).max())
+    assert a.loading().loading and a.bond_counts()[0] < n0
+    assert ctx.flags() == 0
+    a.close()
+    b.close()
