@@ -138,7 +138,7 @@ k_scan_cells(uint32_t *__restrict__ pnC, uint32_t C, uint32_t *__restrict__ curs
     v[k] = (b0 + k < C) ? cnt[b0 + k] : 0u;
     local += v[k];
   }
-  uint32_t run = part[blockIdx.x] + block_prefix(local, s_sum);
+  uint32_t run = (part ? part[blockIdx.x] : 0u) + block_prefix(local, s_sum);
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
     if (b0 + k < C) {
@@ -550,7 +550,7 @@ k_scan_excl(const uint32_t *__restrict__ cnt, uint32_t n, uint32_t *__restrict__
     v[k] = (b0 + k < n) ? cnt[b0 + k] : 0u;
     local += v[k];
   }
-  uint32_t run = part[blockIdx.x] + block_prefix(local, s_sum);
+  uint32_t run = (part ? part[blockIdx.x] : 0u) + block_prefix(local, s_sum);
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
     if (b0 + k < n) {
@@ -747,9 +747,12 @@ int cell_distribution_ws(rfd_ctx *ctx, uint32_t *ws, const float *R, uint32_t N,
   k_fill_u32<<<rfd_blocks(2 * (size_t)C, 256, 4096), 256, 0, s>>>(pnC, 2 * (size_t)C, 0u);
   k_cell_index<<<rfd_blocks(N, 256), 256, 0, s>>>(R, N, inv_a, Nx, Ny, Nz, CI, pnC + C,
                                                   ctx->d_flags);
-  k_chunk_sums<<<chunks, kScanThreads, 0, s>>>(pnC + C, C, part);
-  k_scan_chunk_sums<<<1, kScanThreads, 0, s>>>(part, chunks);
-  k_scan_cells<<<chunks, kScanThreads, 0, s>>>(pnC, C, cursor, part);
+  if (chunks > 1) {
+    k_chunk_sums<<<chunks, kScanThreads, 0, s>>>(pnC + C, C, part);
+    k_scan_chunk_sums<<<1, kScanThreads, 0, s>>>(part, chunks);
+  }
+  // one chunk: its carry is zero (no chunk sums to form)
+  k_scan_cells<<<chunks, kScanThreads, 0, s>>>(pnC, C, cursor, chunks > 1 ? part : nullptr);
   k_scatter<<<rfd_blocks(N, 256), 256, 0, s>>>(CI, N, cursor, tmp);
   k_rank_cells<<<rfd_blocks(C, 4), 256, 0, s>>>(tmp, pnC, C, N, CIs);
   RFD_LAUNCHED(ctx, "rfd_cell_distribution");
@@ -890,9 +893,11 @@ int rfd_il_reconstruct_bpm(rfd_ctx *ctx, const float *R, uint32_t N,
   k_fill_u32<<<rfd_blocks(Cf, 256, 4096), 256, 0, q>>>(cnt, Cf, 0u);
   k_place_of<<<rfd_blocks(N, 256), 256, 0, q>>>(CIs, N, placeOf);
   k_fine_index<<<rfd_blocks(N, 256), 256, 0, q>>>(R, N, f, fci, cnt);
-  k_chunk_sums<<<chunks, kScanThreads, 0, q>>>(cnt, Cf, part);
-  k_scan_chunk_sums<<<1, kScanThreads, 0, q>>>(part, chunks);
-  k_scan_excl<<<chunks, kScanThreads, 0, q>>>(cnt, Cf, excl, part);
+  if (chunks > 1) {
+    k_chunk_sums<<<chunks, kScanThreads, 0, q>>>(cnt, Cf, part);
+    k_scan_chunk_sums<<<1, kScanThreads, 0, q>>>(part, chunks);
+  }
+  k_scan_excl<<<chunks, kScanThreads, 0, q>>>(cnt, Cf, excl, chunks > 1 ? part : nullptr);
   k_fill_u32<<<rfd_blocks(Cf, 256, 4096), 256, 0, q>>>(cnt, Cf, 0u);
   k_fine_scatter<<<rfd_blocks(N, 256), 256, 0, q>>>(R, N, fci, placeOf, excl, cnt, Rf);
   const uint32_t per = (blocks + kXcds - 1) / kXcds;
