@@ -493,10 +493,17 @@ int rfd_system_get_tiles(rfd_system *sys, int *force_tile, int *nbr_block, int *
  * specimen in the FP32 physical mode and the single-evaluation force pass,
  * rfd_system_step runs the platens, integration and cut box of each particle
  * inside the force pass, pairs of steps alternating between two particle
- * buffers.  On by default; the results are the same bits with it off.  It
- * also moves the FIRE measure of rfd_system_fire_load_step (platens at rest,
- * power, norms, reactions) into the epilogue of the quasi-static force pass;
- * there only the order of the FP64 partial sums differs.
+ * buffers.  On by default; the results are the same bits with it off.  A
+ * system of at most 256 force tiles also runs the loading control
+ * (SumForcesULoading, md_results.cu:34) in the force pass, by its last
+ * workgroup, so that each step is one launch (same bits).  An FP64 system
+ * whose force pass has at most 256 workgroups runs each step as one launch
+ * too; there the platen reactions are summed per workgroup of that launch, so
+ * the reaction sums (and the history) may differ from the separate pass's in
+ * their last bits, never the particle state.  Fusion also moves the FIRE
+ * measure of rfd_system_fire_load_step (platens at rest, power, norms,
+ * reactions) into the epilogue of the quasi-static force pass; there only the
+ * order of the FP64 partial sums differs.
  * rfd_system_get_fusion reports whether steps currently run fused. */
 int rfd_system_set_fusion(rfd_system *sys, int on);
 int rfd_system_get_fusion(rfd_system *sys, int *fused);

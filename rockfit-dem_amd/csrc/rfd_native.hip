@@ -5202,7 +5202,7 @@ int rfd_system_set_fusion(rfd_system *s, int on) {
 
 int rfd_system_get_fusion(rfd_system *s, int *fused) {
   if (!valid(s) || !fused) return RFD_E_ARG;
-  *fused = can_fuse(s) ? 1 : 0;
+  *fused = can_fuse(s) || can_fuse64(s) ? 1 : 0;
   return RFD_OK;
 }
 

@@ -82,3 +82,40 @@ def test_contacts_only(ctx, broken):
     assert ctx.flags() == 0
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("fp64", [False, True], ids=["fp32", "fp64"])
+@pytest.mark.parametrize("n", [2, 3, 70])
+def test_one_launch_steps_of_tiny_systems(ctx, n, fp64):
+    """Chains of 2, 3 and 70 bonded spheres (one partial force tile, two
+    tiles) between uniaxial platens that close on them: the one-launch steps
+    of small systems (the particle update on the force pass's sums, the
+    loading control by the pass's last workgroup) against the same system with
+    the separate particle pass and control launch.  The state is the same
+    bits; the loading history agrees to 1e-12 of its largest stress (FP64
+    sums its platen reactions per workgroup of the launch that forms them)."""
+    from test_gpu_fire_oracle import _chain
+
+    a, _ = _chain(ctx, n, fp64)
+    b, _ = _chain(ctx, n, fp64)
+    for s in (a, b):
+        L = s.loading()
+        L.v_load = -2e-3
+        L.v = L.v_load
+        s.set_loading(L)
+    b.set_fusion(False)
+    assert a.fused() and not b.fused()
+    for k in (2, 40, 38):
+        a.step(k)
+        for _ in range(k):
+            b.step(1)
+        sa, sb = a.state(), b.state()
+        for key in ("R", "V", "W", "F", "M"):
+            assert np.array_equal(sa[key], sb[key]), (n, fp64, k, key)
+    ha, hb = a.history(), b.history()
+    assert ha.shape == hb.shape == (80, 3)
+    assert np.abs(hb[:, 1:]).max() > 0  # the platens carry load
+    np.testing.assert_allclose(ha, hb, rtol=0, atol=1e-12 * np.abs(hb).max())
+    assert ctx.flags() == 0
+    a.close()
+    b.close()
