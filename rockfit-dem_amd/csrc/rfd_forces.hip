@@ -26,6 +26,17 @@ __device__ __forceinline__ void st3(float *p, size_t s, float x, float y, float 
   p[3 * s + 2] = z;
 }
 
+// A particle's F and M, read before the slot terms are formed so that the
+// row sums at the end wait for LDS only.
+struct RowFM {
+  float fx, fy, fz, mx, my, mz;
+};
+__device__ __forceinline__ RowFM load_fm(const float *__restrict__ F,
+                                         const float *__restrict__ M, uint32_t ip, size_t N1,
+                                         size_t N2) {
+  return RowFM{F[ip], F[ip + N1], F[ip + N2], M[ip], M[ip + N1], M[ip + N2]};
+}
+
 // d_CalculateForcesDEM_22 (md_calculateforcesdem3.cu:424-483).  A workgroup
 // takes the rows of P particles (P K <= 512 slots), lists their live bond
 // slots in slot order (a ballot per wave), evaluates the bond law of each
@@ -42,30 +53,49 @@ __host__ __device__ inline uint32_t bpm_rows(uint32_t K) {
   return min(kBpmSlots / K, (uint32_t)kSlotThreads);
 }
 
-// Lists, in slot order, the row-local slots ls < nsl of a tile for which
-// keep(ls) holds (one ballot per wave and sweep of kSlotThreads slots) and
-// returns how many.  Every thread of the workgroup calls it.
+// Lists, in slot order, the row-local slots ls < nsl <= kBpmSlots of a tile
+// for which keep(ls, val) holds, stores the bits of the val it sets beside
+// each listed slot in s_val, and returns how many.  The keep() tests of all
+// sweeps of kSlotThreads slots are made first, so their loads are in flight
+// together; one ballot per wave and sweep, one barrier.  Every thread of the
+// workgroup calls it.
+constexpr uint32_t kListSweeps = kBpmSlots / kSlotThreads;
 template <class Keep>
-__device__ __forceinline__ uint32_t list_slots(uint32_t nsl, uint16_t *s_slot,
-                                               uint32_t *s_wave, Keep keep) {
+__device__ __forceinline__ uint32_t list_slots(uint32_t nsl, uint16_t *s_slot, float *s_val,
+                                               uint32_t (*s_wave)[kSlotThreads / 64],
+                                               Keep keep) {
   const uint32_t t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  bool kept[kListSweeps];
+  uint32_t val[kListSweeps];
+#pragma unroll
+  for (uint32_t r = 0; r < kListSweeps; ++r) {
+    const uint32_t ls = r * kSlotThreads + t;
+    val[r] = 0;
+    kept[r] = ls < nsl && keep(ls, val[r]);
+  }
+  uint64_t bal[kListSweeps];
+#pragma unroll
+  for (uint32_t r = 0; r < kListSweeps; ++r) {
+    bal[r] = __ballot(kept[r]);
+    if (lane == 0) s_wave[r][wid] = (uint32_t)__popcll(bal[r]);
+  }
+  __syncthreads();
   uint32_t n = 0;
-  for (uint32_t r0 = 0; r0 < nsl; r0 += kSlotThreads) {
-    const uint32_t ls = r0 + t;
-    const bool kept = ls < nsl && keep(ls);
-    const uint64_t bal = __ballot(kept);
-    if (lane == 0) s_wave[wid] = (uint32_t)__popcll(bal);
-    __syncthreads();
+#pragma unroll
+  for (uint32_t r = 0; r < kListSweeps; ++r) {
     uint32_t before = 0, all = 0;
     for (uint32_t w = 0; w < kSlotThreads / 64; ++w) {
-      before += w < wid ? s_wave[w] : 0u;
-      all += s_wave[w];
+      before += w < wid ? s_wave[r][w] : 0u;
+      all += s_wave[r][w];
     }
-    if (kept)
-      s_slot[n + before + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = (uint16_t)ls;
+    if (kept[r]) {
+      const uint32_t at = n + before + (uint32_t)__popcll(bal[r] & ((1ull << lane) - 1ull));
+      s_slot[at] = (uint16_t)(r * kSlotThreads + t);
+      s_val[at] = __uint_as_float(val[r]);
+    }
     n += all;
-    __syncthreads();
   }
+  __syncthreads();
   return n;
 }
 
@@ -98,7 +128,7 @@ k_forces_bpm(const float *__restrict__ R, const float *__restrict__ V,
   // force, moment of the tangential force (3 floats each)
   __shared__ float s_c[15][kBpmSlots];
   __shared__ uint16_t s_slot[kBpmSlots];  // row-local slot (lp K + k) of each listed slot
-  __shared__ uint32_t s_wave[kSlotThreads / 64];
+  __shared__ uint32_t s_wave[kListSweeps][kSlotThreads / 64];
   const uint32_t P = bpm_rows(K);
   const uint32_t t = threadIdx.x;
   const uint32_t blk = xcd_block(per);
@@ -110,16 +140,24 @@ k_forces_bpm(const float *__restrict__ R, const float *__restrict__ V,
   const size_t N1 = N, N2 = 2 * (size_t)N;
   const float area_k = b_r * b_r * kPiF;          // b_r*b_r*MCf_pi
   const float j4 = kPiF * b_r * b_r * b_r * b_r;  // MCf_pi*b_r^4
-  // 1. the live bond slots, in slot order
-  const uint32_t nlive = list_slots(nsl, s_slot, s_wave, [&](uint32_t ls) {
-    return IL[g0 + ls] != kSentinel && ILtype[g0 + ls] == 1;
+  // the row's F and M, read now so that the row sums wait for LDS only
+  const uint32_t ip = p0 + t;
+  RowFM fm{};
+  if (t < np) fm = load_fm(F, M, ip, N1, N2);
+  // 1. the live bond slots, in slot order, and their partners (in the last
+  // term's column: the one thread that evaluates a listed slot reads its
+  // partner there before it stores the term)
+  const uint32_t nlive = list_slots(nsl, s_slot, s_c[14], s_wave, [&](uint32_t ls, uint32_t &j) {
+    j = IL[g0 + ls];
+    const uint8_t ty = ILtype[g0 + ls];  // read with the id, not after it
+    return j != kSentinel && ty == 1;
   });
   // 2. the law of each listed slot
   for (uint32_t q = t; q < nlive; q += kSlotThreads) {
     const uint32_t ls = s_slot[q];
     const uint32_t i = p0 + ls / K;
     const size_t s = g0 + ls;
-    const uint32_t j = IL[s];
+    const uint32_t j = __float_as_uint(s_c[14][q]);
     const float Rix = R[i], Riy = R[i + N1], Riz = R[i + N2];
     const float Vix = V[i], Viy = V[i + N1], Viz = V[i + N2];
     const float Wix = W[i], Wiy = W[i + N1], Wiz = W[i + N2];
@@ -215,7 +253,6 @@ k_forces_bpm(const float *__restrict__ R, const float *__restrict__ V,
   if (t >= np) return;
   uint32_t q0, q1;
   row_range(s_slot, nlive, t, K, q0, q1);
-  const uint32_t ip = p0 + t;
   float fsx = 0, fsy = 0, fsz = 0, msx = 0, msy = 0, msz = 0;
   for (uint32_t q = q0; q < q1; ++q) {
     fsx = fsx + s_c[0][q];
@@ -234,25 +271,175 @@ k_forces_bpm(const float *__restrict__ R, const float *__restrict__ V,
     msy = msy + s_c[13][q];
     msz = msz + s_c[14][q];
   }
-  F[ip] += fsx;
-  F[ip + N1] += fsy;
-  F[ip + N2] += fsz;
-  M[ip] += msx;
-  M[ip + N1] += msy;
-  M[ip + N2] += msz;
+  F[ip] = fm.fx + fsx;
+  F[ip + N1] = fm.fy + fsy;
+  F[ip + N2] = fm.fz + fsz;
+  M[ip] = fm.mx + msx;
+  M[ip + N1] = fm.my + msy;
+  M[ip + N2] = fm.mz + msz;
 }
 
 // d_CalculateForcesDEM_21 (md_calculateforcesdem3.cu:338-422), slot-parallel
 // like k_forces_bpm: the terms of each contact slot go to LDS, the row sums
 // are formed in slot order, the rolling-moment wipe (:330-335) included.
+struct ContactLaw {
+  float dt, Estar, Gstar, pA, mu, mur, m, r;
+};
+
+// The law of a contact slot s (row particle i, partner j, centre distance
+// rm < 2r along r = (rx, ry, rz)): stores the slot's new tangential spring in
+// Oijt and the twelve terms of the row sums in column c of s_c (normal force,
+// tangential force, its moment, rolling moment); returns 1, or 2 when the
+// particle does not spin and the reference clears ms instead of adding a
+// rolling moment.
+template <uint32_t C>
+__device__ __forceinline__ uint8_t contact_law(const float *__restrict__ V,
+                                               const float *__restrict__ W,
+                                               float *__restrict__ Oijt, uint32_t i,
+                                               uint32_t j, size_t s, size_t N1, size_t N2,
+                                               float rx, float ry, float rz, float rm,
+                                               const ContactLaw &L, float (&s_c)[12][C],
+                                               uint32_t c) {
+  const float r = L.r;
+  const float inv_rm = 1.0f / rm;
+  const float nx = rx * inv_rm, ny = ry * inv_rm, nz = rz * inv_rm;
+  const float Vix = V[i], Viy = V[i + N1], Viz = V[i + N2];
+  const float Wix = W[i], Wiy = W[i + N1], Wiz = W[i + N2];
+  // dd_Calculate_fijn_ppHM_2 (:235-252)
+  const float q = sqrtf(r * fabsf(r - 0.5f * rm));
+  const float Kn = 2.0f * L.Estar * q;
+  const float Kt = 8.0f * L.Gstar * q;
+  const float Wjx = W[j], Wjy = W[j + N1], Wjz = W[j + N2];
+  const float wsx = Wjx + Wix, wsy = Wjy + Wiy, wsz = Wjz + Wiz;
+  const float vx = V[j] - Vix + r * (ny * wsz - nz * wsy);
+  const float vy = V[j + N1] - Viy + r * (nz * wsx - nx * wsz);
+  const float vz = V[j + N2] - Viz + r * (nx * wsy - ny * wsx);
+  const float vn = nx * vx + ny * vy + nz * vz;
+  const float fn = (rm - 2.0f * r) * Kn * kTwoThirdsF;
+  const float dmp = -1.8257f * L.pA * vn * sqrtf(Kn * 0.5f * L.m);
+  s_c[0][c] = nx * (fn + dmp);
+  s_c[1][c] = ny * (fn + dmp);
+  s_c[2][c] = nz * (fn + dmp);
+  const float vtx = vx - nx * vn, vty = vy - ny * vn, vtz = vz - nz * vn;
+  // dd_Calculate_fijtmijt_ppHM_2 (:255-314)
+  const f3 O = ld3(Oijt, s);
+  float c1 = nx * O.x + ny * O.y + nz * O.z;
+  float ox = O.x - nx * c1, oy = O.y - ny * c1, oz = O.z - nz * c1;
+  c1 = ox * ox + oy * oy + oz * oz;
+  if ((double)c1 > 1e-18) {
+    c1 = 1.0f / c1;
+    c1 = c1 * (O.x * O.x + O.y * O.y + O.z * O.z);
+    c1 = sqrtf(c1);
+    ox = ox * c1;
+    oy = oy * c1;
+    oz = oz * c1;
+  }
+  ox = ox + vtx * L.dt;
+  oy = oy + vty * L.dt;
+  oz = oz + vtz * L.dt;
+  float ftx = ox * Kt, fty = oy * Kt, ftz = oz * Kt;
+  const float ftm = ftx * ftx + fty * fty + ftz * ftz;
+  if (ftm > L.mu * L.mu * fn * fn) {
+    c1 = L.mu * fabsf(fn) * rsqrt_rn(ftm);
+    ftx = ftx * c1;
+    fty = fty * c1;
+    ftz = ftz * c1;
+    c1 = 1.0f / Kt;
+    ox = ftx * c1;
+    oy = fty * c1;
+    oz = ftz * c1;
+  } else {
+    c1 = -1.8257f * L.pA * sqrtf(Kt * 0.5f * L.m);
+    ftx = ftx + vtx * c1;
+    fty = fty + vty * c1;
+    ftz = ftz + vtz * c1;
+  }
+  st3(Oijt, s, ox, oy, oz);
+  s_c[3][c] = ftx;
+  s_c[4][c] = fty;
+  s_c[5][c] = ftz;
+  s_c[6][c] = (ny * ftz - nz * fty) * r;
+  s_c[7][c] = (nz * ftx - nx * ftz) * r;
+  s_c[8][c] = (nx * fty - ny * ftx) * r;
+  // dd_Calculate_mijroll_ppHM_2 (:316-336): the else branch clears ms
+  const float wim = Wix * Wix + Wiy * Wiy + Wiz * Wiz;
+  if ((double)wim > 1e-18) {
+    const float cr = -L.mur * fabsf(fn) * rsqrt_rn(wim);
+    s_c[9][c] = Wix * cr;
+    s_c[10][c] = Wiy * cr;
+    s_c[11][c] = Wiz * cr;
+    return 1;
+  }
+  return 2;
+}
+
+// Adds the terms of one live slot q (s_live value lv) to the row sums, as the
+// reference's row loop does: fs += normal, fs += tangential, ms += moment,
+// then ms += rolling moment or, for lv == 2, ms = 0.
+template <uint32_t C>
+__device__ __forceinline__ void add_contact_terms(const float (&s_c)[12][C], uint32_t q,
+                                                  uint32_t lv, float (&f)[3], float (&mm)[3]) {
+  f[0] = f[0] + s_c[0][q];
+  f[1] = f[1] + s_c[1][q];
+  f[2] = f[2] + s_c[2][q];
+  f[0] = f[0] + s_c[3][q];
+  f[1] = f[1] + s_c[4][q];
+  f[2] = f[2] + s_c[5][q];
+  mm[0] = mm[0] + s_c[6][q];
+  mm[1] = mm[1] + s_c[7][q];
+  mm[2] = mm[2] + s_c[8][q];
+  if (lv == 1) {
+    mm[0] = mm[0] + s_c[9][q];
+    mm[1] = mm[1] + s_c[10][q];
+    mm[2] = mm[2] + s_c[11][q];
+  } else {
+    mm[0] = 0;
+    mm[1] = 0;
+    mm[2] = 0;
+  }
+}
+
+// Row t's sums of the workgroup (slots t K .. t K + K - 1 of s_c and s_live),
+// in slot order, added to the particle's F and M read earlier (fm) and
+// stored.  WORDS: K is a multiple of 4 and s_live is read four slots per
+// 32-bit word, skipping words without a live slot.
+template <bool WORDS, uint32_t C>
+__device__ __forceinline__ void contact_row_sums(const float (&s_c)[12][C],
+                                                 const uint8_t *s_live, uint32_t t, uint32_t K,
+                                                 const RowFM &fm, uint32_t ip, size_t N1,
+                                                 size_t N2, float *__restrict__ F,
+                                                 float *__restrict__ M) {
+  float f[3] = {0, 0, 0}, mm[3] = {0, 0, 0};
+  if constexpr (WORDS) {
+    const uint32_t *w4 = reinterpret_cast<const uint32_t *>(s_live);
+    for (uint32_t w = t * K / 4; w < (t + 1) * K / 4; ++w) {
+      uint32_t lw = w4[w];
+      while (lw) {
+        const uint32_t u = __builtin_ctz(lw) >> 3;
+        add_contact_terms(s_c, 4 * w + u, (lw >> (8 * u)) & 0xFFu, f, mm);
+        lw &= ~(0xFFu << (8 * u));
+      }
+    }
+  } else {
+    for (uint32_t q = t * K; q < (t + 1) * K; ++q)
+      if (s_live[q]) add_contact_terms(s_c, q, s_live[q], f, mm);
+  }
+  F[ip] = fm.fx + f[0];
+  F[ip + N1] = fm.fy + f[1];
+  F[ip + N2] = fm.fz + f[2];
+  M[ip] = fm.mx + mm[0];
+  M[ip + N1] = fm.my + mm[1];
+  M[ip + N2] = fm.mz + mm[2];
+}
+
+// One thread per slot, floor(256 / K) rows per workgroup: for row lengths
+// that are not a multiple of 4.
 __global__ void __launch_bounds__(kSlotThreads)
 k_forces_contact(const float *__restrict__ R, const float *__restrict__ V,
                  const float *__restrict__ W, const uint32_t *__restrict__ IL,
                  const uint8_t *__restrict__ ILtype, float *__restrict__ Oijt,
                  float *__restrict__ F, float *__restrict__ M, uint32_t N, uint32_t K,
-                 float dt, float Estar, float Gstar, float pA, float mu, float mur,
-                 float m, float r, uint32_t nblk, uint32_t per) {
-  // per slot: normal force, tangential force, its moment, rolling moment
+                 ContactLaw L, uint32_t nblk, uint32_t per) {
   __shared__ float s_c[12][kSlotThreads];
   __shared__ uint8_t s_live[kSlotThreads];  // 0 none, 1 contact, 2 contact that wipes ms
   const uint32_t P = kSlotThreads / K;
@@ -262,6 +449,10 @@ k_forces_contact(const float *__restrict__ R, const float *__restrict__ V,
   const uint32_t p0 = blk * P;
   const uint32_t i = p0 + lp;
   const size_t N1 = N, N2 = 2 * (size_t)N;
+  const uint32_t ip = p0 + t;
+  const bool row = t < P && ip < N;
+  RowFM fm{};
+  if (row) fm = load_fm(F, M, ip, N1, N2);
   uint8_t live = 0;
   if (lp < P && i < N) {
     const size_t s = (size_t)i * K + k;
@@ -269,118 +460,100 @@ k_forces_contact(const float *__restrict__ R, const float *__restrict__ V,
     if (j == kSentinel) {
       st3(Oijt, s, 0.0f, 0.0f, 0.0f);
     } else if (ILtype[s] == 0) {
-      const float Rix = R[i], Riy = R[i + N1], Riz = R[i + N2];
-      const float rx = R[j] - Rix, ry = R[j + N1] - Riy, rz = R[j + N2] - Riz;
+      const float rx = R[j] - R[i], ry = R[j + N1] - R[i + N1], rz = R[j + N2] - R[i + N2];
       const float rm = norm3(rx, ry, rz);
-      const float inv_rm = 1.0f / rm;
-      const float nx = rx * inv_rm, ny = ry * inv_rm, nz = rz * inv_rm;
-      if (!(rm < 2.0f * r)) {
+      if (!(rm < 2.0f * L.r))
         st3(Oijt, s, 0.0f, 0.0f, 0.0f);
-      } else {
-        const float Vix = V[i], Viy = V[i + N1], Viz = V[i + N2];
-        const float Wix = W[i], Wiy = W[i + N1], Wiz = W[i + N2];
-        // dd_Calculate_fijn_ppHM_2 (:235-252)
-        const float q = sqrtf(r * fabsf(r - 0.5f * rm));
-        const float Kn = 2.0f * Estar * q;
-        const float Kt = 8.0f * Gstar * q;
-        const float Wjx = W[j], Wjy = W[j + N1], Wjz = W[j + N2];
-        const float wsx = Wjx + Wix, wsy = Wjy + Wiy, wsz = Wjz + Wiz;
-        const float vx = V[j] - Vix + r * (ny * wsz - nz * wsy);
-        const float vy = V[j + N1] - Viy + r * (nz * wsx - nx * wsz);
-        const float vz = V[j + N2] - Viz + r * (nx * wsy - ny * wsx);
-        const float vn = nx * vx + ny * vy + nz * vz;
-        const float fn = (rm - 2.0f * r) * Kn * kTwoThirdsF;
-        const float dmp = -1.8257f * pA * vn * sqrtf(Kn * 0.5f * m);
-        s_c[0][t] = nx * (fn + dmp);
-        s_c[1][t] = ny * (fn + dmp);
-        s_c[2][t] = nz * (fn + dmp);
-        const float vtx = vx - nx * vn, vty = vy - ny * vn, vtz = vz - nz * vn;
-        // dd_Calculate_fijtmijt_ppHM_2 (:255-314)
-        const f3 O = ld3(Oijt, s);
-        float c1 = nx * O.x + ny * O.y + nz * O.z;
-        float ox = O.x - nx * c1, oy = O.y - ny * c1, oz = O.z - nz * c1;
-        c1 = ox * ox + oy * oy + oz * oz;
-        if ((double)c1 > 1e-18) {
-          c1 = 1.0f / c1;
-          c1 = c1 * (O.x * O.x + O.y * O.y + O.z * O.z);
-          c1 = sqrtf(c1);
-          ox = ox * c1;
-          oy = oy * c1;
-          oz = oz * c1;
-        }
-        ox = ox + vtx * dt;
-        oy = oy + vty * dt;
-        oz = oz + vtz * dt;
-        float ftx = ox * Kt, fty = oy * Kt, ftz = oz * Kt;
-        const float ftm = ftx * ftx + fty * fty + ftz * ftz;
-        if (ftm > mu * mu * fn * fn) {
-          c1 = mu * fabsf(fn) * rsqrt_rn(ftm);
-          ftx = ftx * c1;
-          fty = fty * c1;
-          ftz = ftz * c1;
-          c1 = 1.0f / Kt;
-          ox = ftx * c1;
-          oy = fty * c1;
-          oz = ftz * c1;
-        } else {
-          c1 = -1.8257f * pA * sqrtf(Kt * 0.5f * m);
-          ftx = ftx + vtx * c1;
-          fty = fty + vty * c1;
-          ftz = ftz + vtz * c1;
-        }
-        st3(Oijt, s, ox, oy, oz);
-        s_c[3][t] = ftx;
-        s_c[4][t] = fty;
-        s_c[5][t] = ftz;
-        s_c[6][t] = (ny * ftz - nz * fty) * r;
-        s_c[7][t] = (nz * ftx - nx * ftz) * r;
-        s_c[8][t] = (nx * fty - ny * ftx) * r;
-        // dd_Calculate_mijroll_ppHM_2 (:316-336): the else branch clears ms
-        const float wim = Wix * Wix + Wiy * Wiy + Wiz * Wiz;
-        if ((double)wim > 1e-18) {
-          const float cr = -mur * fabsf(fn) * rsqrt_rn(wim);
-          s_c[9][t] = Wix * cr;
-          s_c[10][t] = Wiy * cr;
-          s_c[11][t] = Wiz * cr;
-          live = 1;
-        } else {
-          live = 2;
-        }
-      }
+      else
+        live = contact_law(V, W, Oijt, i, j, s, N1, N2, rx, ry, rz, rm, L, s_c, t);
     }
   }
   s_live[t] = live;
   __syncthreads();
+  if (!row) return;
+  contact_row_sums<false>(s_c, s_live, t, K, fm, ip, N1, N2, F, M);
+}
+
+// Four consecutive slots per thread (K a multiple of 4, so that they share
+// a row and their ids, types and springs are 16-, 4- and 16-byte aligned),
+// 1024 slots per workgroup: one 16-byte id load and one 4-byte type load per
+// four slots, the partners' positions gathered together, and the springs of
+// four empty or apart slots cleared by three 16-byte stores.  Most slots are
+// empty, bonds, or candidates that do not touch, so this is the common path.
+constexpr uint32_t kContact4Slots = 4 * kSlotThreads;
+__global__ void __launch_bounds__(kSlotThreads)
+k_forces_contact4(const float *__restrict__ R, const float *__restrict__ V,
+                  const float *__restrict__ W, const uint32_t *__restrict__ IL,
+                  const uint8_t *__restrict__ ILtype, float *__restrict__ Oijt,
+                  float *__restrict__ F, float *__restrict__ M, uint32_t N, uint32_t K,
+                  ContactLaw L, uint32_t nblk, uint32_t per) {
+  __shared__ float s_c[12][kContact4Slots];
+  __shared__ uint32_t s_live4[kSlotThreads];  // s_live of slots 4t .. 4t + 3, a byte each
+  const uint32_t P = kContact4Slots / K;
+  const uint32_t t = threadIdx.x;
+  const uint32_t blk = xcd_block(per);
+  if (blk >= nblk) return;
+  const uint32_t p0 = blk * P;
+  const uint32_t i = p0 + 4 * t / K;
+  const size_t N1 = N, N2 = 2 * (size_t)N;
   const uint32_t ip = p0 + t;
-  if (t >= P || ip >= N) return;
-  float fsx = 0, fsy = 0, fsz = 0, msx = 0, msy = 0, msz = 0;
-  for (uint32_t q = t * K; q < (t + 1) * K; ++q) {
-    if (!s_live[q]) continue;
-    fsx = fsx + s_c[0][q];
-    fsy = fsy + s_c[1][q];
-    fsz = fsz + s_c[2][q];
-    fsx = fsx + s_c[3][q];
-    fsy = fsy + s_c[4][q];
-    fsz = fsz + s_c[5][q];
-    msx = msx + s_c[6][q];
-    msy = msy + s_c[7][q];
-    msz = msz + s_c[8][q];
-    if (s_live[q] == 1) {
-      msx = msx + s_c[9][q];
-      msy = msy + s_c[10][q];
-      msz = msz + s_c[11][q];
+  const bool row = t < P && ip < N;
+  RowFM fm{};
+  if (row) fm = load_fm(F, M, ip, N1, N2);
+  uint32_t live4 = 0;
+  if (4 * t < P * K && i < N) {
+    const size_t s0 = (size_t)p0 * K + 4 * t;
+    const uint4 jv = *reinterpret_cast<const uint4 *>(IL + s0);
+    const uint32_t tv = *reinterpret_cast<const uint32_t *>(ILtype + s0);
+    const uint32_t js[4] = {jv.x, jv.y, jv.z, jv.w};
+    bool cand[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      cand[u] = js[u] != kSentinel && ((tv >> (8 * u)) & 0xFFu) == 0;
+    // a slot's spring is cleared unless it holds a bond or a touching contact
+    bool clear[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) clear[u] = js[u] == kSentinel;
+    if (cand[0] | cand[1] | cand[2] | cand[3]) {
+      const float Rix = R[i], Riy = R[i + N1], Riz = R[i + N2];
+      float rx[4] = {}, ry[4] = {}, rz[4] = {};
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (!cand[u]) continue;
+        rx[u] = R[js[u]] - Rix;
+        ry[u] = R[js[u] + N1] - Riy;
+        rz[u] = R[js[u] + N2] - Riz;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (!cand[u]) continue;
+        const float rm = norm3(rx[u], ry[u], rz[u]);
+        if (!(rm < 2.0f * L.r)) {
+          clear[u] = true;
+          continue;
+        }
+        const uint32_t live = contact_law(V, W, Oijt, i, js[u], s0 + u, N1, N2, rx[u], ry[u],
+                                          rz[u], rm, L, s_c, 4 * t + u);
+        live4 |= live << (8 * u);
+      }
+    }
+    if (clear[0] & clear[1] & clear[2] & clear[3]) {
+      float4 *o = reinterpret_cast<float4 *>(Oijt + 3 * s0);
+      const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      o[0] = z;
+      o[1] = z;
+      o[2] = z;
     } else {
-      msx = 0;
-      msy = 0;
-      msz = 0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (clear[u]) st3(Oijt, s0 + u, 0.0f, 0.0f, 0.0f);
     }
   }
-  F[ip] += fsx;
-  F[ip + N1] += fsy;
-  F[ip + N2] += fsz;
-  M[ip] += msx;
-  M[ip + N1] += msy;
-  M[ip + N2] += msz;
+  s_live4[t] = live4;
+  __syncthreads();
+  if (!row) return;
+  contact_row_sums<true>(s_c, reinterpret_cast<const uint8_t *>(s_live4), t, K, fm, ip, N1, N2,
+                         F, M);
 }
 
 // d_SumUpForcesDEM (md_sumupforcesdem.cu:19-60)
@@ -624,11 +797,19 @@ int rfd_forces_contact(rfd_ctx *ctx, const float *R, const float *V, const float
                          (uint64_t)N * K < 0xFFFFFFFFull,
                 "rfd_forces_contact: bad arguments");
   RFD_CHECK_ARG(ctx, K <= (uint32_t)kMaxK, "rfd_forces_contact: K=%u > %d", K, kMaxK);
-  const uint32_t P = kSlotThreads / K;
+  const ContactLaw L{dt, Estar, Gstar, pA, mu, mur, m, r};
+  const bool by4 = K % 4 == 0 && reinterpret_cast<uintptr_t>(IL) % 16 == 0 &&
+                   reinterpret_cast<uintptr_t>(Oijt) % 16 == 0 &&
+                   reinterpret_cast<uintptr_t>(ILtype) % 4 == 0;
+  const uint32_t P = (by4 ? kContact4Slots : kSlotThreads) / K;
   const unsigned blocks = (N + P - 1) / P;
   const uint32_t per = (blocks + kXcds - 1) / kXcds;
-  k_forces_contact<<<per * kXcds, kSlotThreads, 0, ctx->stream>>>(
-      R, V, W, IL, ILtype, Oijt, F, M, N, K, dt, Estar, Gstar, pA, mu, mur, m, r, blocks, per);
+  if (by4)
+    k_forces_contact4<<<per * kXcds, kSlotThreads, 0, ctx->stream>>>(R, V, W, IL, ILtype, Oijt,
+                                                                      F, M, N, K, L, blocks, per);
+  else
+    k_forces_contact<<<per * kXcds, kSlotThreads, 0, ctx->stream>>>(R, V, W, IL, ILtype, Oijt, F,
+                                                                     M, N, K, L, blocks, per);
   RFD_LAUNCHED(ctx, "rfd_forces_contact");
   return RFD_OK;
 }
