@@ -644,39 +644,71 @@ k_il_reconstruct_fine(const float *__restrict__ R, uint32_t N, const uint32_t *_
       uint32_t nf = 0;
       uint32_t fc[3];
       if (fine_coords(f, xi, yi, zi, fc)) {
-        const uint32_t hu = (uint32_t)f.h;
+        // the grid is built with f.h == kFineHalf
+        constexpr int kRows = 2 * kFineHalf + 1;
+        const uint32_t hu = (uint32_t)kFineHalf;
         const uint32_t xlo = fc[0] > hu ? fc[0] - hu : 0;
         const uint32_t xhi = fc[0] + hu < f.nx ? fc[0] + hu : f.nx - 1;
-        for (int dz = -f.h; dz <= f.h; ++dz) {
+        // a candidate within range whose place lies in one of the 8 search
+        // cells' ranges goes into the sorted keys
+        auto test = [&](const float4 &pj) {
+          const float rr = rec_r2(pj, xi, yi, zi);
+          if (rr < aacut && (double)rr > 1e-10) {
+            const uint32_t pl = __float_as_uint(pj.w);
+            uint32_t q = 8;
+#pragma unroll
+            for (uint32_t qq = 0; qq < 8; ++qq)
+              if (pl - cfirst[qq] < ccount[qq]) q = qq;
+            if (q < 8) {
+              uint32_t x = (q << kFineIdBits) | pl;
+#pragma unroll
+              for (int u = 0; u < kFineFound; ++u) {
+                const uint32_t lo2 = min(key[u], x), hi2 = max(key[u], x);
+                key[u] = lo2;
+                x = hi2;
+              }
+              ++nf;
+            }
+          }
+        };
+        for (int dz = -kFineHalf; dz <= kFineHalf; ++dz) {
           const int z = (int)fc[2] + dz;
           if (z < 0 || z >= (int)f.nz) continue;
-          for (int dy = -f.h; dy <= f.h; ++dy) {
-            const int y = (int)fc[1] + dy;
-            if (y < 0 || y >= (int)f.ny) continue;
-            const uint32_t base = f.nx * ((uint32_t)y + f.ny * (uint32_t)z);
-            const uint32_t lo = excl[base + xlo], hi = excl[base + xhi + 1];
-            for (uint32_t e = lo; e < hi; ++e) {
-              const float4 pj = Rf[e];
-              const float rr = rec_r2(pj, xi, yi, zi);
-              if (rr < aacut && (double)rr > 1e-10) {
-                // kept if its place lies in one of the 8 search cells' ranges
-                const uint32_t pl = __float_as_uint(pj.w);
-                uint32_t q = 8;
+          // the plane's x-row ranges, read together
+          uint32_t rlo[kRows], rhi[kRows];
 #pragma unroll
-                for (uint32_t qq = 0; qq < 8; ++qq)
-                  if (pl - cfirst[qq] < ccount[qq]) q = qq;
-                if (q < 8) {
-                  uint32_t x = (q << kFineIdBits) | pl;
-#pragma unroll
-                  for (int u = 0; u < kFineFound; ++u) {
-                    const uint32_t lo2 = min(key[u], x), hi2 = max(key[u], x);
-                    key[u] = lo2;
-                    x = hi2;
-                  }
-                  ++nf;
-                }
-              }
+          for (int r = 0; r < kRows; ++r) {
+            const int y = (int)fc[1] + r - kFineHalf;
+            rlo[r] = rhi[r] = 0;
+            if (y >= 0 && y < (int)f.ny) {
+              const uint32_t base = f.nx * ((uint32_t)y + f.ny * (uint32_t)z);
+              rlo[r] = excl[base + xlo];
+              rhi[r] = excl[base + xhi + 1];
             }
+          }
+          // the plane's candidates as one run v = 0 .. tot - 1 over its rows,
+          // kBatch at a time with their loads in flight together (v is
+          // clamped to the run, so every load reads a candidate of a row)
+          uint32_t cum[kRows + 1];
+          cum[0] = 0;
+#pragma unroll
+          for (int r = 0; r < kRows; ++r) cum[r + 1] = cum[r] + (rhi[r] - rlo[r]);
+          const uint32_t tot = cum[kRows];
+          constexpr uint32_t kBatch = 8;
+          for (uint32_t v = 0; v < tot; v += kBatch) {
+            float4 pj[kBatch];
+#pragma unroll
+            for (uint32_t u = 0; u < kBatch; ++u) {
+              const uint32_t vv = min(v + u, tot - 1);
+              uint32_t e = rlo[0] + vv;
+#pragma unroll
+              for (int r = 1; r < kRows; ++r)
+                if (vv >= cum[r]) e = rlo[r] + (vv - cum[r]);
+              pj[u] = Rf[e];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kBatch; ++u)
+              if (v + u < tot) test(pj[u]);
           }
         }
       } else {
