@@ -170,6 +170,18 @@ __global__ void k_rank_cells(const uint32_t *__restrict__ tmp,
     uint32_t n = pnC[C + c];
     if (n == 0) continue;
     uint32_t s0 = pnC[c];
+    if (n <= 64) {
+      // members read once, one per lane; each compared with the others'
+      // ids read from their lanes
+      const uint32_t id = lane < n ? tmp[s0 + lane] : 0u;
+      uint32_t rank = 0;
+      for (uint32_t t = 0; t < n; ++t) rank += (__shfl(id, (int)t, 64) < id) ? 1u : 0u;
+      if (lane < n) {
+        CIs[s0 + rank] = c;
+        CIs[N + s0 + rank] = id;
+      }
+      continue;
+    }
     for (uint32_t e = lane; e < n; e += 64) {
       uint32_t id = tmp[s0 + e];
       uint32_t rank = 0;

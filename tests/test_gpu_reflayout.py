@@ -386,6 +386,30 @@ def test_cell_distribution_many_chunks_bitexact(ctx):
     assert counts.sum() == N and (counts == 0).sum() > grid.C // 2 and counts.max() > 1
 
 
+@pytest.mark.parametrize("crowd", [60, 65, 200])
+def test_cell_distribution_crowded_cells_bitexact(ctx, crowd):
+    """Cells holding up to 64 particles are ranked by id within one wave's
+    lanes, larger ones by the general loop: `crowd` particles in one cell and
+    crowd - 1 in another (plus the sparse ones that fall there: 63 and 61,
+    67 and 65, 203 and 202 members), in shuffled id order."""
+    from rfd.reflayout import Grid
+
+    rng = np.random.default_rng(crowd)
+    sparse = rng.uniform(0.0, 1.0, (3000, 3)) * np.array([10.0, 10.0, 10.0]) * 0.999
+    a = rng.uniform(2.05, 2.45, (crowd, 3))           # inside cell (2, 2, 2) of side 1
+    b = rng.uniform(7.05, 7.45, (crowd - 1, 3))       # inside cell (7, 7, 7)
+    pos = np.concatenate([sparse, a, b])[rng.permutation(3000 + 2 * crowd - 1)]
+    R = np.ascontiguousarray(pos.T.reshape(-1)).astype(np.float32)
+    grid = Grid(1.0, 1.0, 10, 10, 10)
+    g, o = RefLayout(ctx, R, 4, grid), OracleLayout(R, 4, grid)
+    ctx.flags()
+    g.cell_distribution()
+    o.cell_distribution()
+    assert o.last_status == 0 and ctx.flags() == 0
+    assert_same(g.host(), o.host(), keys=("CI", "CIs", "pnC"), where=f"crowd {crowd}")
+    assert o.host()["pnC"][grid.C:].max() >= crowd
+
+
 def test_cell_range_flag(ctx):
     """A particle outside the grid raises RFD_FLAG_CELL_RANGE, not a fault."""
     import torch
