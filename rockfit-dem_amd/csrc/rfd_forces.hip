@@ -480,17 +480,26 @@ k_forces_contact(const float *__restrict__ R, const float *__restrict__ V,
 // four slots, the partners' positions gathered together, and the springs of
 // four empty or apart slots cleared by three 16-byte stores.  Most slots are
 // empty, bonds, or candidates that do not touch, so this is the common path.
+// The touching slots of the workgroup are numbered in slot order (a scan of
+// per-thread counts) and their terms kept in kContactCap LDS columns, so the
+// workgroup holds 12 KB of terms instead of 48 KB; a workgroup with more
+// touching slots than that runs the law and the row sums in rounds of
+// kContactCap numbers, each round's terms added in slot order after the last.
 constexpr uint32_t kContact4Slots = 4 * kSlotThreads;
+constexpr uint32_t kContactCap = 256;
 __global__ void __launch_bounds__(kSlotThreads)
 k_forces_contact4(const float *__restrict__ R, const float *__restrict__ V,
                   const float *__restrict__ W, const uint32_t *__restrict__ IL,
                   const uint8_t *__restrict__ ILtype, float *__restrict__ Oijt,
                   float *__restrict__ F, float *__restrict__ M, uint32_t N, uint32_t K,
                   ContactLaw L, uint32_t nblk, uint32_t per) {
-  __shared__ float s_c[12][kContact4Slots];
-  __shared__ uint32_t s_live4[kSlotThreads];  // s_live of slots 4t .. 4t + 3, a byte each
+  __shared__ float s_c[12][kContactCap];
+  // per thread, a 16-bit entry for each of its four slots: 0, or the slot's
+  // number among the touching slots << 2 | the law's result (1 or 2)
+  __shared__ uint64_t s_meta[kSlotThreads];
+  __shared__ uint32_t s_wave[kSlotThreads / 64];
   const uint32_t P = kContact4Slots / K;
-  const uint32_t t = threadIdx.x;
+  const uint32_t t = threadIdx.x, lane = t & 63, wid = t >> 6;
   const uint32_t blk = xcd_block(per);
   if (blk >= nblk) return;
   const uint32_t p0 = blk * P;
@@ -500,12 +509,17 @@ k_forces_contact4(const float *__restrict__ R, const float *__restrict__ V,
   const bool row = t < P && ip < N;
   RowFM fm{};
   if (row) fm = load_fm(F, M, ip, N1, N2);
-  uint32_t live4 = 0;
+  const size_t s0 = (size_t)p0 * K + 4 * t;
+  uint32_t js[4] = {0, 0, 0, 0};
+  bool touch[4] = {false, false, false, false};
+  float rx[4] = {}, ry[4] = {}, rz[4] = {};
   if (4 * t < P * K && i < N) {
-    const size_t s0 = (size_t)p0 * K + 4 * t;
     const uint4 jv = *reinterpret_cast<const uint4 *>(IL + s0);
     const uint32_t tv = *reinterpret_cast<const uint32_t *>(ILtype + s0);
-    const uint32_t js[4] = {jv.x, jv.y, jv.z, jv.w};
+    js[0] = jv.x;
+    js[1] = jv.y;
+    js[2] = jv.z;
+    js[3] = jv.w;
     bool cand[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u)
@@ -516,7 +530,6 @@ k_forces_contact4(const float *__restrict__ R, const float *__restrict__ V,
     for (int u = 0; u < 4; ++u) clear[u] = js[u] == kSentinel;
     if (cand[0] | cand[1] | cand[2] | cand[3]) {
       const float Rix = R[i], Riy = R[i + N1], Riz = R[i + N2];
-      float rx[4] = {}, ry[4] = {}, rz[4] = {};
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         if (!cand[u]) continue;
@@ -527,14 +540,8 @@ k_forces_contact4(const float *__restrict__ R, const float *__restrict__ V,
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         if (!cand[u]) continue;
-        const float rm = norm3(rx[u], ry[u], rz[u]);
-        if (!(rm < 2.0f * L.r)) {
-          clear[u] = true;
-          continue;
-        }
-        const uint32_t live = contact_law(V, W, Oijt, i, js[u], s0 + u, N1, N2, rx[u], ry[u],
-                                          rz[u], rm, L, s_c, 4 * t + u);
-        live4 |= live << (8 * u);
+        touch[u] = norm3(rx[u], ry[u], rz[u]) < 2.0f * L.r;
+        clear[u] = !touch[u];
       }
     }
     if (clear[0] & clear[1] & clear[2] & clear[3]) {
@@ -549,11 +556,71 @@ k_forces_contact4(const float *__restrict__ R, const float *__restrict__ V,
         if (clear[u]) st3(Oijt, s0 + u, 0.0f, 0.0f, 0.0f);
     }
   }
-  s_live4[t] = live4;
+  // number the touching slots in slot order: the count of this thread (0-4)
+  // is summed over the lanes below by its three bits' ballots
+  const uint32_t cnt = (uint32_t)touch[0] + touch[1] + touch[2] + touch[3];
+  const uint64_t below = (1ull << lane) - 1ull;
+  uint32_t before = 0, wtot = 0;
+#pragma unroll
+  for (int bit = 0; bit < 3; ++bit) {
+    const uint64_t bb = __ballot((cnt >> bit) & 1u);
+    before += (uint32_t)__popcll(bb & below) << bit;
+    wtot += (uint32_t)__popcll(bb) << bit;
+  }
+  if (lane == 0) s_wave[wid] = wtot;
   __syncthreads();
+  uint32_t base = before, total = 0;
+  for (uint32_t w = 0; w < kSlotThreads / 64; ++w) {
+    base += w < wid ? s_wave[w] : 0u;
+    total += s_wave[w];
+  }
+  float f[3] = {0, 0, 0}, mm[3] = {0, 0, 0};
+  uint64_t meta = 0;
+  for (uint32_t w0 = 0; w0 < total; w0 += kContactCap) {
+    // one copy of the law, its slot's values picked by selects (the arrays
+    // stay in registers)
+    auto pick = [](const auto (&a)[4], uint32_t u) {
+      return u == 0 ? a[0] : u == 1 ? a[1] : u == 2 ? a[2] : a[3];
+    };
+    uint32_t num = base;
+#pragma unroll 1
+    for (uint32_t u = 0; u < 4; ++u) {
+      const bool tu = u == 0 ? touch[0] : u == 1 ? touch[1] : u == 2 ? touch[2] : touch[3];
+      if (!tu) continue;
+      if (num >= w0 && num < w0 + kContactCap) {
+        const float x = pick(rx, u), y = pick(ry, u), z = pick(rz, u);
+        const uint32_t code = contact_law(V, W, Oijt, i, pick(js, u), s0 + u, N1, N2, x, y, z,
+                                          norm3(x, y, z), L, s_c, num - w0);
+        meta |= (uint64_t)((num << 2) | code) << (16 * u);
+      }
+      ++num;
+    }
+    s_meta[t] = meta;
+    __syncthreads();
+    if (row) {
+      // the row's slots in order, four per entry of s_meta; only those
+      // numbered in this round's window
+      for (uint32_t q4 = t * K / 4; q4 < (t + 1) * K / 4; ++q4) {
+        uint64_t m = s_meta[q4];
+        while (m) {
+          const uint32_t u = __builtin_ctzll(m) >> 4;
+          const uint32_t v = (uint32_t)(m >> (16 * u)) & 0xFFFFu;
+          m &= ~(0xFFFFull << (16 * u));
+          const uint32_t num2 = v >> 2;
+          if (num2 >= w0 && num2 < w0 + kContactCap)
+            add_contact_terms(s_c, num2 - w0, v & 3u, f, mm);
+        }
+      }
+    }
+    __syncthreads();
+  }
   if (!row) return;
-  contact_row_sums<true>(s_c, reinterpret_cast<const uint8_t *>(s_live4), t, K, fm, ip, N1, N2,
-                         F, M);
+  F[ip] = fm.fx + f[0];
+  F[ip + N1] = fm.fy + f[1];
+  F[ip + N2] = fm.fz + f[2];
+  M[ip] = fm.mx + mm[0];
+  M[ip + N1] = fm.my + mm[1];
+  M[ip + N2] = fm.mz + mm[2];
 }
 
 // d_SumUpForcesDEM (md_sumupforcesdem.cu:19-60)

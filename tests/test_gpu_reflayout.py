@@ -207,6 +207,49 @@ def test_forces_bitexact_with_motion(ctx):
     assert np.abs(o.F).max() > 0
 
 
+@pytest.mark.parametrize("K", [32, 27])
+def test_contact_forces_dense_bitexact(ctx, K):
+    """Every neighbour pair a touching contact: the lattice cylinder shrunk by
+    3 % about its centre, lists built without bonds, so each particle touches
+    about 12 others. At K = 32 a workgroup of the four-slot contact kernel
+    then holds over 256 touching slots and runs the law and the row sums in
+    rounds; K = 27 takes the one-slot kernel."""
+    import torch
+
+    case = cases.reference_uniaxial(8.0, 12.0, seed=6, K=K)
+    N, m = case.N, case.mat
+    R = case.R.reshape(3, N).astype(np.float64)
+    c = R.mean(axis=1, keepdims=True)
+    R = np.ascontiguousarray((c + (R - c) * 0.97).reshape(-1)).astype(np.float32)
+    grid = case.grid()
+    g, o = RefLayout(ctx, R, K, grid), OracleLayout(R, K, grid)
+    for s in (g, o):
+        s.cell_distribution()
+        s.il_construct(float(np.float32(m.a_farcut) ** 2))
+    dv, dw = perturb(N, 61, 1e-3), perturb(N, 62, 2e-3)
+    for it in range(3):
+        g.V.copy_(torch.from_numpy(dv * (it + 1)).cuda())
+        g.W.copy_(torch.from_numpy(dw * (it + 1)).cuda())
+        o.V[:] = dv * (it + 1)
+        o.W[:] = dw * (it + 1)
+        for s in (g, o):
+            s.zero_fm()
+            s.forces_contact(m)
+        assert_same(g.host(), o.host(), where=f"K={K} iteration {it}")
+    # touching slots per 1024-slot group (32 rows at K = 32)
+    IL = o.IL.reshape(N, K)
+    Rr = R.reshape(3, N)
+    live = np.zeros(N, np.int64)
+    for k in range(K):
+        j = IL[:, k]
+        ok = j != 0xFFFFFFFF
+        d = np.linalg.norm(Rr[:, j[ok]] - Rr[:, ok], axis=0)
+        live[ok] += d < 2 * m.p_r
+    per_group = np.add.reduceat(live, np.arange(0, N, 1024 // K))
+    print(f"K={K}: touching per particle {live.mean():.1f}, per group max {per_group.max()}")
+    assert per_group.max() > 256 and ctx.flags() == 0
+
+
 @pytest.mark.parametrize("k", [1, 2, 5, 64])
 def test_forces_short_and_long_rows_bitexact(ctx, k):
     """The force kernels at row lengths far from 32: the rows of a K = 32
