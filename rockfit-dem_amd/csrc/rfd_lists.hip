@@ -41,20 +41,44 @@ __device__ __forceinline__ bool cell_coords(const float *__restrict__ R, uint32_
   return ix[0] < Nx && ix[1] < Ny && ix[2] < Nz;
 }
 
+// Runs of equal keys in consecutive lanes of a wave (every lane active): the
+// lane's offset in its run, the run's first lane, and at that lane the run's
+// length.  Consecutive particles mostly share a cell, so a run takes one
+// atomic where each lane took one.
+struct WaveRun {
+  uint32_t first, off, len;
+};
+__device__ __forceinline__ WaveRun wave_run(uint32_t key) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t prev = __shfl_up(key, 1, 64);
+  const uint64_t heads = __ballot(lane == 0 || prev != key);
+  const uint64_t upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+  const uint32_t first = 63u - (uint32_t)__clzll(upto);
+  const uint64_t after = heads & ~(lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+  const uint32_t next = after ? (uint32_t)__ffsll((long long)after) - 1u : 64u;
+  return WaveRun{first, lane - first, next - lane};
+}
+
 __global__ void k_cell_index(const float *__restrict__ R, uint32_t N, float inv_a,
                              uint32_t Nx, uint32_t Ny, uint32_t Nz,
                              uint32_t *__restrict__ CI, uint32_t *__restrict__ cnt,
                              uint32_t *__restrict__ flags) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
-    uint32_t ix[3];
-    if (!cell_coords(R, N, i, inv_a, Nx, Ny, Nz, ix)) {
-      CI[i] = kSentinel;
-      atomicOr(flags, RFD_FLAG_CELL_RANGE);
-      continue;
+  // whole waves step together (the runs need every lane)
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint32_t i0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); i0 < N;
+       i0 += grid_stride()) {
+    const uint32_t i = i0 + lane;
+    uint32_t c = kSentinel;
+    if (i < N) {
+      uint32_t ix[3];
+      if (!cell_coords(R, N, i, inv_a, Nx, Ny, Nz, ix))
+        atomicOr(flags, RFD_FLAG_CELL_RANGE);
+      else
+        c = ix[0] + ix[1] * Nx + ix[2] * Ny * Nx;
+      CI[i] = c;
     }
-    uint32_t c = ix[0] + ix[1] * Nx + ix[2] * Ny * Nx;
-    CI[i] = c;
-    atomicAdd(&cnt[c], 1u);
+    const WaveRun run = wave_run(c);
+    if (run.off == 0 && c != kSentinel) atomicAdd(&cnt[c], run.len);
   }
 }
 
@@ -151,11 +175,16 @@ k_scan_cells(uint32_t *__restrict__ pnC, uint32_t C, uint32_t *__restrict__ curs
 
 __global__ void k_scatter(const uint32_t *__restrict__ CI, uint32_t N,
                           uint32_t *__restrict__ cursor, uint32_t *__restrict__ tmp) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
-    uint32_t c = CI[i];
-    if (c == kSentinel) continue;
-    uint32_t p = atomicAdd(&cursor[c], 1u);
-    tmp[p] = i;
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint32_t i0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); i0 < N;
+       i0 += grid_stride()) {
+    const uint32_t i = i0 + lane;
+    const uint32_t c = i < N ? CI[i] : kSentinel;
+    const WaveRun run = wave_run(c);
+    uint32_t p = 0;
+    if (run.off == 0 && c != kSentinel) p = atomicAdd(&cursor[c], run.len);
+    p = __shfl(p, (int)run.first, 64) + run.off;
+    if (c != kSentinel) tmp[p] = i;
   }
 }
 
