@@ -566,15 +566,19 @@ __global__ void k_place_of(const uint32_t *__restrict__ CIs, uint32_t N,
 
 __global__ void k_fine_index(const float *__restrict__ R, uint32_t N, FineGrid f,
                              uint32_t *__restrict__ fci, uint32_t *__restrict__ cnt) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
-    uint32_t c3[3];
-    if (!fine_coords(f, R[i], R[i + N], R[i + 2 * (size_t)N], c3)) {
-      fci[i] = kSentinel;
-      continue;
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint32_t i0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); i0 < N;
+       i0 += grid_stride()) {
+    const uint32_t i = i0 + lane;
+    uint32_t c = kSentinel;
+    if (i < N) {
+      uint32_t c3[3];
+      if (fine_coords(f, R[i], R[i + N], R[i + 2 * (size_t)N], c3))
+        c = c3[0] + f.nx * (c3[1] + f.ny * c3[2]);
+      fci[i] = c;
     }
-    const uint32_t c = c3[0] + f.nx * (c3[1] + f.ny * c3[2]);
-    fci[i] = c;
-    atomicAdd(&cnt[c], 1u);
+    const WaveRun run = wave_run(c);
+    if (run.off == 0 && c != kSentinel) atomicAdd(&cnt[c], run.len);
   }
 }
 
@@ -609,11 +613,17 @@ __global__ void k_fine_scatter(const float *__restrict__ R, uint32_t N,
                                const uint32_t *__restrict__ placeOf,
                                const uint32_t *__restrict__ excl, uint32_t *__restrict__ cnt,
                                float4 *__restrict__ Rf) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += grid_stride()) {
-    const uint32_t c = fci[i];
-    if (c == kSentinel) continue;
-    const uint32_t p = excl[c] + atomicAdd(&cnt[c], 1u);
-    Rf[p] = make_float4(R[i], R[i + N], R[i + 2 * (size_t)N], __uint_as_float(placeOf[i]));
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint32_t i0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); i0 < N;
+       i0 += grid_stride()) {
+    const uint32_t i = i0 + lane;
+    const uint32_t c = i < N ? fci[i] : kSentinel;
+    const WaveRun run = wave_run(c);
+    uint32_t p = 0;
+    if (run.off == 0 && c != kSentinel) p = excl[c] + atomicAdd(&cnt[c], run.len);
+    p = __shfl(p, (int)run.first, 64) + run.off;
+    if (c != kSentinel)
+      Rf[p] = make_float4(R[i], R[i + N], R[i + 2 * (size_t)N], __uint_as_float(placeOf[i]));
   }
 }
 
